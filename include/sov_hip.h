@@ -1,0 +1,256 @@
+/*
+ * sov_hip.h -- the C-ABI over this project's hand-written gfx950 kernels.
+ *
+ * This is the drop-in boundary under the reference's L0 Backend seam
+ * (reference include/sovereign/backend.hpp:82-165).  The C++ wrapper
+ * sovereign-solver_amd/src/l0/backend_hip.cpp implements BackendKind::Hip on
+ * top of it, and src/l1/pdhg.cpp drives the fused PDHG kernels through it.
+ * Plain pointers and sizes only; nothing throws across it.
+ *
+ * Return codes equal sov::Status (reference include/sovereign/status.hpp:25-34):
+ * 0 Ok, 1 Unsupported, 2 NotImplemented, 3 InvalidArgument, 4 DimensionMismatch,
+ * 5 OutOfMemory, 6 BackendError, 7 NumericalFailure.  sov_hip_last_error()
+ * returns the calling thread's last message.
+ *
+ * Threading: a context owns one HIP stream and its own scratch; use one
+ * context per host thread.  Every call is ordered on that stream; calls that
+ * return host scalars synchronise that stream only (never the device).
+ *
+ * Device pointers are those returned by sov_hip_malloc (or any allocation of
+ * the same device).  "Vector" arguments whose element type depends on a
+ * precision argument are void*: SOV_PREC_F64 -> double, SOV_PREC_F32 -> float.
+ */
+#ifndef SOV_HIP_H
+#define SOV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SOV_OK = 0,
+    SOV_UNSUPPORTED = 1,
+    SOV_NOT_IMPLEMENTED = 2,
+    SOV_INVALID_ARGUMENT = 3,
+    SOV_DIMENSION_MISMATCH = 4,
+    SOV_OUT_OF_MEMORY = 5,
+    SOV_BACKEND_ERROR = 6,
+    SOV_NUMERICAL_FAILURE = 7
+};
+
+enum { SOV_PREC_F64 = 0, SOV_PREC_F32 = 1 };
+
+typedef struct sov_hip_ctx sov_hip_ctx;
+typedef struct sov_hip_csr sov_hip_csr;
+typedef struct sov_hip_graph sov_hip_graph;
+
+/* ---- runtime / context ------------------------------------------------
+ * Replaces the reference HIP backend's constructor and device query
+ * (src/l0/backend_hip.cpp:56-90). */
+int sov_hip_device_count(int* count);
+int sov_hip_ctx_create(int device, sov_hip_ctx** out);
+void sov_hip_ctx_destroy(sov_hip_ctx* ctx);
+const char* sov_hip_ctx_device_name(const sov_hip_ctx* ctx);
+/* Number of compute units and the device's L2/MALL sizes, for schedules. */
+int sov_hip_ctx_props(const sov_hip_ctx* ctx, int* cus, int64_t* l2_bytes, int64_t* mall_bytes);
+const char* sov_hip_last_error(void);
+
+/* ---- memory -----------------------------------------------------------
+ * Backend::allocate / deallocate / copy_to_device / copy_to_host /
+ * copy_device_to_device / fill_zero / synchronize (backend.hpp:100-108;
+ * reference HIP versions backend_hip.cpp:92-124).  Copies are ordered on the
+ * context stream; h2d/d2h return when the copy is complete.  sov_hip_sync
+ * synchronises the context's stream, not the device. */
+int sov_hip_malloc(sov_hip_ctx* ctx, size_t bytes, void** out);
+int sov_hip_free(sov_hip_ctx* ctx, void* ptr);
+int sov_hip_memcpy_h2d(sov_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sov_hip_memcpy_d2h(sov_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sov_hip_memcpy_d2d(sov_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sov_hip_fill_zero(sov_hip_ctx* ctx, void* dst, size_t bytes);
+int sov_hip_sync(sov_hip_ctx* ctx);
+
+/* ---- level 1, FP64 ----------------------------------------------------
+ * Backend::axpy/scale/dot/norm2/max_abs (backend.hpp:111-115; reference
+ * rocBLAS calls backend_hip.cpp:126-146, Unsupported max_abs :169-175).
+ * dot/nrm2/amax write a HOST scalar; reductions are two-stage with a fixed
+ * order, so results are run-to-run reproducible.  n == 0 is a no-op (dot and
+ * nrm2 write 0). */
+int sov_hip_axpy_f64(sov_hip_ctx* ctx, int64_t n, double alpha, const double* x, double* y);
+int sov_hip_scal_f64(sov_hip_ctx* ctx, int64_t n, double alpha, double* x);
+int sov_hip_dot_f64(sov_hip_ctx* ctx, int64_t n, const double* x, const double* y, double* result);
+int sov_hip_nrm2_f64(sov_hip_ctx* ctx, int64_t n, const double* x, double* result);
+int sov_hip_amax_f64(sov_hip_ctx* ctx, int64_t n, const double* x, double* result);
+/* Backend::project_box (backend.hpp:135; reference kernel
+ * backend_hip.cpp:46-53).  lo or hi may be NULL (unbounded side). */
+int sov_hip_project_box_f64(sov_hip_ctx* ctx, int64_t n, const double* lo, const double* hi,
+                            double* x);
+/* Precision images. */
+int sov_hip_convert_f64_f32(sov_hip_ctx* ctx, int64_t n, const double* src, float* dst);
+int sov_hip_convert_f32_f64(sov_hip_ctx* ctx, int64_t n, const float* src, double* dst);
+
+/* ---- CSR handle -------------------------------------------------------
+ * Replaces DeviceCsr's device half and the per-call cuSPARSE descriptors of
+ * the reference (backend_cuda.cu:242-297).  The handle takes A (m x n, nnz,
+ * int32 row_ptr/col_idx with strictly ascending columns, FP64 values) already
+ * on the device and builds A^T as its own CSR ON THE DEVICE (rows of A^T in
+ * ascending A-row order), so the transposed product is a gather with no
+ * atomics and no extra host->device traffic.  It also builds the row-length
+ * schedules both orientations use.
+ *   flags & SOV_CSR_OWN_VALUES: copy the values so sov_hip_csr_scale may
+ *     rescale them in place (otherwise A's value array is borrowed and must
+ *     outlive the handle; the structure arrays are always borrowed).
+ * Row-length schedules: each orientation's rows are grouped by length class
+ * and each class is processed by G lanes per row, G in {1,..,64}; the lane
+ * counts come from sov_hip_csr_tune (an autotuner that times candidates) or
+ * from a length heuristic. */
+enum { SOV_CSR_OWN_VALUES = 1 };
+int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz,
+                       const int32_t* d_row_ptr, const int32_t* d_col_idx, const double* d_values,
+                       int flags, sov_hip_csr** out);
+void sov_hip_csr_destroy(sov_hip_csr* csr);
+int sov_hip_csr_dims(const sov_hip_csr* csr, int32_t* m, int32_t* n, int32_t* nnz);
+/* Build the FP32 value images of both orientations from the current FP64 values. */
+int sov_hip_csr_make_f32(sov_hip_ctx* ctx, sov_hip_csr* csr);
+/* A <- diag(row_scale) A diag(col_scale) on the owned values (both
+ * orientations), value = (v * r_i) * c_j; entries whose magnitude falls to
+ * <= 1e-14 become exact zeros (the reference drops them when it rebuilds
+ * the scaled problem, scaling.cpp:194-204 / sparse.cpp:238). */
+int sov_hip_csr_scale(sov_hip_ctx* ctx, sov_hip_csr* csr, const double* d_row_scale,
+                      const double* d_col_scale);
+/* Copy the current FP64 values of A^T into dst (nnz) / its row_ptr (n+1) and
+ * col_idx (nnz) -- for tests. */
+int sov_hip_csr_transpose_arrays(sov_hip_ctx* ctx, const sov_hip_csr* csr, int32_t* d_tptr,
+                                 int32_t* d_tidx, double* d_tval);
+/* Set the lanes-per-row of each length class: lanes[c] for class c in
+ * [0, 8), both orientations (orient 0 = A, 1 = A^T).  0 keeps the heuristic. */
+int sov_hip_csr_set_lanes(sov_hip_csr* csr, int orient, const int32_t lanes[8]);
+int sov_hip_csr_get_lanes(const sov_hip_csr* csr, int orient, int32_t lanes[8]);
+/* Autotuner: for each orientation and length class, time every G in
+ * {1,2,4,8,16,32,64} on the SpMV of precision prec and keep the fastest. */
+int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* csr, int prec);
+
+/* ---- sparse level 2 ---------------------------------------------------
+ * Backend::spmv / spmv_transpose (backend.hpp:124-129; reference cuSPARSE
+ * calls backend_cuda.cu:220-227).  y := alpha*op(A)*x + beta*y, op = A
+ * (transpose == 0) or A^T (transpose == 1).  beta == 0 overwrites y (NaN-safe).
+ * prec selects the value image and the element type of x and y. */
+int sov_hip_spmv(sov_hip_ctx* ctx, const sov_hip_csr* csr, int transpose, int prec,
+                 double alpha, const void* x, double beta, void* y);
+
+/* ---- scaling (kernel d) -----------------------------------------------
+ * Ruiz and Pock-Chambolle factors (reference scaling.cpp:36-101) from the
+ * handle's CURRENT values.  One Ruiz sweep: row/column inf-norms of
+ * |a_ij| r_i c_j, r_i /= sqrt(row norm), c_j /= sqrt(col norm) where the norm
+ * is positive; *worst receives max |norm - 1| (host scalar).  Pock-Chambolle:
+ * r_i /= sqrt(sum_j m^(2-alpha)), c_j /= sqrt(sum_i m^alpha), m = |a| r c.
+ * Column reductions run over the A^T CSR (no atomics); sums are serial per
+ * row in ascending index order, so factors match the host restatement. */
+int sov_hip_scale_ruiz_sweep(sov_hip_ctx* ctx, const sov_hip_csr* csr, double* d_row_scale,
+                             double* d_col_scale, double* worst);
+int sov_hip_scale_pock_chambolle(sov_hip_ctx* ctx, const sov_hip_csr* csr, double alpha,
+                                 double* d_row_scale, double* d_col_scale);
+
+/* ---- fused PDHG step kernels (kernels a+b) ----------------------------
+ * One iteration of reference pdhg.cpp:295-331 is two launches:
+ *   primal: per column j, g = c_j - (A^T y)_j; x+ = clamp(x - tau g, lo, hi);
+ *           x_bar = 2 x+ - x; x = x+; x_sum = (first ? x+ : x_sum + x+)
+ *   dual:   per row i, s+ = clamp(s - tau y, lo, hi); s_bar = 2 s+ - s;
+ *           y+ = y + sigma (s_bar - (A x_bar)_i); s = s+; y = y+;
+ *           y_sum = (first ? y+ : y_sum + y+)
+ * All vectors have the element type of prec; bounds may hold +-inf. */
+typedef struct sov_hip_pdhg_state {
+    int32_t prec;
+    int32_t m, n;
+    const void* c;       /* n: internal-sense (minimize) scaled objective */
+    const void* col_lo;  /* n */
+    const void* col_hi;  /* n */
+    const void* row_lo;  /* m */
+    const void* row_hi;  /* m */
+    void* x;             /* n */
+    void* x_bar;         /* n */
+    void* x_sum;         /* n */
+    void* s;             /* m */
+    void* y;             /* m */
+    void* y_sum;         /* m */
+} sov_hip_pdhg_state;
+
+int sov_hip_pdhg_primal_step(sov_hip_ctx* ctx, const sov_hip_csr* csr,
+                             const sov_hip_pdhg_state* st, double tau, int first);
+int sov_hip_pdhg_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* csr,
+                           const sov_hip_pdhg_state* st, double tau, double sigma, int first);
+
+/* ---- KKT / restart reductions (kernel c) ------------------------------
+ * Evaluates, in FP64 on the device, for the current point (x, y) and the
+ * Cesaro average (x_sum, y_sum) / since_restart:
+ *   kkt_*   the scaled-space max-norm residual of pdhg.cpp:87-136
+ *           (bound_tol = tol_scaled)
+ *   orig_*  the original-space residual of pdhg.cpp:145-199
+ *           (bound_tol = tol_orig), via x = c_s x_s, y = sign r_s y_s
+ *   rel_*   relative KKT [r_p, r_d, gap] and the primal/dual objectives
+ *           (problem sense, with obj_const) of the original-space point.
+ * Max reductions propagate NaN (a NaN anywhere reads back as NaN).  If
+ * act_avg is not NULL it receives A_s (x_sum / since_restart) (m, FP64),
+ * the activity a restart needs. */
+typedef struct sov_hip_check_args {
+    const sov_hip_pdhg_state* st;
+    int64_t since_restart;
+    double tol_scaled;
+    double tol_orig;
+    double obj_sign;
+    double obj_const;
+    const double* row_scale;    /* m */
+    const double* col_scale;    /* n */
+    const double* orig_row_lo;  /* m */
+    const double* orig_row_hi;  /* m */
+    const double* orig_col_lo;  /* n */
+    const double* orig_col_hi;  /* n */
+    const double* orig_c;       /* n, problem sense */
+    double* act_avg;            /* m, optional */
+} sov_hip_check_args;
+
+typedef struct sov_hip_check_out {
+    double kkt_cur, kkt_avg;
+    double orig_cur, orig_avg;
+    double rel_cur[3], rel_avg[3];
+    double pobj_cur, dobj_cur, pobj_avg, dobj_avg;
+} sov_hip_check_out;
+
+int sov_hip_pdhg_check(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_check_args* a,
+                       sov_hip_check_out* out);
+
+/* Restart to the average (pdhg.cpp:393-404): x = clamp(x_sum/k, col box),
+ * y = y_sum/k, s = clamp(act_avg, row box).  If x_last/y_last are not NULL
+ * (FP64, n and m), movement[0] = ||x_new - x_last||_2^2 and
+ * movement[1] = ||y_new - y_last||_2^2 are returned (host) and x_last/y_last
+ * are overwritten with the new point (primal-weight bookkeeping). */
+int sov_hip_pdhg_restart(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, int64_t since_restart,
+                         const double* act_avg, double* x_last, double* y_last,
+                         double movement[2]);
+
+/* Set (x, s, y) from FP64 sources: x = clamp(x_src, col box), y = y_src,
+ * s = clamp(act, row box); NULL sources leave the vector as is.  Used for
+ * warm starts and for the FP32 -> FP64 switch. */
+int sov_hip_pdhg_set_point(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double* x_src,
+                           const double* y_src, const double* act);
+
+/* ---- launch graphs ----------------------------------------------------
+ * Capture the context stream's launches between begin and end into a graph
+ * and replay it.  Only the step kernels above may be captured (no host
+ * scalars, no allocation inside a capture). */
+int sov_hip_graph_begin(sov_hip_ctx* ctx);
+int sov_hip_graph_end(sov_hip_ctx* ctx, sov_hip_graph** out);
+int sov_hip_graph_launch(sov_hip_ctx* ctx, sov_hip_graph* g);
+void sov_hip_graph_destroy(sov_hip_graph* g);
+
+/* ---- timing -------------------------------------------------------------
+ * Event pair on the context stream, for in-process kernel timing. */
+int sov_hip_timer_start(sov_hip_ctx* ctx);
+int sov_hip_timer_stop(sov_hip_ctx* ctx, double* milliseconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOV_HIP_H */

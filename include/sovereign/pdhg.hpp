@@ -1,0 +1,152 @@
+// PDHG / PDLP engine.  API mirrors include/sovereign/pdhg.hpp:74-172 of the
+// reference: the same PdhgStatus/PdhgOptions/PdhgResult/Pdhg names, fields and
+// defaults, so `sovereign-cli solve --engine pdhg`, race_solve and
+// Crossover::solve take this engine unchanged.
+//
+// The iteration (reference pdhg.hpp:9-31) is Chambolle-Pock on the
+// slack-augmented saddle point  min_{x in X, s in S} max_y c'x + y'(s - Ax):
+//     x+ = Proj_X(x - tau (c - A'y))
+//     s+ = Proj_S(s - tau y)
+//     y+ = y + sigma ((2 s+ - s) - A (2 x+ - x))
+// with Cesaro-average restarts checked every restart_check_period iterations.
+//
+// Two loops sit behind one solve():
+//   * Primitive -- the reference's loop written against the virtual Backend
+//     primitives; runs on any backend and is the parity mode.
+//   * Fused     -- selected automatically on a Hip backend: per iteration one
+//     kernel fuses A'y with the primal update/projection/extrapolation/Cesaro
+//     sum and one fuses A x_bar with the slack and dual updates; scaling and
+//     all KKT/restart reductions run on the GPU in FP64, and only a few
+//     scalars cross the bus per check.
+//
+// Everything below the "additive" markers has no reference counterpart and
+// defaults to the reference's behaviour.
+#pragma once
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "sovereign/backend.hpp"
+#include "sovereign/cancellation.hpp"
+#include "sovereign/duals.hpp"
+#include "sovereign/numeric.hpp"
+#include "sovereign/problem.hpp"
+
+namespace sov {
+
+enum class PdhgStatus {
+    Optimal,
+    IterationLimit,
+    TimeLimit,
+    NumericalFailure,
+    NotSolved,
+    Cancelled,
+    // additive
+    PrimalInfeasible,   ///< a Farkas certificate in PdhgResult::certificate verifies
+    DualInfeasible,     ///< a primal ray in PdhgResult::ray verifies (LP unbounded)
+};
+
+const char* to_string(PdhgStatus s) noexcept;
+
+/// Additive: which loop solve() runs.
+enum class PdhgEngine { Auto, Primitive, Fused };
+
+/// Additive: the convergence test.  Reference = the original-space max-norm
+/// residual of pdhg.cpp:145-199 (< tolerance).  RelativeKkt = max(r_p, r_d,
+/// gap) <= tolerance with
+///   r_p = ||row violation of Ax||_2 / (1 + ||finite row bounds||_2)
+///   r_d = ||(c - A'y - lambda, y - proj(y))||_2 / (1 + ||c||_2)
+///   gap = |p - d| / (1 + |p| + |d|)
+/// all in original units, lambda/proj(y) being the projections onto the sign
+/// cones the column/row bounds allow.
+enum class PdhgTermination { Reference, RelativeKkt };
+
+struct PdhgOptions {
+    Real tolerance = 1e-4;
+    long long max_iterations = 1000000;
+    double time_limit_seconds = 0.0;
+    long long restart_check_period = 40;
+    Real restart_sufficient_decrease = 0.5;
+    Real step_size_safety = 0.9;
+    int power_iterations = 30;
+    bool verbose = false;
+    std::function<void(long long iteration, const std::vector<Real>& x_original,
+                       const std::vector<Real>& y_original, Real original_residual)>
+        checkpoint;
+
+    // ---- additive --------------------------------------------------------
+    PdhgEngine engine = PdhgEngine::Auto;
+    PdhgTermination termination = PdhgTermination::Reference;
+    /// F64: everything FP64.  F32: FP32 matrix values and iterates for the
+    /// SpMVs and updates, FP64 for every reduction, restart and weight
+    /// decision, then an FP64 polish (below).  Fused engine only.
+    Precision precision = Precision::F64;
+    /// FP32 runs switch to FP64 iterates (same matrix, FP64 values) once the
+    /// candidate's relative KKT is below fp32_switch_factor * tolerance or
+    /// has stopped improving for fp32_stall_checks checks.
+    bool fp64_polish = true;
+    Real fp32_switch_factor = 20.0;
+    int fp32_stall_checks = 8;
+    /// PDLP primal weight omega: tau = eta/omega, sigma = eta*omega, updated at
+    /// restarts from the restart-to-restart movement.  Off = the reference
+    /// (omega = 1 throughout).
+    bool use_primal_weight = false;
+    Real initial_primal_weight = 1.0;
+    Real primal_weight_smoothing = 0.5;
+    /// Certificate checks at every check period (fused engine).
+    bool detect_infeasibility = false;
+    /// Optional warm start: original-space row duals (length num_rows).
+    std::vector<Real> warm_start_dual;
+    /// Capture each check interval as a HIP graph and replay it.
+    bool use_graphs = true;
+    /// Device ordinal for a fused solve that creates its own backend context.
+    int device = 0;
+};
+
+struct PdhgResult {
+    PdhgStatus status = PdhgStatus::NotSolved;
+    Real objective = 0.0;
+    std::vector<Real> primal;
+    std::vector<Real> dual;
+    std::vector<Real> reduced_costs;
+    Real primal_infeasibility = 0.0;
+    Real dual_infeasibility = 0.0;
+    Real duality_gap = 0.0;
+    long long iterations = 0;
+    long long restarts = 0;
+    double seconds = 0.0;
+    Real operator_norm_estimate = 0.0;
+    std::string message;
+
+    // ---- additive --------------------------------------------------------
+    Real relative_primal_residual = 0.0;   ///< r_p of the reported point
+    Real relative_dual_residual = 0.0;     ///< r_d
+    Real relative_gap = 0.0;               ///< gap
+    Real primal_weight = 1.0;              ///< final omega
+    long long fp64_polish_iterations = 0;  ///< iterations run after an FP32->FP64 switch
+    std::string engine;                    ///< "primitive" or "fused"
+    FarkasCertificate certificate;         ///< set when PrimalInfeasible
+    PrimalRay ray;                         ///< set when DualInfeasible
+    double setup_seconds = 0.0;            ///< scaling, upload, operator norm
+    double loop_seconds = 0.0;             ///< the iteration loop alone
+};
+
+class Pdhg {
+public:
+    explicit Pdhg(PdhgOptions options = {}) : options_(std::move(options)) {}
+
+    PdhgResult solve(const Problem& problem, Backend& backend,
+                     const CancellationToken* cancel = nullptr) const;
+
+    const PdhgOptions& options() const noexcept { return options_; }
+
+private:
+    PdhgResult solve_primitive(const Problem& problem, Backend& backend,
+                               const CancellationToken* cancel) const;
+    PdhgResult solve_fused(const Problem& problem, Backend& backend,
+                           const CancellationToken* cancel) const;
+    PdhgOptions options_;
+};
+
+}  // namespace sov

@@ -1,0 +1,229 @@
+// Internal types and device helpers shared by the sov_hip translation units.
+// gfx950 (CDNA4): 64-wide wavefronts; all reductions are written for wave64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sov_hip.h"
+
+namespace sovhip {
+
+// ---------------------------------------------------------------------------
+// error plumbing: nothing throws across the C-ABI
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define SOV_HIP_TRY(expr)                                            \
+    do {                                                             \
+        hipError_t _e = (expr);                                      \
+        if (_e != hipSuccess) return ::sovhip::hip_fail(_e, #expr);  \
+    } while (0)
+
+constexpr int kBlock = 256;          // threads per workgroup (4 waves)
+constexpr int kWave = 64;
+constexpr int kClasses = 8;          // row-length classes of a schedule
+constexpr int kCheckSlots = 16;      // FP64 partials per block in the check kernels
+
+// Row-length class boundaries: class c holds rows with length <= kClassMax[c].
+constexpr int kClassMax[kClasses] = {2, 4, 8, 16, 32, 64, 256, 0x7fffffff};
+// Default lanes per row for each class.
+constexpr int kDefaultLanes[kClasses] = {1, 2, 4, 4, 8, 16, 32, 64};
+
+// Schedule passed by value to the row kernels.  Rows of class c are
+// list[offset[c] .. offset[c] + count[c]); they are processed by G[c] lanes
+// per row, kBlock / G[c] rows per workgroup, in workgroups
+// [block_begin[c], block_begin[c+1]).
+struct Sched {
+    int32_t G[kClasses];
+    int32_t count[kClasses];
+    int32_t offset[kClasses];
+    int32_t block_begin[kClasses + 1];
+    const int32_t* list;
+};
+
+struct Orient {
+    int32_t rows = 0, cols = 0;
+    const int32_t* ptr = nullptr;   // rows + 1
+    const int32_t* idx = nullptr;   // nnz
+    double* val64 = nullptr;        // nnz (owned or borrowed, see owns_val64)
+    float* val32 = nullptr;         // nnz, owned, optional
+    int32_t* list = nullptr;        // rows, owned: rows grouped by class
+    int32_t class_count[kClasses] = {};
+    int32_t class_offset[kClasses] = {};
+    int32_t lanes[kClasses] = {};
+    int64_t max_len = 0;
+    bool owns_val64 = false;
+    Sched sched() const;
+    int blocks() const;
+};
+
+}  // namespace sovhip
+
+struct sov_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    int cus = 0;
+    int64_t l2_bytes = 0;
+    int64_t mall_bytes = 0;
+    // FP64 scratch for two-stage reductions (device) and pinned host staging
+    double* d_red = nullptr;
+    size_t d_red_count = 0;
+    double* h_pinned = nullptr;   // 64 doubles
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool capturing = false;
+    int ensure_red(size_t count);
+};
+
+struct sov_hip_csr {
+    int32_t m = 0, n = 0, nnz = 0;
+    sovhip::Orient a;     // A   : m rows, gathers n-vectors
+    sovhip::Orient t;     // A^T : n rows, gathers m-vectors
+    int32_t* t_ptr = nullptr;   // owned A^T structure
+    int32_t* t_idx = nullptr;
+    int32_t* t_src = nullptr;   // owned: A position of each A^T entry
+    double* partials = nullptr; // owned: kCheckSlots * max(blocks) FP64
+    int64_t partial_blocks = 0;
+    int device = 0;
+};
+
+struct sov_hip_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace sovhip {
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// max that keeps a NaN from either side (residual maxima must not lose one)
+__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Block-wide reduction of NS sums and NM maxima; thread 0 writes them to
+// out[0..NS) then out[NS..NS+NM).  All threads of the block must call it.
+template <int NS, int NM>
+__device__ __forceinline__ void block_reduce_write(const double* s, const double* mx,
+                                                   double* out) {
+    constexpr int W = kBlock / kWave;
+    __shared__ double lds[(NS + NM) * W];
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const double r = wave_sum(s[i]);
+        if (lane == 0) lds[i * W + wv] = r;
+    }
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        const double r = wave_max(mx[i]);
+        if (lane == 0) lds[(NS + i) * W + wv] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            double acc = 0.0;
+            for (int w = 0; w < W; ++w) acc += lds[i * W + w];
+            out[i] = acc;
+        }
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            double acc = 0.0;
+            for (int w = 0; w < W; ++w) acc = nmax(acc, lds[(NS + i) * W + w]);
+            out[NS + i] = acc;
+        }
+    }
+}
+
+// The class a workgroup belongs to (uniform per block).
+__device__ __forceinline__ int block_class(const Sched& s) {
+    int c = 0;
+#pragma unroll
+    for (int k = 1; k < kClasses; ++k)
+        if ((int)blockIdx.x >= s.block_begin[k]) c = k;
+    return c;
+}
+
+// Dot product of one CSR row with a gathered vector, G lanes per row.  Every
+// lane of the group returns the full sum.  Accumulation type Acc.
+template <int G, typename Acc, typename V, typename X>
+__device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
+                                       const int32_t* __restrict__ idx,
+                                       const V* __restrict__ val, const X* __restrict__ x,
+                                       int row, int lane) {
+    Acc acc = Acc(0);
+    if (row >= 0) {
+        const int b = ptr[row], e = ptr[row + 1];
+        for (int k = b + lane; k < e; k += G) acc += Acc(val[k]) * Acc(x[idx[k]]);
+    }
+    return group_sum<G>(acc);
+}
+
+// Two dots of the same row (current and second vector scaled by s2).
+template <int G, typename V, typename X>
+__device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
+                                         const int32_t* __restrict__ idx,
+                                         const V* __restrict__ val, const X* __restrict__ x1,
+                                         const X* __restrict__ x2, double s2, int row, int lane,
+                                         double& d1, double& d2) {
+    double a1 = 0.0, a2 = 0.0;
+    if (row >= 0) {
+        const int b = ptr[row], e = ptr[row + 1];
+        for (int k = b + lane; k < e; k += G) {
+            const double v = double(val[k]);
+            const int j = idx[k];
+            a1 += v * double(x1[j]);
+            a2 += v * (double(x2[j]) * s2);
+        }
+    }
+    d1 = group_sum<G>(a1);
+    d2 = group_sum<G>(a2);
+}
+
+// Dispatch a functor templated on G with the block's lane count.
+template <class F>
+__device__ __forceinline__ void dispatch_G(int G, F& f) {
+    switch (G) {
+        case 1: f.template run<1>(); break;
+        case 2: f.template run<2>(); break;
+        case 4: f.template run<4>(); break;
+        case 8: f.template run<8>(); break;
+        case 16: f.template run<16>(); break;
+        case 32: f.template run<32>(); break;
+        default: f.template run<64>(); break;
+    }
+}
+
+// Host helpers
+int launch_blocks(int64_t n);      // blocks for an elementwise kernel over n
+hipError_t check_launch();         // hipGetLastError wrapper
+int build_schedule(sov_hip_ctx* ctx, Orient& o);   // class lists from ptr (host)
+int ensure_partials(sov_hip_csr* csr);
+
+}  // namespace sovhip

@@ -1,0 +1,877 @@
+// PDHG engine (API: include/sovereign/pdhg.hpp).
+//
+// solve() runs one of two loops:
+//   solve_primitive -- reference src/l1/pdhg.cpp:218-502 written against the
+//     virtual Backend, with every Status checked (the reference discards
+//     them, e.g. :297, :316).  Any backend; the parity mode.
+//   solve_fused     -- the MI355X loop on a Hip backend: device scaling
+//     (sov_hip_scale_*), device power iteration, two fused launches per
+//     iteration (sov_hip_pdhg_primal_step / _dual_step, replayed as HIP graphs
+//     per check interval), FP64 device checks (sov_hip_pdhg_check) and device
+//     restarts.  Same decisions as the reference: candidate = average when its
+//     scaled residual is no worse (:354), converged when the candidate's
+//     original-space residual < tolerance (:383), restart to the average when
+//     its residual <= decrease * residual at the last restart (:391-393).
+#include "sovereign/pdhg.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <utility>
+
+#include "sov_hip.h"
+#include "sovereign/scaling.hpp"
+
+namespace sov {
+
+const char* to_string(PdhgStatus s) noexcept {
+    switch (s) {
+        case PdhgStatus::Optimal: return "optimal";
+        case PdhgStatus::IterationLimit: return "iteration_limit";
+        case PdhgStatus::TimeLimit: return "time_limit";
+        case PdhgStatus::NumericalFailure: return "numerical_failure";
+        case PdhgStatus::NotSolved: return "not_solved";
+        case PdhgStatus::Cancelled: return "cancelled";
+        case PdhgStatus::PrimalInfeasible: return "primal_infeasible";
+        case PdhgStatus::DualInfeasible: return "dual_infeasible";
+    }
+    return "unknown";
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double seconds_since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+/// inf over [lo, hi] of coeff * v (reference pdhg.cpp:36-40).
+Real inf_over_box(Real coeff, Real lo, Real hi) {
+    if (coeff == 0.0) return 0.0;
+    if (coeff > 0.0) return is_finite_bound(lo) ? coeff * lo : -kInfinity;
+    return is_finite_bound(hi) ? coeff * hi : -kInfinity;
+}
+
+Real dual_class(Real val, Real lo, Real hi, Real point, Real tol) {
+    const bool at_lo = is_finite_bound(lo) && std::abs(point - lo) < tol;
+    const bool at_hi = is_finite_bound(hi) && std::abs(point - hi) < tol;
+    if (!at_lo && !at_hi) return std::abs(val);
+    if (at_lo && !at_hi) return std::max(Real{0}, -val);
+    if (at_hi && !at_lo) return std::max(Real{0}, val);
+    return 0.0;
+}
+
+/// Scaled-space max-norm residual (reference pdhg.cpp:87-136).
+Real kkt_residual(const Problem& s, std::span<const Real> c, std::span<const Real> x, std::span<const Real> y,
+                  Real tol) {
+    std::vector<Real> act(static_cast<std::size_t>(s.num_rows()));
+    s.matrix().multiply(x, act);
+    Real pr = 0.0, dr = 0.0;
+    for (std::size_t i = 0; i < act.size(); ++i) {
+        const Real lo = s.row_lower()[i], hi = s.row_upper()[i];
+        if (is_finite_bound(lo)) pr = std::max(pr, lo - act[i]);
+        if (is_finite_bound(hi)) pr = std::max(pr, act[i] - hi);
+        dr = std::max(dr, dual_class(y[i], lo, hi, act[i], tol));
+    }
+    std::vector<Real> aty(static_cast<std::size_t>(s.num_cols()));
+    s.matrix().multiply_transpose(y, aty);
+    for (std::size_t j = 0; j < aty.size(); ++j)
+        dr = std::max(dr, dual_class(c[j] - aty[j], s.col_lower()[j], s.col_upper()[j], x[j], tol));
+    return std::max(pr, dr);
+}
+
+/// Original-space residual of an already-unscaled point (reference
+/// pdhg.cpp:145-199 after its unscale at :148-152).
+Real original_residual(const Problem& p, Real sign, std::span<const Real> x, std::span<const Real> y, Real tol) {
+    std::vector<Real> act(static_cast<std::size_t>(p.num_rows()));
+    p.matrix().multiply(x, act);
+    Real pr = 0.0, dr = 0.0;
+    for (std::size_t i = 0; i < act.size(); ++i) {
+        const Real lo = p.row_lower()[i], hi = p.row_upper()[i];
+        if (is_finite_bound(lo)) pr = std::max(pr, lo - act[i]);
+        if (is_finite_bound(hi)) pr = std::max(pr, act[i] - hi);
+        dr = std::max(dr, dual_class(sign * y[i], lo, hi, act[i], tol));
+    }
+    std::vector<Real> aty(static_cast<std::size_t>(p.num_cols()));
+    p.matrix().multiply_transpose(y, aty);
+    for (std::size_t j = 0; j < aty.size(); ++j)
+        dr = std::max(dr, dual_class(sign * (p.objective()[j] - aty[j]), p.col_lower()[j], p.col_upper()[j], x[j], tol));
+    return std::max(pr, dr);
+}
+
+struct RelKkt {
+    Real rp = 0, rd = 0, gap = 0, pobj = 0, dobj = 0;
+    Real worst() const { return std::max({rp, rd, gap}); }
+};
+
+/// Relative KKT of an original-space point (definitions in pdhg.hpp).
+RelKkt relative_kkt(const Problem& p, std::span<const Real> x, std::span<const Real> y) {
+    const Real sg = p.sense() == ObjSense::Maximize ? -1.0 : 1.0;
+    std::vector<Real> act(static_cast<std::size_t>(p.num_rows())), aty(static_cast<std::size_t>(p.num_cols()));
+    p.matrix().multiply(x, act);
+    p.matrix().multiply_transpose(y, aty);
+    Real pv2 = 0, bn2 = 0, dr2 = 0, cn2 = 0, pobj = 0, dobj = 0;
+    for (std::size_t i = 0; i < act.size(); ++i) {
+        const Real lo = p.row_lower()[i], hi = p.row_upper()[i];
+        Real v = 0.0;
+        if (is_finite_bound(lo) && act[i] < lo) v = lo - act[i];
+        if (is_finite_bound(hi) && act[i] > hi) v = act[i] - hi;
+        pv2 += v * v;
+        if (is_finite_bound(lo)) bn2 += lo * lo;
+        if (is_finite_bound(hi) && !(is_finite_bound(lo) && lo == hi)) bn2 += hi * hi;
+        const Real yi = sg * y[i];
+        Real yp = yi;
+        if (!is_finite_bound(lo) && yp > 0.0) yp = 0.0;
+        if (!is_finite_bound(hi) && yp < 0.0) yp = 0.0;
+        dr2 += (yi - yp) * (yi - yp);
+        if (yp > 0.0) dobj += yp * lo;
+        else if (yp < 0.0) dobj += yp * hi;
+    }
+    for (std::size_t j = 0; j < aty.size(); ++j) {
+        const Real ci = sg * p.objective()[j];
+        const Real rc = ci - sg * aty[j];
+        const Real lo = p.col_lower()[j], hi = p.col_upper()[j];
+        Real lam = rc;
+        if (!is_finite_bound(lo) && lam > 0.0) lam = 0.0;
+        if (!is_finite_bound(hi) && lam < 0.0) lam = 0.0;
+        dr2 += (rc - lam) * (rc - lam);
+        cn2 += ci * ci;
+        if (lam > 0.0) dobj += lam * lo;
+        else if (lam < 0.0) dobj += lam * hi;
+        pobj += ci * x[j];
+    }
+    RelKkt r;
+    r.pobj = sg * pobj + p.objective_constant();
+    r.dobj = sg * dobj + p.objective_constant();
+    r.rp = std::sqrt(pv2) / (1.0 + std::sqrt(bn2));
+    r.rd = std::sqrt(dr2) / (1.0 + std::sqrt(cn2));
+    r.gap = std::abs(r.pobj - r.dobj) / (1.0 + std::abs(r.pobj) + std::abs(r.dobj));
+    return r;
+}
+
+/// Unscale a scaled, internal-sense point (reference pdhg.cpp:205-214).
+std::pair<std::vector<Real>, std::vector<Real>> unscale_point(Real sign, const Scaling& sc, std::span<const Real> xs,
+                                                              std::span<const Real> ys) {
+    std::vector<Real> x(xs.begin(), xs.end()), y(ys.begin(), ys.end());
+    sc.unscale_primal(x);
+    for (Real& v : y) v *= sign;
+    sc.unscale_dual(y);
+    return {std::move(x), std::move(y)};
+}
+
+/// Result population from the final original-space point (reference
+/// pdhg.cpp:428-498) plus the additive relative-KKT fields.
+void finish_result(const Problem& p, const PdhgOptions& o, std::vector<Real> x, std::vector<Real> y, PdhgResult& r) {
+    r.primal = std::move(x);
+    r.dual = std::move(y);
+    r.objective = p.evaluate_objective(r.primal);
+    const auto n = static_cast<std::size_t>(p.num_cols()), m = static_cast<std::size_t>(p.num_rows());
+    std::vector<Real> aty(n);
+    p.matrix().multiply_transpose(r.dual, aty);
+    r.reduced_costs.resize(n);
+    for (std::size_t j = 0; j < n; ++j) r.reduced_costs[j] = p.objective()[j] - aty[j];
+    std::vector<Real> act(m);
+    p.matrix().multiply(r.primal, act);
+    r.primal_infeasibility = 0.0;
+    for (std::size_t i = 0; i < m; ++i) {
+        if (is_finite_bound(p.row_lower()[i])) r.primal_infeasibility = std::max(r.primal_infeasibility, p.row_lower()[i] - act[i]);
+        if (is_finite_bound(p.row_upper()[i])) r.primal_infeasibility = std::max(r.primal_infeasibility, act[i] - p.row_upper()[i]);
+    }
+    DualSolution d;
+    d.row_duals = r.dual;
+    d.reduced_costs = r.reduced_costs;
+    r.dual_infeasibility = d.complementary_slackness_violation(p, r.primal, std::max(o.tolerance, Real{1e-6}));
+    Real dobj = 0.0;
+    bool finite = true;
+    for (std::size_t j = 0; j < n && finite; ++j) {
+        const Real t = inf_over_box(r.reduced_costs[j], p.col_lower()[j], p.col_upper()[j]);
+        if (!std::isfinite(t)) finite = false;
+        else dobj += t;
+    }
+    for (std::size_t i = 0; i < m && finite; ++i) {
+        const Real t = inf_over_box(r.dual[i], p.row_lower()[i], p.row_upper()[i]);
+        if (!std::isfinite(t)) finite = false;
+        else dobj += t;
+    }
+    r.duality_gap = finite ? std::abs(r.objective - dobj) / (1.0 + std::abs(r.objective) + std::abs(dobj)) : kInfinity;
+    const RelKkt k = relative_kkt(p, r.primal, r.dual);
+    r.relative_primal_residual = k.rp;
+    r.relative_dual_residual = k.rd;
+    r.relative_gap = k.gap;
+}
+
+bool prevalidate(const Problem& p, PdhgResult& r) {
+    std::string why;
+    if (p.validate(&why) != Status::Ok) {
+        r.status = PdhgStatus::NumericalFailure;
+        r.message = "invalid problem: " + why;
+        return false;
+    }
+    if (p.has_crossed_bounds()) {
+        r.status = PdhgStatus::NumericalFailure;
+        r.message = "a bound pair is crossed (lower > upper)";
+        return false;
+    }
+    return true;
+}
+
+Real next_primal_weight(Real omega, Real dx2, Real dy2, Real theta) {
+    const Real dx = std::sqrt(dx2), dy = std::sqrt(dy2);
+    if (dx > 1e-10 && dy > 1e-10) return std::exp(theta * std::log(dy / dx) + (1.0 - theta) * std::log(omega));
+    return omega;
+}
+
+}  // namespace
+
+// ===========================================================================
+// dispatch
+// ===========================================================================
+
+PdhgResult Pdhg::solve(const Problem& problem, Backend& backend, const CancellationToken* cancel) const {
+    const bool fused_ok = backend.kind() == BackendKind::Hip && backend.native_context() != nullptr;
+    PdhgEngine e = options_.engine;
+    if (e == PdhgEngine::Auto) e = fused_ok ? PdhgEngine::Fused : PdhgEngine::Primitive;
+    if (e == PdhgEngine::Fused && !fused_ok) {
+        PdhgResult r;
+        r.status = PdhgStatus::NumericalFailure;
+        r.message = "the fused engine needs a Hip backend";
+        return r;
+    }
+    if (e == PdhgEngine::Primitive && options_.precision != Precision::F64) {
+        PdhgResult r;
+        r.status = PdhgStatus::NumericalFailure;
+        r.message = "FP32 iterates need the fused engine (Hip backend)";
+        return r;
+    }
+    return e == PdhgEngine::Fused ? solve_fused(problem, backend, cancel) : solve_primitive(problem, backend, cancel);
+}
+
+// ===========================================================================
+// primitive loop (reference pdhg.cpp:218-502 over Backend primitives)
+// ===========================================================================
+
+PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const CancellationToken* cancel) const {
+    PdhgResult result;
+    result.engine = "primitive";
+    const auto t0 = Clock::now();
+    if (!prevalidate(problem, result)) return result;
+    const PdhgOptions& o = options_;
+    try {
+        auto [scaled, scaling] = Scaling::equilibrate(problem);
+        const Idx n = scaled.num_cols(), m = scaled.num_rows();
+        const auto nu = static_cast<std::size_t>(n), mu = static_cast<std::size_t>(m);
+        DeviceCsr A = DeviceCsr::upload(be, scaled.matrix());
+        const Real sign = scaled.sense() == ObjSense::Maximize ? -1.0 : 1.0;
+        std::vector<Real> c_host(scaled.objective().begin(), scaled.objective().end());
+        for (Real& v : c_host) v *= sign;
+        DeviceBuffer c = DeviceBuffer::upload(be, c_host);
+        DeviceBuffer col_lo = DeviceBuffer::upload(be, scaled.col_lower());
+        DeviceBuffer col_hi = DeviceBuffer::upload(be, scaled.col_upper());
+        DeviceBuffer row_lo = DeviceBuffer::upload(be, scaled.row_lower());
+        DeviceBuffer row_hi = DeviceBuffer::upload(be, scaled.row_upper());
+
+        auto ok = [](Status s, const char* what) { check(s, what); };
+
+        // power iteration (reference pdhg.cpp:48-79)
+        Real est = 0.0;
+        {
+            DeviceBuffer u = DeviceBuffer::upload(be, std::vector<Real>(nu, 1.0));
+            DeviceBuffer v = DeviceBuffer::upload(be, std::vector<Real>(mu, 1.0));
+            DeviceBuffer w(be, mu);
+            for (int it = 0; it < o.power_iterations; ++it) {
+                ok(be.spmv(A.view(), -1.0, u.data(), 0.0, w.data()), "power: spmv");
+                ok(be.axpy(m, 1.0, v.data(), w.data()), "power: axpy");
+                Real wn = 0.0;
+                ok(be.norm2(m, w.data(), &wn), "power: norm2");
+                if (wn < 1e-300) break;
+                ok(be.scale(m, 1.0 / wn, w.data()), "power: scale");
+                ok(be.spmv_transpose(A.view(), -1.0, w.data(), 0.0, u.data()), "power: spmv_transpose");
+                ok(be.copy_device_to_device(w.data(), v.data(), mu * sizeof(Real)), "power: copy");
+                Real un = 0.0, vn = 0.0;
+                ok(be.norm2(n, u.data(), &un), "power: norm2");
+                ok(be.norm2(m, v.data(), &vn), "power: norm2");
+                est = std::sqrt(un * un + vn * vn);
+                if (est < 1e-300) break;
+                ok(be.scale(n, 1.0 / est, u.data()), "power: scale");
+                ok(be.scale(m, 1.0 / est, v.data()), "power: scale");
+            }
+        }
+        result.operator_norm_estimate = est;
+        const Real eta = est > 1e-9 ? o.step_size_safety / est : o.step_size_safety;
+        Real omega = o.use_primal_weight ? o.initial_primal_weight : 1.0;
+        Real tau = eta / omega, sigma = eta * omega;
+
+        DeviceBuffer x(be, nu), s(be, mu), y(be, mu);
+        ok(be.project_box(n, col_lo.data(), col_hi.data(), x.data()), "project_box");
+        ok(be.project_box(m, row_lo.data(), row_hi.data(), s.data()), "project_box");
+        if (!o.warm_start_dual.empty()) {
+            if (o.warm_start_dual.size() != mu) throw Error(Status::DimensionMismatch, "warm_start_dual");
+            std::vector<Real> ys(mu);
+            for (std::size_t i = 0; i < mu; ++i) ys[i] = sign * o.warm_start_dual[i] / scaling.row_scale()[i];
+            y = DeviceBuffer::upload(be, ys);
+        }
+        DeviceBuffer x_new(be, nu), s_new(be, mu), x_bar(be, nu), s_bar(be, mu), grad(be, nu), kb(be, mu);
+        DeviceBuffer x_sum(be, nu), y_sum(be, mu);
+        std::vector<Real> x_last = x.to_host(), y_last = y.to_host();
+        long long since = 0;
+        Real res_at_restart = kInfinity;
+        const std::size_t nb = nu * sizeof(Real), mb = mu * sizeof(Real);
+        std::vector<Real> best_x, best_y;
+        bool converged = false;
+        const Real otol = std::max(o.tolerance, Real{1e-6});
+        const auto t_loop = Clock::now();
+        result.setup_seconds = seconds_since(t0);
+
+        for (result.iterations = 0; result.iterations < o.max_iterations; ++result.iterations) {
+            if (cancel != nullptr && cancel->is_cancelled()) { result.status = PdhgStatus::Cancelled; break; }
+            if (o.time_limit_seconds > 0.0 && seconds_since(t0) > o.time_limit_seconds) {
+                result.status = PdhgStatus::TimeLimit;
+                break;
+            }
+            ok(be.copy_device_to_device(c.data(), grad.data(), nb), "copy");
+            ok(be.spmv_transpose(A.view(), -1.0, y.data(), 1.0, grad.data()), "spmv_transpose");
+            ok(be.copy_device_to_device(x.data(), x_new.data(), nb), "copy");
+            ok(be.axpy(n, -tau, grad.data(), x_new.data()), "axpy");
+            ok(be.project_box(n, col_lo.data(), col_hi.data(), x_new.data()), "project_box");
+            ok(be.copy_device_to_device(s.data(), s_new.data(), mb), "copy");
+            ok(be.axpy(m, -tau, y.data(), s_new.data()), "axpy");
+            ok(be.project_box(m, row_lo.data(), row_hi.data(), s_new.data()), "project_box");
+            ok(be.copy_device_to_device(x_new.data(), x_bar.data(), nb), "copy");
+            ok(be.scale(n, 2.0, x_bar.data()), "scale");
+            ok(be.axpy(n, -1.0, x.data(), x_bar.data()), "axpy");
+            ok(be.copy_device_to_device(s_new.data(), s_bar.data(), mb), "copy");
+            ok(be.scale(m, 2.0, s_bar.data()), "scale");
+            ok(be.axpy(m, -1.0, s.data(), s_bar.data()), "axpy");
+            ok(be.spmv(A.view(), -1.0, x_bar.data(), 0.0, kb.data()), "spmv");
+            ok(be.axpy(m, 1.0, s_bar.data(), kb.data()), "axpy");
+            ok(be.axpy(m, sigma, kb.data(), y.data()), "axpy");
+            ok(be.copy_device_to_device(x_new.data(), x.data(), nb), "copy");
+            ok(be.copy_device_to_device(s_new.data(), s.data(), mb), "copy");
+            if (since == 0) {
+                ok(be.copy_device_to_device(x.data(), x_sum.data(), nb), "copy");
+                ok(be.copy_device_to_device(y.data(), y_sum.data(), mb), "copy");
+            } else {
+                ok(be.axpy(n, 1.0, x.data(), x_sum.data()), "axpy");
+                ok(be.axpy(m, 1.0, y.data(), y_sum.data()), "axpy");
+            }
+            ++since;
+            if ((result.iterations + 1) % o.restart_check_period != 0) continue;
+
+            const std::vector<Real> xh = x.to_host(), yh = y.to_host();
+            const Real cur = kkt_residual(scaled, c_host, xh, yh, o.tolerance);
+            std::vector<Real> xa = x_sum.to_host(), ya = y_sum.to_host();
+            const Real inv = 1.0 / static_cast<Real>(since);
+            for (Real& v : xa) v *= inv;
+            for (Real& v : ya) v *= inv;
+            const Real avg = kkt_residual(scaled, c_host, xa, ya, o.tolerance);
+            if (!std::isfinite(cur) || !std::isfinite(avg)) {
+                result.status = PdhgStatus::NumericalFailure;
+                result.message = "non-finite KKT residual at iteration " + std::to_string(result.iterations);
+                return result;
+            }
+            const bool avg_better = avg <= cur;
+            const std::vector<Real>& cx = avg_better ? xa : xh;
+            const std::vector<Real>& cy = avg_better ? ya : yh;
+            auto [ux, uy] = unscale_point(sign, scaling, cx, cy);
+            const Real ores = original_residual(problem, sign, ux, uy, otol);
+            if (!std::isfinite(ores)) {
+                result.status = PdhgStatus::NumericalFailure;
+                result.message = "non-finite KKT residual (original units) at iteration " + std::to_string(result.iterations);
+                return result;
+            }
+            if (o.checkpoint) o.checkpoint(result.iterations, ux, uy, ores);
+            bool stop = ores < o.tolerance;
+            if (o.termination == PdhgTermination::RelativeKkt) stop = relative_kkt(problem, ux, uy).worst() <= o.tolerance;
+            if (stop) {
+                best_x = cx;
+                best_y = cy;
+                converged = true;
+                ++result.iterations;
+                break;
+            }
+            if (!std::isfinite(res_at_restart)) res_at_restart = cur;
+            if (avg <= o.restart_sufficient_decrease * res_at_restart) {
+                x = DeviceBuffer::upload(be, xa);
+                y = DeviceBuffer::upload(be, ya);
+                ok(be.project_box(n, col_lo.data(), col_hi.data(), x.data()), "project_box");
+                std::vector<Real> act(mu, 0.0);
+                scaled.matrix().multiply(xa, act);
+                s = DeviceBuffer::upload(be, act);
+                ok(be.project_box(m, row_lo.data(), row_hi.data(), s.data()), "project_box");
+                if (o.use_primal_weight) {
+                    const std::vector<Real> xn = x.to_host();
+                    Real dx2 = 0.0, dy2 = 0.0;
+                    for (std::size_t j = 0; j < nu; ++j) dx2 += (xn[j] - x_last[j]) * (xn[j] - x_last[j]);
+                    for (std::size_t i = 0; i < mu; ++i) dy2 += (ya[i] - y_last[i]) * (ya[i] - y_last[i]);
+                    omega = next_primal_weight(omega, dx2, dy2, o.primal_weight_smoothing);
+                    tau = eta / omega;
+                    sigma = eta * omega;
+                    x_last = xn;
+                    y_last = ya;
+                }
+                res_at_restart = avg;
+                since = 0;
+                ++result.restarts;
+                if (o.verbose)
+                    std::fprintf(stderr, "  [pdhg] restart at iter %lld, residual %.3g\n", result.iterations + 1,
+                                 static_cast<double>(avg));
+            }
+        }
+        result.loop_seconds = seconds_since(t_loop);
+        if (best_x.empty()) {
+            best_x = x.to_host();
+            best_y = y.to_host();
+        }
+        if (converged) result.status = PdhgStatus::Optimal;
+        else if (result.status == PdhgStatus::NotSolved) result.status = PdhgStatus::IterationLimit;
+        auto [fx, fy] = unscale_point(sign, scaling, best_x, best_y);
+        result.primal_weight = omega;
+        finish_result(problem, o, std::move(fx), std::move(fy), result);
+    } catch (const Error& e) {
+        result.status = PdhgStatus::NumericalFailure;
+        result.message = std::string("backend failure: ") + e.what();
+    }
+    result.seconds = seconds_since(t0);
+    return result;
+}
+
+// ===========================================================================
+// fused device loop
+// ===========================================================================
+
+namespace {
+
+/// Device allocation owned by the fused loop.
+struct DevMem {
+    sov_hip_ctx* ctx = nullptr;
+    void* p = nullptr;
+    DevMem() = default;
+    DevMem(sov_hip_ctx* c, std::size_t bytes) : ctx(c) {
+        const int rc = sov_hip_malloc(c, bytes, &p);
+        if (rc != SOV_OK) throw Error(static_cast<Status>(rc), std::string("device allocation: ") + sov_hip_last_error());
+    }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : ctx(o.ctx), p(std::exchange(o.p, nullptr)) {}
+    DevMem& operator=(DevMem&& o) noexcept {
+        if (this != &o) {
+            reset();
+            ctx = o.ctx;
+            p = std::exchange(o.p, nullptr);
+        }
+        return *this;
+    }
+    ~DevMem() { reset(); }
+    void reset() {
+        if (p != nullptr) (void)sov_hip_free(ctx, p);
+        p = nullptr;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+void hip_ok(int rc, const char* what) {
+    if (rc != SOV_OK) throw Error(static_cast<Status>(rc), std::string(what) + ": " + sov_hip_last_error());
+}
+
+struct CsrHandle {
+    sov_hip_csr* h = nullptr;
+    ~CsrHandle() { sov_hip_csr_destroy(h); }
+};
+
+struct GraphHandle {
+    sov_hip_graph* g = nullptr;
+    ~GraphHandle() { sov_hip_graph_destroy(g); }
+    void reset() {
+        sov_hip_graph_destroy(g);
+        g = nullptr;
+    }
+};
+
+/// The device-resident iterate set of one precision.
+struct DeviceState {
+    int prec = SOV_PREC_F64;
+    std::size_t es = 8;
+    DevMem c, clo, chi, rlo, rhi, x, xbar, xsum, s, y, ysum;
+    sov_hip_pdhg_state st{};
+
+    void allocate(sov_hip_ctx* ctx, int p, std::size_t m, std::size_t n) {
+        prec = p;
+        es = p == SOV_PREC_F64 ? 8 : 4;
+        const std::size_t nb = std::max<std::size_t>(n, 1) * es, mb = std::max<std::size_t>(m, 1) * es;
+        c = DevMem(ctx, nb); clo = DevMem(ctx, nb); chi = DevMem(ctx, nb);
+        x = DevMem(ctx, nb); xbar = DevMem(ctx, nb); xsum = DevMem(ctx, nb);
+        rlo = DevMem(ctx, mb); rhi = DevMem(ctx, mb);
+        s = DevMem(ctx, mb); y = DevMem(ctx, mb); ysum = DevMem(ctx, mb);
+        for (DevMem* d : {&x, &xbar, &xsum, &s, &y, &ysum}) hip_ok(sov_hip_fill_zero(ctx, d->p, d == &s || d == &y || d == &ysum ? mb : nb), "fill");
+        st.prec = p;
+        st.m = static_cast<int32_t>(m);
+        st.n = static_cast<int32_t>(n);
+        st.c = c.p; st.col_lo = clo.p; st.col_hi = chi.p; st.row_lo = rlo.p; st.row_hi = rhi.p;
+        st.x = x.p; st.x_bar = xbar.p; st.x_sum = xsum.p; st.s = s.p; st.y = y.p; st.y_sum = ysum.p;
+    }
+
+    /// Upload an FP64 host vector into a device vector of this precision.
+    void upload(sov_hip_ctx* ctx, const DevMem& dst, std::span<const Real> v) const {
+        if (v.empty()) return;
+        if (prec == SOV_PREC_F64) {
+            hip_ok(sov_hip_memcpy_h2d(ctx, dst.p, v.data(), v.size() * 8), "upload");
+        } else {
+            std::vector<float> f(v.size());
+            for (std::size_t i = 0; i < v.size(); ++i) f[i] = static_cast<float>(v[i]);
+            hip_ok(sov_hip_memcpy_h2d(ctx, dst.p, f.data(), f.size() * 4), "upload");
+        }
+    }
+
+    /// Download a device vector as FP64, optionally times `scale`.
+    std::vector<Real> download(sov_hip_ctx* ctx, const DevMem& src, std::size_t len, Real scale = 1.0) const {
+        std::vector<Real> out(len);
+        if (len == 0) return out;
+        if (prec == SOV_PREC_F64) {
+            hip_ok(sov_hip_memcpy_d2h(ctx, out.data(), src.p, len * 8), "download");
+        } else {
+            std::vector<float> f(len);
+            hip_ok(sov_hip_memcpy_d2h(ctx, f.data(), src.p, len * 4), "download");
+            for (std::size_t i = 0; i < len; ++i) out[i] = static_cast<Real>(f[i]);
+        }
+        if (scale != 1.0)
+            for (Real& v : out) v *= scale;
+        return out;
+    }
+};
+
+}  // namespace
+
+PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const CancellationToken* cancel) const {
+    PdhgResult result;
+    result.engine = "fused";
+    const auto t0 = Clock::now();
+    if (!prevalidate(problem, result)) return result;
+    const PdhgOptions& o = options_;
+    auto* ctx = static_cast<sov_hip_ctx*>(backend.native_context());
+    try {
+        const CsrMatrix& A0 = problem.matrix();
+        const auto m = static_cast<std::size_t>(problem.num_rows()), n = static_cast<std::size_t>(problem.num_cols());
+        const auto nnz = static_cast<std::size_t>(A0.nnz());
+        const Real sign = problem.sense() == ObjSense::Maximize ? -1.0 : 1.0;
+
+        // -- the matrix crosses the bus once (3 copies), then everything is device work
+        DevMem d_rp(ctx, (m + 1) * sizeof(int32_t)), d_ci(ctx, std::max<std::size_t>(nnz, 1) * sizeof(int32_t)),
+            d_v(ctx, std::max<std::size_t>(nnz, 1) * sizeof(double));
+        hip_ok(sov_hip_memcpy_h2d(ctx, d_rp.p, A0.row_ptr().data(), (m + 1) * sizeof(int32_t)), "upload A");
+        if (nnz > 0) {
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_ci.p, A0.col_idx().data(), nnz * sizeof(int32_t)), "upload A");
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_v.p, A0.values().data(), nnz * sizeof(double)), "upload A");
+        }
+        CsrHandle csr;
+        hip_ok(sov_hip_csr_create(ctx, static_cast<int32_t>(m), static_cast<int32_t>(n), static_cast<int32_t>(nnz),
+                                  d_rp.as<int32_t>(), d_ci.as<int32_t>(), d_v.as<double>(), SOV_CSR_OWN_VALUES, &csr.h),
+               "csr_create");
+        d_v.reset();   // the handle owns a copy of the values
+
+        // -- kernel (d): Ruiz + Pock-Chambolle on the device (reference scaling.cpp:19-104)
+        DevMem d_rs(ctx, std::max<std::size_t>(m, 1) * 8), d_cs(ctx, std::max<std::size_t>(n, 1) * 8);
+        {
+            const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, d_rs.p, ones_m.data(), m * 8), "upload");
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, d_cs.p, ones_n.data(), n * 8), "upload");
+            const ScalingOptions so;
+            if (so.apply_ruiz)
+                for (int sweep = 0; sweep < so.ruiz_iterations; ++sweep) {
+                    double worst = 0.0;
+                    hip_ok(sov_hip_scale_ruiz_sweep(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>(), &worst), "ruiz");
+                    if (worst < so.ruiz_tolerance) break;
+                }
+            if (so.apply_pock_chambolle)
+                hip_ok(sov_hip_scale_pock_chambolle(ctx, csr.h, so.pock_chambolle_alpha, d_rs.as<double>(),
+                                                    d_cs.as<double>()),
+                       "pock_chambolle");
+        }
+        std::vector<Real> rs(m), cs(n);
+        if (m) hip_ok(sov_hip_memcpy_d2h(ctx, rs.data(), d_rs.p, m * 8), "download scaling");
+        if (n) hip_ok(sov_hip_memcpy_d2h(ctx, cs.data(), d_cs.p, n * 8), "download scaling");
+        const Scaling scaling = Scaling::from_factors(rs, cs);
+        hip_ok(sov_hip_csr_scale(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>()), "csr_scale");
+
+        // scaled vectors (reference scaling.cpp:172-192, pdhg.cpp:250-252)
+        std::vector<Real> c_s(n), clo_s(n), chi_s(n), rlo_s(m), rhi_s(m);
+        for (std::size_t j = 0; j < n; ++j) {
+            c_s[j] = (problem.objective()[j] * cs[j]) * sign;
+            const Real lo = problem.col_lower()[j], hi = problem.col_upper()[j];
+            clo_s[j] = is_finite_bound(lo) ? lo / cs[j] : lo;
+            chi_s[j] = is_finite_bound(hi) ? hi / cs[j] : hi;
+        }
+        for (std::size_t i = 0; i < m; ++i) {
+            const Real lo = problem.row_lower()[i], hi = problem.row_upper()[i];
+            rlo_s[i] = is_finite_bound(lo) ? lo * rs[i] : lo;
+            rhi_s[i] = is_finite_bound(hi) ? hi * rs[i] : hi;
+        }
+        // original-space arrays for the device checks (FP64)
+        DevMem d_olo_r(ctx, std::max<std::size_t>(m, 1) * 8), d_ohi_r(ctx, std::max<std::size_t>(m, 1) * 8),
+            d_olo_c(ctx, std::max<std::size_t>(n, 1) * 8), d_ohi_c(ctx, std::max<std::size_t>(n, 1) * 8),
+            d_oc(ctx, std::max<std::size_t>(n, 1) * 8), d_act(ctx, std::max<std::size_t>(m, 1) * 8);
+        if (m) {
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_olo_r.p, problem.row_lower().data(), m * 8), "upload");
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_ohi_r.p, problem.row_upper().data(), m * 8), "upload");
+        }
+        if (n) {
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_olo_c.p, problem.col_lower().data(), n * 8), "upload");
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_ohi_c.p, problem.col_upper().data(), n * 8), "upload");
+            hip_ok(sov_hip_memcpy_h2d(ctx, d_oc.p, problem.objective().data(), n * 8), "upload");
+        }
+
+        // -- step size: device power iteration in FP64 (reference pdhg.cpp:48-79)
+        Real est = 0.0;
+        {
+            DevMem u(ctx, std::max<std::size_t>(n, 1) * 8), v(ctx, std::max<std::size_t>(m, 1) * 8),
+                w(ctx, std::max<std::size_t>(m, 1) * 8);
+            const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, u.p, ones_n.data(), n * 8), "upload");
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, v.p, ones_m.data(), m * 8), "upload");
+            const auto mm = static_cast<int64_t>(m), nn = static_cast<int64_t>(n);
+            for (int it = 0; it < o.power_iterations; ++it) {
+                hip_ok(sov_hip_spmv(ctx, csr.h, 0, SOV_PREC_F64, -1.0, u.p, 0.0, w.p), "power spmv");
+                hip_ok(sov_hip_axpy_f64(ctx, mm, 1.0, v.as<double>(), w.as<double>()), "power axpy");
+                double wn = 0.0;
+                hip_ok(sov_hip_nrm2_f64(ctx, mm, w.as<double>(), &wn), "power nrm2");
+                if (wn < 1e-300) break;
+                hip_ok(sov_hip_scal_f64(ctx, mm, 1.0 / wn, w.as<double>()), "power scal");
+                hip_ok(sov_hip_spmv(ctx, csr.h, 1, SOV_PREC_F64, -1.0, w.p, 0.0, u.p), "power spmv_t");
+                hip_ok(sov_hip_memcpy_d2d(ctx, v.p, w.p, m * 8), "power copy");
+                double un = 0.0, vn = 0.0;
+                hip_ok(sov_hip_nrm2_f64(ctx, nn, u.as<double>(), &un), "power nrm2");
+                hip_ok(sov_hip_nrm2_f64(ctx, mm, v.as<double>(), &vn), "power nrm2");
+                est = std::sqrt(un * un + vn * vn);
+                if (est < 1e-300) break;
+                hip_ok(sov_hip_scal_f64(ctx, nn, 1.0 / est, u.as<double>()), "power scal");
+                hip_ok(sov_hip_scal_f64(ctx, mm, 1.0 / est, v.as<double>()), "power scal");
+            }
+        }
+        result.operator_norm_estimate = est;
+        const Real eta = est > 1e-9 ? o.step_size_safety / est : o.step_size_safety;
+        Real omega = o.use_primal_weight ? o.initial_primal_weight : 1.0;
+
+        // -- iterate state
+        int prec = o.precision == Precision::F32 ? SOV_PREC_F32 : SOV_PREC_F64;
+        if (prec == SOV_PREC_F32) hip_ok(sov_hip_csr_make_f32(ctx, csr.h), "make_f32");
+        DeviceState S;
+        auto init_state = [&](int p) {
+            S.allocate(ctx, p, m, n);
+            S.upload(ctx, S.c, c_s);
+            S.upload(ctx, S.clo, clo_s);
+            S.upload(ctx, S.chi, chi_s);
+            S.upload(ctx, S.rlo, rlo_s);
+            S.upload(ctx, S.rhi, rhi_s);
+        };
+        init_state(prec);
+        // x = Proj_X(0), s = Proj_S(0), y = 0 or the warm start (reference pdhg.cpp:268-270)
+        {
+            const std::vector<Real> zm(m, 0.0), zn(n, 0.0);
+            DevMem zx(ctx, std::max<std::size_t>(n, 1) * 8), zs(ctx, std::max<std::size_t>(m, 1) * 8);
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, zx.p, zn.data(), n * 8), "upload");
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, zs.p, zm.data(), m * 8), "upload");
+            DevMem wy;
+            const double* ysrc = nullptr;
+            if (!o.warm_start_dual.empty()) {
+                if (o.warm_start_dual.size() != m) throw Error(Status::DimensionMismatch, "warm_start_dual length");
+                std::vector<Real> ys(m);
+                for (std::size_t i = 0; i < m; ++i) ys[i] = sign * o.warm_start_dual[i] / rs[i];
+                wy = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
+                if (m) hip_ok(sov_hip_memcpy_h2d(ctx, wy.p, ys.data(), m * 8), "upload warm start");
+                ysrc = wy.as<double>();
+            }
+            hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), ysrc, zs.as<double>()), "set_point");
+            hip_ok(sov_hip_sync(ctx), "sync");
+        }
+        if (o.use_graphs && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, prec), "tune");
+
+        // primal-weight bookkeeping (FP64 copies of the last restart point)
+        DevMem d_xlast, d_ylast;
+        if (o.use_primal_weight) {
+            d_xlast = DevMem(ctx, std::max<std::size_t>(n, 1) * 8);
+            d_ylast = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
+            const std::vector<Real> xl = S.download(ctx, S.x, n), yl = S.download(ctx, S.y, m);
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, d_xlast.p, xl.data(), n * 8), "upload");
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, d_ylast.p, yl.data(), m * 8), "upload");
+        }
+
+        sov_hip_check_args ca{};
+        ca.tol_scaled = o.tolerance;
+        ca.tol_orig = std::max(o.tolerance, Real{1e-6});
+        ca.obj_sign = sign;
+        ca.obj_const = problem.objective_constant();
+        ca.row_scale = d_rs.as<double>();
+        ca.col_scale = d_cs.as<double>();
+        ca.orig_row_lo = d_olo_r.as<double>();
+        ca.orig_row_hi = d_ohi_r.as<double>();
+        ca.orig_col_lo = d_olo_c.as<double>();
+        ca.orig_col_hi = d_ohi_c.as<double>();
+        ca.orig_c = d_oc.as<double>();
+        ca.act_avg = d_act.as<double>();
+
+        const long long period = std::max<long long>(o.restart_check_period, 1);
+        GraphHandle g_first, g_cont;
+        Real graph_tau = -1.0, graph_sigma = -1.0;
+        int graph_prec = -1;
+        auto run_steps = [&](long long count, bool first_is_fresh, Real tau, Real sigma) {
+            for (long long k = 0; k < count; ++k) {
+                const int first = (first_is_fresh && k == 0) ? 1 : 0;
+                hip_ok(sov_hip_pdhg_primal_step(ctx, csr.h, &S.st, tau, first), "primal_step");
+                hip_ok(sov_hip_pdhg_dual_step(ctx, csr.h, &S.st, tau, sigma, first), "dual_step");
+            }
+        };
+        auto run_interval = [&](long long count, bool fresh, Real tau, Real sigma) {
+            if (!o.use_graphs || count != period) {
+                run_steps(count, fresh, tau, sigma);
+                return;
+            }
+            if (graph_tau != tau || graph_sigma != sigma || graph_prec != S.prec) {
+                g_first.reset();
+                g_cont.reset();
+                graph_tau = tau;
+                graph_sigma = sigma;
+                graph_prec = S.prec;
+            }
+            GraphHandle& g = fresh ? g_first : g_cont;
+            if (g.g == nullptr) {
+                hip_ok(sov_hip_graph_begin(ctx), "graph_begin");
+                run_steps(count, fresh, tau, sigma);
+                hip_ok(sov_hip_graph_end(ctx, &g.g), "graph_end");
+            }
+            hip_ok(sov_hip_graph_launch(ctx, g.g), "graph_launch");
+        };
+
+        long long since = 0;
+        Real res_at_restart = kInfinity;
+        bool converged = false;
+        std::vector<Real> best_x, best_y;   // scaled, internal sense, FP64
+        const auto t_loop = Clock::now();
+        result.setup_seconds = seconds_since(t0);
+        long long polish_start = -1;
+        Real best_rel = kInfinity;
+        int stall = 0;
+
+        result.iterations = 0;
+        while (result.iterations < o.max_iterations) {
+            if (cancel != nullptr && cancel->is_cancelled()) { result.status = PdhgStatus::Cancelled; break; }
+            if (o.time_limit_seconds > 0.0 && seconds_since(t0) > o.time_limit_seconds) {
+                result.status = PdhgStatus::TimeLimit;
+                break;
+            }
+            const long long next_check = (result.iterations / period + 1) * period;
+            const long long count = std::min(next_check, static_cast<long long>(o.max_iterations)) - result.iterations;
+            const Real tau = eta / omega, sigma = eta * omega;
+            run_interval(count, since == 0, tau, sigma);
+            since += count;
+            result.iterations += count;
+            if (result.iterations != next_check) break;   // iteration limit inside the interval
+
+            ca.st = &S.st;
+            ca.since_restart = since;
+            sov_hip_check_out co{};
+            hip_ok(sov_hip_pdhg_check(ctx, csr.h, &ca, &co), "check");
+            const Real cur = co.kkt_cur, avg = co.kkt_avg;
+            if (!std::isfinite(cur) || !std::isfinite(avg)) {
+                result.status = PdhgStatus::NumericalFailure;
+                result.message = "non-finite KKT residual at iteration " + std::to_string(result.iterations - 1);
+                result.seconds = seconds_since(t0);
+                return result;
+            }
+            const bool avg_better = avg <= cur;
+            const Real ores = avg_better ? co.orig_avg : co.orig_cur;
+            const double* rel = avg_better ? co.rel_avg : co.rel_cur;
+            const Real rel_worst = std::max({rel[0], rel[1], rel[2]});
+            if (!std::isfinite(ores)) {
+                result.status = PdhgStatus::NumericalFailure;
+                result.message = "non-finite KKT residual (original units) at iteration " +
+                                 std::to_string(result.iterations - 1);
+                result.seconds = seconds_since(t0);
+                return result;
+            }
+            const Real inv = 1.0 / static_cast<Real>(since);
+            auto candidate = [&]() {
+                std::pair<std::vector<Real>, std::vector<Real>> p;
+                p.first = avg_better ? S.download(ctx, S.xsum, n, inv) : S.download(ctx, S.x, n);
+                p.second = avg_better ? S.download(ctx, S.ysum, m, inv) : S.download(ctx, S.y, m);
+                return p;
+            };
+            if (o.checkpoint) {
+                auto [cx, cy] = candidate();
+                auto [ux, uy] = unscale_point(sign, scaling, cx, cy);
+                o.checkpoint(result.iterations - 1, ux, uy, ores);
+            }
+            const bool stop = o.termination == PdhgTermination::Reference ? (ores < o.tolerance)
+                                                                          : (rel_worst <= o.tolerance);
+            if (stop) {
+                auto [cx, cy] = candidate();
+                best_x = std::move(cx);
+                best_y = std::move(cy);
+                converged = true;
+                break;
+            }
+            // FP32 -> FP64 switch (additive): near tolerance or stalled
+            if (S.prec == SOV_PREC_F32 && o.fp64_polish) {
+                if (rel_worst < best_rel * 0.99) {
+                    best_rel = rel_worst;
+                    stall = 0;
+                } else {
+                    ++stall;
+                }
+                const Real gauge = o.termination == PdhgTermination::Reference ? ores : rel_worst;
+                if (gauge <= o.fp32_switch_factor * o.tolerance || stall >= o.fp32_stall_checks) {
+                    auto [cx, cy] = candidate();
+                    init_state(SOV_PREC_F64);
+                    DevMem hx(ctx, std::max<std::size_t>(n, 1) * 8), hy(ctx, std::max<std::size_t>(m, 1) * 8);
+                    if (n) hip_ok(sov_hip_memcpy_h2d(ctx, hx.p, cx.data(), n * 8), "upload");
+                    if (m) hip_ok(sov_hip_memcpy_h2d(ctx, hy.p, cy.data(), m * 8), "upload");
+                    hip_ok(sov_hip_spmv(ctx, csr.h, 0, SOV_PREC_F64, 1.0, hx.p, 0.0, d_act.p), "polish activity");
+                    hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, hx.as<double>(), hy.as<double>(), d_act.as<double>()),
+                           "polish set_point");
+                    hip_ok(sov_hip_sync(ctx), "sync");
+                    since = 0;
+                    res_at_restart = kInfinity;
+                    polish_start = result.iterations;
+                    if (o.verbose)
+                        std::fprintf(stderr, "  [pdhg] fp64 polish from iter %lld (rel kkt %.3g)\n", result.iterations,
+                                     static_cast<double>(rel_worst));
+                    continue;
+                }
+            }
+            if (!std::isfinite(res_at_restart)) res_at_restart = cur;
+            if (avg <= o.restart_sufficient_decrease * res_at_restart) {
+                double mv[2] = {0.0, 0.0};
+                hip_ok(sov_hip_pdhg_restart(ctx, &S.st, since, d_act.as<double>(),
+                                            o.use_primal_weight ? d_xlast.as<double>() : nullptr,
+                                            o.use_primal_weight ? d_ylast.as<double>() : nullptr, mv),
+                       "restart");
+                if (o.use_primal_weight) omega = next_primal_weight(omega, mv[0], mv[1], o.primal_weight_smoothing);
+                res_at_restart = avg;
+                since = 0;
+                ++result.restarts;
+                if (o.verbose)
+                    std::fprintf(stderr, "  [pdhg] restart at iter %lld, residual %.3g, omega %.3g\n",
+                                 result.iterations, static_cast<double>(avg), static_cast<double>(omega));
+            }
+        }
+        hip_ok(sov_hip_sync(ctx), "sync");
+        result.loop_seconds = seconds_since(t_loop);
+        if (best_x.empty()) {
+            best_x = S.download(ctx, S.x, n);
+            best_y = S.download(ctx, S.y, m);
+        }
+        if (converged) result.status = PdhgStatus::Optimal;
+        else if (result.status == PdhgStatus::NotSolved) result.status = PdhgStatus::IterationLimit;
+        if (polish_start >= 0) result.fp64_polish_iterations = result.iterations - polish_start;
+        result.primal_weight = omega;
+        auto [fx, fy] = unscale_point(sign, scaling, best_x, best_y);
+        finish_result(problem, o, std::move(fx), std::move(fy), result);
+    } catch (const Error& e) {
+        result.status = PdhgStatus::NumericalFailure;
+        result.message = std::string("device failure: ") + e.what();
+    }
+    result.seconds = seconds_since(t0);
+    return result;
+}
+
+}  // namespace sov
