@@ -1,0 +1,283 @@
+#!/usr/bin/env python3
+"""Benchmark of the PDHG hot path on one MI355X.
+
+Workload: the 1M-column Zipf LP (m = 500,000 rows, n = 1,000,000 columns,
+column nonzeros clip(Zipf(1.5), 2, 200), seed 0; the actual nnz is printed),
+FP32 iterates and matrix values, FP64 checks.  A step is one PDHG iteration
+(the fused primal step over A^T and the fused dual step over A), with the
+KKT/restart check every 40 iterations exactly as the engine runs it.  Setup
+(upload, device Ruiz/Pock-Chambolle scaling, FP32 images, lane autotuning,
+power iteration) happens before the timed region; inputs are resident in HBM.
+
+Prints one JSON line.  value = algorithmic HBM bytes of the SpMV kernels per
+iteration x iterations/s (GB/s); the roofline object gives the dominant
+kernel's achieved bandwidth measured with HIP events on the engine stream;
+cpu_baseline times the CPU oracle (single thread) on a bounded sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sovereign-solver_amd"))
+
+HBM_PEAK_GBS = 8000.0
+PERIOD = 40
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class DeviceProblem:
+    """The scaled LP and the FP32 iterate state on the device (the same calls
+    the fused engine makes, src/l1/pdhg.cpp solve_fused)."""
+
+    def __init__(self, ctx, a, prec):
+        from sovereign_amd import device as D
+        self.D, self.ctx, L = D, ctx, ctx.L
+        m, n = a["m"], a["n"]
+        self.m, self.n, self.nnz = m, n, a["nnz"]
+        self.A = D.Csr(ctx, m, n, a["row_ptr"], a["col_idx"], a["val"], own_values=True)
+        rs, cs = ctx.upload(np.ones(m)), ctx.upload(np.ones(n))
+        w = C.c_double()
+        for _ in range(8):
+            D.check(L.sov_hip_scale_ruiz_sweep(ctx.h, self.A.h, rs, cs, C.byref(w)))
+            if w.value < 1e-2:
+                break
+        D.check(L.sov_hip_scale_pock_chambolle(ctx.h, self.A.h, 1.0, rs, cs))
+        D.check(L.sov_hip_csr_scale(ctx.h, self.A.h, rs, cs))
+        r, c = ctx.download(rs, m), ctx.download(cs, n)
+        self.rs, self.cs = rs, cs
+        sign = -1.0 if a["maximize"] else 1.0
+        fin = np.isfinite
+        cs_h = c
+        c_s = (a["c"] * cs_h) * sign
+        clo = np.where(fin(a["col_lo"]), a["col_lo"] / cs_h, a["col_lo"])
+        chi = np.where(fin(a["col_hi"]), a["col_hi"] / cs_h, a["col_hi"])
+        rlo = np.where(fin(a["row_lo"]), a["row_lo"] * r, a["row_lo"])
+        rhi = np.where(fin(a["row_hi"]), a["row_hi"] * r, a["row_hi"])
+        self.prec = prec
+        dt = np.float32 if prec == 1 else np.float64
+        if prec == 1:
+            D.check(L.sov_hip_csr_make_f32(ctx.h, self.A.h))
+        D.check(L.sov_hip_csr_tune(ctx.h, self.A.h, prec))
+        st = D.PdhgState()
+        st.prec, st.m, st.n = prec, m, n
+        st.c, st.col_lo, st.col_hi = ctx.upload(c_s.astype(dt)), ctx.upload(clo.astype(dt)), ctx.upload(chi.astype(dt))
+        st.row_lo, st.row_hi = ctx.upload(rlo.astype(dt)), ctx.upload(rhi.astype(dt))
+        for f, ln in (("x", n), ("x_bar", n), ("x_sum", n), ("s", m), ("y", m), ("y_sum", m)):
+            setattr(st, f, ctx.upload(np.zeros(ln, dt)))
+        self.st = st
+        D.check(L.sov_hip_pdhg_set_point(ctx.h, C.byref(st), ctx.upload(np.zeros(n)), None,
+                                         ctx.upload(np.zeros(m))))
+        # power iteration on the scaled operator (FP64), as the engine does
+        u, v, wv = ctx.upload(np.ones(n)), ctx.upload(np.ones(m)), ctx.upload(np.zeros(m))
+        est = 0.0
+        for _ in range(30):
+            D.check(L.sov_hip_spmv(ctx.h, self.A.h, 0, 0, -1.0, u, 0.0, wv))
+            D.check(L.sov_hip_axpy_f64(ctx.h, m, 1.0, v, wv))
+            wn = C.c_double()
+            D.check(L.sov_hip_nrm2_f64(ctx.h, m, wv, C.byref(wn)))
+            D.check(L.sov_hip_scal_f64(ctx.h, m, 1.0 / wn.value, wv))
+            D.check(L.sov_hip_spmv(ctx.h, self.A.h, 1, 0, -1.0, wv, 0.0, u))
+            D.check(L.sov_hip_memcpy_d2d(ctx.h, v, wv, 8 * m))
+            un, vn = C.c_double(), C.c_double()
+            D.check(L.sov_hip_nrm2_f64(ctx.h, n, u, C.byref(un)))
+            D.check(L.sov_hip_nrm2_f64(ctx.h, m, v, C.byref(vn)))
+            est = (un.value ** 2 + vn.value ** 2) ** 0.5
+            D.check(L.sov_hip_scal_f64(ctx.h, n, 1.0 / est, u))
+            D.check(L.sov_hip_scal_f64(ctx.h, m, 1.0 / est, v))
+        self.op_norm = est
+        self.tau = self.sigma = 0.9 / est
+        ca = D.CheckArgs()
+        ca.st = C.pointer(st)
+        ca.tol_scaled, ca.tol_orig, ca.obj_sign, ca.obj_const = 1e-6, 1e-6, sign, a["obj_const"]
+        ca.row_scale, ca.col_scale = rs, cs
+        ca.orig_row_lo, ca.orig_row_hi = ctx.upload(a["row_lo"]), ctx.upload(a["row_hi"])
+        ca.orig_col_lo, ca.orig_col_hi, ca.orig_c = ctx.upload(a["col_lo"]), ctx.upload(a["col_hi"]), ctx.upload(a["c"])
+        ca.act_avg = ctx.upload(np.zeros(m))
+        self.ca = ca
+        self.since = 0
+        self.res_at_restart = float("inf")
+        self.iterations = 0
+        self.restarts = 0
+        self.last_check = None
+        D.check(L.sov_hip_sync(ctx.h))
+
+    def primal(self, first):
+        self.D.check(self.ctx.L.sov_hip_pdhg_primal_step(self.ctx.h, self.A.h, C.byref(self.st), self.tau, first))
+
+    def dual(self, first):
+        self.D.check(self.ctx.L.sov_hip_pdhg_dual_step(self.ctx.h, self.A.h, C.byref(self.st), self.tau,
+                                                       self.sigma, first))
+
+    def iterate(self, count):
+        """count iterations with the engine's check/restart every PERIOD."""
+        for _ in range(count):
+            first = 1 if self.since == 0 else 0
+            self.primal(first)
+            self.dual(first)
+            self.since += 1
+            self.iterations += 1
+            if self.iterations % PERIOD == 0:
+                self.check_and_restart()
+
+    def check_and_restart(self):
+        D, L = self.D, self.ctx.L
+        self.ca.since_restart = self.since
+        out = D.CheckOut()
+        D.check(L.sov_hip_pdhg_check(self.ctx.h, self.A.h, C.byref(self.ca), C.byref(out)))
+        self.last_check = out
+        cur, avg = out.kkt_cur, out.kkt_avg
+        if not np.isfinite(self.res_at_restart):
+            self.res_at_restart = cur
+        if avg <= 0.5 * self.res_at_restart:
+            D.check(L.sov_hip_pdhg_restart(self.ctx.h, C.byref(self.st), self.since, self.ca.act_avg, None, None,
+                                           None))
+            self.res_at_restart = avg
+            self.since = 0
+            self.restarts += 1
+
+
+def spmv_bytes(m, n, nnz, sv, sx):
+    """Algorithmic bytes of the two fused SpMV kernels (each vector element
+    counted once; gather misses not credited):
+      primal (A^T y):  nnz (sv + 4) + 4 (n + 1) + 4 n (row list) + sx m (y)
+                       + sx n * 8 (x, c, lo, hi, x_sum read; x, x_bar, x_sum written)
+      dual   (A x_bar): nnz (sv + 4) + 4 (m + 1) + 4 m + sx n (x_bar)
+                       + sx m * 8 (s, y, lo, hi, y_sum read; s, y, y_sum written)"""
+    primal = nnz * (sv + 4) + 4 * (n + 1) + 4 * n + sx * m + 8 * sx * n
+    dual = nnz * (sv + 4) + 4 * (m + 1) + 4 * m + sx * n + 8 * sx * m
+    return primal, dual
+
+
+def cpu_baseline(a, budget_iters):
+    """CPU oracle (single thread) on the same instance: per-iteration time of
+    the reference loop measured as (t(setup + k iterations) - t(setup)) / k."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    M = O.Model(a)
+    t0 = time.perf_counter()
+    O.pdhg(M, tolerance=1e-12, max_iterations=0)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.pdhg(M, tolerance=1e-12, max_iterations=budget_iters)
+    t_run = time.perf_counter() - t0
+    per_it = max(t_run - t_setup, 1e-9) / budget_iters
+    return per_it, t_setup
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--m", type=int, default=500_000)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is described "
+              "in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import sovereign_amd as S
+    from sovereign_amd import device as D
+
+    t0 = time.perf_counter()
+    lp = S.LP.random_zipf(args.m, args.n, args.seed)
+    a = lp.arrays()
+    log(f"instance: m={a['m']} n={a['n']} nnz={a['nnz']} ({time.perf_counter() - t0:.1f}s to generate)")
+    prec = 1 if args.precision == "f32" else 0
+    ctx = D.Ctx(0)
+    t0 = time.perf_counter()
+    P = DeviceProblem(ctx, a, prec)
+    log(f"setup: {time.perf_counter() - t0:.2f}s  op_norm={P.op_norm:.6g}")
+    lanes_a, lanes_t = (C.c_int32 * 8)(), (C.c_int32 * 8)()
+    ctx.L.sov_hip_csr_get_lanes(P.A.h, 0, lanes_a)
+    ctx.L.sov_hip_csr_get_lanes(P.A.h, 1, lanes_t)
+    log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}")
+
+    P.iterate(args.warmup)
+    D.check(ctx.L.sov_hip_sync(ctx.h))
+    t0 = time.perf_counter()
+    P.iterate(args.steps)
+    D.check(ctx.L.sov_hip_sync(ctx.h))
+    elapsed = time.perf_counter() - t0
+    its = args.steps / elapsed
+    log(f"timed: {args.steps} iterations in {elapsed:.3f}s -> {its:.1f} it/s; restarts so far {P.restarts}")
+
+    # per-kernel durations with events on the engine stream
+    K = 50
+    ev = D.Events(ctx, 3 * K + 1)
+    for k in range(K):
+        first = 1 if P.since == 0 else 0
+        ev.record(3 * k)
+        P.primal(first)
+        ev.record(3 * k + 1)
+        P.dual(first)
+        ev.record(3 * k + 2)
+        P.since += 1
+        P.iterations += 1
+    D.check(ctx.L.sov_hip_sync(ctx.h))
+    tp = sum(ev.elapsed(3 * k, 3 * k + 1) for k in range(K)) / K
+    td = sum(ev.elapsed(3 * k + 1, 3 * k + 2) for k in range(K)) / K
+    ev.destroy()
+    sv = sx = 4 if prec == 1 else 8
+    bp, bd = spmv_bytes(a["m"], a["n"], a["nnz"], sv, sx)
+    log(f"primal step {tp * 1e3:.1f} us ({bp / tp / 1e6:.0f} GB/s), dual step {td * 1e3:.1f} us "
+        f"({bd / td / 1e6:.0f} GB/s)")
+    dom_name, dom_t, dom_b = ("primal_step (A^T y fused)", tp, bp) if tp >= td else ("dual_step (A x_bar fused)", td, bd)
+    achieved = dom_b / dom_t / 1e6
+    value = (bp + bd) * its / 1e9
+    chk = P.last_check
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        per_it, t_setup = cpu_baseline(a, args.cpu_iters)
+        cpu = {"value": round((bp + bd) / per_it / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle (oracle/oracle.c, FP64, 1 thread) on the same 1M-column instance: "
+                         f"{args.cpu_iters} reference-loop iterations after setup ({t_setup:.1f}s), "
+                         f"{per_it * 1e3:.1f} ms/iteration = {1.0 / per_it:.3f} it/s; value uses the same "
+                         f"algorithmic bytes per iteration as the GPU line"}
+        log(f"cpu oracle: {per_it * 1e3:.1f} ms/iteration")
+
+    out = {
+        "metric": "sec to 1e-6 rel KKT (cold vs learned warm start); batched LPs/s; SpMV HBM GB/s",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / its, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == 1 else "f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"PDHG iteration on the Zipf LP m={a['m']} n={a['n']} nnz={a['nnz']} "
+                               f"(clip(Zipf(1.5),2,200) nonzeros per column, seed {args.seed}); "
+                               f"{'FP32' if prec == 1 else 'FP64'} iterates, FP64 checks every {PERIOD}",
+                   "iterations_per_s": round(its, 2), "restarts": P.restarts,
+                   "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
+                   "kkt_scaled_at_last_check": chk.kkt_cur if chk else None},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(dom_b)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

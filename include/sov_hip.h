@@ -249,6 +249,13 @@ void sov_hip_graph_destroy(sov_hip_graph* g);
  * Event pair on the context stream, for in-process kernel timing. */
 int sov_hip_timer_start(sov_hip_ctx* ctx);
 int sov_hip_timer_stop(sov_hip_ctx* ctx, double* milliseconds);
+/* An array of `count` events; record event i on the context stream; after a
+ * sov_hip_sync, elapsed time between events i and j. */
+typedef struct sov_hip_events sov_hip_events;
+int sov_hip_events_create(sov_hip_ctx* ctx, int count, sov_hip_events** out);
+int sov_hip_events_record(sov_hip_ctx* ctx, sov_hip_events* ev, int i);
+int sov_hip_events_elapsed(sov_hip_events* ev, int i, int j, double* milliseconds);
+void sov_hip_events_destroy(sov_hip_events* ev);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,113 @@
+/*
+ * sovereign_c.h -- C entry points over the C++ engine (lib/libsovereign.so),
+ * for FFI callers (Python ctypes in tests/ and bench.py, or any other
+ * language).  The C++ API of include/sovereign/ headers is the primary
+ * interface; this is a flat mirror of the pieces a caller of
+ * `sovereign-cli solve --engine pdhg` needs: build or read a model, solve it
+ * with PDHG on a chosen backend, read the result.  Nothing throws across it;
+ * functions return sov::Status codes (0 = Ok) and sov_c_last_error() gives
+ * the calling thread's message.
+ */
+#ifndef SOVEREIGN_C_H
+#define SOVEREIGN_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sov_lp sov_lp;   /* an immutable sov::Problem */
+
+const char* sov_c_last_error(void);
+
+/* ---- models --------------------------------------------------------- */
+sov_lp* sov_lp_from_csr(int32_t m, int32_t n, int32_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                        const double* val, const double* c, const double* col_lo, const double* col_hi,
+                        const double* row_lo, const double* row_hi, int32_t maximize, double obj_const);
+/* fmt: 0 = CPLEX LP text, 1 = MPS text */
+sov_lp* sov_lp_from_text(const char* text, int32_t fmt);
+sov_lp* sov_lp_read_file(const char* path);
+void sov_lp_destroy(sov_lp* lp);
+int sov_lp_dims(const sov_lp* lp, int32_t* m, int32_t* n, int32_t* nnz);
+/* Copy the model out; any pointer may be NULL. */
+int sov_lp_arrays(const sov_lp* lp, int32_t* row_ptr, int32_t* col_idx, double* val, double* c, double* col_lo,
+                  double* col_hi, double* row_lo, double* row_hi, int32_t* maximize, double* obj_const);
+
+/* ---- seeded generators (include/sovereign/generators.hpp) -------------- */
+sov_lp* sov_gen_random_lp(int32_t m, int32_t n, int32_t k, uint64_t seed);
+sov_lp* sov_gen_random_lp_zipf(int32_t m, int32_t n, double a, int32_t kmin, int32_t kmax, uint64_t seed);
+sov_lp* sov_gen_transportation(int32_t sources, int32_t sinks, uint64_t seed);
+int sov_gen_batch_sizes(int32_t count, int32_t lo, int32_t hi, uint64_t seed, int32_t* m_out);
+
+/* ---- host scaling (Scaling::equilibrate, for checks) ------------------ */
+int sov_scaling_factors(const sov_lp* lp, int32_t apply_ruiz, int32_t apply_pc, double* row_scale,
+                        double* col_scale);
+
+/* ---- PDHG ---------------------------------------------------------------
+ * Mirrors sov::PdhgOptions / PdhgResult (include/sovereign/pdhg.hpp). */
+enum { SOV_BACKEND_HOST = 0, SOV_BACKEND_HIP = 2 };
+enum { SOV_ENGINE_AUTO = 0, SOV_ENGINE_PRIMITIVE = 1, SOV_ENGINE_FUSED = 2 };
+enum { SOV_TERM_REFERENCE = 0, SOV_TERM_RELATIVE_KKT = 1 };
+
+typedef struct sov_pdhg_opts {
+    double tolerance;
+    int64_t max_iterations;
+    double time_limit_seconds;
+    int64_t restart_check_period;
+    double restart_sufficient_decrease;
+    double step_size_safety;
+    int32_t power_iterations;
+    int32_t verbose;
+    int32_t engine;        /* SOV_ENGINE_* */
+    int32_t termination;   /* SOV_TERM_* */
+    int32_t precision;     /* 0 = f64, 1 = f32 */
+    int32_t fp64_polish;
+    double fp32_switch_factor;
+    int32_t fp32_stall_checks;
+    int32_t use_primal_weight;
+    double initial_primal_weight;
+    double primal_weight_smoothing;
+    int32_t detect_infeasibility;
+    int32_t use_graphs;
+    const double* warm_start_dual;   /* NULL or num_rows values (original units) */
+    int64_t trace_cap;               /* checkpoint trace: residual and iteration of each check */
+    double* trace_res;
+    int64_t* trace_it;
+} sov_pdhg_opts;
+
+typedef struct sov_pdhg_result {
+    int32_t status;        /* sov::PdhgStatus */
+    int32_t reserved;
+    double objective;
+    double primal_infeasibility;
+    double dual_infeasibility;
+    double duality_gap;
+    int64_t iterations;
+    int64_t restarts;
+    double seconds;
+    double setup_seconds;
+    double loop_seconds;
+    double operator_norm_estimate;
+    double relative_primal;
+    double relative_dual;
+    double relative_gap;
+    double primal_weight;
+    int64_t fp64_polish_iterations;
+    int64_t trace_len;
+} sov_pdhg_result;
+
+void sov_pdhg_default_opts(sov_pdhg_opts* o);
+/* backend: SOV_BACKEND_*.  x[n], y[m], rc[n] may be NULL.  msg (optional)
+ * receives PdhgResult::message.  Returns 0 when the solve ran (whatever its
+ * PdhgStatus), else a Status code (e.g. the backend is unavailable). */
+int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, double* x, double* y, double* rc,
+                   sov_pdhg_result* res, char* msg, int32_t msg_len);
+
+/* Backend report line (available backends, default, index width). */
+int sov_backend_report(char* buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -43,10 +43,11 @@ Problem g_parsed;
 
 }  // namespace
 
+#define REF_API __attribute__((visibility("default")))
 extern "C" {
 
 // stats: [objective, primal_inf, dual_inf, gap, iterations, restarts, seconds, op_norm]
-int ref_pdhg_solve(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
+REF_API int ref_pdhg_solve(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
                    const double* c, const double* clo, const double* chi, const double* rlo,
                    const double* rhi, int32_t maximize, double obj_const, double tol,
                    int64_t max_it, int64_t period, double time_limit, double* x, double* y,
@@ -91,7 +92,7 @@ int ref_pdhg_solve(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, c
     }
 }
 
-int ref_scaling(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
+REF_API int ref_scaling(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
                 int32_t apply_ruiz, int32_t apply_pc, double* rs, double* cs) {
     try {
         std::vector<Idx> rpv(rp, rp + m + 1), civ(ci, ci + rp[m]);
@@ -110,7 +111,7 @@ int ref_scaling(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, cons
 }
 
 // Parse an LP (is_mps == 0) or MPS text; sizes out, arrays via ref_parsed_get.
-int ref_parse(const char* text, int32_t is_mps, int32_t* m, int32_t* n, int32_t* nnz) {
+REF_API int ref_parse(const char* text, int32_t is_mps, int32_t* m, int32_t* n, int32_t* nnz) {
     try {
         g_parsed = is_mps ? read_mps_string(text).problem : read_lp_string(text).problem;
         *m = g_parsed.num_rows();
@@ -122,7 +123,7 @@ int ref_parse(const char* text, int32_t is_mps, int32_t* m, int32_t* n, int32_t*
     }
 }
 
-int ref_parsed_get(int32_t* rp, int32_t* ci, double* v, double* c, double* clo, double* chi,
+REF_API int ref_parsed_get(int32_t* rp, int32_t* ci, double* v, double* c, double* clo, double* chi,
                    double* rlo, double* rhi, int32_t* maximize, double* obj_const) {
     const Problem& p = g_parsed;
     const auto& A = p.matrix();

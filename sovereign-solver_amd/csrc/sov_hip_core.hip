@@ -436,4 +436,48 @@ int sov_hip_timer_stop(sov_hip_ctx* ctx, double* ms) {
     return SOV_OK;
 }
 
+struct sov_hip_events {
+    std::vector<hipEvent_t> ev;
+};
+
+int sov_hip_events_create(sov_hip_ctx* ctx, int count, sov_hip_events** out) {
+    if (ctx == nullptr || out == nullptr || count <= 0) return fail(SOV_INVALID_ARGUMENT, "events_create args");
+    DeviceGuard g(ctx->device);
+    auto* e = new (std::nothrow) sov_hip_events();
+    if (e == nullptr) return fail(SOV_OUT_OF_MEMORY, "events");
+    e->ev.assign(static_cast<size_t>(count), nullptr);
+    for (auto& x : e->ev) {
+        const hipError_t r = hipEventCreate(&x);
+        if (r != hipSuccess) {
+            sov_hip_events_destroy(e);
+            return hip_fail(r, "hipEventCreate");
+        }
+    }
+    *out = e;
+    return SOV_OK;
+}
+
+int sov_hip_events_record(sov_hip_ctx* ctx, sov_hip_events* e, int i) {
+    if (ctx == nullptr || e == nullptr || i < 0 || i >= (int)e->ev.size())
+        return fail(SOV_INVALID_ARGUMENT, "events_record args");
+    SOV_HIP_TRY(hipEventRecord(e->ev[static_cast<size_t>(i)], ctx->stream));
+    return SOV_OK;
+}
+
+int sov_hip_events_elapsed(sov_hip_events* e, int i, int j, double* ms) {
+    if (e == nullptr || ms == nullptr || i < 0 || j < 0 || i >= (int)e->ev.size() || j >= (int)e->ev.size())
+        return fail(SOV_INVALID_ARGUMENT, "events_elapsed args");
+    float f = 0.f;
+    SOV_HIP_TRY(hipEventElapsedTime(&f, e->ev[static_cast<size_t>(i)], e->ev[static_cast<size_t>(j)]));
+    *ms = f;
+    return SOV_OK;
+}
+
+void sov_hip_events_destroy(sov_hip_events* e) {
+    if (e == nullptr) return;
+    for (auto& x : e->ev)
+        if (x) (void)hipEventDestroy(x);
+    delete e;
+}
+
 }  // extern "C"

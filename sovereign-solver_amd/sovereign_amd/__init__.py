@@ -1,0 +1,290 @@
+"""Python binding of the MI355X PDHG engine (ctypes over the in-tree builds).
+
+Loads ``lib/libsovereign.so`` (C++ engine, include/sovereign_c.h) and
+``lib/libsov_hip.so`` (HIP kernels, include/sov_hip.h) from this package's
+directory.  Nothing here computes: every solve and every kernel runs in the
+native libraries, and a missing library raises instead of falling back.
+
+Mirrors the reference's solve entry (apps/sovereign_cli.cpp:196-236 /
+src/l1/pdhg.cpp:218): ``solve(lp, backend="hip", **options)`` returns the
+PdhgResult fields plus the primal/dual/reduced-cost vectors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
+
+STATUS = ["optimal", "iteration_limit", "time_limit", "numerical_failure", "not_solved",
+          "cancelled", "primal_infeasible", "dual_infeasible"]
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+
+
+class PdhgOpts(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double), ("max_iterations", C.c_int64), ("time_limit_seconds", C.c_double),
+        ("restart_check_period", C.c_int64), ("restart_sufficient_decrease", C.c_double),
+        ("step_size_safety", C.c_double), ("power_iterations", C.c_int32), ("verbose", C.c_int32),
+        ("engine", C.c_int32), ("termination", C.c_int32), ("precision", C.c_int32), ("fp64_polish", C.c_int32),
+        ("fp32_switch_factor", C.c_double), ("fp32_stall_checks", C.c_int32), ("use_primal_weight", C.c_int32),
+        ("initial_primal_weight", C.c_double), ("primal_weight_smoothing", C.c_double),
+        ("detect_infeasibility", C.c_int32), ("use_graphs", C.c_int32), ("warm_start_dual", _f64p),
+        ("trace_cap", C.c_int64), ("trace_res", _f64p), ("trace_it", _i64p),
+    ]
+
+
+class PdhgRes(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("reserved", C.c_int32), ("objective", C.c_double),
+        ("primal_infeasibility", C.c_double), ("dual_infeasibility", C.c_double), ("duality_gap", C.c_double),
+        ("iterations", C.c_int64), ("restarts", C.c_int64), ("seconds", C.c_double), ("setup_seconds", C.c_double),
+        ("loop_seconds", C.c_double), ("operator_norm_estimate", C.c_double), ("relative_primal", C.c_double),
+        ("relative_dual", C.c_double), ("relative_gap", C.c_double), ("primal_weight", C.c_double),
+        ("fp64_polish_iterations", C.c_int64), ("trace_len", C.c_int64),
+    ]
+
+
+_engine = None
+_hip = None
+
+
+def lib_path(name: str) -> str:
+    return os.path.join(LIB_DIR, name)
+
+
+def engine() -> C.CDLL:
+    """libsovereign.so (raises if it was not built)."""
+    global _engine
+    if _engine is None:
+        p = lib_path("libsovereign.so")
+        if not os.path.exists(p):
+            raise RuntimeError(f"{p} is missing: run __graft_entry__.build() first")
+        _engine = C.CDLL(p)
+        L = _engine
+        L.sov_c_last_error.restype = C.c_char_p
+        L.sov_lp_from_csr.restype = C.c_void_p
+        L.sov_lp_from_csr.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                      _f64p, _f64p, C.c_int32, C.c_double]
+        L.sov_lp_from_text.restype = C.c_void_p
+        L.sov_lp_from_text.argtypes = [C.c_char_p, C.c_int32]
+        L.sov_lp_read_file.restype = C.c_void_p
+        L.sov_lp_read_file.argtypes = [C.c_char_p]
+        L.sov_lp_destroy.argtypes = [C.c_void_p]
+        L.sov_lp_dims.argtypes = [C.c_void_p, _i32p, _i32p, _i32p]
+        L.sov_lp_arrays.argtypes = [C.c_void_p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _i32p,
+                                    _f64p]
+        L.sov_gen_random_lp.restype = C.c_void_p
+        L.sov_gen_random_lp.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
+        L.sov_gen_random_lp_zipf.restype = C.c_void_p
+        L.sov_gen_random_lp_zipf.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_uint64]
+        L.sov_gen_transportation.restype = C.c_void_p
+        L.sov_gen_transportation.argtypes = [C.c_int32, C.c_int32, C.c_uint64]
+        L.sov_gen_batch_sizes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _i32p]
+        L.sov_scaling_factors.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p]
+        L.sov_pdhg_default_opts.argtypes = [C.POINTER(PdhgOpts)]
+        L.sov_pdhg_solve.argtypes = [C.c_void_p, C.c_int32, C.POINTER(PdhgOpts), _f64p, _f64p, _f64p,
+                                     C.POINTER(PdhgRes), C.c_char_p, C.c_int32]
+        L.sov_backend_report.argtypes = [C.c_char_p, C.c_int32]
+    return _engine
+
+
+def hip() -> C.CDLL:
+    """libsov_hip.so, the C-ABI of include/sov_hip.h (raises if not built)."""
+    global _hip
+    if _hip is None:
+        p = lib_path("libsov_hip.so")
+        if not os.path.exists(p):
+            raise RuntimeError(f"{p} is missing: run __graft_entry__.build() first")
+        _hip = C.CDLL(p, mode=C.RTLD_GLOBAL)
+        _hip.sov_hip_last_error.restype = C.c_char_p
+        _hip.sov_hip_ctx_device_name.restype = C.c_char_p
+    return _hip
+
+
+def ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def last_error() -> str:
+    return engine().sov_c_last_error().decode()
+
+
+def backend_report() -> str:
+    buf = C.create_string_buffer(512)
+    engine().sov_backend_report(buf, 512)
+    return buf.value.decode()
+
+
+def hip_device_count() -> int:
+    n = C.c_int(0)
+    hip().sov_hip_device_count(C.byref(n))
+    return n.value
+
+
+class LP:
+    """An immutable model held by the native library (sov::Problem)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("model construction failed: " + last_error())
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and engine() is not None:
+            engine().sov_lp_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def from_csr(cls, m, n, row_ptr, col_idx, val, c, col_lo, col_hi, row_lo, row_hi, maximize=False,
+                 obj_const=0.0):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        ci = np.ascontiguousarray(col_idx, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (c, col_lo, col_hi, row_lo, row_hi)]
+        h = engine().sov_lp_from_csr(m, n, len(v), ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(v, C.c_double),
+                                     *[ptr(a, C.c_double) for a in arrs], int(bool(maximize)), float(obj_const))
+        return cls(h)
+
+    @classmethod
+    def from_text(cls, text: str, mps: bool = False):
+        return cls(engine().sov_lp_from_text(text.encode(), 1 if mps else 0))
+
+    @classmethod
+    def read(cls, path: str):
+        return cls(engine().sov_lp_read_file(path.encode()))
+
+    @classmethod
+    def random(cls, m, n, k, seed):
+        return cls(engine().sov_gen_random_lp(m, n, k, seed))
+
+    @classmethod
+    def random_zipf(cls, m, n, seed, a=1.5, kmin=2, kmax=200):
+        return cls(engine().sov_gen_random_lp_zipf(m, n, a, kmin, kmax, seed))
+
+    @classmethod
+    def transportation(cls, sources, sinks, seed):
+        return cls(engine().sov_gen_transportation(sources, sinks, seed))
+
+    def dims(self):
+        m, n, z = C.c_int32(), C.c_int32(), C.c_int32()
+        engine().sov_lp_dims(self._h, C.byref(m), C.byref(n), C.byref(z))
+        return m.value, n.value, z.value
+
+    def arrays(self) -> dict:
+        m, n, nnz = self.dims()
+        out = dict(m=m, n=n, nnz=nnz, row_ptr=np.zeros(m + 1, np.int32), col_idx=np.zeros(max(nnz, 1), np.int32),
+                   val=np.zeros(max(nnz, 1)), c=np.zeros(max(n, 1)), col_lo=np.zeros(max(n, 1)),
+                   col_hi=np.zeros(max(n, 1)), row_lo=np.zeros(max(m, 1)), row_hi=np.zeros(max(m, 1)))
+        mx, k = C.c_int32(), C.c_double()
+        engine().sov_lp_arrays(self._h, ptr(out["row_ptr"], C.c_int32), ptr(out["col_idx"], C.c_int32),
+                               ptr(out["val"], C.c_double), ptr(out["c"], C.c_double), ptr(out["col_lo"], C.c_double),
+                               ptr(out["col_hi"], C.c_double), ptr(out["row_lo"], C.c_double),
+                               ptr(out["row_hi"], C.c_double), C.byref(mx), C.byref(k))
+        for key, ln in (("col_idx", nnz), ("val", nnz), ("c", n), ("col_lo", n), ("col_hi", n), ("row_lo", m),
+                        ("row_hi", m)):
+            out[key] = out[key][:ln]
+        out["maximize"] = bool(mx.value)
+        out["obj_const"] = k.value
+        return out
+
+    def scaling(self, ruiz=True, pc=True):
+        m, n, _ = self.dims()
+        rs, cs = np.zeros(max(m, 1)), np.zeros(max(n, 1))
+        engine().sov_scaling_factors(self._h, int(ruiz), int(pc), ptr(rs, C.c_double), ptr(cs, C.c_double))
+        return rs[:m], cs[:n]
+
+
+def batch_sizes(count, lo, hi, seed):
+    out = np.zeros(count, np.int32)
+    engine().sov_gen_batch_sizes(count, lo, hi, seed, ptr(out, C.c_int32))
+    return out
+
+
+@dataclass
+class Result:
+    status: str
+    objective: float
+    primal_infeasibility: float
+    dual_infeasibility: float
+    duality_gap: float
+    iterations: int
+    restarts: int
+    seconds: float
+    setup_seconds: float
+    loop_seconds: float
+    operator_norm_estimate: float
+    relative_kkt: tuple
+    primal_weight: float
+    fp64_polish_iterations: int
+    x: np.ndarray
+    y: np.ndarray
+    rc: np.ndarray
+    message: str = ""
+    trace_res: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    trace_it: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+
+
+_ENGINES = {"auto": 0, "primitive": 1, "fused": 2}
+
+
+def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, time_limit=0.0,
+          restart_check_period=40, engine_kind="auto", termination="reference", precision="f64",
+          fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
+          use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None) -> Result:
+    """Pdhg(options).solve(problem, backend) through the C entry point."""
+    L = engine()
+    o = PdhgOpts()
+    L.sov_pdhg_default_opts(C.byref(o))
+    o.tolerance = tolerance
+    o.max_iterations = max_iterations
+    o.time_limit_seconds = time_limit
+    o.restart_check_period = restart_check_period
+    o.engine = _ENGINES[engine_kind]
+    o.termination = 1 if termination == "relative" else 0
+    o.precision = 1 if precision == "f32" else 0
+    o.fp64_polish = int(fp64_polish)
+    o.use_primal_weight = int(primal_weight)
+    o.initial_primal_weight = initial_primal_weight
+    o.use_graphs = int(use_graphs)
+    o.verbose = int(verbose)
+    if fp32_switch_factor is not None:
+        o.fp32_switch_factor = fp32_switch_factor
+    if fp32_stall_checks is not None:
+        o.fp32_stall_checks = fp32_stall_checks
+    m, n, _ = lp.dims()
+    keep = []
+    if warm_start_dual is not None:
+        w = np.ascontiguousarray(warm_start_dual, np.float64)
+        assert w.shape == (m,)
+        keep.append(w)
+        o.warm_start_dual = ptr(w, C.c_double)
+    tr = np.zeros(max(trace_cap, 1))
+    ti = np.zeros(max(trace_cap, 1), np.int64)
+    if trace_cap > 0:
+        o.trace_cap = trace_cap
+        o.trace_res = ptr(tr, C.c_double)
+        o.trace_it = ptr(ti, C.c_int64)
+    x, y, rc = np.zeros(max(n, 1)), np.zeros(max(m, 1)), np.zeros(max(n, 1))
+    r = PdhgRes()
+    msg = C.create_string_buffer(1024)
+    rc_code = L.sov_pdhg_solve(lp.handle, 2 if backend == "hip" else 0, C.byref(o), ptr(x, C.c_double),
+                               ptr(y, C.c_double), ptr(rc, C.c_double), C.byref(r), msg, 1024)
+    if rc_code != 0:
+        raise RuntimeError(f"sov_pdhg_solve failed ({rc_code}): {last_error()}")
+    nt = min(int(r.trace_len), trace_cap)
+    return Result(STATUS[r.status], r.objective, r.primal_infeasibility, r.dual_infeasibility, r.duality_gap,
+                  int(r.iterations), int(r.restarts), r.seconds, r.setup_seconds, r.loop_seconds,
+                  r.operator_norm_estimate, (r.relative_primal, r.relative_dual, r.relative_gap), r.primal_weight,
+                  int(r.fp64_polish_iterations), x[:n], y[:m], rc[:n], msg.value.decode(), tr[:nt], ti[:nt])

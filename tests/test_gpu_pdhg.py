@@ -1,0 +1,142 @@
+"""GPU parity of the PDHG engine (C++ host loop over the HIP C-ABI) against the
+CPU oracle and the reference's recorded results.
+
+Bar (north star): same termination status, objective within 1e-6 relative,
+relative KKT under the run's tolerance; FP64 trajectories agree with the
+oracle to rounding, FP32 runs are stated against the FP64 oracle solve.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import sovereign_amd as S
+
+pytestmark = pytest.mark.gpu
+
+OBJ_RTOL = 1e-6
+
+
+def lp_of(d):
+    d = {k: ([float(v) for v in d[k]] if k in ("col_lo", "col_hi", "row_lo", "row_hi") else d[k]) for k in d}
+    return S.LP.from_csr(d["m"], d["n"], d["row_ptr"], d["col_idx"], d["val"], d["c"], d["col_lo"], d["col_hi"],
+                         d["row_lo"], d["row_hi"], d["maximize"], d["obj_const"])
+
+
+def rel(a, b):
+    return abs(a - b) / (1.0 + abs(b))
+
+
+@pytest.mark.parametrize("engine", ["primitive", "fused"])
+def test_hand_lps_match_reference(gpu, golden, engine):
+    for name, case in golden["lps"].items():
+        g = case["result"]
+        cap = 20000 if name.startswith("smoke_") else 1000000
+        print(f"[{engine}] {name}", flush=True)
+        r = S.solve(lp_of(case["model"]), backend="hip", engine_kind=engine, max_iterations=cap,
+                    restart_check_period=case.get("period", 40))
+        print(f"   gpu {r.status} {r.objective:.12g} it={r.iterations} rs={r.restarts} | "
+              f"ref {g['status']} {g['objective']} it={g['iterations']} rs={g['restarts']}", flush=True)
+        assert r.status == g["status"], name
+        if g["status"] == "optimal":
+            assert rel(r.objective, g["objective"]) < OBJ_RTOL, name
+            assert abs(r.iterations - g["iterations"]) <= 2 * case.get("period", 40), name
+        assert abs(r.operator_norm_estimate - g["operator_norm"]) < 1e-12 * g["operator_norm"]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_lp_fused_matches_reference(gpu, golden, seed):
+    case = golden["random"][f"c0_200x500_s{seed}_tol0.0001"]
+    g = case["result"]
+    lp = S.LP.random(200, 500, 5, seed)
+    r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-4, max_iterations=case["max_iterations"])
+    print(f"seed {seed}: gpu {r.status} {r.objective:.12g} it={r.iterations} | ref {g['status']} "
+          f"{g['objective']:.12g} it={g['iterations']}", flush=True)
+    assert r.status == g["status"]
+    assert rel(r.objective, g["objective"]) < OBJ_RTOL
+    assert np.max(np.abs(r.x - np.array(g["x"]))) < 1e-5 * (1 + np.max(np.abs(g["x"])))
+
+
+def test_short_trajectory_is_the_oracles_to_rounding(gpu):
+    lp = S.LP.random(300, 800, 5, 4)
+    M = O.Model(lp.arrays())
+    for engine in ("primitive", "fused"):
+        r = S.solve(lp, backend="hip", engine_kind=engine, tolerance=1e-12, max_iterations=400)
+        o = O.pdhg(M, tolerance=1e-12, max_iterations=400)
+        assert r.status == o["status"] == "iteration_limit"
+        assert r.restarts == o["restarts"]
+        err = np.max(np.abs(r.x - o["x"])) / (1 + np.max(np.abs(o["x"])))
+        erry = np.max(np.abs(r.y - o["y"])) / (1 + np.max(np.abs(o["y"])))
+        print(f"{engine}: 400-iteration x error {err:.3g}, y error {erry:.3g}", flush=True)
+        assert err < 1e-10 and erry < 1e-10
+
+
+def test_graph_replay_is_deterministic(gpu):
+    lp = S.LP.random(200, 500, 5, 1)
+    a = S.solve(lp, backend="hip", engine_kind="fused", use_graphs=True)
+    b = S.solve(lp, backend="hip", engine_kind="fused", use_graphs=True)
+    c = S.solve(lp, backend="hip", engine_kind="fused", use_graphs=False)
+    assert a.iterations == b.iterations and np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    assert a.status == c.status and abs(a.iterations - c.iterations) <= 80
+    assert rel(a.objective, c.objective) < OBJ_RTOL
+
+
+def test_relative_kkt_fp64_and_fp32_against_fp64_oracle(gpu):
+    lp = S.LP.random(200, 500, 5, 1)
+    M = O.Model(lp.arrays())
+    o = O.pdhg(M, tolerance=1e-6, termination=1, max_iterations=400000)
+    assert o["status"] == "optimal"
+    for prec in ("f64", "f32"):
+        r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative", precision=prec,
+                    max_iterations=400000)
+        print(f"{prec}: {r.status} obj {r.objective:.12g} (oracle {o['objective']:.12g}) it={r.iterations} "
+              f"polish={r.fp64_polish_iterations} rel={r.relative_kkt}", flush=True)
+        assert r.status == "optimal"
+        assert max(r.relative_kkt) <= 1e-6 * 1.0001
+        assert rel(r.objective, o["objective"]) < OBJ_RTOL
+
+
+def test_checkpoint_trace_and_time_limit(gpu, golden):
+    case = golden["lps"]["two_variable_optimum"]
+    r = S.solve(lp_of(case["model"]), backend="hip", engine_kind="fused", trace_cap=1000)
+    g = case["result"]
+    assert len(r.trace_res) == len(g["trace"])
+    np.testing.assert_allclose(r.trace_res, g["trace"], rtol=1e-9, atol=1e-12)
+    t = S.solve(lp_of(case["model"]), backend="hip", engine_kind="fused", time_limit=1e-9)
+    assert t.status == "time_limit" and t.iterations == 0 and len(t.x) == 2
+
+
+def test_transportation_lp(gpu):
+    lp = S.LP.transportation(20, 30, 0)
+    M = O.Model(lp.arrays())
+    o = O.pdhg(M, tolerance=1e-4, max_iterations=300000)
+    r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-4, max_iterations=300000)
+    print(f"transport: gpu {r.status} {r.objective:.10g} it={r.iterations} | oracle {o['status']} "
+          f"{o['objective']:.10g} it={o['iterations']}", flush=True)
+    assert r.status == o["status"]
+    if o["status"] == "optimal":
+        assert rel(r.objective, o["objective"]) < OBJ_RTOL
+
+
+def test_full_size_zipf_spmv_against_oracle(gpu):
+    # The 1M-column Zipf instance itself (m = 500k): both products at full size.
+    from sovereign_amd.device import Csr, Ctx
+    lp = S.LP.random_zipf(500_000, 1_000_000, 0)
+    a = lp.arrays()
+    print("nnz", a["nnz"], flush=True)
+    M = O.Model(a)
+    ctx = Ctx(0)
+    try:
+        A = Csr(ctx, a["m"], a["n"], a["row_ptr"], a["col_idx"], a["val"])
+        rng = np.random.default_rng(0)
+        x, y = rng.standard_normal(a["n"]), rng.standard_normal(a["m"])
+        got = A.spmv(x)
+        want = O.spmv(M, x)
+        scale = O.spmv(O.Model(dict(a, val=np.abs(a["val"]))), np.abs(x)) + 1e-300
+        assert np.max(np.abs(got - want) / scale) < 1e-14
+        got = A.spmv(y, transpose=True)
+        want = O.spmv_t(M, y)
+        scale = O.spmv_t(O.Model(dict(a, val=np.abs(a["val"]))), np.abs(y)) + 1e-300
+        assert np.max(np.abs(got - want) / scale) < 1e-14
+        A.destroy()
+    finally:
+        ctx.close()

@@ -1,0 +1,148 @@
+"""CPU tests of the product's host side: the C++ engine on the Host backend,
+the model readers, the generators, and the exported C-ABI surface.  The
+kernels themselves are exercised only by the gpu-marked tests."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import sovereign_amd as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def fixture_model(d):
+    return {k: ([float(v) for v in d[k]] if k in ("col_lo", "col_hi", "row_lo", "row_hi") else d[k]) for k in d}
+
+
+def lp_of(d):
+    d = fixture_model(d)
+    return S.LP.from_csr(d["m"], d["n"], d["row_ptr"], d["col_idx"], d["val"], d["c"], d["col_lo"], d["col_hi"],
+                         d["row_lo"], d["row_hi"], d["maximize"], d["obj_const"])
+
+
+def test_host_primitive_engine_is_bitwise_the_reference(golden):
+    for name, case in golden["lps"].items():
+        g = case["result"]
+        cap = 20000 if name.startswith("smoke_") else 1000000
+        r = S.solve(lp_of(case["model"]), backend="host", max_iterations=cap,
+                    restart_check_period=case.get("period", 40), trace_cap=100000)
+        assert r.status == g["status"], name
+        assert r.iterations == g["iterations"] and r.restarts == g["restarts"], name
+        assert r.operator_norm_estimate == g["operator_norm"]
+        np.testing.assert_array_equal(r.trace_res, g["trace"])
+        np.testing.assert_array_equal(r.x, g["x"])
+        np.testing.assert_array_equal(r.y, g["y"])
+
+
+def test_host_primitive_engine_on_random_lp(golden):
+    case = golden["random"]["c0_200x500_s1_tol0.0001"]
+    r = S.solve(S.LP.random(200, 500, 5, 1), backend="host", tolerance=1e-4, trace_cap=100000)
+    g = case["result"]
+    assert (r.status, r.iterations, r.restarts) == (g["status"], g["iterations"], g["restarts"])
+    assert r.objective == g["objective"]
+    np.testing.assert_array_equal(r.x, g["x"])
+
+
+def test_reader_matches_the_reference_parser(golden):
+    from golden.make_golden import TEST_PDHG_LPS
+    for name, text in TEST_PDHG_LPS.items():
+        a = S.LP.from_text(text).arrays()
+        g = fixture_model(golden["lps"][name]["model"])
+        for k in ("row_ptr", "col_idx", "val", "c", "col_lo", "col_hi", "row_lo", "row_hi"):
+            np.testing.assert_array_equal(a[k], g[k], err_msg=f"{name}:{k}")
+        assert a["maximize"] == g["maximize"]
+
+
+@pytest.mark.parametrize("fname", ["tiny_lp", "tiny_degenerate", "tiny_infeasible", "tiny_unbounded"])
+def test_mps_reader_matches_the_reference_parser(golden, fname):
+    path = os.path.join(ROOT, "tests", "golden", "smoke", fname + ".mps")
+    a = S.LP.read(path).arrays()
+    g = fixture_model(golden["lps"]["smoke_" + fname]["model"])
+    for k in ("row_ptr", "col_idx", "val", "c", "col_lo", "col_hi", "row_lo", "row_hi"):
+        np.testing.assert_array_equal(a[k], g[k], err_msg=f"{fname}:{k}")
+
+
+def test_generators_are_pinned(golden):
+    from golden.make_golden import digest
+    pins = golden["generators"]
+    for name, lp in (("random_200x500_k5_s0", S.LP.random(200, 500, 5, 0)),
+                     ("random_2000x5000_k5_s0", S.LP.random(2000, 5000, 5, 0)),
+                     ("zipf_5000x10000_s0", S.LP.random_zipf(5000, 10000, 0)),
+                     ("transport_20x30_s0", S.LP.transportation(20, 30, 0))):
+        a = lp.arrays()
+        p = pins[name]
+        assert (a["m"], a["n"], a["nnz"]) == (p["m"], p["n"], p["nnz"])
+        assert digest(a["row_ptr"], a["col_idx"], a["val"], a["c"], a["row_lo"]) == p["digest"], name
+    assert S.batch_sizes(256, 200, 20000, 0).tolist() == pins["batch_sizes_256_s0"]
+
+
+def test_generator_constructions():
+    # random_lp: feasible at x0 by construction (rows equal A x0), bounds [0, 10]
+    a = S.LP.random(50, 120, 5, 7).arrays()
+    assert np.all(np.diff(a["row_ptr"]) >= 0) and a["nnz"] == 600
+    np.testing.assert_array_equal(a["row_lo"], a["row_hi"])
+    assert np.all(a["col_lo"] == 0) and np.all(a["col_hi"] == 10)
+    # transportation: balanced supplies and demands, unit coefficients, integer costs
+    t = S.LP.transportation(7, 9, 3).arrays()
+    assert t["nnz"] == 2 * 63 and np.all(t["val"] == 1.0)
+    assert abs(t["row_lo"][:7].sum() - t["row_lo"][7:].sum()) < 1e-9
+    assert np.all(t["c"] == np.round(t["c"])) and t["c"].min() >= 1 and t["c"].max() <= 100
+    # zipf columns: counts clipped into [2, 200]
+    z = S.LP.random_zipf(3000, 4000, 1).arrays()
+    counts = np.bincount(z["col_idx"], minlength=4000)
+    assert counts.min() >= 2 and counts.max() <= 200
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sov_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("sov_hip.h", "libsov_hip.so"), ("sovereign_c.h", "libsovereign.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    path = os.path.join(ROOT, "sovereign-solver_amd", "lib", lib)
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+    names = _declared(header)
+    assert len(names) > 10
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_hip_library_loads_without_a_device():
+    L = S.hip()
+    assert L.sov_hip_last_error() is not None
+    # creating a context on a machine without a device is a clean error code
+    if S.hip_device_count() == 0:
+        import ctypes as C
+        ctx = C.c_void_p()
+        assert L.sov_hip_ctx_create(0, C.byref(ctx)) == 1
+
+
+def test_time_limit_before_first_check_reports_a_point():
+    # reference tests/test_pdhg.cpp:125-147
+    text = "Maximize\n obj: x + y\nSubject To\n c1: x + 2 y <= 4\n c2: 4 x + 2 y <= 12\nEnd\n"
+    r = S.solve(S.LP.from_text(text), backend="host", time_limit=1e-9)
+    assert r.status == "time_limit" and r.iterations == 0 and len(r.x) == 2
+
+
+def test_crossed_bounds_are_a_numerical_failure():
+    lp = S.LP.from_csr(1, 1, [0, 1], [0], [1.0], [1.0], [2.0], [1.0], [0.0], [5.0])
+    r = S.solve(lp, backend="host")
+    assert r.status == "numerical_failure"
+
+
+def test_backend_report_and_cli():
+    assert "host" in S.backend_report()
+    cli = os.path.join(ROOT, "sovereign-solver_amd", "bin", "sovereign-cli")
+    path = os.path.join(ROOT, "tests", "golden", "smoke", "tiny_lp.mps")
+    out = subprocess.run([cli, "solve", path, "--json", "--backend", "host"], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    assert '"status": "optimal"' in out.stdout
+    bad = subprocess.run([cli, "solve", path, "--engine", "simplex"], capture_output=True, text=True)
+    assert bad.returncode == 2
