@@ -230,6 +230,12 @@ int sov_hip_pdhg_restart(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, int64_t
                          const double* act_avg, double* x_last, double* y_last,
                          double movement[2]);
 
+/* Restart in place at the current point (adaptive restarts keep it when it
+ * beats the average): movement[0] = ||x - x_last||_2^2, movement[1] =
+ * ||y - y_last||_2^2 (host), then x_last = x, y_last = y (FP64 copies). */
+int sov_hip_pdhg_movement(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, double* x_last,
+                          double* y_last, double movement[2]);
+
 /* Set (x, s, y) from FP64 sources: x = clamp(x_src, col box), y = y_src,
  * s = clamp(act, row box); NULL sources leave the vector as is.  Used for
  * warm starts and for the FP32 -> FP64 switch. */

@@ -62,6 +62,15 @@ enum class PdhgEngine { Auto, Primitive, Fused };
 /// cones the column/row bounds allow.
 enum class PdhgTermination { Reference, RelativeKkt };
 
+/// Additive: the restart rule.  Reference = pdhg.cpp:391-408 (restart to the
+/// average when its scaled residual <= restart_sufficient_decrease x the
+/// residual at the last restart).  Adaptive = the published PDLP rule on the
+/// relative KKT K of the candidate (the better of average and current):
+/// restart when K <= beta_sufficient K_last, or K <= beta_necessary K_last
+/// and K grew since the previous check, or the run since the last restart is
+/// >= beta_artificial of all iterations; the restart moves to the candidate.
+enum class PdhgRestartScheme { Reference, Adaptive };
+
 struct PdhgOptions {
     Real tolerance = 1e-4;
     long long max_iterations = 1000000;
@@ -94,6 +103,10 @@ struct PdhgOptions {
     bool use_primal_weight = false;
     Real initial_primal_weight = 1.0;
     Real primal_weight_smoothing = 0.5;
+    PdhgRestartScheme restart_scheme = PdhgRestartScheme::Reference;
+    Real restart_beta_sufficient = 0.2;
+    Real restart_beta_necessary = 0.8;
+    Real restart_beta_artificial = 0.36;
     /// Certificate checks at every check period (fused engine).
     bool detect_infeasibility = false;
     /// Optional warm start: original-space row duals (length num_rows).

@@ -74,6 +74,11 @@ typedef struct sov_pdhg_opts {
     int64_t trace_cap;               /* checkpoint trace: residual and iteration of each check */
     double* trace_res;
     int64_t* trace_it;
+    int32_t restart_scheme;          /* 0 = reference rule, 1 = adaptive (PdhgRestartScheme) */
+    int32_t reserved;
+    double restart_beta_sufficient;
+    double restart_beta_necessary;
+    double restart_beta_artificial;
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

@@ -347,6 +347,26 @@ void orc_default_options(orc_options* o) {
     o->apply_ruiz = 1;
     o->apply_pc = 1;
     o->warm_y = NULL;
+    o->restart_scheme = 0;
+    o->beta_sufficient = 0.2;
+    o->beta_necessary = 0.8;
+    o->beta_artificial = 0.36;
+}
+
+/* Relative KKT (max of r_p, r_d, gap) of a scaled internal-sense point. */
+static double rel_worst_scaled(const orc_lp* p, double sign, const double* rs, const double* cs,
+                               const double* xs, const double* ys, double kk[5]) {
+    double* ux = (double*)malloc(sizeof(double) * ((size_t)p->n + 1));
+    double* uy = (double*)malloc(sizeof(double) * ((size_t)p->m + 1));
+    for (int32_t j = 0; j < p->n; ++j) ux[j] = xs[j] * cs[j];
+    for (int32_t i = 0; i < p->m; ++i) { uy[i] = ys[i] * sign; uy[i] *= rs[i]; }
+    orc_relative_kkt(p, ux, uy, kk);
+    free(ux);
+    free(uy);
+    double w = kk[0];
+    if (kk[1] > w) w = kk[1];
+    if (kk[2] > w) w = kk[2];
+    return w;
 }
 
 static void clamp_vec(int32_t n, const double* lo, const double* hi, double* x) {
@@ -461,6 +481,7 @@ void orc_pdhg_solve(const orc_lp* p, const orc_options* o, double* xo, double* y
     memcpy(ylast, y, sizeof(double) * (size_t)m);
     int64_t since = 0;
     double res_at_restart = INFINITY;
+    double prev_cand = INFINITY;
     const double* bx = NULL;
     const double* by = NULL;
     int converged = 0;
@@ -503,8 +524,15 @@ void orc_pdhg_solve(const orc_lp* p, const orc_options* o, double* xo, double* y
         for (int32_t i = 0; i < m; ++i) yavg[i] = ysum[i] * inv;
         const double avg = orc_kkt_residual(&S, sc, xavg, yavg, o->tolerance);
         if (!isfinite(cur) || !isfinite(avg)) { res->status = ORC_NUMERICAL_FAILURE; goto done; }
-        /* pdhg.cpp:354-371 */
-        const int avg_better = avg <= cur;
+        /* pdhg.cpp:354-371 (reference); adaptive: the candidate with the
+         * smaller relative KKT */
+        double kkc[5], kka[5];
+        double rel_c = 0.0, rel_a = 0.0;
+        if (o->restart_scheme == 1 || o->termination == 1) {
+            rel_c = rel_worst_scaled(p, obj_sign, rs, cs, x, y, kkc);
+            rel_a = rel_worst_scaled(p, obj_sign, rs, cs, xavg, yavg, kka);
+        }
+        const int avg_better = o->restart_scheme == 1 ? (rel_a <= rel_c) : (avg <= cur);
         const double* cx = avg_better ? xavg : x;
         const double* cy = avg_better ? yavg : y;
         const double ores = orc_original_space_residual(p, obj_sign, rs, cs, cx, cy, otol);
@@ -512,20 +540,8 @@ void orc_pdhg_solve(const orc_lp* p, const orc_options* o, double* xo, double* y
         if (trace != NULL && ntrace < trace_cap) trace[ntrace] = ores;
         ++ntrace;
         /* pdhg.cpp:383-389, or the relative-KKT extension */
-        int stop = 0;
-        if (o->termination == 0) {
-            stop = ores < o->tolerance;
-        } else {
-            double* ux = (double*)malloc(nb);
-            double* uy = (double*)malloc(mb);
-            double kk[5];
-            for (int32_t j = 0; j < n; ++j) ux[j] = cx[j] * cs[j];
-            for (int32_t i = 0; i < m; ++i) { uy[i] = cy[i] * obj_sign; uy[i] *= rs[i]; }
-            orc_relative_kkt(p, ux, uy, kk);
-            free(ux);
-            free(uy);
-            stop = kk[0] <= o->tolerance && kk[1] <= o->tolerance && kk[2] <= o->tolerance;
-        }
+        const double rel_cand = avg_better ? rel_a : rel_c;
+        const int stop = (o->termination == 0) ? (ores < o->tolerance) : (rel_cand <= o->tolerance);
         if (stop) {
             if (avg_better) {
                 memcpy(x, xavg, sizeof(double) * (size_t)n);
@@ -537,14 +553,27 @@ void orc_pdhg_solve(const orc_lp* p, const orc_options* o, double* xo, double* y
             ++res->iterations;
             break;
         }
-        /* pdhg.cpp:391-408 */
-        if (!isfinite(res_at_restart)) res_at_restart = cur;
-        if (avg <= o->restart_sufficient_decrease * res_at_restart) {
-            memcpy(x, xavg, sizeof(double) * (size_t)n);
-            memcpy(y, yavg, sizeof(double) * (size_t)m);
-            clamp_vec(n, sclo, schi, x);
-            orc_spmv(m, srp, sci, sv, 1.0, xavg, 0.0, s);
-            clamp_vec(m, srlo, srhi, s);
+        int do_restart = 0;
+        if (o->restart_scheme == 0) {
+            /* pdhg.cpp:391-393 */
+            if (!isfinite(res_at_restart)) res_at_restart = cur;
+            do_restart = avg <= o->restart_sufficient_decrease * res_at_restart;
+        } else {
+            if (!isfinite(res_at_restart)) res_at_restart = rel_c;
+            if (rel_cand <= o->beta_sufficient * res_at_restart) do_restart = 1;
+            else if (rel_cand <= o->beta_necessary * res_at_restart && rel_cand > prev_cand) do_restart = 1;
+            else if ((double)since >= o->beta_artificial * (double)(res->iterations + 1)) do_restart = 1;
+            prev_cand = rel_cand;
+        }
+        if (do_restart) {
+            if (o->restart_scheme == 0 || avg_better) {
+                /* pdhg.cpp:394-400: to the average, s from its activity */
+                memcpy(x, xavg, sizeof(double) * (size_t)n);
+                memcpy(y, yavg, sizeof(double) * (size_t)m);
+                clamp_vec(n, sclo, schi, x);
+                orc_spmv(m, srp, sci, sv, 1.0, xavg, 0.0, s);
+                clamp_vec(m, srlo, srhi, s);
+            }
             if (o->use_primal_weight) {
                 /* PDLP primal-weight update (no reference counterpart); at
                  * omega == 1 with use_primal_weight == 0 it never runs. */
@@ -561,7 +590,8 @@ void orc_pdhg_solve(const orc_lp* p, const orc_options* o, double* xo, double* y
                 memcpy(xlast, x, sizeof(double) * (size_t)n);
                 memcpy(ylast, y, sizeof(double) * (size_t)m);
             }
-            res_at_restart = avg;
+            res_at_restart = o->restart_scheme == 0 ? avg : rel_cand;
+            prev_cand = INFINITY;
             since = 0;
             ++res->restarts;
         }

@@ -23,6 +23,9 @@
  *                                       objectives and residual trace
  *   orc_relative_kkt                    hand-computed cases, 0 at an optimum
  *   orc_cs_violation / orc_farkas_*     hand cases of test_duals.cpp
+ *   adaptive restarts                   same decisions as the engine on the
+ *                                       GPU (trajectory parity), textbook
+ *                                       optima on the tiny LPs
  *   primal weight / relative-KKT stop   reduces to reference mode at omega=1
  *                                       (bitwise), textbook optima on the
  *                                       tiny LPs
@@ -132,6 +135,10 @@ typedef struct orc_options {
     int32_t ruiz_iterations;       /* scaling.hpp:42 */
     int32_t apply_ruiz, apply_pc;  /* scaling.hpp:51-52 */
     const double* warm_y;          /* optional ORIGINAL-space dual start (NULL: y=0) */
+    int32_t restart_scheme;        /* 0: reference rule (pdhg.cpp:391-408); 1: adaptive */
+    double beta_sufficient;        /* adaptive: restart when K(cand) <= b_s K(last restart) */
+    double beta_necessary;         /* ... or <= b_n K(last) and K(cand) grew since the last check */
+    double beta_artificial;        /* ... or since-restart >= b_a * total iterations */
 } orc_options;
 
 void orc_default_options(orc_options* o);

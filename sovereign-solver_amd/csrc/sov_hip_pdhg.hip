@@ -365,6 +365,20 @@ __global__ void k_set_clamped(int32_t n, const double* __restrict__ src, const T
     }
 }
 
+template <typename T>
+__global__ void k_movement(int32_t len, const T* __restrict__ v, double* __restrict__ last,
+                           double* __restrict__ partial) {
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+        const double t = double(v[i]);
+        const double d = t - last[i];
+        acc += d * d;
+        last[i] = t;
+    }
+    double sm[1] = {acc}, mx[1] = {0.0};
+    block_reduce_write<1, 0>(sm, mx, partial + blockIdx.x);
+}
+
 constexpr int kRestartBlocks = 512;
 
 static inline double nmax_host(double a, double b) { return (a > b || a != a) ? a : b; }
@@ -513,6 +527,26 @@ int restart_impl(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, int64_t since, 
 }
 
 template <typename T>
+int movement_impl(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, double* xl, double* yl, double movement[2]) {
+    int rc = ctx->ensure_red(2 * kRestartBlocks + 8);
+    if (rc != SOV_OK) return rc;
+    double* px = ctx->d_red;
+    double* py = px + kRestartBlocks;
+    double* res = py + kRestartBlocks;
+    const int bx = std::min(launch_blocks(st->n), kRestartBlocks);
+    const int by = std::min(launch_blocks(st->m), kRestartBlocks);
+    k_movement<T><<<bx, kBlock, 0, ctx->stream>>>(st->n, static_cast<const T*>(st->x), xl, px);
+    k_movement<T><<<by, kBlock, 0, ctx->stream>>>(st->m, static_cast<const T*>(st->y), yl, py);
+    k_sum_partials2<<<2, kBlock, 0, ctx->stream>>>(px, bx, py, by, res);
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    movement[0] = ctx->h_pinned[0];
+    movement[1] = ctx->h_pinned[1];
+    return SOV_OK;
+}
+
+template <typename T>
 int set_point_impl(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double* xs, const double* ys,
                    const double* act) {
     if (xs != nullptr && st->n > 0)
@@ -574,6 +608,18 @@ int sov_hip_pdhg_restart(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, int64_t
     DeviceGuard3 g(ctx->device);
     return st->prec == SOV_PREC_F64 ? restart_impl<double>(ctx, st, since, act, xl, yl, movement)
                                     : restart_impl<float>(ctx, st, since, act, xl, yl, movement);
+}
+
+int sov_hip_pdhg_movement(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, double* xl, double* yl,
+                          double movement[2]) {
+    if (ctx == nullptr || movement == nullptr) return fail(SOV_INVALID_ARGUMENT, "movement args");
+    int rc = state_ok(nullptr, st);
+    if (rc != SOV_OK) return rc;
+    if ((st->n > 0 && xl == nullptr) || (st->m > 0 && yl == nullptr)) return fail(SOV_INVALID_ARGUMENT, "movement: null last point");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "movement during capture");
+    DeviceGuard3 g(ctx->device);
+    return st->prec == SOV_PREC_F64 ? movement_impl<double>(ctx, st, xl, yl, movement)
+                                    : movement_impl<float>(ctx, st, xl, yl, movement);
 }
 
 int sov_hip_pdhg_set_point(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double* xs, const double* ys,

@@ -151,6 +151,10 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->primal_weight_smoothing = d.primal_weight_smoothing;
     o->detect_infeasibility = 0;
     o->use_graphs = d.use_graphs ? 1 : 0;
+    o->restart_scheme = 0;
+    o->restart_beta_sufficient = d.restart_beta_sufficient;
+    o->restart_beta_necessary = d.restart_beta_necessary;
+    o->restart_beta_artificial = d.restart_beta_artificial;
 }
 
 int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, double* x, double* y, double* rc,
@@ -177,6 +181,10 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
         p.primal_weight_smoothing = o->primal_weight_smoothing;
         p.detect_infeasibility = o->detect_infeasibility != 0;
         p.use_graphs = o->use_graphs != 0;
+        p.restart_scheme = o->restart_scheme == 1 ? sov::PdhgRestartScheme::Adaptive : sov::PdhgRestartScheme::Reference;
+        p.restart_beta_sufficient = o->restart_beta_sufficient;
+        p.restart_beta_necessary = o->restart_beta_necessary;
+        p.restart_beta_artificial = o->restart_beta_artificial;
         if (o->warm_start_dual != nullptr)
             p.warm_start_dual.assign(o->warm_start_dual, o->warm_start_dual + lp->p.num_rows());
         int64_t nt = 0;

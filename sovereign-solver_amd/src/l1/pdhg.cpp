@@ -315,7 +315,7 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
         DeviceBuffer x_sum(be, nu), y_sum(be, mu);
         std::vector<Real> x_last = x.to_host(), y_last = y.to_host();
         long long since = 0;
-        Real res_at_restart = kInfinity;
+        Real res_at_restart = kInfinity, prev_cand = kInfinity;
         const std::size_t nb = nu * sizeof(Real), mb = mu * sizeof(Real);
         std::vector<Real> best_x, best_y;
         bool converged = false;
@@ -370,7 +370,15 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
                 result.message = "non-finite KKT residual at iteration " + std::to_string(result.iterations);
                 return result;
             }
-            const bool avg_better = avg <= cur;
+            const bool adaptive = o.restart_scheme == PdhgRestartScheme::Adaptive;
+            Real rel_c = 0.0, rel_a = 0.0;
+            if (adaptive || o.termination == PdhgTermination::RelativeKkt) {
+                auto [uxc, uyc] = unscale_point(sign, scaling, xh, yh);
+                auto [uxa, uya] = unscale_point(sign, scaling, xa, ya);
+                rel_c = relative_kkt(problem, uxc, uyc).worst();
+                rel_a = relative_kkt(problem, uxa, uya).worst();
+            }
+            const bool avg_better = adaptive ? rel_a <= rel_c : avg <= cur;
             const std::vector<Real>& cx = avg_better ? xa : xh;
             const std::vector<Real>& cy = avg_better ? ya : yh;
             auto [ux, uy] = unscale_point(sign, scaling, cx, cy);
@@ -381,8 +389,8 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
                 return result;
             }
             if (o.checkpoint) o.checkpoint(result.iterations, ux, uy, ores);
-            bool stop = ores < o.tolerance;
-            if (o.termination == PdhgTermination::RelativeKkt) stop = relative_kkt(problem, ux, uy).worst() <= o.tolerance;
+            const Real rel_cand = avg_better ? rel_a : rel_c;
+            const bool stop = o.termination == PdhgTermination::RelativeKkt ? rel_cand <= o.tolerance : ores < o.tolerance;
             if (stop) {
                 best_x = cx;
                 best_y = cy;
@@ -390,32 +398,45 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
                 ++result.iterations;
                 break;
             }
-            if (!std::isfinite(res_at_restart)) res_at_restart = cur;
-            if (avg <= o.restart_sufficient_decrease * res_at_restart) {
-                x = DeviceBuffer::upload(be, xa);
-                y = DeviceBuffer::upload(be, ya);
-                ok(be.project_box(n, col_lo.data(), col_hi.data(), x.data()), "project_box");
-                std::vector<Real> act(mu, 0.0);
-                scaled.matrix().multiply(xa, act);
-                s = DeviceBuffer::upload(be, act);
-                ok(be.project_box(m, row_lo.data(), row_hi.data(), s.data()), "project_box");
+            bool do_restart = false;
+            if (!adaptive) {
+                if (!std::isfinite(res_at_restart)) res_at_restart = cur;
+                do_restart = avg <= o.restart_sufficient_decrease * res_at_restart;
+            } else {
+                if (!std::isfinite(res_at_restart)) res_at_restart = rel_c;
+                do_restart = rel_cand <= o.restart_beta_sufficient * res_at_restart ||
+                             (rel_cand <= o.restart_beta_necessary * res_at_restart && rel_cand > prev_cand) ||
+                             static_cast<Real>(since) >= o.restart_beta_artificial * static_cast<Real>(result.iterations + 1);
+                prev_cand = rel_cand;
+            }
+            if (do_restart) {
+                if (!adaptive || avg_better) {
+                    x = DeviceBuffer::upload(be, xa);
+                    y = DeviceBuffer::upload(be, ya);
+                    ok(be.project_box(n, col_lo.data(), col_hi.data(), x.data()), "project_box");
+                    std::vector<Real> act(mu, 0.0);
+                    scaled.matrix().multiply(xa, act);
+                    s = DeviceBuffer::upload(be, act);
+                    ok(be.project_box(m, row_lo.data(), row_hi.data(), s.data()), "project_box");
+                }
                 if (o.use_primal_weight) {
-                    const std::vector<Real> xn = x.to_host();
+                    const std::vector<Real> xn = x.to_host(), yn = y.to_host();
                     Real dx2 = 0.0, dy2 = 0.0;
                     for (std::size_t j = 0; j < nu; ++j) dx2 += (xn[j] - x_last[j]) * (xn[j] - x_last[j]);
-                    for (std::size_t i = 0; i < mu; ++i) dy2 += (ya[i] - y_last[i]) * (ya[i] - y_last[i]);
+                    for (std::size_t i = 0; i < mu; ++i) dy2 += (yn[i] - y_last[i]) * (yn[i] - y_last[i]);
                     omega = next_primal_weight(omega, dx2, dy2, o.primal_weight_smoothing);
                     tau = eta / omega;
                     sigma = eta * omega;
                     x_last = xn;
-                    y_last = ya;
+                    y_last = yn;
                 }
-                res_at_restart = avg;
+                res_at_restart = adaptive ? rel_cand : avg;
+                prev_cand = kInfinity;
                 since = 0;
                 ++result.restarts;
                 if (o.verbose)
                     std::fprintf(stderr, "  [pdhg] restart at iter %lld, residual %.3g\n", result.iterations + 1,
-                                 static_cast<double>(avg));
+                                 static_cast<double>(adaptive ? rel_cand : avg));
             }
         }
         result.loop_seconds = seconds_since(t_loop);
@@ -743,7 +764,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         };
 
         long long since = 0;
-        Real res_at_restart = kInfinity;
+        Real res_at_restart = kInfinity, prev_cand = kInfinity;
         bool converged = false;
         std::vector<Real> best_x, best_y;   // scaled, internal sense, FP64
         const auto t_loop = Clock::now();
@@ -778,10 +799,12 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 result.seconds = seconds_since(t0);
                 return result;
             }
-            const bool avg_better = avg <= cur;
+            const bool adaptive = o.restart_scheme == PdhgRestartScheme::Adaptive;
+            const Real rel_c = std::max({co.rel_cur[0], co.rel_cur[1], co.rel_cur[2]});
+            const Real rel_a = std::max({co.rel_avg[0], co.rel_avg[1], co.rel_avg[2]});
+            const bool avg_better = adaptive ? rel_a <= rel_c : avg <= cur;
             const Real ores = avg_better ? co.orig_avg : co.orig_cur;
-            const double* rel = avg_better ? co.rel_avg : co.rel_cur;
-            const Real rel_worst = std::max({rel[0], rel[1], rel[2]});
+            const Real rel_worst = avg_better ? rel_a : rel_c;
             if (!std::isfinite(ores)) {
                 result.status = PdhgStatus::NumericalFailure;
                 result.message = "non-finite KKT residual (original units) at iteration " +
@@ -831,6 +854,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                     hip_ok(sov_hip_sync(ctx), "sync");
                     since = 0;
                     res_at_restart = kInfinity;
+                    prev_cand = kInfinity;
                     polish_start = result.iterations;
                     if (o.verbose)
                         std::fprintf(stderr, "  [pdhg] fp64 polish from iter %lld (rel kkt %.3g)\n", result.iterations,
@@ -838,20 +862,35 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                     continue;
                 }
             }
-            if (!std::isfinite(res_at_restart)) res_at_restart = cur;
-            if (avg <= o.restart_sufficient_decrease * res_at_restart) {
+            bool do_restart = false;
+            if (!adaptive) {
+                if (!std::isfinite(res_at_restart)) res_at_restart = cur;
+                do_restart = avg <= o.restart_sufficient_decrease * res_at_restart;
+            } else {
+                if (!std::isfinite(res_at_restart)) res_at_restart = rel_c;
+                do_restart = rel_worst <= o.restart_beta_sufficient * res_at_restart ||
+                             (rel_worst <= o.restart_beta_necessary * res_at_restart && rel_worst > prev_cand) ||
+                             static_cast<Real>(since) >= o.restart_beta_artificial * static_cast<Real>(result.iterations);
+                prev_cand = rel_worst;
+            }
+            if (do_restart) {
                 double mv[2] = {0.0, 0.0};
-                hip_ok(sov_hip_pdhg_restart(ctx, &S.st, since, d_act.as<double>(),
-                                            o.use_primal_weight ? d_xlast.as<double>() : nullptr,
-                                            o.use_primal_weight ? d_ylast.as<double>() : nullptr, mv),
-                       "restart");
+                double* xl = o.use_primal_weight ? d_xlast.as<double>() : nullptr;
+                double* yl = o.use_primal_weight ? d_ylast.as<double>() : nullptr;
+                if (!adaptive || avg_better) {
+                    hip_ok(sov_hip_pdhg_restart(ctx, &S.st, since, d_act.as<double>(), xl, yl, mv), "restart");
+                } else if (o.use_primal_weight) {
+                    hip_ok(sov_hip_pdhg_movement(ctx, &S.st, xl, yl, mv), "movement");
+                }
                 if (o.use_primal_weight) omega = next_primal_weight(omega, mv[0], mv[1], o.primal_weight_smoothing);
-                res_at_restart = avg;
+                res_at_restart = adaptive ? rel_worst : avg;
+                prev_cand = kInfinity;
                 since = 0;
                 ++result.restarts;
                 if (o.verbose)
                     std::fprintf(stderr, "  [pdhg] restart at iter %lld, residual %.3g, omega %.3g\n",
-                                 result.iterations, static_cast<double>(avg), static_cast<double>(omega));
+                                 result.iterations, static_cast<double>(adaptive ? rel_worst : avg),
+                                 static_cast<double>(omega));
             }
         }
         hip_ok(sov_hip_sync(ctx), "sync");

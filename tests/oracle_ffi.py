@@ -30,7 +30,8 @@ class OrcOptions(C.Structure):
                 ("power_iterations", C.c_int32), ("termination", C.c_int32), ("use_primal_weight", C.c_int32),
                 ("primal_weight", C.c_double), ("primal_weight_smoothing", C.c_double),
                 ("ruiz_iterations", C.c_int32), ("apply_ruiz", C.c_int32), ("apply_pc", C.c_int32),
-                ("warm_y", _f64p)]
+                ("warm_y", _f64p), ("restart_scheme", C.c_int32), ("beta_sufficient", C.c_double),
+                ("beta_necessary", C.c_double), ("beta_artificial", C.c_double)]
 
 
 class OrcResult(C.Structure):
@@ -205,11 +206,12 @@ def farkas_certifies(M: Model, y, slack=1e-6):
 
 
 def pdhg(M: Model, tolerance=1e-4, max_iterations=1000000, period=40, termination=0, primal_weight=False,
-         omega=1.0, warm_y=None, trace_cap=0):
+         omega=1.0, warm_y=None, trace_cap=0, restart_scheme=0):
     o = OrcOptions()
     oracle().orc_default_options(C.byref(o))
     o.tolerance, o.max_iterations, o.restart_check_period = tolerance, max_iterations, period
     o.termination, o.use_primal_weight, o.primal_weight = termination, int(primal_weight), omega
+    o.restart_scheme = restart_scheme
     keep = None
     if warm_y is not None:
         keep = np.ascontiguousarray(warm_y, np.float64)

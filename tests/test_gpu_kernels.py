@@ -81,6 +81,7 @@ def test_spmv_hand_answers(ctx):
     np.testing.assert_array_equal(A.spmv([1, 2, 3, 4], transpose=True), [5, 6, 27])
     out = A.spmv([1, 2, 3], beta=0.0, y=[np.nan] * 4)
     assert not np.isnan(out).any() and out[1] == 18
+    check(ctx.L.sov_hip_csr_make_f32(ctx.h, A.h))
     np.testing.assert_array_equal(A.spmv([1, 2, 3], prec=1), np.array([-1, 18, 1, 15], np.float32))
     A.destroy()
 
