@@ -117,16 +117,34 @@ class DeviceProblem:
         self.D.check(self.ctx.L.sov_hip_pdhg_dual_step(self.ctx.h, self.A.h, C.byref(self.st), self.tau,
                                                        self.sigma, first))
 
+    def _interval_graph(self, fresh):
+        """The PERIOD-iteration interval captured once as a HIP graph (one for
+        an interval that starts a new Cesaro average, one that continues it),
+        replayed like the engine replays it."""
+        if not hasattr(self, "graphs"):
+            self.graphs = {}
+        if fresh not in self.graphs:
+            L = self.ctx.L
+            self.D.check(L.sov_hip_graph_begin(self.ctx.h))
+            for k in range(PERIOD):
+                first = 1 if (fresh and k == 0) else 0
+                self.primal(first)
+                self.dual(first)
+            g = C.c_void_p()
+            self.D.check(L.sov_hip_graph_end(self.ctx.h, C.byref(g)))
+            self.graphs[fresh] = g
+        return self.graphs[fresh]
+
     def iterate(self, count):
-        """count iterations with the engine's check/restart every PERIOD."""
-        for _ in range(count):
-            first = 1 if self.since == 0 else 0
-            self.primal(first)
-            self.dual(first)
-            self.since += 1
-            self.iterations += 1
-            if self.iterations % PERIOD == 0:
-                self.check_and_restart()
+        """count iterations (a multiple of PERIOD) with the engine's
+        check/restart after every PERIOD."""
+        assert count % PERIOD == 0
+        for _ in range(count // PERIOD):
+            g = self._interval_graph(self.since == 0)
+            self.D.check(self.ctx.L.sov_hip_graph_launch(self.ctx.h, g))
+            self.since += PERIOD
+            self.iterations += PERIOD
+            self.check_and_restart()
 
     def check_and_restart(self):
         D, L = self.D, self.ctx.L
@@ -176,7 +194,7 @@ def cpu_baseline(a, budget_iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--m", type=int, default=500_000)
@@ -185,6 +203,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    args.steps = max(PERIOD, (args.steps + PERIOD - 1) // PERIOD * PERIOD)
+    args.warmup = (args.warmup + PERIOD - 1) // PERIOD * PERIOD
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is described "
               "in DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -205,7 +225,10 @@ def main():
     lanes_a, lanes_t = (C.c_int32 * 8)(), (C.c_int32 * 8)()
     ctx.L.sov_hip_csr_get_lanes(P.A.h, 0, lanes_a)
     ctx.L.sov_hip_csr_get_lanes(P.A.h, 1, lanes_t)
-    log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}")
+    seg_a, seg_t = C.c_int(), C.c_int()
+    ctx.L.sov_hip_csr_get_segments(P.A.h, 0, C.byref(seg_a))
+    ctx.L.sov_hip_csr_get_segments(P.A.h, 1, C.byref(seg_t))
+    log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}; column segments A={seg_a.value} A^T={seg_t.value}")
 
     P.iterate(args.warmup)
     D.check(ctx.L.sov_hip_sync(ctx.h))
@@ -268,6 +291,7 @@ def main():
                                f"(clip(Zipf(1.5),2,200) nonzeros per column, seed {args.seed}); "
                                f"{'FP32' if prec == 1 else 'FP64'} iterates, FP64 checks every {PERIOD}",
                    "iterations_per_s": round(its, 2), "restarts": P.restarts,
+                   "column_segments": {"A": seg_a.value, "A^T": seg_t.value},
                    "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
                    "kkt_scaled_at_last_check": chk.kkt_cur if chk else None},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -128,8 +128,18 @@ int sov_hip_csr_transpose_arrays(sov_hip_ctx* ctx, const sov_hip_csr* csr, int32
  * [0, 8), both orientations (orient 0 = A, 1 = A^T).  0 keeps the heuristic. */
 int sov_hip_csr_set_lanes(sov_hip_csr* csr, int orient, const int32_t lanes[8]);
 int sov_hip_csr_get_lanes(const sov_hip_csr* csr, int orient, int32_t lanes[8]);
-/* Autotuner: for each orientation and length class, time every G in
- * {1,2,4,8,16,32,64} on the SpMV of precision prec and keep the fastest. */
+/* Column segments: split an orientation into `count` column ranges of equal
+ * width, stored segment-major, so each pass gathers from a slice of the
+ * vector that stays resident in an XCD's L2.  The step kernels and SpMV run
+ * the segments in order and carry per-row partial sums between them (a
+ * fixed summation order: results are reproducible).  count = 1 removes the
+ * split.  The checks and the scaling kernels always use the whole CSR. */
+int sov_hip_csr_set_segments(sov_hip_ctx* ctx, sov_hip_csr* csr, int orient, int count);
+int sov_hip_csr_get_segments(const sov_hip_csr* csr, int orient, int* count);
+/* Autotuner: for each orientation, try 1, 2, 4 and 8 column segments (slices
+ * of at least 64k elements); for each, time every G in {1,2,4,8,16,32,64} per
+ * length class of every segment on the SpMV of precision prec; keep the
+ * fastest segment count with its lanes. */
 int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* csr, int prec);
 
 /* ---- sparse level 2 ---------------------------------------------------

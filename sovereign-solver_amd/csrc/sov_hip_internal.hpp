@@ -38,7 +38,9 @@ constexpr int kDefaultLanes[kClasses] = {1, 2, 4, 4, 8, 16, 32, 64};
 // Schedule passed by value to the row kernels.  Rows of class c are
 // list[offset[c] .. offset[c] + count[c]); they are processed by G[c] lanes
 // per row, kBlock / G[c] rows per workgroup, in workgroups
-// [block_begin[c], block_begin[c+1]).
+// [block_begin[c], block_begin[c+1]).  Kernels read the per-class values with
+// pick() (unrolled selects on kernel-argument scalars), never by a runtime
+// array index, so nothing spills to scratch.
 struct Sched {
     int32_t G[kClasses];
     int32_t count[kClasses];
@@ -47,21 +49,57 @@ struct Sched {
     const int32_t* list;
 };
 
-struct Orient {
-    int32_t rows = 0, cols = 0;
+// One CSR "part": the whole matrix of an orientation, or one column segment
+// of it (entries whose column lies in [k W, (k+1) W)).
+struct Part {
+    int32_t rows = 0, cols = 0, nnz = 0;
     const int32_t* ptr = nullptr;   // rows + 1
-    const int32_t* idx = nullptr;   // nnz
-    double* val64 = nullptr;        // nnz (owned or borrowed, see owns_val64)
-    float* val32 = nullptr;         // nnz, owned, optional
-    int32_t* list = nullptr;        // rows, owned: rows grouped by class
+    const int32_t* idx = nullptr;   // nnz, global column indices, ascending per row
+    double* val64 = nullptr;        // nnz
+    float* val32 = nullptr;         // nnz, optional
+    const int32_t* src = nullptr;   // nnz: position of each entry in A's value array; null = A itself
+    int32_t* list = nullptr;        // rows grouped by length class (owned)
     int32_t class_count[kClasses] = {};
     int32_t class_offset[kClasses] = {};
     int32_t lanes[kClasses] = {};
     int64_t max_len = 0;
-    bool owns_val64 = false;
     Sched sched() const;
     int blocks() const;
 };
+
+// An orientation (A or A^T): the whole CSR for checks/scaling, and the
+// column-segmented copy the step kernels and SpMV stream through when the
+// gathered vector is larger than a comfortable share of an XCD's L2.
+struct Orient {
+    Part whole;
+    std::vector<Part> seg;          // empty: unsegmented (steps use `whole`)
+    int32_t seg_width = 0;
+    // owned storage of the segmented copy
+    int32_t* seg_ptr = nullptr;     // nseg * (rows + 1)
+    int32_t* seg_idx = nullptr;     // nnz
+    int32_t* seg_src = nullptr;     // nnz
+    double* seg_val64 = nullptr;    // nnz
+    float* seg_val32 = nullptr;     // nnz
+    double* acc64 = nullptr;        // rows: running partial sums across segments
+    float* acc32 = nullptr;         // rows
+    int nparts() const { return seg.empty() ? 1 : static_cast<int>(seg.size()); }
+    const Part& part(int k) const { return seg.empty() ? whole : seg[static_cast<size_t>(k)]; }
+    Part& part(int k) { return seg.empty() ? whole : seg[static_cast<size_t>(k)]; }
+    int max_blocks() const;
+    void free_segments();
+};
+
+// How a part's launch combines with the others of its orientation.
+enum SegMode : int {
+    kOnly = 0,    // unsegmented: sum is the full row product
+    kFirst = 1,   // acc = sum
+    kMiddle = 2,  // acc += sum
+    kLast = 3,    // sum += acc, then the epilogue
+};
+
+__host__ __device__ constexpr int seg_mode(int k, int nparts) {
+    return nparts == 1 ? kOnly : (k == 0 ? kFirst : (k == nparts - 1 ? kLast : kMiddle));
+}
 
 }  // namespace sovhip
 
@@ -88,7 +126,8 @@ struct sov_hip_csr {
     int32_t* t_ptr = nullptr;   // owned A^T structure
     int32_t* t_idx = nullptr;
     int32_t* t_src = nullptr;   // owned: A position of each A^T entry
-    double* partials = nullptr; // owned: kCheckSlots * max(blocks) FP64
+    bool owns_a_val64 = false;
+    double* partials = nullptr; // owned: kCheckSlots * (blocks of both whole parts + 64) FP64
     int64_t partial_blocks = 0;
     int device = 0;
 };
@@ -170,6 +209,40 @@ __device__ __forceinline__ int block_class(const Sched& s) {
     return c;
 }
 
+// a[c] for a runtime c without a dynamic index (keeps the kernel-argument
+// arrays in scalar registers).
+template <int N>
+__device__ __forceinline__ int pick(const int32_t (&a)[N], int c) {
+    int v = a[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) v = (c == k) ? a[k] : v;
+    return v;
+}
+
+// Where a workgroup's rows come from: resolved once per block.
+struct ClassView {
+    int cls, G, begin, count, offset;
+};
+
+__device__ __forceinline__ ClassView class_view(const Sched& s) {
+    ClassView v;
+    v.cls = block_class(s);
+    v.G = pick(s.G, v.cls);
+    v.begin = pick(s.block_begin, v.cls);
+    v.count = pick(s.count, v.cls);
+    v.offset = pick(s.offset, v.cls);
+    return v;
+}
+
+// The row of this thread's group of G lanes (-1 past the class) and its lane.
+template <int G>
+__device__ __forceinline__ int row_of(const Sched& s, const ClassView& v, int& lane) {
+    constexpr int RPB = kBlock / G;
+    const int local = ((int)blockIdx.x - v.begin) * RPB + (int)threadIdx.x / G;
+    lane = (int)threadIdx.x % G;
+    return (local < v.count) ? s.list[v.offset + local] : -1;
+}
+
 // Dot product of one CSR row with a gathered vector, G lanes per row.  Every
 // lane of the group returns the full sum.  Accumulation type Acc.
 template <int G, typename Acc, typename V, typename X>
@@ -180,6 +253,7 @@ __device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
     Acc acc = Acc(0);
     if (row >= 0) {
         const int b = ptr[row], e = ptr[row + 1];
+#pragma unroll 4
         for (int k = b + lane; k < e; k += G) acc += Acc(val[k]) * Acc(x[idx[k]]);
     }
     return group_sum<G>(acc);
@@ -208,22 +282,22 @@ __device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
 
 // Dispatch a functor templated on G with the block's lane count.
 template <class F>
-__device__ __forceinline__ void dispatch_G(int G, F& f) {
-    switch (G) {
-        case 1: f.template run<1>(); break;
-        case 2: f.template run<2>(); break;
-        case 4: f.template run<4>(); break;
-        case 8: f.template run<8>(); break;
-        case 16: f.template run<16>(); break;
-        case 32: f.template run<32>(); break;
-        default: f.template run<64>(); break;
+__device__ __forceinline__ void dispatch_G(const ClassView& v, F& f) {
+    switch (v.G) {
+        case 1: f.template run<1>(v); break;
+        case 2: f.template run<2>(v); break;
+        case 4: f.template run<4>(v); break;
+        case 8: f.template run<8>(v); break;
+        case 16: f.template run<16>(v); break;
+        case 32: f.template run<32>(v); break;
+        default: f.template run<64>(v); break;
     }
 }
 
 // Host helpers
 int launch_blocks(int64_t n);      // blocks for an elementwise kernel over n
 hipError_t check_launch();         // hipGetLastError wrapper
-int build_schedule(sov_hip_ctx* ctx, Orient& o);   // class lists from ptr (host)
+int build_schedule(sov_hip_ctx* ctx, Part& p);    // class lists from ptr (host)
 int ensure_partials(sov_hip_csr* csr);
 
 }  // namespace sovhip

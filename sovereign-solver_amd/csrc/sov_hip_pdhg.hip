@@ -42,18 +42,20 @@ struct PrimalOp {
     T* x;
     T* xbar;
     T* xsum;
+    T* acc;
     T tau;
     int first;
+    int mode;
     Sched s;
-    int cls;
     template <int G>
-    __device__ void run() {
-        constexpr int RPB = kBlock / G;
-        const int local = (blockIdx.x - s.block_begin[cls]) * RPB + threadIdx.x / G;
-        const int lane = threadIdx.x % G;
-        const int row = (local < s.count[cls]) ? s.list[s.offset[cls] + local] : -1;
-        const T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
+    __device__ void run(const ClassView& v) {
+        int lane;
+        const int row = row_of<G>(s, v, lane);
+        T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
         if (row >= 0 && lane == 0) {
+            if (mode == kFirst) { acc[row] = aty; return; }
+            if (mode == kMiddle) { acc[row] = acc[row] + aty; return; }
+            if (mode == kLast) aty = acc[row] + aty;
             const T g = c[row] - aty;
             const T xo = x[row];
             T xn = xo + (-tau) * g;
@@ -80,18 +82,20 @@ struct DualOp {
     T* s_;
     T* y;
     T* ysum;
+    T* acc;
     T tau, sigma;
     int first;
+    int mode;
     Sched s;
-    int cls;
     template <int G>
-    __device__ void run() {
-        constexpr int RPB = kBlock / G;
-        const int local = (blockIdx.x - s.block_begin[cls]) * RPB + threadIdx.x / G;
-        const int lane = threadIdx.x % G;
-        const int row = (local < s.count[cls]) ? s.list[s.offset[cls] + local] : -1;
-        const T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
+    __device__ void run(const ClassView& v) {
+        int lane;
+        const int row = row_of<G>(s, v, lane);
+        T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
         if (row >= 0 && lane == 0) {
+            if (mode == kFirst) { acc[row] = ax; return; }
+            if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
+            if (mode == kLast) ax = acc[row] + ax;
             const T so = s_[row], yo = y[row];
             T sn = so + (-tau) * yo;
             const T l = lo[row], h = hi[row];
@@ -108,8 +112,8 @@ struct DualOp {
 
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_step(Op op) {
-    op.cls = block_class(op.s);
-    dispatch_G(op.s.G[op.cls], op);
+    const ClassView v = class_view(op.s);
+    dispatch_G(v, op);
 }
 
 // ---------------------------------------------------------------------------
@@ -165,16 +169,11 @@ struct CheckRowOp {
     double* partial;
     double inv, tol_s, tol_o;
     Sched s;
-    int cls;
     // sums: pv2_c, pv2_a, bn2, yres2_c, yres2_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
-    double sm[7];
-    double mx[4];
     template <int G>
-    __device__ void run() {
-        constexpr int RPB = kBlock / G;
-        const int local = (blockIdx.x - s.block_begin[cls]) * RPB + threadIdx.x / G;
-        const int lane = threadIdx.x % G;
-        const int row = (local < s.count[cls]) ? s.list[s.offset[cls] + local] : -1;
+    __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
+        int lane;
+        const int row = row_of<G>(s, v, lane);
         double ac, aa;
         row_dot2<G>(ptr, idx, val, x, xsum, inv, row, lane, ac, aa);
         if (row >= 0 && lane == 0) {
@@ -225,16 +224,11 @@ struct CheckColOp {
     double* partial;
     double inv, tol_s, tol_o, sign;
     Sched s;
-    int cls;
     // sums: dres2_c, dres2_a, cn2, pobj_c, pobj_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
-    double sm[7];
-    double mx[4];
     template <int G>
-    __device__ void run() {
-        constexpr int RPB = kBlock / G;
-        const int local = (blockIdx.x - s.block_begin[cls]) * RPB + threadIdx.x / G;
-        const int lane = threadIdx.x % G;
-        const int row = (local < s.count[cls]) ? s.list[s.offset[cls] + local] : -1;
+    __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
+        int lane;
+        const int row = row_of<G>(s, v, lane);
         double tc, ta;
         row_dot2<G>(ptr, idx, val, y, ysum, inv, row, lane, tc, ta);
         if (row >= 0 && lane == 0) {
@@ -262,14 +256,27 @@ struct CheckColOp {
 };
 
 template <class Op>
+struct CheckRunner {
+    const Op& op;
+    double (&sm)[7];
+    double (&mx)[4];
+    template <int G>
+    __device__ void run(const ClassView& v) {
+        op.template run<G>(v, sm, mx);
+    }
+};
+
+template <class Op>
 __global__ void __launch_bounds__(kBlock) k_check(Op op) {
-    op.cls = block_class(op.s);
+    double sm[7], mx[4];
 #pragma unroll
-    for (int i = 0; i < 7; ++i) op.sm[i] = 0.0;
+    for (int i = 0; i < 7; ++i) sm[i] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) op.mx[i] = 0.0;
-    dispatch_G(op.s.G[op.cls], op);
-    block_reduce_write<7, 4>(op.sm, op.mx, op.partial + (int64_t)blockIdx.x * kCheckSlots);
+    for (int i = 0; i < 4; ++i) mx[i] = 0.0;
+    const ClassView v = class_view(op.s);
+    CheckRunner<Op> r{op, sm, mx};
+    dispatch_G(v, r);
+    block_reduce_write<7, 4>(sm, mx, op.partial + (int64_t)blockIdx.x * kCheckSlots);
 }
 
 // Block b reduces slot (b % 11) of region (b / 11): slots 0..6 sums, 7..10 maxima.
@@ -387,12 +394,14 @@ template <typename T>
 static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a, sov_hip_check_out* out) {
     const sov_hip_pdhg_state* st = a->st;
     const double inv = 1.0 / static_cast<double>(a->since_restart > 0 ? a->since_restart : 1);
-    const Sched sa = h->a.sched(), stt = h->t.sched();
+    const Part& PA = h->a.whole;
+    const Part& PT = h->t.whole;
+    const Sched sa = PA.sched(), stt = PT.sched();
     const int rb = sa.block_begin[kClasses], cb = stt.block_begin[kClasses];
-    const T* val_a = std::is_same<T, double>::value ? reinterpret_cast<const T*>(h->a.val64)
-                                                    : reinterpret_cast<const T*>(h->a.val32);
-    const T* val_t = std::is_same<T, double>::value ? reinterpret_cast<const T*>(h->t.val64)
-                                                    : reinterpret_cast<const T*>(h->t.val32);
+    const T* val_a = std::is_same<T, double>::value ? reinterpret_cast<const T*>(PA.val64)
+                                                    : reinterpret_cast<const T*>(PA.val32);
+    const T* val_t = std::is_same<T, double>::value ? reinterpret_cast<const T*>(PT.val64)
+                                                    : reinterpret_cast<const T*>(PT.val32);
     if (val_a == nullptr || val_t == nullptr) return fail(SOV_INVALID_ARGUMENT, "check: value image missing");
     double* rowpart = h->partials;
     double* colpart = h->partials + (int64_t)(rb + 1) * kCheckSlots;
@@ -401,7 +410,7 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
     if (rc != SOV_OK) return rc;
     if (rb > 0) {
         CheckRowOp<T> op{};
-        op.ptr = h->a.ptr; op.idx = h->a.idx; op.val = val_a;
+        op.ptr = PA.ptr; op.idx = PA.idx; op.val = val_a;
         op.x = static_cast<const T*>(st->x); op.xsum = static_cast<const T*>(st->x_sum);
         op.y = static_cast<const T*>(st->y); op.ysum = static_cast<const T*>(st->y_sum);
         op.lo_s = static_cast<const T*>(st->row_lo); op.hi_s = static_cast<const T*>(st->row_hi);
@@ -412,7 +421,7 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
     }
     if (cb > 0) {
         CheckColOp<T> op{};
-        op.ptr = h->t.ptr; op.idx = h->t.idx; op.val = val_t;
+        op.ptr = PT.ptr; op.idx = PT.idx; op.val = val_t;
         op.y = static_cast<const T*>(st->y); op.ysum = static_cast<const T*>(st->y_sum);
         op.x = static_cast<const T*>(st->x); op.xsum = static_cast<const T*>(st->x_sum);
         op.c_s = static_cast<const T*>(st->c);
@@ -456,17 +465,32 @@ using namespace sovhip;
 namespace {
 
 template <typename T>
+const T* part_vals(const Part& p) {
+    return std::is_same<T, double>::value ? reinterpret_cast<const T*>(p.val64) : reinterpret_cast<const T*>(p.val32);
+}
+
+template <typename T>
+T* orient_acc(const Orient& o) {
+    return std::is_same<T, double>::value ? reinterpret_cast<T*>(o.acc64) : reinterpret_cast<T*>(o.acc32);
+}
+
+template <typename T>
 int primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau, int first) {
-    const Sched s = h->t.sched();
-    const int blocks = s.block_begin[kClasses];
-    if (blocks == 0) return SOV_OK;
-    const T* val = std::is_same<T, double>::value ? reinterpret_cast<const T*>(h->t.val64)
-                                                  : reinterpret_cast<const T*>(h->t.val32);
-    if (val == nullptr && h->nnz > 0) return fail(SOV_INVALID_ARGUMENT, "primal_step: value image missing");
-    PrimalOp<T> op{h->t.ptr, h->t.idx, val, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
-                   static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi), static_cast<T*>(st->x),
-                   static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum), T(tau), first, s, 0};
-    k_step<PrimalOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    const Orient& o = h->t;
+    const int np = o.nparts();
+    for (int k = 0; k < np; ++k) {
+        const Part& p = o.part(k);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
+        const T* val = part_vals<T>(p);
+        if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "primal_step: value image missing");
+        PrimalOp<T> op{p.ptr, p.idx, val, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
+                       static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi), static_cast<T*>(st->x),
+                       static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum), orient_acc<T>(o), T(tau), first,
+                       seg_mode(k, np), s};
+        k_step<PrimalOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    }
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
 }
@@ -474,16 +498,20 @@ int primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state
 template <typename T>
 int dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau, double sigma,
               int first) {
-    const Sched s = h->a.sched();
-    const int blocks = s.block_begin[kClasses];
-    if (blocks == 0) return SOV_OK;
-    const T* val = std::is_same<T, double>::value ? reinterpret_cast<const T*>(h->a.val64)
-                                                  : reinterpret_cast<const T*>(h->a.val32);
-    if (val == nullptr && h->nnz > 0) return fail(SOV_INVALID_ARGUMENT, "dual_step: value image missing");
-    DualOp<T> op{h->a.ptr, h->a.idx, val, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
-                 static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
-                 static_cast<T*>(st->y_sum), T(tau), T(sigma), first, s, 0};
-    k_step<DualOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    const Orient& o = h->a;
+    const int np = o.nparts();
+    for (int k = 0; k < np; ++k) {
+        const Part& p = o.part(k);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
+        const T* val = part_vals<T>(p);
+        if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "dual_step: value image missing");
+        DualOp<T> op{p.ptr, p.idx, val, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
+                     static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
+                     static_cast<T*>(st->y_sum), orient_acc<T>(o), T(tau), T(sigma), first, seg_mode(k, np), s};
+        k_step<DualOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    }
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
 }

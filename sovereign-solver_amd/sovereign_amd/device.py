@@ -42,6 +42,8 @@ def lib():
     L.sov_hip_csr_set_lanes.argtypes = [_v, C.c_int, C.POINTER(C.c_int32)]
     L.sov_hip_csr_get_lanes.argtypes = [_v, C.c_int, C.POINTER(C.c_int32)]
     L.sov_hip_csr_tune.argtypes = [_v, _v, C.c_int]
+    L.sov_hip_csr_set_segments.argtypes = [_v, _v, C.c_int, C.c_int]
+    L.sov_hip_csr_get_segments.argtypes = [_v, C.c_int, C.POINTER(C.c_int)]
     L.sov_hip_spmv.argtypes = [_v, _v, C.c_int, C.c_int, d, _v, d, _v]
     L.sov_hip_scale_ruiz_sweep.argtypes = [_v, _v, _v, _v, C.POINTER(d)]
     L.sov_hip_scale_pock_chambolle.argtypes = [_v, _v, d, _v, _v]

@@ -194,3 +194,29 @@ def test_device_scaling_matches_oracle(ctx, seed):
     assert np.max(np.abs(gcs - ocs) / ocs) < 4e-16
     print("bitwise equal factors:", np.mean(grs == ors), np.mean(gcs == ocs))
     A.destroy()
+
+
+@pytest.mark.parametrize("K", [2, 3, 8])
+def test_column_segments_keep_results(ctx, K):
+    rng = np.random.default_rng(21 + K)
+    m, n = 4000, 9000
+    rp, ci, v = random_csr(rng, m, n, zipf=True)
+    M = O.Model(dict(m=m, n=n, row_ptr=rp, col_idx=ci, val=v, c=np.zeros(n), col_lo=np.zeros(n),
+                     col_hi=np.zeros(n), row_lo=np.zeros(m), row_hi=np.zeros(m)))
+    A = Csr(ctx, m, n, rp, ci, v)
+    check(ctx.L.sov_hip_csr_set_segments(ctx.h, A.h, 0, K))
+    check(ctx.L.sov_hip_csr_set_segments(ctx.h, A.h, 1, K))
+    cnt = C.c_int()
+    check(ctx.L.sov_hip_csr_get_segments(A.h, 0, C.byref(cnt)))
+    assert cnt.value == K
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+    Ad = np.abs(M.dense())
+    for got, want, scale in ((A.spmv(x, alpha=2.0, beta=-1.0, y=np.ones(m)), 2 * O.spmv(M, x) - 1,
+                              2 * Ad @ np.abs(x) + 1),
+                             (A.spmv(y, transpose=True), O.spmv_t(M, y), Ad.T @ np.abs(y) + 1e-300)):
+        assert np.max(np.abs(got - want) / scale) < 1e-14
+    check(ctx.L.sov_hip_csr_make_f32(ctx.h, A.h))
+    got32 = A.spmv(x, prec=1).astype(np.float64)
+    kmax = int(np.max(np.diff(rp)))
+    assert np.max(np.abs(got32 - O.spmv(M, x)) / (Ad @ np.abs(x) + 1e-30)) < (kmax + 2) * 1.2e-7
+    A.destroy()
