@@ -113,6 +113,11 @@ struct PdhgOptions {
     std::vector<Real> warm_start_dual;
     /// Capture each check interval as a HIP graph and replay it.
     bool use_graphs = true;
+    /// Time SpMV lane counts per row-length class and column-segment counts
+    /// before the loop and keep the fastest (fused engine).  The choice
+    /// changes summation order, so runs with it on are not bit-reproducible
+    /// across processes; off = the fixed default schedule.
+    bool autotune = true;
     /// Device ordinal for a fused solve that creates its own backend context.
     int device = 0;
 };

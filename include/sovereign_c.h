@@ -75,7 +75,7 @@ typedef struct sov_pdhg_opts {
     double* trace_res;
     int64_t* trace_it;
     int32_t restart_scheme;          /* 0 = reference rule, 1 = adaptive (PdhgRestartScheme) */
-    int32_t reserved;
+    int32_t autotune;                /* time lane counts and column segments before the loop */
     double restart_beta_sufficient;
     double restart_beta_necessary;
     double restart_beta_artificial;

@@ -46,27 +46,29 @@ Sched Part::sched() const {
 int Part::blocks() const { return sched().block_begin[kClasses]; }
 
 int Orient::max_blocks() const {
-    int b = whole.blocks();
-    for (const Part& p : seg) b = std::max(b, p.blocks());
+    int b = 0;
+    for (const Part& p : parts) b = std::max(b, p.blocks());
     return b;
 }
 
-void Orient::free_segments() {
-    for (Part& p : seg) (void)hipFree(p.list);
-    seg.clear();
+void Orient::free_parts() {
+    parts.clear();
     seg_width = 0;
-    (void)hipFree(seg_ptr);
-    (void)hipFree(seg_idx);
-    (void)hipFree(seg_src);
-    (void)hipFree(seg_val64);
-    (void)hipFree(seg_val32);
+    (void)hipFree(p_list);
+    (void)hipFree(p_ptr);
+    (void)hipFree(p_idx);
+    (void)hipFree(p_src);
+    (void)hipFree(p_val64);
+    (void)hipFree(p_val32);
     (void)hipFree(acc64);
     (void)hipFree(acc32);
-    seg_ptr = seg_idx = seg_src = nullptr;
-    seg_val64 = nullptr;
-    seg_val32 = nullptr;
+    (void)hipFree(chk_acc);
+    p_list = p_ptr = p_idx = p_src = nullptr;
+    p_val64 = nullptr;
+    p_val32 = nullptr;
     acc64 = nullptr;
     acc32 = nullptr;
+    chk_acc = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -211,49 +213,11 @@ __global__ void k_copy_f64(int64_t n, const double* __restrict__ s, double* __re
         d[i] = s[i];
 }
 
-__global__ void k_to_f32(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
+template <typename T>
+__global__ void k_fill_value(int64_t n, T v, T* __restrict__ d) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        d[i] = (float)s[i];
-}
-
-// Host: group rows by length class (stable), upload the list.
-int build_schedule(sov_hip_ctx* ctx, Part& o) {
-    std::vector<int32_t> ptr(static_cast<size_t>(o.rows) + 1);
-    SOV_HIP_TRY(hipMemcpyAsync(ptr.data(), o.ptr, sizeof(int32_t) * ptr.size(), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    std::vector<int32_t> cls(static_cast<size_t>(o.rows));
-    int32_t cnt[kClasses] = {};
-    o.max_len = 0;
-    for (int32_t r = 0; r < o.rows; ++r) {
-        const int32_t len = ptr[r + 1] - ptr[r];
-        o.max_len = std::max<int64_t>(o.max_len, len);
-        int c = 0;
-        while (c < kClasses - 1 && len > kClassMax[c]) ++c;
-        cls[r] = c;
-        ++cnt[c];
-    }
-    int32_t off = 0;
-    for (int c = 0; c < kClasses; ++c) {
-        o.class_count[c] = cnt[c];
-        o.class_offset[c] = off;
-        off += cnt[c];
-        if (o.lanes[c] == 0) o.lanes[c] = kDefaultLanes[c];
-    }
-    std::vector<int32_t> list(static_cast<size_t>(o.rows));
-    int32_t cur[kClasses];
-    for (int c = 0; c < kClasses; ++c) cur[c] = o.class_offset[c];
-    for (int32_t r = 0; r < o.rows; ++r) list[cur[cls[r]]++] = r;
-    (void)hipFree(o.list);
-    o.list = nullptr;
-    if (o.rows > 0) {
-        SOV_HIP_TRY(hipMalloc(&o.list, sizeof(int32_t) * o.rows));
-        SOV_HIP_TRY(hipMemcpyAsync(o.list, list.data(), sizeof(int32_t) * o.rows, hipMemcpyHostToDevice,
-                                   ctx->stream));
-        SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    }
-    return SOV_OK;
+        d[i] = v;
 }
 
 int ensure_partials(sov_hip_csr* csr) {
@@ -268,7 +232,7 @@ int ensure_partials(sov_hip_csr* csr) {
 }
 
 // ---------------------------------------------------------------------------
-// column segments
+// class-ordered parts
 // ---------------------------------------------------------------------------
 
 // bounds[k * rows + r] = first position of row r whose column is >= k * W,
@@ -292,29 +256,20 @@ __global__ void k_seg_bounds(int32_t rows, int32_t K, int32_t W, const int32_t* 
     }
 }
 
-// Row lengths of segment k into sptr[k * (rows+1) + r + 1] (sptr[.. + 0] = 0).
-__global__ void k_seg_counts(int32_t rows, int32_t K, const int32_t* __restrict__ bounds, int32_t* __restrict__ sptr) {
-    const int64_t total = (int64_t)K * rows;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t k = t / rows, r = t % rows;
-        sptr[k * (rows + 1) + r + 1] = bounds[(k + 1) * rows + r] - bounds[k * rows + r];
-        if (r == 0) sptr[k * (rows + 1)] = 0;
-    }
-}
-
-// Copy the entries of every (row, segment) run into segment-major order.
-__global__ void k_seg_fill(int32_t rows, int32_t K, const int32_t* __restrict__ bounds,
-                           const int32_t* __restrict__ sptr, const int32_t* __restrict__ base,
-                           const int32_t* __restrict__ idx, const int32_t* __restrict__ src_in,
-                           int32_t* __restrict__ sidx, int32_t* __restrict__ ssrc) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
-        for (int32_t k = 0; k < K; ++k) {
-            const int32_t b = bounds[(int64_t)k * rows + r], e = bounds[(int64_t)(k + 1) * rows + r];
-            int64_t d = (int64_t)base[k] + sptr[(int64_t)k * (rows + 1) + r];
-            for (int32_t p = b; p < e; ++p, ++d) {
-                sidx[d] = idx[p];
-                ssrc[d] = src_in ? src_in[p] : p;
-            }
+// Copy the entries of each position of one part: row list[i] of the natural
+// CSR, entries [lo_b[row], hi_b[row]), to [pptr[i], pptr[i+1]) of the part.
+__global__ void k_part_fill(int32_t positions, const int32_t* __restrict__ list, const int32_t* __restrict__ pptr,
+                            const int32_t* __restrict__ lo_b, const int32_t* __restrict__ hi_b,
+                            const int32_t* __restrict__ nidx, const int32_t* __restrict__ nsrc,
+                            int32_t* __restrict__ pidx, int32_t* __restrict__ psrc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < positions;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = list[i];
+        const int32_t b = lo_b[r], e = hi_b[r];
+        int32_t d = pptr[i];
+        for (int32_t q = b; q < e; ++q, ++d) {
+            pidx[d] = nidx[q];
+            psrc[d] = nsrc ? nsrc[q] : q;
         }
     }
 }
@@ -325,102 +280,148 @@ __global__ void k_gather_f64_to_f32(int32_t nnz, const int32_t* __restrict__ src
         out[p] = (float)v[src[p]];
 }
 
-// Build (or rebuild) the K-segment copy of an orientation.  K <= 1 removes it.
-static int build_segments(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32) {
-    o.free_segments();
-    const Part& w = o.whole;
-    if (K <= 1 || w.rows == 0 || w.nnz == 0) return SOV_OK;
-    const int32_t rows = w.rows, W = (w.cols + K - 1) / K;
-    if (W <= 0) return SOV_OK;
+static int row_class(int32_t len) {
+    int c = 0;
+    while (c < kClasses - 1 && len > kClassMax[c]) ++c;
+    return c;
+}
+
+// (Re)build the K class-ordered parts of an orientation from its natural CSR
+// and A's current FP64 values.  Every row appears in the first and the last
+// part (they initialise the running sums and run the epilogue); a middle part
+// holds only the rows with entries in its column segment.
+int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32) {
+    o.free_parts();
+    const Natural& nat = o.nat;
+    const int32_t rows = nat.rows, cols = nat.cols, nnz = nat.nnz;
+    if (K < 1 || cols < 2 * K || nnz == 0) K = 1;
+    const int32_t W = (K == 1) ? std::max<int32_t>(cols, 1) : (cols + K - 1) / K;
+    const int64_t r1 = std::max<int32_t>(rows, 1);
+    std::vector<int32_t> hb(static_cast<size_t>(K + 1) * rows);
     int32_t* bounds = nullptr;
-    int32_t* base_d = nullptr;
     auto fail_free = [&](hipError_t e, const char* what) {
         (void)hipFree(bounds);
-        (void)hipFree(base_d);
-        o.free_segments();
+        o.free_parts();
         return hip_fail(e, what);
     };
-    hipError_t e = hipMalloc(&bounds, sizeof(int32_t) * (int64_t)(K + 1) * rows);
-    if (e == hipSuccess) e = hipMalloc(&o.seg_ptr, sizeof(int32_t) * (int64_t)K * (rows + 1));
-    if (e == hipSuccess) e = hipMalloc(&o.seg_idx, sizeof(int32_t) * w.nnz);
-    if (e == hipSuccess) e = hipMalloc(&o.seg_src, sizeof(int32_t) * w.nnz);
-    if (e == hipSuccess) e = hipMalloc(&o.seg_val64, sizeof(double) * w.nnz);
-    if (e == hipSuccess) e = hipMalloc(&o.acc64, sizeof(double) * rows);
-    if (e == hipSuccess) e = hipMalloc(&o.acc32, sizeof(float) * rows);
-    if (e == hipSuccess) e = hipMalloc(&base_d, sizeof(int32_t) * (K + 1));
-    if (e != hipSuccess) return fail_free(e, "segment buffers");
-    k_seg_bounds<<<launch_blocks(rows), kBlock, 0, ctx->stream>>>(rows, K, W, w.ptr, w.idx, bounds);
-    k_seg_counts<<<launch_blocks((int64_t)K * rows), kBlock, 0, ctx->stream>>>(rows, K, bounds, o.seg_ptr);
-    e = check_launch();
-    if (e != hipSuccess) return fail_free(e, "segment counts");
-    std::vector<int32_t> base(static_cast<size_t>(K) + 1, 0);
-    for (int k = 0; k < K; ++k) {
-        const int rc = inclusive_scan(ctx, o.seg_ptr + (int64_t)k * (rows + 1), rows + 1);
-        if (rc != SOV_OK) {
-            (void)hipFree(bounds);
-            (void)hipFree(base_d);
-            o.free_segments();
-            return rc;
-        }
-        int32_t nk = 0;
-        e = hipMemcpy(&nk, o.seg_ptr + (int64_t)k * (rows + 1) + rows, sizeof(int32_t), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return fail_free(e, "segment sizes");
-        base[static_cast<size_t>(k) + 1] = base[static_cast<size_t>(k)] + nk;
-    }
-    e = hipMemcpyAsync(base_d, base.data(), sizeof(int32_t) * (K + 1), hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return fail_free(e, "segment bases");
-    k_seg_fill<<<launch_blocks(rows), kBlock, 0, ctx->stream>>>(rows, K, bounds, o.seg_ptr, base_d, w.idx, w.src,
-                                                                 o.seg_idx, o.seg_src);
-    k_gather_vals<<<launch_blocks(w.nnz), kBlock, 0, ctx->stream>>>(w.nnz, o.seg_src, h->a.whole.val64, o.seg_val64);
-    e = check_launch();
-    if (e == hipSuccess && want_f32) {
-        e = hipMalloc(&o.seg_val32, sizeof(float) * w.nnz);
+    hipError_t e = hipMalloc(&bounds, sizeof(int32_t) * (int64_t)(K + 1) * r1);
+    if (e != hipSuccess) return fail_free(e, "part bounds");
+    if (rows > 0) {
+        k_seg_bounds<<<launch_blocks(rows), kBlock, 0, ctx->stream>>>(rows, K, W, nat.ptr, nat.idx, bounds);
+        e = check_launch();
         if (e == hipSuccess)
-            k_gather_f64_to_f32<<<launch_blocks(w.nnz), kBlock, 0, ctx->stream>>>(w.nnz, o.seg_src, h->a.whole.val64,
-                                                                                o.seg_val32);
+            e = hipMemcpyAsync(hb.data(), bounds, sizeof(int32_t) * hb.size(), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return fail_free(e, "part bounds");
     }
+    // host: class order of each part, position pointers
+    std::vector<int32_t> all_list, all_ptr;
+    all_list.reserve(static_cast<size_t>(K) * rows);
+    all_ptr.reserve(static_cast<size_t>(K) * (rows + 1));
+    std::vector<int64_t> list_off(K), ptr_off(K), nnz_off(K + 1, 0);
+    o.parts.assign(static_cast<size_t>(K), Part{});
+    std::vector<int8_t> cls(static_cast<size_t>(rows));
+    for (int k = 0; k < K; ++k) {
+        Part& p = o.parts[static_cast<size_t>(k)];
+        const int32_t* lo_b = hb.data() + static_cast<size_t>(k) * rows;
+        const int32_t* hi_b = lo_b + rows;
+        const bool middle = seg_mode(k, K) == kMiddle;
+        int32_t cnt[kClasses] = {};
+        p.max_len = 0;
+        for (int32_t r = 0; r < rows; ++r) {
+            const int32_t len = hi_b[r] - lo_b[r];
+            if (middle && len == 0) {
+                cls[r] = -1;
+                continue;
+            }
+            p.max_len = std::max<int64_t>(p.max_len, len);
+            cls[r] = static_cast<int8_t>(row_class(len));
+            ++cnt[cls[r]];
+        }
+        int32_t off = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            p.class_count[c] = cnt[c];
+            p.class_offset[c] = off;
+            p.lanes[c] = kDefaultLanes[c];
+            off += cnt[c];
+        }
+        p.rows = off;
+        p.cols = cols;
+        list_off[k] = static_cast<int64_t>(all_list.size());
+        ptr_off[k] = static_cast<int64_t>(all_ptr.size());
+        all_list.resize(all_list.size() + off);
+        int32_t* lst = all_list.data() + list_off[k];
+        int32_t cur[kClasses];
+        for (int c = 0; c < kClasses; ++c) cur[c] = p.class_offset[c];
+        for (int32_t r = 0; r < rows; ++r)
+            if (cls[r] >= 0) lst[cur[cls[r]]++] = r;
+        all_ptr.push_back(0);
+        int32_t run = 0;
+        for (int32_t i = 0; i < off; ++i) {
+            run += hi_b[lst[i]] - lo_b[lst[i]];
+            all_ptr.push_back(run);
+        }
+        p.nnz = run;
+        nnz_off[k + 1] = nnz_off[k] + run;
+    }
+    if (nnz_off[K] != nnz) return fail_free(hipErrorUnknown, "part fill: entry count mismatch");
+    const int64_t nz = std::max<int32_t>(nnz, 1);
+    e = hipMalloc(&o.p_list, sizeof(int32_t) * std::max<size_t>(all_list.size(), 1));
+    if (e == hipSuccess) e = hipMalloc(&o.p_ptr, sizeof(int32_t) * all_ptr.size());
+    if (e == hipSuccess) e = hipMalloc(&o.p_idx, sizeof(int32_t) * nz);
+    if (e == hipSuccess) e = hipMalloc(&o.p_src, sizeof(int32_t) * nz);
+    if (e == hipSuccess) e = hipMalloc(&o.p_val64, sizeof(double) * nz);
+    if (e == hipSuccess && want_f32) e = hipMalloc(&o.p_val32, sizeof(float) * nz);
+    if (e == hipSuccess) e = hipMalloc(&o.acc64, sizeof(double) * r1);
+    if (e == hipSuccess) e = hipMalloc(&o.acc32, sizeof(float) * r1);
+    if (e == hipSuccess) e = hipMalloc(&o.chk_acc, sizeof(double) * 2 * r1);
+    if (e != hipSuccess) return fail_free(e, "part buffers");
+    if (!all_list.empty())
+        e = hipMemcpyAsync(o.p_list, all_list.data(), sizeof(int32_t) * all_list.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(o.p_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e != hipSuccess) return fail_free(e, "part upload");
+    for (int k = 0; k < K; ++k) {
+        Part& p = o.parts[static_cast<size_t>(k)];
+        p.list = o.p_list + list_off[k];
+        p.ptr = o.p_ptr + ptr_off[k];
+        p.idx = o.p_idx + nnz_off[k];
+        p.src = o.p_src + nnz_off[k];
+        p.val64 = o.p_val64 + nnz_off[k];
+        p.val32 = o.p_val32 ? o.p_val32 + nnz_off[k] : nullptr;
+        if (p.rows > 0 && p.nnz > 0)
+            k_part_fill<<<launch_blocks(p.rows), kBlock, 0, ctx->stream>>>(
+                p.rows, p.list, p.ptr, bounds + (int64_t)k * rows, bounds + (int64_t)(k + 1) * rows, nat.idx,
+                nat.src, o.p_idx + nnz_off[k], o.p_src + nnz_off[k]);
+    }
+    if (nnz > 0) {
+        k_gather_vals<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, o.p_src, h->a.nat.val64, o.p_val64);
+        if (o.p_val32)
+            k_gather_f64_to_f32<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, o.p_src, h->a.nat.val64,
+                                                                                o.p_val32);
+    }
+    e = check_launch();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(bounds);
     bounds = nullptr;
-    (void)hipFree(base_d);
-    base_d = nullptr;
-    if (e != hipSuccess) return fail_free(e, "segment fill");
-    o.seg_width = W;
-    o.seg.resize(static_cast<size_t>(K));
-    for (int k = 0; k < K; ++k) {
-        Part& p = o.seg[static_cast<size_t>(k)];
-        p.rows = rows;
-        p.cols = w.cols;
-        p.nnz = base[static_cast<size_t>(k) + 1] - base[static_cast<size_t>(k)];
-        p.ptr = o.seg_ptr + (int64_t)k * (rows + 1);
-        p.idx = o.seg_idx + base[static_cast<size_t>(k)];
-        p.src = o.seg_src + base[static_cast<size_t>(k)];
-        p.val64 = o.seg_val64 + base[static_cast<size_t>(k)];
-        p.val32 = o.seg_val32 ? o.seg_val32 + base[static_cast<size_t>(k)] : nullptr;
-        const int rc = build_schedule(ctx, p);
-        if (rc != SOV_OK) {
-            o.free_segments();
-            return rc;
-        }
-    }
+    if (e != hipSuccess) return fail_free(e, "part fill");
+    o.seg_width = K > 1 ? W : 0;
     return SOV_OK;
 }
 
 // Re-derive every value image of both orientations from A's FP64 values.
 static int refresh_values(sov_hip_ctx* ctx, sov_hip_csr* h) {
     if (h->nnz == 0) return SOV_OK;
-    const double* av = h->a.whole.val64;
-    k_gather_vals<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, h->t_src, av, h->t.whole.val64);
+    const double* av = h->a.nat.val64;
+    k_gather_vals<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, h->t_src, av, h->t_val64);
     for (Orient* o : {&h->a, &h->t}) {
-        if (o->whole.val32 != nullptr && o != &h->a)
-            k_gather_f64_to_f32<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, h->t_src, av, o->whole.val32);
-        if (o->seg_val64 != nullptr)
-            k_gather_vals<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, o->seg_src, av, o->seg_val64);
-        if (o->seg_val32 != nullptr)
-            k_gather_f64_to_f32<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, o->seg_src, av, o->seg_val32);
+        if (o->p_val64 != nullptr)
+            k_gather_vals<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, o->p_src, av, o->p_val64);
+        if (o->p_val32 != nullptr)
+            k_gather_f64_to_f32<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, o->p_src, av, o->p_val32);
     }
-    if (h->a.whole.val32 != nullptr)
-        k_to_f32<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, av, h->a.whole.val32);
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
 }
@@ -443,9 +444,10 @@ struct SpmvOp {
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        const int row = row_of<G>(s, v, lane);
-        T sum = row_dot<G, T>(ptr, idx, val, x, row, lane);
-        if (row >= 0 && lane == 0) {
+        const int pos = pos_of<G>(v, lane);
+        T sum = row_dot<G, T>(ptr, idx, val, x, pos, lane);
+        if (pos >= 0 && lane == 0) {
+            const int row = s.list[pos];
             if (mode == kFirst) { acc[row] = sum; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + sum; return; }
             if (mode == kLast) sum = acc[row] + sum;
@@ -699,7 +701,7 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
         return rc;
     };
     const int64_t nz = std::max<int32_t>(nnz, 1);
-    Part& A = h->a.whole;
+    Natural& A = h->a.nat;
     A.rows = m;
     A.cols = n;
     A.nnz = nnz;
@@ -713,13 +715,12 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
     } else {
         A.val64 = const_cast<double*>(d_val);
     }
-    // A^T structure, built on the device
-    Part& T = h->t.whole;
+    // natural A^T, built on the device
     int32_t *cursor = nullptr, *trow_u = nullptr, *tsrc_u = nullptr;
     hipError_t e = hipMalloc(&h->t_ptr, sizeof(int32_t) * ((int64_t)n + 1));
     if (e == hipSuccess) e = hipMalloc(&h->t_idx, sizeof(int32_t) * nz);
     if (e == hipSuccess) e = hipMalloc(&h->t_src, sizeof(int32_t) * nz);
-    if (e == hipSuccess) e = hipMalloc(&T.val64, sizeof(double) * nz);
+    if (e == hipSuccess) e = hipMalloc(&h->t_val64, sizeof(double) * nz);
     if (e == hipSuccess) e = hipMalloc(&cursor, sizeof(int32_t) * ((int64_t)n + 1));
     if (e == hipSuccess) e = hipMalloc(&trow_u, sizeof(int32_t) * nz);
     if (e == hipSuccess) e = hipMalloc(&tsrc_u, sizeof(int32_t) * nz);
@@ -735,7 +736,7 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
         k_fill_transpose<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(m, nnz, d_rp, d_ci, cursor, trow_u, tsrc_u);
         k_rank_transpose<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(n, nnz, h->t_ptr, trow_u, tsrc_u,
                                                                           h->t_idx, h->t_src);
-        k_gather_vals<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, h->t_src, A.val64, T.val64);
+        k_gather_vals<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, h->t_src, A.val64, h->t_val64);
         e = check_launch();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(e, "csr transpose");
@@ -744,14 +745,16 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
     (void)hipFree(trow_u);
     (void)hipFree(tsrc_u);
     if (rc != SOV_OK) return cleanup(rc);
+    Natural& T = h->t.nat;
     T.rows = n;
     T.cols = m;
     T.nnz = nnz;
     T.ptr = h->t_ptr;
     T.idx = h->t_idx;
+    T.val64 = h->t_val64;
     T.src = h->t_src;
-    rc = build_schedule(ctx, A);
-    if (rc == SOV_OK) rc = build_schedule(ctx, T);
+    rc = build_parts(ctx, h, h->a, 1, false);
+    if (rc == SOV_OK) rc = build_parts(ctx, h, h->t, 1, false);
     if (rc == SOV_OK) rc = ensure_partials(h);
     if (rc != SOV_OK) return cleanup(rc);
     *out = h;
@@ -762,17 +765,13 @@ void sov_hip_csr_destroy(sov_hip_csr* h) {
     if (h == nullptr) return;
     DeviceGuard2 g(h->device);
     (void)hipDeviceSynchronize();
-    h->a.free_segments();
-    h->t.free_segments();
-    if (h->owns_a_val64) (void)hipFree(h->a.whole.val64);
-    (void)hipFree(h->a.whole.val32);
-    (void)hipFree(h->a.whole.list);
-    (void)hipFree(h->t.whole.val64);
-    (void)hipFree(h->t.whole.val32);
-    (void)hipFree(h->t.whole.list);
+    h->a.free_parts();
+    h->t.free_parts();
+    if (h->owns_a_val64) (void)hipFree(h->a.nat.val64);
     (void)hipFree(h->t_ptr);
     (void)hipFree(h->t_idx);
     (void)hipFree(h->t_src);
+    (void)hipFree(h->t_val64);
     (void)hipFree(h->partials);
     delete h;
 }
@@ -790,18 +789,16 @@ int sov_hip_csr_make_f32(sov_hip_ctx* ctx, sov_hip_csr* h) {
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "make_f32 during capture");
     DeviceGuard2 g(ctx->device);
     const int64_t nz = std::max<int32_t>(h->nnz, 1);
-    if (h->a.whole.val32 == nullptr) SOV_HIP_TRY(hipMalloc(&h->a.whole.val32, sizeof(float) * nz));
-    if (h->t.whole.val32 == nullptr) SOV_HIP_TRY(hipMalloc(&h->t.whole.val32, sizeof(float) * nz));
     for (Orient* o : {&h->a, &h->t}) {
-        if (o->seg_val64 != nullptr && o->seg_val32 == nullptr) {
-            SOV_HIP_TRY(hipMalloc(&o->seg_val32, sizeof(float) * nz));
-            int64_t off = 0;
-            for (Part& p : o->seg) {
-                p.val32 = o->seg_val32 + off;
-                off += p.nnz;
-            }
+        if (o->p_val32 != nullptr) continue;
+        SOV_HIP_TRY(hipMalloc(&o->p_val32, sizeof(float) * nz));
+        int64_t off = 0;
+        for (Part& p : o->parts) {
+            p.val32 = o->p_val32 + off;
+            off += p.nnz;
         }
     }
+    h->f32 = true;
     return refresh_values(ctx, h);
 }
 
@@ -811,8 +808,8 @@ int sov_hip_csr_scale(sov_hip_ctx* ctx, sov_hip_csr* h, const double* rs, const 
     if (!h->owns_a_val64) return fail(SOV_INVALID_ARGUMENT, "csr_scale needs SOV_CSR_OWN_VALUES");
     DeviceGuard2 g(ctx->device);
     if (h->nnz == 0) return SOV_OK;
-    k_scale_vals<<<launch_blocks(h->m), kBlock, 0, ctx->stream>>>(h->m, h->a.whole.ptr, h->a.whole.idx,
-                                                                   h->a.whole.val64, rs, cs);
+    k_scale_vals<<<launch_blocks(h->m), kBlock, 0, ctx->stream>>>(h->m, h->a.nat.ptr, h->a.nat.idx,
+                                                                   h->a.nat.val64, rs, cs);
     SOV_HIP_TRY(check_launch());
     return refresh_values(ctx, h);
 }
@@ -826,7 +823,7 @@ int sov_hip_csr_transpose_arrays(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t
     if (d_tidx && h->nnz > 0)
         SOV_HIP_TRY(hipMemcpyAsync(d_tidx, h->t_idx, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToDevice, ctx->stream));
     if (d_tval && h->nnz > 0)
-        SOV_HIP_TRY(hipMemcpyAsync(d_tval, h->t.whole.val64, sizeof(double) * h->nnz, hipMemcpyDeviceToDevice,
+        SOV_HIP_TRY(hipMemcpyAsync(d_tval, h->t_val64, sizeof(double) * h->nnz, hipMemcpyDeviceToDevice,
                                    ctx->stream));
     return SOV_OK;
 }
@@ -840,10 +837,8 @@ int sov_hip_csr_set_lanes(sov_hip_csr* h, int orient, const int32_t lanes[8]) {
         if (G != 0 && (G < 1 || G > 64 || (G & (G - 1)) != 0))
             return fail(SOV_INVALID_ARGUMENT, "lanes must be a power of two <= 64");
     }
-    for (int k = -1; k < (int)o.seg.size(); ++k) {
-        Part& p = k < 0 ? o.whole : o.seg[static_cast<size_t>(k)];
+    for (Part& p : o.parts)
         for (int c = 0; c < kClasses; ++c) p.lanes[c] = lanes[c] == 0 ? kDefaultLanes[c] : lanes[c];
-    }
     return ensure_partials(h);
 }
 
@@ -861,7 +856,7 @@ int sov_hip_csr_set_segments(sov_hip_ctx* ctx, sov_hip_csr* h, int orient, int c
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "set_segments during capture");
     DeviceGuard2 g(ctx->device);
     Orient& o = orient == 0 ? h->a : h->t;
-    const int rc = build_segments(ctx, h, o, count, h->a.whole.val32 != nullptr);
+    const int rc = build_parts(ctx, h, o, count, h->f32);
     if (rc != SOV_OK) return rc;
     return ensure_partials(h);
 }
@@ -878,7 +873,7 @@ int sov_hip_spmv(sov_hip_ctx* ctx, const sov_hip_csr* h, int transpose, int prec
     if (ctx == nullptr || h == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv args");
     if (prec != SOV_PREC_F64 && prec != SOV_PREC_F32) return fail(SOV_INVALID_ARGUMENT, "spmv: precision");
     const Orient& o = transpose ? h->t : h->a;
-    if (o.whole.rows > 0 && y == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv: y is null");
+    if (o.nat.rows > 0 && y == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv: y is null");
     if (h->nnz > 0 && x == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv: x is null");
     DeviceGuard2 g(ctx->device);
     return spmv_orient(ctx, o, prec, alpha, x, beta, y);
@@ -887,45 +882,48 @@ int sov_hip_spmv(sov_hip_ctx* ctx, const sov_hip_csr* h, int transpose, int prec
 int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
     if (ctx == nullptr || h == nullptr) return fail(SOV_INVALID_ARGUMENT, "tune args");
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "tune during capture");
-    if (prec == SOV_PREC_F32 && h->a.whole.val32 == nullptr) return fail(SOV_INVALID_ARGUMENT, "tune f32 needs make_f32");
+    if (prec == SOV_PREC_F32 && !h->f32) return fail(SOV_INVALID_ARGUMENT, "tune f32 needs make_f32");
     DeviceGuard2 g(ctx->device);
     const size_t es = prec == SOV_PREC_F64 ? sizeof(double) : sizeof(float);
-    void *xv = nullptr, *yv = nullptr;
     const int64_t len = std::max<int64_t>(std::max(h->m, h->n), 1);
+    void *xv = nullptr, *yv = nullptr;
     SOV_HIP_TRY(hipMalloc(&xv, es * len));
     hipError_t e = hipMalloc(&yv, es * len);
     if (e != hipSuccess) { (void)hipFree(xv); return hip_fail(e, "tune buffers"); }
-    (void)hipMemsetAsync(xv, 0, es * len, ctx->stream);
-    (void)hipMemsetAsync(yv, 0, es * len, ctx->stream);
     int rc = SOV_OK;
-    const bool f32 = h->a.whole.val32 != nullptr;
+    if (es == sizeof(double)) {
+        k_fill_value<double><<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, 1.0, static_cast<double*>(xv));
+    } else {
+        k_fill_value<float><<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, 1.0f, static_cast<float*>(xv));
+    }
+    e = check_launch();
+    if (e != hipSuccess) rc = hip_fail(e, "tune fill");
     for (int orient = 0; orient < 2 && rc == SOV_OK; ++orient) {
         Orient& o = orient == 0 ? h->a : h->t;
         // segment counts worth trying: keep each gathered slice >= 64k elements
-        const int64_t cols = o.whole.cols;
+        const int64_t cols = o.nat.cols;
         int best_K = 1;
         float best_ms = 1e30f;
+        std::vector<std::vector<int32_t>> best_lanes;
         for (int K : {1, 2, 4, 8}) {
             if (K > 1 && cols / K < 65536) break;
-            rc = build_segments(ctx, h, o, K, f32);
-            if (rc != SOV_OK) break;
+            rc = build_parts(ctx, h, o, K, h->f32);
             for (int k = 0; k < o.nparts() && rc == SOV_OK; ++k) rc = tune_part_lanes(ctx, o, o.part(k), prec, xv, yv);
             if (rc != SOV_OK) break;
             const float ms = time_spmv(ctx, o, prec, xv, yv, 4);
             if (ms < best_ms) {
                 best_ms = ms;
                 best_K = K;
+                best_lanes.clear();
+                for (const Part& p : o.parts) best_lanes.emplace_back(p.lanes, p.lanes + kClasses);
             }
         }
         if (rc == SOV_OK && best_K != o.nparts()) {
-            rc = build_segments(ctx, h, o, best_K, f32);
-            for (int k = 0; k < o.nparts() && rc == SOV_OK; ++k) rc = tune_part_lanes(ctx, o, o.part(k), prec, xv, yv);
-        }
-        // the whole part (checks, primitive SpMV fallback) keeps tuned lanes too
-        if (rc == SOV_OK && !o.seg.empty()) {
-            Orient whole_only;
-            whole_only.whole = o.whole;
-            rc = tune_part_lanes(ctx, whole_only, o.whole, prec, xv, yv);
+            rc = build_parts(ctx, h, o, best_K, h->f32);
+            if (rc == SOV_OK && static_cast<int>(best_lanes.size()) == o.nparts())
+                for (int k = 0; k < o.nparts(); ++k)
+                    std::copy(best_lanes[static_cast<size_t>(k)].begin(), best_lanes[static_cast<size_t>(k)].end(),
+                              o.part(k).lanes);
         }
     }
     (void)hipStreamSynchronize(ctx->stream);
@@ -945,8 +943,8 @@ int sov_hip_scale_ruiz_sweep(sov_hip_ctx* ctx, const sov_hip_csr* h, double* rs,
     double* rmax = ctx->d_red;
     double* cmax = rmax + h->m;
     double* part = cmax + h->n;
-    const Part& A = h->a.whole;
-    const Part& T = h->t.whole;
+    const Natural& A = h->a.nat;
+    const Natural& T = h->t.nat;
     if (h->m > 0)
         k_ruiz_rowmax<<<launch_blocks(h->m), kBlock, 0, ctx->stream>>>(h->m, A.ptr, A.idx, A.val64, rs, cs, rmax);
     if (h->n > 0)
@@ -970,8 +968,8 @@ int sov_hip_scale_pock_chambolle(sov_hip_ctx* ctx, const sov_hip_csr* h, double 
     if (rc != SOV_OK) return rc;
     double* rsum = ctx->d_red;
     double* csum = rsum + h->m;
-    const Part& A = h->a.whole;
-    const Part& T = h->t.whole;
+    const Natural& A = h->a.nat;
+    const Natural& T = h->t.nat;
     if (h->m > 0)
         k_pc_rowsum<<<launch_blocks(h->m), kBlock, 0, ctx->stream>>>(h->m, A.ptr, A.idx, A.val64, rs, cs,
                                                                       2.0 - alpha, rsum);

@@ -49,16 +49,19 @@ struct Sched {
     const int32_t* list;
 };
 
-// One CSR "part": the whole matrix of an orientation, or one column segment
-// of it (entries whose column lies in [k W, (k+1) W)).
+// One CSR "part" of an orientation: all its columns, or one column segment
+// (entries whose column lies in [k W, (k+1) W)).  A part stores its rows in
+// class order: position i holds row list[i], ptr/idx/values are indexed by
+// position, so each length class streams one contiguous range of the arrays
+// (rows of different classes never share a cache line of the matrix).
 struct Part {
     int32_t rows = 0, cols = 0, nnz = 0;
-    const int32_t* ptr = nullptr;   // rows + 1
+    const int32_t* list = nullptr;  // rows: original row of each position
+    const int32_t* ptr = nullptr;   // rows + 1, by position
     const int32_t* idx = nullptr;   // nnz, global column indices, ascending per row
     double* val64 = nullptr;        // nnz
     float* val32 = nullptr;         // nnz, optional
-    const int32_t* src = nullptr;   // nnz: position of each entry in A's value array; null = A itself
-    int32_t* list = nullptr;        // rows grouped by length class (owned)
+    const int32_t* src = nullptr;   // nnz: position of each entry in A's value array
     int32_t class_count[kClasses] = {};
     int32_t class_offset[kClasses] = {};
     int32_t lanes[kClasses] = {};
@@ -67,26 +70,38 @@ struct Part {
     int blocks() const;
 };
 
-// An orientation (A or A^T): the whole CSR for checks/scaling, and the
-// column-segmented copy the step kernels and SpMV stream through when the
-// gathered vector is larger than a comfortable share of an XCD's L2.
+// The natural (row-ordered) CSR of an orientation: the source the parts are
+// built from, and what the scaling kernels walk row by row.
+struct Natural {
+    int32_t rows = 0, cols = 0, nnz = 0;
+    const int32_t* ptr = nullptr;
+    const int32_t* idx = nullptr;
+    double* val64 = nullptr;
+    const int32_t* src = nullptr;   // null for A itself
+};
+
+// An orientation (A or A^T): the natural CSR and its class-ordered parts
+// (one part, or one per column segment when the gathered vector is larger
+// than a comfortable share of an XCD's L2).
 struct Orient {
-    Part whole;
-    std::vector<Part> seg;          // empty: unsegmented (steps use `whole`)
-    int32_t seg_width = 0;
-    // owned storage of the segmented copy
-    int32_t* seg_ptr = nullptr;     // nseg * (rows + 1)
-    int32_t* seg_idx = nullptr;     // nnz
-    int32_t* seg_src = nullptr;     // nnz
-    double* seg_val64 = nullptr;    // nnz
-    float* seg_val32 = nullptr;     // nnz
+    Natural nat;
+    std::vector<Part> parts;
+    int32_t seg_width = 0;          // 0 when unsegmented
+    // owned storage of the parts
+    int32_t* p_list = nullptr;      // nparts * rows
+    int32_t* p_ptr = nullptr;       // nparts * (rows + 1)
+    int32_t* p_idx = nullptr;       // nnz
+    int32_t* p_src = nullptr;       // nnz
+    double* p_val64 = nullptr;      // nnz
+    float* p_val32 = nullptr;       // nnz
     double* acc64 = nullptr;        // rows: running partial sums across segments
     float* acc32 = nullptr;         // rows
-    int nparts() const { return seg.empty() ? 1 : static_cast<int>(seg.size()); }
-    const Part& part(int k) const { return seg.empty() ? whole : seg[static_cast<size_t>(k)]; }
-    Part& part(int k) { return seg.empty() ? whole : seg[static_cast<size_t>(k)]; }
+    double* chk_acc = nullptr;      // 2 * rows: the checks' two running dots
+    int nparts() const { return static_cast<int>(parts.size()); }
+    const Part& part(int k) const { return parts[static_cast<size_t>(k)]; }
+    Part& part(int k) { return parts[static_cast<size_t>(k)]; }
     int max_blocks() const;
-    void free_segments();
+    void free_parts();
 };
 
 // How a part's launch combines with the others of its orientation.
@@ -123,11 +138,13 @@ struct sov_hip_csr {
     int32_t m = 0, n = 0, nnz = 0;
     sovhip::Orient a;     // A   : m rows, gathers n-vectors
     sovhip::Orient t;     // A^T : n rows, gathers m-vectors
-    int32_t* t_ptr = nullptr;   // owned A^T structure
+    int32_t* t_ptr = nullptr;   // owned natural A^T structure
     int32_t* t_idx = nullptr;
     int32_t* t_src = nullptr;   // owned: A position of each A^T entry
+    double* t_val64 = nullptr;  // owned natural A^T values
     bool owns_a_val64 = false;
-    double* partials = nullptr; // owned: kCheckSlots * (blocks of both whole parts + 64) FP64
+    bool f32 = false;           // parts carry FP32 value images (sov_hip_csr_make_f32)
+    double* partials = nullptr; // owned: kCheckSlots * (largest part blocks of both orientations + 64) FP64
     int64_t partial_blocks = 0;
     int device = 0;
 };
@@ -234,41 +251,42 @@ __device__ __forceinline__ ClassView class_view(const Sched& s) {
     return v;
 }
 
-// The row of this thread's group of G lanes (-1 past the class) and its lane.
+// The position (in the part's class order) of this thread's group of G
+// lanes, -1 past the class, and the lane within the group.
 template <int G>
-__device__ __forceinline__ int row_of(const Sched& s, const ClassView& v, int& lane) {
+__device__ __forceinline__ int pos_of(const ClassView& v, int& lane) {
     constexpr int RPB = kBlock / G;
     const int local = ((int)blockIdx.x - v.begin) * RPB + (int)threadIdx.x / G;
     lane = (int)threadIdx.x % G;
-    return (local < v.count) ? s.list[v.offset + local] : -1;
+    return (local < v.count) ? v.offset + local : -1;
 }
 
-// Dot product of one CSR row with a gathered vector, G lanes per row.  Every
-// lane of the group returns the full sum.  Accumulation type Acc.
+// Dot product of the row at position pos with a gathered vector, G lanes per
+// row.  Every lane of the group returns the full sum.  Accumulation type Acc.
 template <int G, typename Acc, typename V, typename X>
 __device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
                                        const int32_t* __restrict__ idx,
                                        const V* __restrict__ val, const X* __restrict__ x,
-                                       int row, int lane) {
+                                       int pos, int lane) {
     Acc acc = Acc(0);
-    if (row >= 0) {
-        const int b = ptr[row], e = ptr[row + 1];
+    if (pos >= 0) {
+        const int b = ptr[pos], e = ptr[pos + 1];
 #pragma unroll 4
         for (int k = b + lane; k < e; k += G) acc += Acc(val[k]) * Acc(x[idx[k]]);
     }
     return group_sum<G>(acc);
 }
 
-// Two dots of the same row (current and second vector scaled by s2).
+// Two dots of the same row (current and second vector scaled by s2), FP64.
 template <int G, typename V, typename X>
 __device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
                                          const int32_t* __restrict__ idx,
                                          const V* __restrict__ val, const X* __restrict__ x1,
-                                         const X* __restrict__ x2, double s2, int row, int lane,
+                                         const X* __restrict__ x2, double s2, int pos, int lane,
                                          double& d1, double& d2) {
     double a1 = 0.0, a2 = 0.0;
-    if (row >= 0) {
-        const int b = ptr[row], e = ptr[row + 1];
+    if (pos >= 0) {
+        const int b = ptr[pos], e = ptr[pos + 1];
         for (int k = b + lane; k < e; k += G) {
             const double v = double(val[k]);
             const int j = idx[k];
@@ -297,7 +315,7 @@ __device__ __forceinline__ void dispatch_G(const ClassView& v, F& f) {
 // Host helpers
 int launch_blocks(int64_t n);      // blocks for an elementwise kernel over n
 hipError_t check_launch();         // hipGetLastError wrapper
-int build_schedule(sov_hip_ctx* ctx, Part& p);    // class lists from ptr (host)
+int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32);
 int ensure_partials(sov_hip_csr* csr);
 
 }  // namespace sovhip

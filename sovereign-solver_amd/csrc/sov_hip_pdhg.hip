@@ -50,9 +50,10 @@ struct PrimalOp {
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        const int row = row_of<G>(s, v, lane);
-        T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
-        if (row >= 0 && lane == 0) {
+        const int pos = pos_of<G>(v, lane);
+        T aty = row_dot<G, T>(ptr, idx, val, y, pos, lane);
+        if (pos >= 0 && lane == 0) {
+            const int row = s.list[pos];
             if (mode == kFirst) { acc[row] = aty; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + aty; return; }
             if (mode == kLast) aty = acc[row] + aty;
@@ -90,9 +91,10 @@ struct DualOp {
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        const int row = row_of<G>(s, v, lane);
-        T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
-        if (row >= 0 && lane == 0) {
+        const int pos = pos_of<G>(v, lane);
+        T ax = row_dot<G, T>(ptr, idx, val, xbar, pos, lane);
+        if (pos >= 0 && lane == 0) {
+            const int row = s.list[pos];
             if (mode == kFirst) { acc[row] = ax; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
             if (mode == kLast) ax = acc[row] + ax;
@@ -151,6 +153,25 @@ __device__ __forceinline__ double box_inf(double v, double lo, double hi) {
     return v > 0.0 ? v * lo : (v < 0.0 ? v * hi : 0.0);
 }
 
+// Fold one part's two dots of a row into its running pair; true when the row
+// is complete (unsegmented, or the last part) and its epilogue should run.
+__device__ __forceinline__ bool combine_parts(int mode, double* pair, double& d1, double& d2) {
+    if (mode == kOnly) return true;
+    if (mode == kFirst) {
+        pair[0] = d1;
+        pair[1] = d2;
+        return false;
+    }
+    if (mode == kMiddle) {
+        pair[0] += d1;
+        pair[1] += d2;
+        return false;
+    }
+    d1 += pair[0];
+    d2 += pair[1];
+    return true;
+}
+
 template <typename T>
 struct CheckRowOp {
     const int32_t* ptr;
@@ -167,17 +188,20 @@ struct CheckRowOp {
     const double* rs;
     double* act_avg;
     double* partial;
+    double* acc;      // 2 per row: running (current, average) dots across parts
     double inv, tol_s, tol_o;
+    int mode;
     Sched s;
     // sums: pv2_c, pv2_a, bn2, yres2_c, yres2_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
         int lane;
-        const int row = row_of<G>(s, v, lane);
+        const int pos = pos_of<G>(v, lane);
         double ac, aa;
-        row_dot2<G>(ptr, idx, val, x, xsum, inv, row, lane, ac, aa);
-        if (row >= 0 && lane == 0) {
-            const int i = row;
+        row_dot2<G>(ptr, idx, val, x, xsum, inv, pos, lane, ac, aa);
+        if (pos >= 0 && lane == 0) {
+            const int i = s.list[pos];
+            if (!combine_parts(mode, acc + 2 * (int64_t)i, ac, aa)) return;
             if (act_avg != nullptr) act_avg[i] = aa;
             const double yc = double(y[i]), ya = double(ysum[i]) * inv;
             mx[0] = nmax(mx[0], row_res(ac, double(lo_s[i]), double(hi_s[i]), yc, tol_s));
@@ -222,17 +246,20 @@ struct CheckColOp {
     const double* c_o;
     const double* cs;
     double* partial;
+    double* acc;
     double inv, tol_s, tol_o, sign;
+    int mode;
     Sched s;
     // sums: dres2_c, dres2_a, cn2, pobj_c, pobj_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
         int lane;
-        const int row = row_of<G>(s, v, lane);
+        const int pos = pos_of<G>(v, lane);
         double tc, ta;
-        row_dot2<G>(ptr, idx, val, y, ysum, inv, row, lane, tc, ta);
-        if (row >= 0 && lane == 0) {
-            const int j = row;
+        row_dot2<G>(ptr, idx, val, y, ysum, inv, pos, lane, tc, ta);
+        if (pos >= 0 && lane == 0) {
+            const int j = s.list[pos];
+            if (!combine_parts(mode, acc + 2 * (int64_t)j, tc, ta)) return;
             const double xc = double(x[j]), xa = double(xsum[j]) * inv;
             const double cj = double(c_s[j]);
             mx[0] = nmax(mx[0], dual_class(cj - tc, double(lo_s[j]), double(hi_s[j]), xc, tol_s));
@@ -276,7 +303,9 @@ __global__ void __launch_bounds__(kBlock) k_check(Op op) {
     const ClassView v = class_view(op.s);
     CheckRunner<Op> r{op, sm, mx};
     dispatch_G(v, r);
-    block_reduce_write<7, 4>(sm, mx, op.partial + (int64_t)blockIdx.x * kCheckSlots);
+    // mode is uniform over the launch: only a launch that completes rows reduces
+    if (op.mode == kOnly || op.mode == kLast)
+        block_reduce_write<7, 4>(sm, mx, op.partial + (int64_t)blockIdx.x * kCheckSlots);
 }
 
 // Block b reduces slot (b % 11) of region (b / 11): slots 0..6 sums, 7..10 maxima.
@@ -391,45 +420,58 @@ constexpr int kRestartBlocks = 512;
 static inline double nmax_host(double a, double b) { return (a > b || a != a) ? a : b; }
 
 template <typename T>
+static const T* vals_of(const Part& p) {
+    return std::is_same<T, double>::value ? reinterpret_cast<const T*>(p.val64) : reinterpret_cast<const T*>(p.val32);
+}
+
+template <typename T>
 static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a, sov_hip_check_out* out) {
     const sov_hip_pdhg_state* st = a->st;
     const double inv = 1.0 / static_cast<double>(a->since_restart > 0 ? a->since_restart : 1);
-    const Part& PA = h->a.whole;
-    const Part& PT = h->t.whole;
-    const Sched sa = PA.sched(), stt = PT.sched();
-    const int rb = sa.block_begin[kClasses], cb = stt.block_begin[kClasses];
-    const T* val_a = std::is_same<T, double>::value ? reinterpret_cast<const T*>(PA.val64)
-                                                    : reinterpret_cast<const T*>(PA.val32);
-    const T* val_t = std::is_same<T, double>::value ? reinterpret_cast<const T*>(PT.val64)
-                                                    : reinterpret_cast<const T*>(PT.val32);
-    if (val_a == nullptr || val_t == nullptr) return fail(SOV_INVALID_ARGUMENT, "check: value image missing");
+    const Orient& OA = h->a;
+    const Orient& OT = h->t;
+    const int npa = OA.nparts(), npt = OT.nparts();
+    // the launch that completes rows (the last part) owns the block partials
+    const int rb = OA.part(npa - 1).blocks(), cb = OT.part(npt - 1).blocks();
+    for (const Orient* o : {&OA, &OT})
+        for (const Part& p : o->parts)
+            if (p.nnz > 0 && vals_of<T>(p) == nullptr) return fail(SOV_INVALID_ARGUMENT, "check: value image missing");
     double* rowpart = h->partials;
     double* colpart = h->partials + (int64_t)(rb + 1) * kCheckSlots;
     if ((int64_t)(rb + cb + 2) > h->partial_blocks) return fail(SOV_INVALID_ARGUMENT, "check: partials too small");
     int rc = ctx->ensure_red(64);
     if (rc != SOV_OK) return rc;
-    if (rb > 0) {
+    for (int k = 0; k < npa; ++k) {
+        const Part& p = OA.part(k);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
         CheckRowOp<T> op{};
-        op.ptr = PA.ptr; op.idx = PA.idx; op.val = val_a;
+        op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
         op.x = static_cast<const T*>(st->x); op.xsum = static_cast<const T*>(st->x_sum);
         op.y = static_cast<const T*>(st->y); op.ysum = static_cast<const T*>(st->y_sum);
         op.lo_s = static_cast<const T*>(st->row_lo); op.hi_s = static_cast<const T*>(st->row_hi);
         op.lo_o = a->orig_row_lo; op.hi_o = a->orig_row_hi; op.rs = a->row_scale;
-        op.act_avg = a->act_avg; op.partial = rowpart;
-        op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.s = sa;
-        k_check<CheckRowOp<T>><<<rb, kBlock, 0, ctx->stream>>>(op);
+        op.act_avg = a->act_avg; op.partial = rowpart; op.acc = OA.chk_acc;
+        op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.mode = seg_mode(k, npa); op.s = s;
+        k_check<CheckRowOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
-    if (cb > 0) {
+    for (int k = 0; k < npt; ++k) {
+        const Part& p = OT.part(k);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
         CheckColOp<T> op{};
-        op.ptr = PT.ptr; op.idx = PT.idx; op.val = val_t;
+        op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
         op.y = static_cast<const T*>(st->y); op.ysum = static_cast<const T*>(st->y_sum);
         op.x = static_cast<const T*>(st->x); op.xsum = static_cast<const T*>(st->x_sum);
         op.c_s = static_cast<const T*>(st->c);
         op.lo_s = static_cast<const T*>(st->col_lo); op.hi_s = static_cast<const T*>(st->col_hi);
         op.lo_o = a->orig_col_lo; op.hi_o = a->orig_col_hi; op.c_o = a->orig_c; op.cs = a->col_scale;
-        op.partial = colpart;
-        op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign; op.s = stt;
-        k_check<CheckColOp<T>><<<cb, kBlock, 0, ctx->stream>>>(op);
+        op.partial = colpart; op.acc = OT.chk_acc;
+        op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign;
+        op.mode = seg_mode(k, npt); op.s = s;
+        k_check<CheckColOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     k_check_final<<<22, kBlock, 0, ctx->stream>>>(rowpart, rb, colpart, cb, ctx->d_red);
     SOV_HIP_TRY(check_launch());
@@ -465,11 +507,6 @@ using namespace sovhip;
 namespace {
 
 template <typename T>
-const T* part_vals(const Part& p) {
-    return std::is_same<T, double>::value ? reinterpret_cast<const T*>(p.val64) : reinterpret_cast<const T*>(p.val32);
-}
-
-template <typename T>
 T* orient_acc(const Orient& o) {
     return std::is_same<T, double>::value ? reinterpret_cast<T*>(o.acc64) : reinterpret_cast<T*>(o.acc32);
 }
@@ -483,7 +520,7 @@ int primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state
         const Sched s = p.sched();
         const int blocks = s.block_begin[kClasses];
         if (blocks == 0) continue;
-        const T* val = part_vals<T>(p);
+        const T* val = vals_of<T>(p);
         if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "primal_step: value image missing");
         PrimalOp<T> op{p.ptr, p.idx, val, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
                        static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi), static_cast<T*>(st->x),
@@ -505,7 +542,7 @@ int dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* 
         const Sched s = p.sched();
         const int blocks = s.block_begin[kClasses];
         if (blocks == 0) continue;
-        const T* val = part_vals<T>(p);
+        const T* val = vals_of<T>(p);
         if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "dual_step: value image missing");
         DualOp<T> op{p.ptr, p.idx, val, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
                      static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),

@@ -38,7 +38,7 @@ class PdhgOpts(C.Structure):
         ("initial_primal_weight", C.c_double), ("primal_weight_smoothing", C.c_double),
         ("detect_infeasibility", C.c_int32), ("use_graphs", C.c_int32), ("warm_start_dual", _f64p),
         ("trace_cap", C.c_int64), ("trace_res", _f64p), ("trace_it", _i64p),
-        ("restart_scheme", C.c_int32), ("reserved", C.c_int32), ("restart_beta_sufficient", C.c_double),
+        ("restart_scheme", C.c_int32), ("autotune", C.c_int32), ("restart_beta_sufficient", C.c_double),
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
     ]
 
@@ -245,7 +245,7 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           restart_check_period=40, engine_kind="auto", termination="reference", precision="f64",
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
-          restarts="reference") -> Result:
+          restarts="reference", autotune=True) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point."""
     L = engine()
     o = PdhgOpts()
@@ -263,6 +263,7 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     o.use_graphs = int(use_graphs)
     o.verbose = int(verbose)
     o.restart_scheme = 1 if restarts == "adaptive" else 0
+    o.autotune = int(autotune)
     if fp32_switch_factor is not None:
         o.fp32_switch_factor = fp32_switch_factor
     if fp32_stall_checks is not None:

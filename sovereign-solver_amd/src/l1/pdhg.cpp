@@ -705,7 +705,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), ysrc, zs.as<double>()), "set_point");
             hip_ok(sov_hip_sync(ctx), "sync");
         }
-        if (o.use_graphs && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, prec), "tune");
+        if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, prec), "tune");
 
         // primal-weight bookkeeping (FP64 copies of the last restart point)
         DevMem d_xlast, d_ylast;
