@@ -53,7 +53,8 @@ class DeviceProblem:
         D.check(L.sov_hip_scale_pock_chambolle(ctx.h, self.A.h, 1.0, rs, cs))
         D.check(L.sov_hip_csr_scale(ctx.h, self.A.h, rs, cs))
         r, c = ctx.download(rs, m), ctx.download(cs, n)
-        self.rs, self.cs = rs, cs
+        # the kernels keep every vector in the handle's device order
+        pr, pc = self.A.order(0), self.A.order(1)
         sign = -1.0 if a["maximize"] else 1.0
         fin = np.isfinite
         cs_h = c
@@ -69,8 +70,8 @@ class DeviceProblem:
         D.check(L.sov_hip_csr_tune(ctx.h, self.A.h, prec))
         st = D.PdhgState()
         st.prec, st.m, st.n = prec, m, n
-        st.c, st.col_lo, st.col_hi = ctx.upload(c_s.astype(dt)), ctx.upload(clo.astype(dt)), ctx.upload(chi.astype(dt))
-        st.row_lo, st.row_hi = ctx.upload(rlo.astype(dt)), ctx.upload(rhi.astype(dt))
+        st.c, st.col_lo, st.col_hi = (ctx.upload(v[pc].astype(dt)) for v in (c_s, clo, chi))
+        st.row_lo, st.row_hi = (ctx.upload(v[pr].astype(dt)) for v in (rlo, rhi))
         for f, ln in (("x", n), ("x_bar", n), ("x_sum", n), ("s", m), ("y", m), ("y_sum", m)):
             setattr(st, f, ctx.upload(np.zeros(ln, dt)))
         self.st = st
@@ -80,12 +81,12 @@ class DeviceProblem:
         u, v, wv = ctx.upload(np.ones(n)), ctx.upload(np.ones(m)), ctx.upload(np.zeros(m))
         est = 0.0
         for _ in range(30):
-            D.check(L.sov_hip_spmv(ctx.h, self.A.h, 0, 0, -1.0, u, 0.0, wv))
+            D.check(L.sov_hip_spmv_device(ctx.h, self.A.h, 0, 0, -1.0, u, 0.0, wv))
             D.check(L.sov_hip_axpy_f64(ctx.h, m, 1.0, v, wv))
             wn = C.c_double()
             D.check(L.sov_hip_nrm2_f64(ctx.h, m, wv, C.byref(wn)))
             D.check(L.sov_hip_scal_f64(ctx.h, m, 1.0 / wn.value, wv))
-            D.check(L.sov_hip_spmv(ctx.h, self.A.h, 1, 0, -1.0, wv, 0.0, u))
+            D.check(L.sov_hip_spmv_device(ctx.h, self.A.h, 1, 0, -1.0, wv, 0.0, u))
             D.check(L.sov_hip_memcpy_d2d(ctx.h, v, wv, 8 * m))
             un, vn = C.c_double(), C.c_double()
             D.check(L.sov_hip_nrm2_f64(ctx.h, n, u, C.byref(un)))
@@ -98,9 +99,9 @@ class DeviceProblem:
         ca = D.CheckArgs()
         ca.st = C.pointer(st)
         ca.tol_scaled, ca.tol_orig, ca.obj_sign, ca.obj_const = 1e-6, 1e-6, sign, a["obj_const"]
-        ca.row_scale, ca.col_scale = rs, cs
-        ca.orig_row_lo, ca.orig_row_hi = ctx.upload(a["row_lo"]), ctx.upload(a["row_hi"])
-        ca.orig_col_lo, ca.orig_col_hi, ca.orig_c = ctx.upload(a["col_lo"]), ctx.upload(a["col_hi"]), ctx.upload(a["c"])
+        ca.row_scale, ca.col_scale = ctx.upload(r[pr]), ctx.upload(c[pc])
+        ca.orig_row_lo, ca.orig_row_hi = ctx.upload(a["row_lo"][pr]), ctx.upload(a["row_hi"][pr])
+        ca.orig_col_lo, ca.orig_col_hi, ca.orig_c = (ctx.upload(a[k][pc]) for k in ("col_lo", "col_hi", "c"))
         ca.act_avg = ctx.upload(np.zeros(m))
         self.ca = ca
         self.since = 0
@@ -202,6 +203,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="end with 3 launches of the f64->f32 convert kernel over 32 Mi elements (reads 256 MiB, "
+                         "writes 128 MiB): the known-size reference for HBM byte counters")
     args = ap.parse_args()
     args.steps = max(PERIOD, (args.steps + PERIOD - 1) // PERIOD * PERIOD)
     args.warmup = (args.warmup + PERIOD - 1) // PERIOD * PERIOD
@@ -300,6 +304,13 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calibrate:
+        cn = 32 << 20
+        src, dst = ctx.alloc(8 * cn), ctx.alloc(4 * cn)
+        D.check(ctx.L.sov_hip_fill_zero(ctx.h, src, 8 * cn))
+        for _ in range(3):
+            D.check(ctx.L.sov_hip_convert_f64_f32(ctx.h, cn, src, dst))
+        D.check(ctx.L.sov_hip_sync(ctx.h))
     ctx.close()
 
 

@@ -102,10 +102,15 @@ int sov_hip_convert_f32_f64(sov_hip_ctx* ctx, int64_t n, const float* src, doubl
  *   flags & SOV_CSR_OWN_VALUES: copy the values so sov_hip_csr_scale may
  *     rescale them in place (otherwise A's value array is borrowed and must
  *     outlive the handle; the structure arrays are always borrowed).
- * Row-length schedules: each orientation's rows are grouped by length class
- * and each class is processed by G lanes per row, G in {1,..,64}; the lane
- * counts come from sov_hip_csr_tune (an autotuner that times candidates) or
- * from a length heuristic. */
+ * Device order: the handle fixes a permutation of the rows (which = 0) and
+ * of the columns (which = 1): the rows of A, resp. of A^T, grouped by
+ * length class (<= 2, 4, 8, 16, 32, 64, 256, more entries), stably.  The
+ * PDHG kernels and sov_hip_spmv_device read and write every vector in this
+ * order (row p of an orientation writes element p; all per-row accesses are
+ * coalesced); sov_hip_spmv and the scaling entry points take natural order.
+ * Row-length schedules: each class is processed by G lanes per row,
+ * G in {1,..,64}; the lane counts come from sov_hip_csr_tune (an autotuner
+ * that times candidates) or from a length heuristic. */
 enum { SOV_CSR_OWN_VALUES = 1 };
 int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz,
                        const int32_t* d_row_ptr, const int32_t* d_col_idx, const double* d_values,
@@ -128,12 +133,13 @@ int sov_hip_csr_transpose_arrays(sov_hip_ctx* ctx, const sov_hip_csr* csr, int32
  * [0, 8), both orientations (orient 0 = A, 1 = A^T).  0 keeps the heuristic. */
 int sov_hip_csr_set_lanes(sov_hip_csr* csr, int orient, const int32_t lanes[8]);
 int sov_hip_csr_get_lanes(const sov_hip_csr* csr, int orient, int32_t lanes[8]);
-/* Column segments: split an orientation into `count` column ranges of equal
- * width, stored segment-major, so each pass gathers from a slice of the
- * vector that stays resident in an XCD's L2.  The step kernels and SpMV run
- * the segments in order and carry per-row partial sums between them (a
- * fixed summation order: results are reproducible).  count = 1 removes the
- * split.  The checks and the scaling kernels always use the whole CSR. */
+/* Column segments: split an orientation into `count` ranges of equal width
+ * of the gathered dimension (in device order), stored segment-major, so each
+ * pass gathers from a slice of the vector that stays resident in an XCD's
+ * L2.  The step, check and SpMV kernels run the segments in order and carry
+ * per-row partial sums between them (a fixed summation order: results are
+ * reproducible).  count = 1 removes the split.  The scaling kernels use the
+ * natural CSR. */
 int sov_hip_csr_set_segments(sov_hip_ctx* ctx, sov_hip_csr* csr, int orient, int count);
 int sov_hip_csr_get_segments(const sov_hip_csr* csr, int orient, int* count);
 /* Autotuner: for each orientation, try 1, 2, 4 and 8 column segments (slices
@@ -149,6 +155,19 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* csr, int prec);
  * prec selects the value image and the element type of x and y. */
 int sov_hip_spmv(sov_hip_ctx* ctx, const sov_hip_csr* csr, int transpose, int prec,
                  double alpha, const void* x, double beta, void* y);
+/* The same product with x and y in device order (no staging copies).
+ * No reference counterpart: the fused engine's power iteration. */
+int sov_hip_spmv_device(sov_hip_ctx* ctx, const sov_hip_csr* csr, int transpose, int prec,
+                        double alpha, const void* x, double beta, void* y);
+/* The device-order permutation of rows (which = 0, m entries) or columns
+ * (which = 1, n entries) into host memory: perm[p] = natural index at
+ * device position p.  No reference counterpart. */
+int sov_hip_csr_order(sov_hip_ctx* ctx, const sov_hip_csr* csr, int which, int32_t* perm);
+/* Reorder a device vector of the row (which = 0) or column (which = 1)
+ * dimension: natural -> device (to_natural = 0) or back (1).  src and dst
+ * must not alias.  No reference counterpart. */
+int sov_hip_csr_permute(sov_hip_ctx* ctx, const sov_hip_csr* csr, int which, int to_natural, int prec,
+                        const void* src, void* dst);
 
 /* ---- scaling (kernel d) -----------------------------------------------
  * Ruiz and Pock-Chambolle factors (reference scaling.cpp:36-101) from the
@@ -170,7 +189,8 @@ int sov_hip_scale_pock_chambolle(sov_hip_ctx* ctx, const sov_hip_csr* csr, doubl
  *   dual:   per row i, s+ = clamp(s - tau y, lo, hi); s_bar = 2 s+ - s;
  *           y+ = y + sigma (s_bar - (A x_bar)_i); s = s+; y = y+;
  *           y_sum = (first ? y+ : y_sum + y+)
- * All vectors have the element type of prec; bounds may hold +-inf. */
+ * All vectors have the element type of prec and are in the handle's device
+ * order (sov_hip_csr_order); bounds may hold +-inf. */
 typedef struct sov_hip_pdhg_state {
     int32_t prec;
     int32_t m, n;
@@ -203,7 +223,7 @@ int sov_hip_pdhg_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* csr,
  *           (problem sense, with obj_const) of the original-space point.
  * Max reductions propagate NaN (a NaN anywhere reads back as NaN).  If
  * act_avg is not NULL it receives A_s (x_sum / since_restart) (m, FP64),
- * the activity a restart needs. */
+ * the activity a restart needs.  Every array is in device order. */
 typedef struct sov_hip_check_args {
     const sov_hip_pdhg_state* st;
     int64_t since_restart;

@@ -39,7 +39,6 @@ Sched Part::sched() const {
         b += (class_count[c] + rpb - 1) / rpb;
     }
     s.block_begin[kClasses] = b;
-    s.list = list;
     return s;
 }
 
@@ -54,7 +53,6 @@ int Orient::max_blocks() const {
 void Orient::free_parts() {
     parts.clear();
     seg_width = 0;
-    (void)hipFree(p_list);
     (void)hipFree(p_ptr);
     (void)hipFree(p_idx);
     (void)hipFree(p_src);
@@ -63,12 +61,21 @@ void Orient::free_parts() {
     (void)hipFree(acc64);
     (void)hipFree(acc32);
     (void)hipFree(chk_acc);
-    p_list = p_ptr = p_idx = p_src = nullptr;
+    p_ptr = p_idx = p_src = nullptr;
     p_val64 = nullptr;
     p_val32 = nullptr;
     acc64 = nullptr;
     acc32 = nullptr;
     chk_acc = nullptr;
+}
+
+void Orient::free_order() {
+    (void)hipFree(perm);
+    (void)hipFree(inv);
+    (void)hipFree(d_ptr);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_src);
+    perm = inv = d_ptr = d_idx = d_src = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -232,44 +239,118 @@ int ensure_partials(sov_hip_csr* csr) {
 }
 
 // ---------------------------------------------------------------------------
-// class-ordered parts
+// device order and parts
 // ---------------------------------------------------------------------------
 
-// bounds[k * rows + r] = first position of row r whose column is >= k * W,
-// for k = 0..K (k = 0 is ptr[r], k = K is ptr[r + 1]).
-__global__ void k_seg_bounds(int32_t rows, int32_t K, int32_t W, const int32_t* __restrict__ ptr,
-                             const int32_t* __restrict__ idx, int32_t* __restrict__ bounds) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t b = ptr[r], e = ptr[r + 1];
-        bounds[r] = b;
-        for (int32_t k = 1; k < K; ++k) {
-            const int64_t key = (int64_t)k * W;
-            int32_t lo = b, hi = e;   // first position with idx >= key
-            while (lo < hi) {
-                const int32_t mid = lo + (hi - lo) / 2;
-                if ((int64_t)idx[mid] < key) lo = mid + 1;
-                else hi = mid;
-            }
-            bounds[(int64_t)k * rows + r] = lo;
+static int row_class(int32_t len) {
+    int c = 0;
+    while (c < kClasses - 1 && len > kClassMax[c]) ++c;
+    return c;
+}
+
+// Device-order CSR of one orientation: row p is natural row perm[p], its
+// column indices mapped to device positions through other_inv.
+__global__ void k_device_fill(int32_t rows, const int32_t* __restrict__ perm, const int32_t* __restrict__ dptr,
+                              const int32_t* __restrict__ nptr, const int32_t* __restrict__ nidx,
+                              const int32_t* __restrict__ nsrc, const int32_t* __restrict__ other_inv,
+                              int32_t* __restrict__ didx, int32_t* __restrict__ dsrc) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < rows; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = perm[p];
+        int32_t d = dptr[p];
+        for (int32_t q = nptr[r]; q < nptr[r + 1]; ++q, ++d) {
+            didx[d] = other_inv[nidx[q]];
+            dsrc[d] = nsrc ? nsrc[q] : q;
         }
-        bounds[(int64_t)K * rows + r] = e;
     }
 }
 
-// Copy the entries of each position of one part: row list[i] of the natural
-// CSR, entries [lo_b[row], hi_b[row]), to [pptr[i], pptr[i+1]) of the part.
-__global__ void k_part_fill(int32_t positions, const int32_t* __restrict__ list, const int32_t* __restrict__ pptr,
-                            const int32_t* __restrict__ lo_b, const int32_t* __restrict__ hi_b,
-                            const int32_t* __restrict__ nidx, const int32_t* __restrict__ nsrc,
-                            int32_t* __restrict__ pidx, int32_t* __restrict__ psrc) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < positions;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t r = list[i];
-        const int32_t b = lo_b[r], e = hi_b[r];
-        int32_t d = pptr[i];
-        for (int32_t q = b; q < e; ++q, ++d) {
-            pidx[d] = nidx[q];
-            psrc[d] = nsrc ? nsrc[q] : q;
+// Host: the stable class order of an orientation's rows from its natural
+// row pointers; uploads perm / inv / device row pointers.
+static int build_order_host(const std::vector<int32_t>& nptr, Orient& o) {
+    const int32_t rows = o.nat.rows;
+    std::vector<int32_t> perm(static_cast<size_t>(rows)), inv(static_cast<size_t>(rows));
+    std::vector<int32_t> dptr(static_cast<size_t>(rows) + 1, 0);
+    int32_t cnt[kClasses] = {};
+    for (int32_t r = 0; r < rows; ++r) ++cnt[row_class(nptr[r + 1] - nptr[r])];
+    int32_t cur[kClasses], off = 0;
+    for (int c = 0; c < kClasses; ++c) {
+        o.class_count[c] = cnt[c];
+        o.class_offset[c] = off;
+        cur[c] = off;
+        off += cnt[c];
+    }
+    for (int32_t r = 0; r < rows; ++r) {
+        const int32_t p = cur[row_class(nptr[r + 1] - nptr[r])]++;
+        perm[p] = r;
+        inv[r] = p;
+    }
+    for (int32_t p = 0; p < rows; ++p) dptr[p + 1] = dptr[p] + (nptr[perm[p] + 1] - nptr[perm[p]]);
+    const int64_t r1 = std::max<int32_t>(rows, 1);
+    SOV_HIP_TRY(hipMalloc(&o.perm, sizeof(int32_t) * r1));
+    SOV_HIP_TRY(hipMalloc(&o.inv, sizeof(int32_t) * r1));
+    SOV_HIP_TRY(hipMalloc(&o.d_ptr, sizeof(int32_t) * (rows + 1)));
+    SOV_HIP_TRY(hipMalloc(&o.d_idx, sizeof(int32_t) * std::max<int32_t>(o.nat.nnz, 1)));
+    SOV_HIP_TRY(hipMalloc(&o.d_src, sizeof(int32_t) * std::max<int32_t>(o.nat.nnz, 1)));
+    if (rows > 0) {
+        SOV_HIP_TRY(hipMemcpy(o.perm, perm.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
+        SOV_HIP_TRY(hipMemcpy(o.inv, inv.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice));
+    }
+    SOV_HIP_TRY(hipMemcpy(o.d_ptr, dptr.data(), sizeof(int32_t) * (rows + 1), hipMemcpyHostToDevice));
+    return SOV_OK;
+}
+
+// Device order of both orientations and their device-order CSRs.
+static int build_order(sov_hip_ctx* ctx, sov_hip_csr* h) {
+    std::vector<int32_t> aptr(static_cast<size_t>(h->m) + 1), tptr(static_cast<size_t>(h->n) + 1);
+    SOV_HIP_TRY(hipMemcpyAsync(aptr.data(), h->a.nat.ptr, sizeof(int32_t) * aptr.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    SOV_HIP_TRY(hipMemcpyAsync(tptr.data(), h->t.nat.ptr, sizeof(int32_t) * tptr.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int rc = build_order_host(aptr, h->a);
+    if (rc == SOV_OK) rc = build_order_host(tptr, h->t);
+    if (rc != SOV_OK) return rc;
+    if (h->nnz > 0) {
+        if (h->m > 0)
+            k_device_fill<<<launch_blocks(h->m), kBlock, 0, ctx->stream>>>(
+                h->m, h->a.perm, h->a.d_ptr, h->a.nat.ptr, h->a.nat.idx, nullptr, h->t.inv, h->a.d_idx, h->a.d_src);
+        if (h->n > 0)
+            k_device_fill<<<launch_blocks(h->n), kBlock, 0, ctx->stream>>>(
+                h->n, h->t.perm, h->t.d_ptr, h->t.nat.ptr, h->t.nat.idx, h->t.nat.src, h->a.inv, h->t.d_idx,
+                h->t.d_src);
+        SOV_HIP_TRY(check_launch());
+    }
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SOV_OK;
+}
+
+// lens[k * (rows + 1) + r + 1] = entries of device row r in column segment k
+// (device column / W == k); lens[k * (rows + 1)] = 0.
+__global__ void k_seg_lens(int32_t rows, int32_t K, int32_t W, const int32_t* __restrict__ ptr,
+                           const int32_t* __restrict__ idx, int32_t* __restrict__ lens) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = ptr[r], e = ptr[r + 1];
+        for (int32_t k = 0; k < K; ++k) {
+            int32_t c = 0;
+            for (int32_t q = b; q < e; ++q) c += (idx[q] / W == k) ? 1 : 0;
+            lens[(int64_t)k * (rows + 1) + r + 1] = c;
+            if (r == 0) lens[(int64_t)k * (rows + 1)] = 0;
+        }
+    }
+}
+
+// Copy the entries of segment k of every device row, in row order.
+__global__ void k_seg_fill(int32_t rows, int32_t k, int32_t W, const int32_t* __restrict__ ptr,
+                           const int32_t* __restrict__ idx, const int32_t* __restrict__ src,
+                           const int32_t* __restrict__ pptr, int32_t* __restrict__ pidx,
+                           int32_t* __restrict__ psrc) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        int32_t d = pptr[r];
+        for (int32_t q = ptr[r]; q < ptr[r + 1]; ++q) {
+            if (idx[q] / W != k) continue;
+            pidx[d] = idx[q];
+            psrc[d] = src[q];
+            ++d;
         }
     }
 }
@@ -280,94 +361,19 @@ __global__ void k_gather_f64_to_f32(int32_t nnz, const int32_t* __restrict__ src
         out[p] = (float)v[src[p]];
 }
 
-static int row_class(int32_t len) {
-    int c = 0;
-    while (c < kClasses - 1 && len > kClassMax[c]) ++c;
-    return c;
-}
-
-// (Re)build the K class-ordered parts of an orientation from its natural CSR
-// and A's current FP64 values.  Every row appears in the first and the last
-// part (they initialise the running sums and run the epilogue); a middle part
-// holds only the rows with entries in its column segment.
+// (Re)build the K column-segment parts of an orientation from its
+// device-order CSR and A's current FP64 values.
 int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32) {
     o.free_parts();
-    const Natural& nat = o.nat;
-    const int32_t rows = nat.rows, cols = nat.cols, nnz = nat.nnz;
+    const int32_t rows = o.nat.rows, cols = o.nat.cols, nnz = o.nat.nnz;
     if (K < 1 || cols < 2 * K || nnz == 0) K = 1;
     const int32_t W = (K == 1) ? std::max<int32_t>(cols, 1) : (cols + K - 1) / K;
-    const int64_t r1 = std::max<int32_t>(rows, 1);
-    std::vector<int32_t> hb(static_cast<size_t>(K + 1) * rows);
-    int32_t* bounds = nullptr;
+    const int64_t r1 = std::max<int32_t>(rows, 1), nz = std::max<int32_t>(nnz, 1);
     auto fail_free = [&](hipError_t e, const char* what) {
-        (void)hipFree(bounds);
         o.free_parts();
         return hip_fail(e, what);
     };
-    hipError_t e = hipMalloc(&bounds, sizeof(int32_t) * (int64_t)(K + 1) * r1);
-    if (e != hipSuccess) return fail_free(e, "part bounds");
-    if (rows > 0) {
-        k_seg_bounds<<<launch_blocks(rows), kBlock, 0, ctx->stream>>>(rows, K, W, nat.ptr, nat.idx, bounds);
-        e = check_launch();
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hb.data(), bounds, sizeof(int32_t) * hb.size(), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) return fail_free(e, "part bounds");
-    }
-    // host: class order of each part, position pointers
-    std::vector<int32_t> all_list, all_ptr;
-    all_list.reserve(static_cast<size_t>(K) * rows);
-    all_ptr.reserve(static_cast<size_t>(K) * (rows + 1));
-    std::vector<int64_t> list_off(K), ptr_off(K), nnz_off(K + 1, 0);
-    o.parts.assign(static_cast<size_t>(K), Part{});
-    std::vector<int8_t> cls(static_cast<size_t>(rows));
-    for (int k = 0; k < K; ++k) {
-        Part& p = o.parts[static_cast<size_t>(k)];
-        const int32_t* lo_b = hb.data() + static_cast<size_t>(k) * rows;
-        const int32_t* hi_b = lo_b + rows;
-        const bool middle = seg_mode(k, K) == kMiddle;
-        int32_t cnt[kClasses] = {};
-        p.max_len = 0;
-        for (int32_t r = 0; r < rows; ++r) {
-            const int32_t len = hi_b[r] - lo_b[r];
-            if (middle && len == 0) {
-                cls[r] = -1;
-                continue;
-            }
-            p.max_len = std::max<int64_t>(p.max_len, len);
-            cls[r] = static_cast<int8_t>(row_class(len));
-            ++cnt[cls[r]];
-        }
-        int32_t off = 0;
-        for (int c = 0; c < kClasses; ++c) {
-            p.class_count[c] = cnt[c];
-            p.class_offset[c] = off;
-            p.lanes[c] = kDefaultLanes[c];
-            off += cnt[c];
-        }
-        p.rows = off;
-        p.cols = cols;
-        list_off[k] = static_cast<int64_t>(all_list.size());
-        ptr_off[k] = static_cast<int64_t>(all_ptr.size());
-        all_list.resize(all_list.size() + off);
-        int32_t* lst = all_list.data() + list_off[k];
-        int32_t cur[kClasses];
-        for (int c = 0; c < kClasses; ++c) cur[c] = p.class_offset[c];
-        for (int32_t r = 0; r < rows; ++r)
-            if (cls[r] >= 0) lst[cur[cls[r]]++] = r;
-        all_ptr.push_back(0);
-        int32_t run = 0;
-        for (int32_t i = 0; i < off; ++i) {
-            run += hi_b[lst[i]] - lo_b[lst[i]];
-            all_ptr.push_back(run);
-        }
-        p.nnz = run;
-        nnz_off[k + 1] = nnz_off[k] + run;
-    }
-    if (nnz_off[K] != nnz) return fail_free(hipErrorUnknown, "part fill: entry count mismatch");
-    const int64_t nz = std::max<int32_t>(nnz, 1);
-    e = hipMalloc(&o.p_list, sizeof(int32_t) * std::max<size_t>(all_list.size(), 1));
-    if (e == hipSuccess) e = hipMalloc(&o.p_ptr, sizeof(int32_t) * all_ptr.size());
+    hipError_t e = hipMalloc(&o.p_ptr, sizeof(int32_t) * (int64_t)K * (rows + 1));
     if (e == hipSuccess) e = hipMalloc(&o.p_idx, sizeof(int32_t) * nz);
     if (e == hipSuccess) e = hipMalloc(&o.p_src, sizeof(int32_t) * nz);
     if (e == hipSuccess) e = hipMalloc(&o.p_val64, sizeof(double) * nz);
@@ -376,25 +382,47 @@ int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f3
     if (e == hipSuccess) e = hipMalloc(&o.acc32, sizeof(float) * r1);
     if (e == hipSuccess) e = hipMalloc(&o.chk_acc, sizeof(double) * 2 * r1);
     if (e != hipSuccess) return fail_free(e, "part buffers");
-    if (!all_list.empty())
-        e = hipMemcpyAsync(o.p_list, all_list.data(), sizeof(int32_t) * all_list.size(), hipMemcpyHostToDevice,
-                           ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(o.p_ptr, all_ptr.data(), sizeof(int32_t) * all_ptr.size(), hipMemcpyHostToDevice,
-                           ctx->stream);
-    if (e != hipSuccess) return fail_free(e, "part upload");
+    std::vector<int64_t> base(static_cast<size_t>(K) + 1, 0);
+    if (rows > 0) {
+        k_seg_lens<<<launch_blocks(rows), kBlock, 0, ctx->stream>>>(rows, K, W, o.d_ptr, o.d_idx, o.p_ptr);
+        e = check_launch();
+        if (e != hipSuccess) return fail_free(e, "part lengths");
+        for (int k = 0; k < K; ++k) {
+            const int rc = inclusive_scan(ctx, o.p_ptr + (int64_t)k * (rows + 1), rows + 1);
+            if (rc != SOV_OK) {
+                o.free_parts();
+                return rc;
+            }
+            int32_t nk = 0;
+            e = hipMemcpy(&nk, o.p_ptr + (int64_t)k * (rows + 1) + rows, sizeof(int32_t), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return fail_free(e, "part sizes");
+            base[static_cast<size_t>(k) + 1] = base[static_cast<size_t>(k)] + nk;
+        }
+    } else {
+        e = hipMemsetAsync(o.p_ptr, 0, sizeof(int32_t) * K, ctx->stream);
+        if (e != hipSuccess) return fail_free(e, "part pointers");
+    }
+    if (base[static_cast<size_t>(K)] != nnz) return fail_free(hipErrorUnknown, "part fill: entry count mismatch");
+    o.parts.assign(static_cast<size_t>(K), Part{});
     for (int k = 0; k < K; ++k) {
         Part& p = o.parts[static_cast<size_t>(k)];
-        p.list = o.p_list + list_off[k];
-        p.ptr = o.p_ptr + ptr_off[k];
-        p.idx = o.p_idx + nnz_off[k];
-        p.src = o.p_src + nnz_off[k];
-        p.val64 = o.p_val64 + nnz_off[k];
-        p.val32 = o.p_val32 ? o.p_val32 + nnz_off[k] : nullptr;
-        if (p.rows > 0 && p.nnz > 0)
-            k_part_fill<<<launch_blocks(p.rows), kBlock, 0, ctx->stream>>>(
-                p.rows, p.list, p.ptr, bounds + (int64_t)k * rows, bounds + (int64_t)(k + 1) * rows, nat.idx,
-                nat.src, o.p_idx + nnz_off[k], o.p_src + nnz_off[k]);
+        const int64_t b = base[static_cast<size_t>(k)];
+        p.rows = rows;
+        p.cols = cols;
+        p.nnz = static_cast<int32_t>(base[static_cast<size_t>(k) + 1] - b);
+        p.ptr = o.p_ptr + (int64_t)k * (rows + 1);
+        p.idx = o.p_idx + b;
+        p.src = o.p_src + b;
+        p.val64 = o.p_val64 + b;
+        p.val32 = o.p_val32 ? o.p_val32 + b : nullptr;
+        for (int c = 0; c < kClasses; ++c) {
+            p.class_count[c] = o.class_count[c];
+            p.class_offset[c] = o.class_offset[c];
+            p.lanes[c] = kDefaultLanes[c];
+        }
+        if (rows > 0 && p.nnz > 0)
+            k_seg_fill<<<launch_blocks(rows), kBlock, 0, ctx->stream>>>(rows, k, W, o.d_ptr, o.d_idx, o.d_src, p.ptr,
+                                                                        o.p_idx + b, o.p_src + b);
     }
     if (nnz > 0) {
         k_gather_vals<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, o.p_src, h->a.nat.val64, o.p_val64);
@@ -404,10 +432,33 @@ int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f3
     }
     e = check_launch();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    (void)hipFree(bounds);
-    bounds = nullptr;
     if (e != hipSuccess) return fail_free(e, "part fill");
     o.seg_width = K > 1 ? W : 0;
+    return SOV_OK;
+}
+
+// dst[p] = src[perm[p]] (natural -> device) or dst[perm[p]] = src[p].
+template <typename T>
+__global__ void k_permute(int32_t n, const int32_t* __restrict__ perm, int to_natural, const T* __restrict__ src,
+                          T* __restrict__ dst) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        if (to_natural) dst[perm[p]] = src[p];
+        else dst[p] = src[perm[p]];
+    }
+}
+
+static int permute(sov_hip_ctx* ctx, const Orient& o, int to_natural, int prec, const void* src, void* dst) {
+    const int32_t n = o.nat.rows;
+    if (n == 0) return SOV_OK;
+    if (prec == SOV_PREC_F64)
+        k_permute<double><<<launch_blocks(n), kBlock, 0, ctx->stream>>>(n, o.perm, to_natural,
+                                                                        static_cast<const double*>(src),
+                                                                        static_cast<double*>(dst));
+    else
+        k_permute<float><<<launch_blocks(n), kBlock, 0, ctx->stream>>>(n, o.perm, to_natural,
+                                                                       static_cast<const float*>(src),
+                                                                       static_cast<float*>(dst));
+    SOV_HIP_TRY(check_launch());
     return SOV_OK;
 }
 
@@ -444,10 +495,9 @@ struct SpmvOp {
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        const int pos = pos_of<G>(v, lane);
-        T sum = row_dot<G, T>(ptr, idx, val, x, pos, lane);
-        if (pos >= 0 && lane == 0) {
-            const int row = s.list[pos];
+        const int row = row_of<G>(v, lane);
+        T sum = row_dot<G, T>(ptr, idx, val, x, row, lane);
+        if (row >= 0 && lane == 0) {
             if (mode == kFirst) { acc[row] = sum; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + sum; return; }
             if (mode == kLast) sum = acc[row] + sum;
@@ -489,6 +539,20 @@ static int spmv_orient(sov_hip_ctx* ctx, const Orient& o, int prec, double alpha
         if (rc != SOV_OK) return rc;
     }
     return SOV_OK;
+}
+
+// SpMV on natural-order vectors: stage x (and y when beta != 0) in device
+// order, multiply, scatter y back.
+static int spmv_natural(sov_hip_ctx* ctx, sov_hip_csr* h, int transpose, int prec, double alpha, const void* x,
+                        double beta, void* y) {
+    const Orient& o = transpose ? h->t : h->a;       // rows = outputs
+    const Orient& g = transpose ? h->a : h->t;       // rows = the gathered dimension
+    int rc = SOV_OK;
+    if (g.nat.rows > 0) rc = permute(ctx, g, 0, prec, x, h->vx);
+    if (rc == SOV_OK && beta != 0.0 && o.nat.rows > 0) rc = permute(ctx, o, 0, prec, y, h->vy);
+    if (rc == SOV_OK) rc = spmv_orient(ctx, o, prec, alpha, h->vx, beta, h->vy);
+    if (rc == SOV_OK && o.nat.rows > 0) rc = permute(ctx, o, 1, prec, h->vy, y);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -753,7 +817,14 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
     T.idx = h->t_idx;
     T.val64 = h->t_val64;
     T.src = h->t_src;
-    rc = build_parts(ctx, h, h->a, 1, false);
+    {
+        const int64_t len = std::max<int64_t>(std::max<int64_t>(m, n), 1);
+        hipError_t e2 = hipMalloc(&h->vx, sizeof(double) * len);
+        if (e2 == hipSuccess) e2 = hipMalloc(&h->vy, sizeof(double) * len);
+        if (e2 != hipSuccess) return cleanup(hip_fail(e2, "csr staging"));
+    }
+    rc = build_order(ctx, h);
+    if (rc == SOV_OK) rc = build_parts(ctx, h, h->a, 1, false);
     if (rc == SOV_OK) rc = build_parts(ctx, h, h->t, 1, false);
     if (rc == SOV_OK) rc = ensure_partials(h);
     if (rc != SOV_OK) return cleanup(rc);
@@ -767,6 +838,10 @@ void sov_hip_csr_destroy(sov_hip_csr* h) {
     (void)hipDeviceSynchronize();
     h->a.free_parts();
     h->t.free_parts();
+    h->a.free_order();
+    h->t.free_order();
+    (void)hipFree(h->vx);
+    (void)hipFree(h->vy);
     if (h->owns_a_val64) (void)hipFree(h->a.nat.val64);
     (void)hipFree(h->t_ptr);
     (void)hipFree(h->t_idx);
@@ -875,8 +950,43 @@ int sov_hip_spmv(sov_hip_ctx* ctx, const sov_hip_csr* h, int transpose, int prec
     const Orient& o = transpose ? h->t : h->a;
     if (o.nat.rows > 0 && y == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv: y is null");
     if (h->nnz > 0 && x == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv: x is null");
+    if (prec == SOV_PREC_F32 && !h->f32 && h->nnz > 0)
+        return fail(SOV_INVALID_ARGUMENT, "FP32 SpMV needs sov_hip_csr_make_f32 first");
+    DeviceGuard2 g(ctx->device);
+    return spmv_natural(ctx, const_cast<sov_hip_csr*>(h), transpose, prec, alpha, x, beta, y);
+}
+
+int sov_hip_spmv_device(sov_hip_ctx* ctx, const sov_hip_csr* h, int transpose, int prec, double alpha,
+                        const void* x, double beta, void* y) {
+    if (ctx == nullptr || h == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv_device args");
+    if (prec != SOV_PREC_F64 && prec != SOV_PREC_F32) return fail(SOV_INVALID_ARGUMENT, "spmv_device: precision");
+    const Orient& o = transpose ? h->t : h->a;
+    if (o.nat.rows > 0 && y == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv_device: y is null");
+    if (h->nnz > 0 && x == nullptr) return fail(SOV_INVALID_ARGUMENT, "spmv_device: x is null");
     DeviceGuard2 g(ctx->device);
     return spmv_orient(ctx, o, prec, alpha, x, beta, y);
+}
+
+int sov_hip_csr_order(sov_hip_ctx* ctx, const sov_hip_csr* h, int which, int32_t* perm) {
+    if (ctx == nullptr || h == nullptr || perm == nullptr || (which != 0 && which != 1))
+        return fail(SOV_INVALID_ARGUMENT, "csr_order args");
+    DeviceGuard2 g(ctx->device);
+    const Orient& o = which == 0 ? h->a : h->t;
+    if (o.nat.rows == 0) return SOV_OK;
+    SOV_HIP_TRY(hipMemcpyAsync(perm, o.perm, sizeof(int32_t) * o.nat.rows, hipMemcpyDeviceToHost, ctx->stream));
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SOV_OK;
+}
+
+int sov_hip_csr_permute(sov_hip_ctx* ctx, const sov_hip_csr* h, int which, int to_natural, int prec,
+                        const void* src, void* dst) {
+    if (ctx == nullptr || h == nullptr || (which != 0 && which != 1) || (prec != SOV_PREC_F64 && prec != SOV_PREC_F32))
+        return fail(SOV_INVALID_ARGUMENT, "csr_permute args");
+    const Orient& o = which == 0 ? h->a : h->t;
+    if (o.nat.rows > 0 && (src == nullptr || dst == nullptr || src == dst))
+        return fail(SOV_INVALID_ARGUMENT, "csr_permute: null or aliased vectors");
+    DeviceGuard2 g(ctx->device);
+    return permute(ctx, o, to_natural, prec, src, dst);
 }
 
 int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {

@@ -35,9 +35,20 @@ constexpr int kClassMax[kClasses] = {2, 4, 8, 16, 32, 64, 256, 0x7fffffff};
 // Default lanes per row for each class.
 constexpr int kDefaultLanes[kClasses] = {1, 2, 4, 4, 8, 16, 32, 64};
 
+// Device order.  Every vector the PDHG kernels touch is stored permuted:
+// position p of the row dimension holds natural row perm_r[p], position q of
+// the column dimension natural column perm_c[q], where each permutation
+// groups the rows of its orientation (rows of A for perm_r, rows of A^T =
+// columns of A for perm_c) by length class, stably.  Row p of an orientation
+// therefore writes element p of its output, the rows of one class are one
+// contiguous range, and every per-row access of a kernel is coalesced.  The
+// stored matrix indices are device positions of the gathered dimension, so
+// the gathers read the permuted vectors directly.  (For skewed column counts
+// the heavy columns also end up adjacent, which keeps their gathers in L2.)
+
 // Schedule passed by value to the row kernels.  Rows of class c are
-// list[offset[c] .. offset[c] + count[c]); they are processed by G[c] lanes
-// per row, kBlock / G[c] rows per workgroup, in workgroups
+// [offset[c], offset[c] + count[c]); they are processed by G[c] lanes per
+// row, kBlock / G[c] rows per workgroup, in workgroups
 // [block_begin[c], block_begin[c+1]).  Kernels read the per-class values with
 // pick() (unrolled selects on kernel-argument scalars), never by a runtime
 // array index, so nothing spills to scratch.
@@ -46,32 +57,27 @@ struct Sched {
     int32_t count[kClasses];
     int32_t offset[kClasses];
     int32_t block_begin[kClasses + 1];
-    const int32_t* list;
 };
 
-// One CSR "part" of an orientation: all its columns, or one column segment
-// (entries whose column lies in [k W, (k+1) W)).  A part stores its rows in
-// class order: position i holds row list[i], ptr/idx/values are indexed by
-// position, so each length class streams one contiguous range of the arrays
-// (rows of different classes never share a cache line of the matrix).
+// One CSR "part" of an orientation, in device order: all its columns, or one
+// column segment (entries whose device column lies in [k W, (k+1) W)).  Every
+// part holds every row (possibly empty), so all parts share the class ranges.
 struct Part {
     int32_t rows = 0, cols = 0, nnz = 0;
-    const int32_t* list = nullptr;  // rows: original row of each position
-    const int32_t* ptr = nullptr;   // rows + 1, by position
-    const int32_t* idx = nullptr;   // nnz, global column indices, ascending per row
+    const int32_t* ptr = nullptr;   // rows + 1
+    const int32_t* idx = nullptr;   // nnz, device positions in the gathered dimension
     double* val64 = nullptr;        // nnz
     float* val32 = nullptr;         // nnz, optional
-    const int32_t* src = nullptr;   // nnz: position of each entry in A's value array
+    const int32_t* src = nullptr;   // nnz: position of each entry in A's natural value array
     int32_t class_count[kClasses] = {};
     int32_t class_offset[kClasses] = {};
     int32_t lanes[kClasses] = {};
-    int64_t max_len = 0;
     Sched sched() const;
     int blocks() const;
 };
 
-// The natural (row-ordered) CSR of an orientation: the source the parts are
-// built from, and what the scaling kernels walk row by row.
+// The natural (row-ordered, caller-indexed) CSR of an orientation: what the
+// scaling kernels walk row by row and what device order is derived from.
 struct Natural {
     int32_t rows = 0, cols = 0, nnz = 0;
     const int32_t* ptr = nullptr;
@@ -80,15 +86,22 @@ struct Natural {
     const int32_t* src = nullptr;   // null for A itself
 };
 
-// An orientation (A or A^T): the natural CSR and its class-ordered parts
-// (one part, or one per column segment when the gathered vector is larger
-// than a comfortable share of an XCD's L2).
+// An orientation (A or A^T): its natural CSR, its device order, the
+// device-order CSR the parts are cut from, and the parts (one, or one per
+// column segment when the gathered vector is larger than a comfortable share
+// of an XCD's L2).
 struct Orient {
     Natural nat;
+    int32_t* perm = nullptr;        // rows: natural row at each device position
+    int32_t* inv = nullptr;         // rows: device position of each natural row
+    int32_t* d_ptr = nullptr;       // rows + 1: device-order CSR
+    int32_t* d_idx = nullptr;       // nnz (device positions of the gathered dimension)
+    int32_t* d_src = nullptr;       // nnz
+    int32_t class_count[kClasses] = {};
+    int32_t class_offset[kClasses] = {};
     std::vector<Part> parts;
     int32_t seg_width = 0;          // 0 when unsegmented
     // owned storage of the parts
-    int32_t* p_list = nullptr;      // nparts * rows
     int32_t* p_ptr = nullptr;       // nparts * (rows + 1)
     int32_t* p_idx = nullptr;       // nnz
     int32_t* p_src = nullptr;       // nnz
@@ -102,6 +115,7 @@ struct Orient {
     Part& part(int k) { return parts[static_cast<size_t>(k)]; }
     int max_blocks() const;
     void free_parts();
+    void free_order();
 };
 
 // How a part's launch combines with the others of its orientation.
@@ -145,6 +159,8 @@ struct sov_hip_csr {
     bool owns_a_val64 = false;
     bool f32 = false;           // parts carry FP32 value images (sov_hip_csr_make_f32)
     double* partials = nullptr; // owned: kCheckSlots * (largest part blocks of both orientations + 64) FP64
+    void* vx = nullptr;         // owned: max(m, n) FP64, device-order staging of natural-order SpMV operands
+    void* vy = nullptr;
     int64_t partial_blocks = 0;
     int device = 0;
 };
@@ -251,26 +267,26 @@ __device__ __forceinline__ ClassView class_view(const Sched& s) {
     return v;
 }
 
-// The position (in the part's class order) of this thread's group of G
-// lanes, -1 past the class, and the lane within the group.
+// The device-order row of this thread's group of G lanes (-1 past its
+// class) and the lane within the group.
 template <int G>
-__device__ __forceinline__ int pos_of(const ClassView& v, int& lane) {
+__device__ __forceinline__ int row_of(const ClassView& v, int& lane) {
     constexpr int RPB = kBlock / G;
     const int local = ((int)blockIdx.x - v.begin) * RPB + (int)threadIdx.x / G;
     lane = (int)threadIdx.x % G;
     return (local < v.count) ? v.offset + local : -1;
 }
 
-// Dot product of the row at position pos with a gathered vector, G lanes per
+// Dot product of row r with a gathered vector, G lanes per
 // row.  Every lane of the group returns the full sum.  Accumulation type Acc.
 template <int G, typename Acc, typename V, typename X>
 __device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
                                        const int32_t* __restrict__ idx,
                                        const V* __restrict__ val, const X* __restrict__ x,
-                                       int pos, int lane) {
+                                       int r, int lane) {
     Acc acc = Acc(0);
-    if (pos >= 0) {
-        const int b = ptr[pos], e = ptr[pos + 1];
+    if (r >= 0) {
+        const int b = ptr[r], e = ptr[r + 1];
 #pragma unroll 4
         for (int k = b + lane; k < e; k += G) acc += Acc(val[k]) * Acc(x[idx[k]]);
     }
@@ -282,11 +298,11 @@ template <int G, typename V, typename X>
 __device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
                                          const int32_t* __restrict__ idx,
                                          const V* __restrict__ val, const X* __restrict__ x1,
-                                         const X* __restrict__ x2, double s2, int pos, int lane,
+                                         const X* __restrict__ x2, double s2, int r, int lane,
                                          double& d1, double& d2) {
     double a1 = 0.0, a2 = 0.0;
-    if (pos >= 0) {
-        const int b = ptr[pos], e = ptr[pos + 1];
+    if (r >= 0) {
+        const int b = ptr[r], e = ptr[r + 1];
         for (int k = b + lane; k < e; k += G) {
             const double v = double(val[k]);
             const int j = idx[k];

@@ -50,10 +50,9 @@ struct PrimalOp {
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        const int pos = pos_of<G>(v, lane);
-        T aty = row_dot<G, T>(ptr, idx, val, y, pos, lane);
-        if (pos >= 0 && lane == 0) {
-            const int row = s.list[pos];
+        const int row = row_of<G>(v, lane);
+        T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
+        if (row >= 0 && lane == 0) {
             if (mode == kFirst) { acc[row] = aty; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + aty; return; }
             if (mode == kLast) aty = acc[row] + aty;
@@ -91,10 +90,9 @@ struct DualOp {
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        const int pos = pos_of<G>(v, lane);
-        T ax = row_dot<G, T>(ptr, idx, val, xbar, pos, lane);
-        if (pos >= 0 && lane == 0) {
-            const int row = s.list[pos];
+        const int row = row_of<G>(v, lane);
+        T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
+        if (row >= 0 && lane == 0) {
             if (mode == kFirst) { acc[row] = ax; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
             if (mode == kLast) ax = acc[row] + ax;
@@ -196,11 +194,10 @@ struct CheckRowOp {
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
         int lane;
-        const int pos = pos_of<G>(v, lane);
+        const int i = row_of<G>(v, lane);
         double ac, aa;
-        row_dot2<G>(ptr, idx, val, x, xsum, inv, pos, lane, ac, aa);
-        if (pos >= 0 && lane == 0) {
-            const int i = s.list[pos];
+        row_dot2<G>(ptr, idx, val, x, xsum, inv, i, lane, ac, aa);
+        if (i >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)i, ac, aa)) return;
             if (act_avg != nullptr) act_avg[i] = aa;
             const double yc = double(y[i]), ya = double(ysum[i]) * inv;
@@ -254,11 +251,10 @@ struct CheckColOp {
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
         int lane;
-        const int pos = pos_of<G>(v, lane);
+        const int j = row_of<G>(v, lane);
         double tc, ta;
-        row_dot2<G>(ptr, idx, val, y, ysum, inv, pos, lane, tc, ta);
-        if (pos >= 0 && lane == 0) {
-            const int j = s.list[pos];
+        row_dot2<G>(ptr, idx, val, y, ysum, inv, j, lane, tc, ta);
+        if (j >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)j, tc, ta)) return;
             const double xc = double(x[j]), xa = double(xsum[j]) * inv;
             const double cj = double(c_s[j]);

@@ -34,6 +34,8 @@ def lib():
     L.sov_hip_nrm2_f64.argtypes = [_v, _i64, _v, C.POINTER(d)]
     L.sov_hip_amax_f64.argtypes = [_v, _i64, _v, C.POINTER(d)]
     L.sov_hip_project_box_f64.argtypes = [_v, _i64, _v, _v, _v]
+    L.sov_hip_convert_f64_f32.argtypes = [_v, _i64, _v, _v]
+    L.sov_hip_convert_f32_f64.argtypes = [_v, _i64, _v, _v]
     L.sov_hip_csr_create.argtypes = [_v, C.c_int32, C.c_int32, C.c_int32, _v, _v, _v, C.c_int, C.POINTER(_v)]
     L.sov_hip_csr_destroy.argtypes = [_v]
     L.sov_hip_csr_make_f32.argtypes = [_v, _v]
@@ -45,6 +47,9 @@ def lib():
     L.sov_hip_csr_set_segments.argtypes = [_v, _v, C.c_int, C.c_int]
     L.sov_hip_csr_get_segments.argtypes = [_v, C.c_int, C.POINTER(C.c_int)]
     L.sov_hip_spmv.argtypes = [_v, _v, C.c_int, C.c_int, d, _v, d, _v]
+    L.sov_hip_spmv_device.argtypes = [_v, _v, C.c_int, C.c_int, d, _v, d, _v]
+    L.sov_hip_csr_order.argtypes = [_v, _v, C.c_int, _v]
+    L.sov_hip_csr_permute.argtypes = [_v, _v, C.c_int, C.c_int, C.c_int, _v, _v]
     L.sov_hip_scale_ruiz_sweep.argtypes = [_v, _v, _v, _v, C.POINTER(d)]
     L.sov_hip_scale_pock_chambolle.argtypes = [_v, _v, d, _v, _v]
     L.sov_hip_timer_start.argtypes = [_v]
@@ -124,6 +129,14 @@ class Csr:
 
     def destroy(self):
         self.ctx.L.sov_hip_csr_destroy(self.h)
+
+    def order(self, which):
+        """Device order of the rows (which = 0) or columns (1): perm[p] is the
+        natural index stored at device position p."""
+        k = self.m if which == 0 else self.n
+        perm = np.zeros(max(k, 1), np.int32)
+        check(self.ctx.L.sov_hip_csr_order(self.ctx.h, self.h, which, perm.ctypes.data))
+        return perm[:k]
 
     def spmv(self, x, transpose=False, alpha=1.0, beta=0.0, y=None, prec=0):
         dt = np.float64 if prec == 0 else np.float32

@@ -501,6 +501,22 @@ struct CsrHandle {
     ~CsrHandle() { sov_hip_csr_destroy(h); }
 };
 
+/// Host copy of the handle's device order (sov_hip_csr_order): every vector
+/// the fused kernels touch is stored with element p = natural index perm[p].
+struct DeviceOrder {
+    std::vector<int32_t> row, col;
+    static std::vector<Real> to_dev(std::span<const Real> v, const std::vector<int32_t>& perm) {
+        std::vector<Real> out(v.size());
+        for (std::size_t p = 0; p < out.size(); ++p) out[p] = v[static_cast<std::size_t>(perm[p])];
+        return out;
+    }
+    static std::vector<Real> to_nat(std::span<const Real> v, const std::vector<int32_t>& perm) {
+        std::vector<Real> out(v.size());
+        for (std::size_t p = 0; p < out.size(); ++p) out[static_cast<std::size_t>(perm[p])] = v[p];
+        return out;
+    }
+};
+
 struct GraphHandle {
     sov_hip_graph* g = nullptr;
     ~GraphHandle() { sov_hip_graph_destroy(g); }
@@ -590,6 +606,11 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                                   d_rp.as<int32_t>(), d_ci.as<int32_t>(), d_v.as<double>(), SOV_CSR_OWN_VALUES, &csr.h),
                "csr_create");
         d_v.reset();   // the handle owns a copy of the values
+        DeviceOrder ord;
+        ord.row.resize(m);
+        ord.col.resize(n);
+        hip_ok(sov_hip_csr_order(ctx, csr.h, 0, ord.row.data()), "order");
+        hip_ok(sov_hip_csr_order(ctx, csr.h, 1, ord.col.data()), "order");
 
         // -- kernel (d): Ruiz + Pock-Chambolle on the device (reference scaling.cpp:19-104)
         DevMem d_rs(ctx, std::max<std::size_t>(m, 1) * 8), d_cs(ctx, std::max<std::size_t>(n, 1) * 8);
@@ -628,19 +649,29 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             rlo_s[i] = is_finite_bound(lo) ? lo * rs[i] : lo;
             rhi_s[i] = is_finite_bound(hi) ? hi * rs[i] : hi;
         }
-        // original-space arrays for the device checks (FP64)
+        // everything below lives in device order
+        c_s = DeviceOrder::to_dev(c_s, ord.col);
+        clo_s = DeviceOrder::to_dev(clo_s, ord.col);
+        chi_s = DeviceOrder::to_dev(chi_s, ord.col);
+        rlo_s = DeviceOrder::to_dev(rlo_s, ord.row);
+        rhi_s = DeviceOrder::to_dev(rhi_s, ord.row);
+        // original-space arrays and factors for the device checks (FP64)
         DevMem d_olo_r(ctx, std::max<std::size_t>(m, 1) * 8), d_ohi_r(ctx, std::max<std::size_t>(m, 1) * 8),
             d_olo_c(ctx, std::max<std::size_t>(n, 1) * 8), d_ohi_c(ctx, std::max<std::size_t>(n, 1) * 8),
-            d_oc(ctx, std::max<std::size_t>(n, 1) * 8), d_act(ctx, std::max<std::size_t>(m, 1) * 8);
-        if (m) {
-            hip_ok(sov_hip_memcpy_h2d(ctx, d_olo_r.p, problem.row_lower().data(), m * 8), "upload");
-            hip_ok(sov_hip_memcpy_h2d(ctx, d_ohi_r.p, problem.row_upper().data(), m * 8), "upload");
-        }
-        if (n) {
-            hip_ok(sov_hip_memcpy_h2d(ctx, d_olo_c.p, problem.col_lower().data(), n * 8), "upload");
-            hip_ok(sov_hip_memcpy_h2d(ctx, d_ohi_c.p, problem.col_upper().data(), n * 8), "upload");
-            hip_ok(sov_hip_memcpy_h2d(ctx, d_oc.p, problem.objective().data(), n * 8), "upload");
-        }
+            d_oc(ctx, std::max<std::size_t>(n, 1) * 8), d_act(ctx, std::max<std::size_t>(m, 1) * 8),
+            d_rs_dev(ctx, std::max<std::size_t>(m, 1) * 8), d_cs_dev(ctx, std::max<std::size_t>(n, 1) * 8);
+        auto upload_dev = [&](const DevMem& d, std::span<const Real> v, const std::vector<int32_t>& perm) {
+            if (v.empty()) return;
+            const std::vector<Real> pv = DeviceOrder::to_dev(v, perm);
+            hip_ok(sov_hip_memcpy_h2d(ctx, d.p, pv.data(), pv.size() * 8), "upload");
+        };
+        upload_dev(d_olo_r, problem.row_lower(), ord.row);
+        upload_dev(d_ohi_r, problem.row_upper(), ord.row);
+        upload_dev(d_rs_dev, rs, ord.row);
+        upload_dev(d_olo_c, problem.col_lower(), ord.col);
+        upload_dev(d_ohi_c, problem.col_upper(), ord.col);
+        upload_dev(d_oc, problem.objective(), ord.col);
+        upload_dev(d_cs_dev, cs, ord.col);
 
         // -- step size: device power iteration in FP64 (reference pdhg.cpp:48-79)
         Real est = 0.0;
@@ -652,13 +683,13 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             if (m) hip_ok(sov_hip_memcpy_h2d(ctx, v.p, ones_m.data(), m * 8), "upload");
             const auto mm = static_cast<int64_t>(m), nn = static_cast<int64_t>(n);
             for (int it = 0; it < o.power_iterations; ++it) {
-                hip_ok(sov_hip_spmv(ctx, csr.h, 0, SOV_PREC_F64, -1.0, u.p, 0.0, w.p), "power spmv");
+                hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, -1.0, u.p, 0.0, w.p), "power spmv");
                 hip_ok(sov_hip_axpy_f64(ctx, mm, 1.0, v.as<double>(), w.as<double>()), "power axpy");
                 double wn = 0.0;
                 hip_ok(sov_hip_nrm2_f64(ctx, mm, w.as<double>(), &wn), "power nrm2");
                 if (wn < 1e-300) break;
                 hip_ok(sov_hip_scal_f64(ctx, mm, 1.0 / wn, w.as<double>()), "power scal");
-                hip_ok(sov_hip_spmv(ctx, csr.h, 1, SOV_PREC_F64, -1.0, w.p, 0.0, u.p), "power spmv_t");
+                hip_ok(sov_hip_spmv_device(ctx, csr.h, 1, SOV_PREC_F64, -1.0, w.p, 0.0, u.p), "power spmv_t");
                 hip_ok(sov_hip_memcpy_d2d(ctx, v.p, w.p, m * 8), "power copy");
                 double un = 0.0, vn = 0.0;
                 hip_ok(sov_hip_nrm2_f64(ctx, nn, u.as<double>(), &un), "power nrm2");
@@ -698,6 +729,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 if (o.warm_start_dual.size() != m) throw Error(Status::DimensionMismatch, "warm_start_dual length");
                 std::vector<Real> ys(m);
                 for (std::size_t i = 0; i < m; ++i) ys[i] = sign * o.warm_start_dual[i] / rs[i];
+                ys = DeviceOrder::to_dev(ys, ord.row);
                 wy = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
                 if (m) hip_ok(sov_hip_memcpy_h2d(ctx, wy.p, ys.data(), m * 8), "upload warm start");
                 ysrc = wy.as<double>();
@@ -722,8 +754,8 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         ca.tol_orig = std::max(o.tolerance, Real{1e-6});
         ca.obj_sign = sign;
         ca.obj_const = problem.objective_constant();
-        ca.row_scale = d_rs.as<double>();
-        ca.col_scale = d_cs.as<double>();
+        ca.row_scale = d_rs_dev.as<double>();
+        ca.col_scale = d_cs_dev.as<double>();
         ca.orig_row_lo = d_olo_r.as<double>();
         ca.orig_row_hi = d_ohi_r.as<double>();
         ca.orig_col_lo = d_olo_c.as<double>();
@@ -766,7 +798,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         long long since = 0;
         Real res_at_restart = kInfinity, prev_cand = kInfinity;
         bool converged = false;
-        std::vector<Real> best_x, best_y;   // scaled, internal sense, FP64
+        std::vector<Real> best_x, best_y;   // scaled, internal sense, FP64, device order
         const auto t_loop = Clock::now();
         result.setup_seconds = seconds_since(t0);
         long long polish_start = -1;
@@ -821,7 +853,8 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             };
             if (o.checkpoint) {
                 auto [cx, cy] = candidate();
-                auto [ux, uy] = unscale_point(sign, scaling, cx, cy);
+                auto [ux, uy] = unscale_point(sign, scaling, DeviceOrder::to_nat(cx, ord.col),
+                                              DeviceOrder::to_nat(cy, ord.row));
                 o.checkpoint(result.iterations - 1, ux, uy, ores);
             }
             const bool stop = o.termination == PdhgTermination::Reference ? (ores < o.tolerance)
@@ -848,7 +881,8 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                     DevMem hx(ctx, std::max<std::size_t>(n, 1) * 8), hy(ctx, std::max<std::size_t>(m, 1) * 8);
                     if (n) hip_ok(sov_hip_memcpy_h2d(ctx, hx.p, cx.data(), n * 8), "upload");
                     if (m) hip_ok(sov_hip_memcpy_h2d(ctx, hy.p, cy.data(), m * 8), "upload");
-                    hip_ok(sov_hip_spmv(ctx, csr.h, 0, SOV_PREC_F64, 1.0, hx.p, 0.0, d_act.p), "polish activity");
+                    hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, 1.0, hx.p, 0.0, d_act.p),
+                           "polish activity");
                     hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, hx.as<double>(), hy.as<double>(), d_act.as<double>()),
                            "polish set_point");
                     hip_ok(sov_hip_sync(ctx), "sync");
@@ -903,7 +937,8 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         else if (result.status == PdhgStatus::NotSolved) result.status = PdhgStatus::IterationLimit;
         if (polish_start >= 0) result.fp64_polish_iterations = result.iterations - polish_start;
         result.primal_weight = omega;
-        auto [fx, fy] = unscale_point(sign, scaling, best_x, best_y);
+        auto [fx, fy] = unscale_point(sign, scaling, DeviceOrder::to_nat(best_x, ord.col),
+                                      DeviceOrder::to_nat(best_y, ord.row));
         finish_result(problem, o, std::move(fx), std::move(fy), result);
     } catch (const Error& e) {
         result.status = PdhgStatus::NumericalFailure;

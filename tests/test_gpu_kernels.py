@@ -220,3 +220,41 @@ def test_column_segments_keep_results(ctx, K):
     kmax = int(np.max(np.diff(rp)))
     assert np.max(np.abs(got32 - O.spmv(M, x)) / (Ad @ np.abs(x) + 1e-30)) < (kmax + 2) * 1.2e-7
     A.destroy()
+
+
+def test_device_order_and_permute(ctx):
+    """The handle's device order groups the rows of A (and of A^T) by length
+    class, stably; device-order SpMV equals the natural one permuted, and the
+    permute entry point is exact both ways."""
+    rng = np.random.default_rng(5)
+    m, n = 3000, 7000
+    rp, ci, v = random_csr(rng, m, n, zipf=True)
+    M = O.Model(dict(m=m, n=n, row_ptr=rp, col_idx=ci, val=v, c=np.zeros(n), col_lo=np.zeros(n),
+                     col_hi=np.zeros(n), row_lo=np.zeros(m), row_hi=np.zeros(m)))
+    A = Csr(ctx, m, n, rp, ci, v)
+    pr, pc = A.order(0), A.order(1)
+    maxes = [2, 4, 8, 16, 32, 64, 256]
+    cls = lambda lens: np.searchsorted(maxes, lens, side="left")
+    np.testing.assert_array_equal(pr, np.argsort(cls(np.diff(rp)), kind="stable"))
+    np.testing.assert_array_equal(pc, np.argsort(cls(np.bincount(ci[: rp[-1]], minlength=n)), kind="stable"))
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+    Ad = np.abs(M.dense())
+    for transpose, vin, pin, pout, want, scale in ((0, x, pc, pr, O.spmv(M, x), Ad @ np.abs(x)),
+                                                   (1, y, pr, pc, O.spmv_t(M, y), Ad.T @ np.abs(y))):
+        dx = ctx.upload(np.ascontiguousarray(vin[pin]))
+        dy = ctx.upload(np.zeros(len(pout)))
+        check(ctx.L.sov_hip_spmv_device(ctx.h, A.h, transpose, 0, 1.0, dx, 0.0, dy))
+        got = np.empty(len(pout))
+        got[pout] = ctx.download(dy, len(pout))
+        assert np.max(np.abs(got - want) / (scale + 1e-300)) < 1e-14
+        ctx.free(dx)
+        ctx.free(dy)
+    for which, vec, perm in ((0, y, pr), (1, x, pc)):
+        for prec, dt in ((0, np.float64), (1, np.float32)):
+            src = ctx.upload(vec.astype(dt))
+            dev, back = ctx.upload(np.zeros(len(vec), dt)), ctx.upload(np.zeros(len(vec), dt))
+            check(ctx.L.sov_hip_csr_permute(ctx.h, A.h, which, 0, prec, src, dev))
+            check(ctx.L.sov_hip_csr_permute(ctx.h, A.h, which, 1, prec, dev, back))
+            np.testing.assert_array_equal(ctx.download(dev, len(vec), dt), vec.astype(dt)[perm])
+            np.testing.assert_array_equal(ctx.download(back, len(vec), dt), vec.astype(dt))
+    A.destroy()

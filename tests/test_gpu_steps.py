@@ -4,6 +4,9 @@ residual functions, unsegmented and with the column-segmented parts the
 autotuner can choose (the running row sums across parts must not change a
 result beyond summation-order rounding).
 
+All state vectors cross the C-ABI in the handle's device order
+(sov_hip_csr_order); the tests permute on the host.
+
 Oracle functions: orc_spmv / orc_spmv_t (backend_host.cpp:133-172),
 orc_kkt_residual (pdhg.cpp:87-136), orc_original_space_residual
 (pdhg.cpp:145-199), orc_relative_kkt (relative KKT of the reported point)."""
@@ -62,31 +65,44 @@ def random_point(rng, Ms, since):
     return x, y, xs, ys, s
 
 
+COLS = ("x", "x_bar", "x_sum", "c", "col_lo", "col_hi")
+
+
 class Device:
+    """The state of sov_hip_pdhg_state on the device, in the handle's device
+    order; put/get take and return natural-order host vectors."""
+
     def __init__(self, ctx, Ms, K, prec=0):
         self.ctx, self.Ms, self.prec = ctx, Ms, prec
         self.dt = np.float64 if prec == 0 else np.float32
         self.A = Csr(ctx, Ms.m, Ms.n, Ms.row_ptr, Ms.col_idx, Ms.val)
+        self.pr, self.pc = self.A.order(0), self.A.order(1)
+        assert sorted(self.pr) == list(range(Ms.m)) and sorted(self.pc) == list(range(Ms.n))
         if prec == 1:
             check(ctx.L.sov_hip_csr_make_f32(ctx.h, self.A.h))
         for orient in (0, 1):
             check(ctx.L.sov_hip_csr_set_segments(ctx.h, self.A.h, orient, K))
         st = PdhgState()
         st.prec, st.m, st.n = prec, Ms.m, Ms.n
-        up = lambda v: ctx.upload(np.asarray(v, self.dt))
-        st.c, st.col_lo, st.col_hi = up(Ms.c), up(Ms.col_lo), up(Ms.col_hi)
-        st.row_lo, st.row_hi = up(Ms.row_lo), up(Ms.row_hi)
-        for f, k in (("x", Ms.n), ("x_bar", Ms.n), ("x_sum", Ms.n), ("s", Ms.m), ("y", Ms.m), ("y_sum", Ms.m)):
-            setattr(st, f, up(np.zeros(k)))
+        for f, k in (("x", Ms.n), ("x_bar", Ms.n), ("x_sum", Ms.n), ("s", Ms.m), ("y", Ms.m), ("y_sum", Ms.m),
+                     ("c", Ms.n), ("col_lo", Ms.n), ("col_hi", Ms.n), ("row_lo", Ms.m), ("row_hi", Ms.m)):
+            setattr(st, f, ctx.upload(np.zeros(k, self.dt)))
         self.st = st
+        self.put(c=Ms.c, col_lo=Ms.col_lo, col_hi=Ms.col_hi, row_lo=Ms.row_lo, row_hi=Ms.row_hi)
+
+    def perm(self, f):
+        return self.pc if f in COLS else self.pr
 
     def put(self, **kw):
         for f, v in kw.items():
-            v = np.asarray(v, self.dt)
+            v = np.ascontiguousarray(np.asarray(v, self.dt)[self.perm(f)])
             check(self.ctx.L.sov_hip_memcpy_h2d(self.ctx.h, getattr(self.st, f), v.ctypes.data, v.nbytes))
 
     def get(self, f, count):
-        return self.ctx.download(getattr(self.st, f), count, self.dt).astype(np.float64)
+        d = self.ctx.download(getattr(self.st, f), count, self.dt).astype(np.float64)
+        out = np.empty_like(d)
+        out[self.perm(f)] = d
+        return out
 
 
 @pytest.mark.parametrize("K", [1, 3])
@@ -142,9 +158,10 @@ def test_check_matches_oracle_residuals(ctx, K, prec):
     a.since_restart = since
     a.tol_scaled, a.tol_orig = 1e-4, 1e-4
     a.obj_sign, a.obj_const = 1.0, M.obj_const
-    a.row_scale, a.col_scale = ctx.upload(rs), ctx.upload(cs)
-    a.orig_row_lo, a.orig_row_hi = ctx.upload(M.row_lo), ctx.upload(M.row_hi)
-    a.orig_col_lo, a.orig_col_hi, a.orig_c = ctx.upload(M.col_lo), ctx.upload(M.col_hi), ctx.upload(M.c)
+    pr, pc = D.pr, D.pc   # check arrays are in device order too
+    a.row_scale, a.col_scale = ctx.upload(rs[pr]), ctx.upload(cs[pc])
+    a.orig_row_lo, a.orig_row_hi = ctx.upload(M.row_lo[pr]), ctx.upload(M.row_hi[pr])
+    a.orig_col_lo, a.orig_col_hi, a.orig_c = ctx.upload(M.col_lo[pc]), ctx.upload(M.col_hi[pc]), ctx.upload(M.c[pc])
     act = ctx.upload(np.zeros(M.m))
     a.act_avg = act
     out = CheckOut()
@@ -165,5 +182,7 @@ def test_check_matches_oracle_residuals(ctx, K, prec):
             w = O.relative_kkt(M, pt[0] * cs, pt[1] * rs)
             np.testing.assert_allclose(list(rel), w[:3], rtol=1e-10, atol=1e-14)
             np.testing.assert_allclose([p, d], w[3:5], rtol=1e-11)
-        np.testing.assert_allclose(ctx.download(act, M.m), O.spmv(Ms, xa), rtol=1e-11, atol=1e-11)
+        act_nat = np.empty(M.m)
+        act_nat[pr] = ctx.download(act, M.m)
+        np.testing.assert_allclose(act_nat, O.spmv(Ms, xa), rtol=1e-11, atol=1e-11)
     D.A.destroy()
