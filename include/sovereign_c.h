@@ -79,6 +79,8 @@ typedef struct sov_pdhg_opts {
     double restart_beta_sufficient;
     double restart_beta_necessary;
     double restart_beta_artificial;
+    double* farkas_out;              /* NULL or num_rows: the Farkas certificate when status is primal_infeasible */
+    double* ray_out;                 /* NULL or num_cols: the primal ray when status is dual_infeasible */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

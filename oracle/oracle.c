@@ -330,6 +330,35 @@ int32_t orc_farkas_certifies(const orc_lp* p, const double* y, double slack) {
     return (rhi < clo - slack || chi < rlo - slack) ? 1 : 0;
 }
 
+/* A primal ray d of the LP (no reference counterpart; the definition of an
+ * improving direction of unboundedness): with d scaled to max-norm 1,
+ * c'd < -tol in minimize sense, d_j >= -tol where col_lo is finite, d_j <= tol
+ * where col_hi is finite, and (Ad)_i >= -tol / <= tol where row_lo / row_hi
+ * is finite.  Returns 0 for a zero or non-finite d. */
+int32_t orc_primal_ray_certifies(const orc_lp* p, const double* d, double tol) {
+    double s = 0.0, obj = 0.0;
+    for (int32_t j = 0; j < p->n; ++j) {
+        if (!isfinite(d[j])) return 0;
+        if (fabs(d[j]) > s) s = fabs(d[j]);
+    }
+    if (p->n == 0 || s <= 0.0) return 0;
+    const double sg = p->maximize ? -1.0 : 1.0;
+    for (int32_t j = 0; j < p->n; ++j) {
+        const double v = d[j] / s;
+        obj += sg * p->c[j] * v;
+        if (finite_bound(p->col_lo[j]) && v < -tol) return 0;
+        if (finite_bound(p->col_hi[j]) && v > tol) return 0;
+    }
+    if (!(obj < -tol)) return 0;
+    for (int32_t r = 0; r < p->m; ++r) {
+        double a = 0.0;
+        for (int32_t k = p->row_ptr[r]; k < p->row_ptr[r + 1]; ++k) a += p->val[k] * (d[p->col_idx[k]] / s);
+        if (finite_bound(p->row_lo[r]) && a < -tol) return 0;
+        if (finite_bound(p->row_hi[r]) && a > tol) return 0;
+    }
+    return 1;
+}
+
 /* ---------------------------------------------------------------------- */
 void orc_default_options(orc_options* o) {
     memset(o, 0, sizeof(*o));

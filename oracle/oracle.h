@@ -100,6 +100,12 @@ double orc_original_space_residual(const orc_lp* orig, double obj_sign,
  * bounds allow, both in the internal minimize sense. */
 void orc_relative_kkt(const orc_lp* orig, const double* x, const double* y, double out[5]);
 
+/* An improving primal ray of the LP (no reference counterpart; pinned by
+ * hand cases): d scaled to max-norm 1 has c'd < -tol (minimize sense), stays
+ * in the recession cone of the column box and A d in that of the row box,
+ * all within tol. */
+int32_t orc_primal_ray_certifies(const orc_lp* lp, const double* d, double tol);
+
 /* DualSolution::complementary_slackness_violation (duals.cpp:81-113). */
 double orc_cs_violation(const orc_lp* lp, const double* x, const double* row_duals,
                         const double* reduced_costs, double bound_tol);

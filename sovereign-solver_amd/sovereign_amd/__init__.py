@@ -40,6 +40,7 @@ class PdhgOpts(C.Structure):
         ("trace_cap", C.c_int64), ("trace_res", _f64p), ("trace_it", _i64p),
         ("restart_scheme", C.c_int32), ("autotune", C.c_int32), ("restart_beta_sufficient", C.c_double),
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
+        ("farkas_out", _f64p), ("ray_out", _f64p),
     ]
 
 
@@ -236,6 +237,8 @@ class Result:
     message: str = ""
     trace_res: np.ndarray = field(default_factory=lambda: np.zeros(0))
     trace_it: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    farkas: np.ndarray = None     # row multipliers when status == "primal_infeasible"
+    ray: np.ndarray = None        # primal ray when status == "dual_infeasible"
 
 
 _ENGINES = {"auto": 0, "primitive": 1, "fused": 2}
@@ -245,7 +248,7 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           restart_check_period=40, engine_kind="auto", termination="reference", precision="f64",
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
-          restarts="reference", autotune=True) -> Result:
+          restarts="reference", autotune=True, detect_infeasibility=False) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point."""
     L = engine()
     o = PdhgOpts()
@@ -264,6 +267,7 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     o.verbose = int(verbose)
     o.restart_scheme = 1 if restarts == "adaptive" else 0
     o.autotune = int(autotune)
+    o.detect_infeasibility = int(detect_infeasibility)
     if fp32_switch_factor is not None:
         o.fp32_switch_factor = fp32_switch_factor
     if fp32_stall_checks is not None:
@@ -282,6 +286,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
         o.trace_res = ptr(tr, C.c_double)
         o.trace_it = ptr(ti, C.c_int64)
     x, y, rc = np.zeros(max(n, 1)), np.zeros(max(m, 1)), np.zeros(max(n, 1))
+    farkas, ray = np.zeros(max(m, 1)), np.zeros(max(n, 1))
+    o.farkas_out, o.ray_out = ptr(farkas, C.c_double), ptr(ray, C.c_double)
     r = PdhgRes()
     msg = C.create_string_buffer(1024)
     rc_code = L.sov_pdhg_solve(lp.handle, 2 if backend == "hip" else 0, C.byref(o), ptr(x, C.c_double),
@@ -292,4 +298,6 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     return Result(STATUS[r.status], r.objective, r.primal_infeasibility, r.dual_infeasibility, r.duality_gap,
                   int(r.iterations), int(r.restarts), r.seconds, r.setup_seconds, r.loop_seconds,
                   r.operator_norm_estimate, (r.relative_primal, r.relative_dual, r.relative_gap), r.primal_weight,
-                  int(r.fp64_polish_iterations), x[:n], y[:m], rc[:n], msg.value.decode(), tr[:nt], ti[:nt])
+                  int(r.fp64_polish_iterations), x[:n], y[:m], rc[:n], msg.value.decode(), tr[:nt], ti[:nt],
+                  farkas[:m] if STATUS[r.status] == "primal_infeasible" else None,
+                  ray[:n] if STATUS[r.status] == "dual_infeasible" else None)

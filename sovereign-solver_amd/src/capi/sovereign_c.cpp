@@ -220,6 +220,10 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
         res->primal_weight = r.primal_weight;
         res->fp64_polish_iterations = r.fp64_polish_iterations;
         res->trace_len = nt;
+        if (o->farkas_out && !r.certificate.row_multipliers.empty())
+            std::memcpy(o->farkas_out, r.certificate.row_multipliers.data(), r.certificate.row_multipliers.size() * sizeof(double));
+        if (o->ray_out && !r.ray.direction.empty())
+            std::memcpy(o->ray_out, r.ray.direction.data(), r.ray.direction.size() * sizeof(double));
         if (x && !r.primal.empty()) std::memcpy(x, r.primal.data(), r.primal.size() * sizeof(double));
         if (y && !r.dual.empty()) std::memcpy(y, r.dual.data(), r.dual.size() * sizeof(double));
         if (rc && !r.reduced_costs.empty()) std::memcpy(rc, r.reduced_costs.data(), r.reduced_costs.size() * sizeof(double));

@@ -201,6 +201,67 @@ void finish_result(const Problem& p, const PdhgOptions& o, std::vector<Real> x, 
     r.relative_gap = k.gap;
 }
 
+/// Additive infeasibility detection.  On an infeasible or unbounded LP the
+/// PDHG iterates drift along the infimal displacement vector (Applegate,
+/// Diaz, Lu, Lubin, "Infeasibility detection with primal-dual hybrid
+/// gradient for large-scale linear programming", 2021): between two checks
+/// with no restart in between, y moves along a Farkas ray and x along a
+/// primal ray.  The drifts, in original units and scaled to max-norm 1, go to
+/// the exact checkers (FarkasCertificate, PrimalRay); nothing is reported
+/// that they do not verify.
+struct Drift {
+    std::vector<Real> x, y;   // original-space iterates at the previous check
+    bool valid = false;
+
+    void reset() { valid = false; }
+
+    static bool unit(std::vector<Real>& v) {
+        Real s = 0.0;
+        for (Real t : v) {
+            if (!std::isfinite(t)) return false;
+            s = std::max(s, std::abs(t));
+        }
+        if (s <= 0.0) return false;
+        for (Real& t : v) t /= s;
+        return true;
+    }
+
+    /// Test the drift since the previous check; remember (ux, uy).  Sets the
+    /// status and certificate of r and returns true when one verifies.
+    bool test(const Problem& p, std::vector<Real> ux, std::vector<Real> uy, PdhgResult& r) {
+        bool found = false;
+        if (valid) {
+            std::vector<Real> dy(uy.size()), dx(ux.size());
+            for (std::size_t i = 0; i < dy.size(); ++i) dy[i] = uy[i] - y[i];
+            for (std::size_t j = 0; j < dx.size(); ++j) dx[j] = ux[j] - x[j];
+            FarkasCertificate fc;
+            if (unit(dy)) {
+                fc.row_multipliers = std::move(dy);
+                if (fc.certifies_infeasibility(p)) {
+                    r.status = PdhgStatus::PrimalInfeasible;
+                    r.certificate = std::move(fc);
+                    r.message = "primal infeasible: Farkas certificate from the dual drift";
+                    found = true;
+                }
+            }
+            PrimalRay ray;
+            if (!found && unit(dx)) {
+                ray.direction = std::move(dx);
+                if (ray.certifies_unboundedness(p)) {
+                    r.status = PdhgStatus::DualInfeasible;
+                    r.ray = std::move(ray);
+                    r.message = "dual infeasible: improving primal ray from the primal drift";
+                    found = true;
+                }
+            }
+        }
+        x = std::move(ux);
+        y = std::move(uy);
+        valid = true;
+        return found;
+    }
+};
+
 bool prevalidate(const Problem& p, PdhgResult& r) {
     std::string why;
     if (p.validate(&why) != Status::Ok) {
@@ -318,6 +379,7 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
         Real res_at_restart = kInfinity, prev_cand = kInfinity;
         const std::size_t nb = nu * sizeof(Real), mb = mu * sizeof(Real);
         std::vector<Real> best_x, best_y;
+        Drift drift;
         bool converged = false;
         const Real otol = std::max(o.tolerance, Real{1e-6});
         const auto t_loop = Clock::now();
@@ -398,6 +460,15 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
                 ++result.iterations;
                 break;
             }
+            if (o.detect_infeasibility) {
+                auto [dxo, dyo] = unscale_point(sign, scaling, xh, yh);
+                if (drift.test(problem, std::move(dxo), std::move(dyo), result)) {
+                    best_x = xh;
+                    best_y = yh;
+                    ++result.iterations;
+                    break;
+                }
+            }
             bool do_restart = false;
             if (!adaptive) {
                 if (!std::isfinite(res_at_restart)) res_at_restart = cur;
@@ -410,6 +481,7 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
                 prev_cand = rel_cand;
             }
             if (do_restart) {
+                drift.reset();
                 if (!adaptive || avg_better) {
                     x = DeviceBuffer::upload(be, xa);
                     y = DeviceBuffer::upload(be, ya);
@@ -799,6 +871,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         Real res_at_restart = kInfinity, prev_cand = kInfinity;
         bool converged = false;
         std::vector<Real> best_x, best_y;   // scaled, internal sense, FP64, device order
+        Drift drift;
         const auto t_loop = Clock::now();
         result.setup_seconds = seconds_since(t0);
         long long polish_start = -1;
@@ -866,6 +939,16 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 converged = true;
                 break;
             }
+            if (o.detect_infeasibility) {
+                std::vector<Real> xd = S.download(ctx, S.x, n), yd = S.download(ctx, S.y, m);
+                auto [dxo, dyo] = unscale_point(sign, scaling, DeviceOrder::to_nat(xd, ord.col),
+                                                DeviceOrder::to_nat(yd, ord.row));
+                if (drift.test(problem, std::move(dxo), std::move(dyo), result)) {
+                    best_x = std::move(xd);
+                    best_y = std::move(yd);
+                    break;
+                }
+            }
             // FP32 -> FP64 switch (additive): near tolerance or stalled
             if (S.prec == SOV_PREC_F32 && o.fp64_polish) {
                 if (rel_worst < best_rel * 0.99) {
@@ -890,6 +973,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                     res_at_restart = kInfinity;
                     prev_cand = kInfinity;
                     polish_start = result.iterations;
+                    drift.reset();
                     if (o.verbose)
                         std::fprintf(stderr, "  [pdhg] fp64 polish from iter %lld (rel kkt %.3g)\n", result.iterations,
                                      static_cast<double>(rel_worst));
@@ -908,6 +992,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 prev_cand = rel_worst;
             }
             if (do_restart) {
+                drift.reset();
                 double mv[2] = {0.0, 0.0};
                 double* xl = o.use_primal_weight ? d_xlast.as<double>() : nullptr;
                 double* yl = o.use_primal_weight ? d_ylast.as<double>() : nullptr;

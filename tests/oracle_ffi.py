@@ -64,6 +64,7 @@ def oracle() -> C.CDLL:
         _lib.orc_original_space_residual.restype = C.c_double
         _lib.orc_cs_violation.restype = C.c_double
         _lib.orc_farkas_certifies.restype = C.c_int32
+        _lib.orc_primal_ray_certifies.restype = C.c_int32
         lp = C.POINTER(OrcLp)
         _lib.orc_spmv.argtypes = [C.c_int32, _i32p, _i32p, _f64p, C.c_double, _f64p, C.c_double, _f64p]
         _lib.orc_spmv_t.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, C.c_double, _f64p, C.c_double, _f64p]
@@ -76,6 +77,7 @@ def oracle() -> C.CDLL:
         _lib.orc_relative_kkt.argtypes = [lp, _f64p, _f64p, _f64p]
         _lib.orc_cs_violation.argtypes = [lp, _f64p, _f64p, _f64p, C.c_double]
         _lib.orc_farkas_certifies.argtypes = [lp, _f64p, C.c_double]
+        _lib.orc_primal_ray_certifies.argtypes = [lp, _f64p, C.c_double]
         _lib.orc_default_options.argtypes = [C.POINTER(OrcOptions)]
         _lib.orc_pdhg_solve.argtypes = [lp, C.POINTER(OrcOptions), _f64p, _f64p, _f64p, C.POINTER(OrcResult),
                                         _f64p, C.c_int64, _i64p]
@@ -203,6 +205,12 @@ def farkas_certifies(M: Model, y, slack=1e-6):
     s = M.lp()
     y = np.ascontiguousarray(y, np.float64)
     return bool(oracle().orc_farkas_certifies(C.byref(s), _p(y, C.c_double), C.c_double(slack)))
+
+
+def primal_ray_certifies(M: Model, d, tol=1e-6):
+    s = M.lp()
+    d = np.ascontiguousarray(d, np.float64)
+    return bool(oracle().orc_primal_ray_certifies(C.byref(s), _p(d, C.c_double), C.c_double(tol)))
 
 
 def pdhg(M: Model, tolerance=1e-4, max_iterations=1000000, period=40, termination=0, primal_weight=False,

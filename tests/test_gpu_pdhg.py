@@ -5,11 +5,16 @@ Bar (north star): same termination status, objective within 1e-6 relative,
 relative KKT under the run's tolerance; FP64 trajectories agree with the
 oracle to rounding, FP32 runs are stated against the FP64 oracle solve.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle_ffi as O
 import sovereign_amd as S
+from lp_cases import infeasible_and_unbounded
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -153,3 +158,21 @@ def test_full_size_zipf_spmv_against_oracle(gpu):
         A.destroy()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("engine", ["primitive", "fused"])
+def test_infeasibility_certificates_on_the_device(gpu, engine):
+    for fname, status in (("tiny_infeasible", "primal_infeasible"), ("tiny_unbounded", "dual_infeasible")):
+        lp = S.LP.read(os.path.join(ROOT, "tests", "golden", "smoke", fname + ".mps"))
+        r = S.solve(lp, backend="hip", engine_kind=engine, detect_infeasibility=True, max_iterations=200000)
+        assert r.status == status, (fname, r.message)
+        M = O.Model(lp.arrays())
+        assert O.farkas_certifies(M, r.farkas) if status == "primal_infeasible" else O.primal_ray_certifies(M, r.ray)
+    bad, unb = infeasible_and_unbounded()
+    for lp, status in ((bad, "primal_infeasible"), (unb, "dual_infeasible")):
+        r = S.solve(lp, backend="hip", engine_kind=engine, detect_infeasibility=True, max_iterations=400000,
+                    restarts="adaptive", termination="relative", tolerance=1e-6)
+        print(engine, status, r.status, r.iterations, r.message, flush=True)
+        assert r.status == status, r.message
+        M = O.Model(lp.arrays())
+        assert O.farkas_certifies(M, r.farkas) if status == "primal_infeasible" else O.primal_ray_certifies(M, r.ray)

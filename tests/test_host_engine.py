@@ -10,6 +10,7 @@ import pytest
 
 import oracle_ffi as O
 import sovereign_amd as S
+from lp_cases import infeasible_and_unbounded
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -146,3 +147,43 @@ def test_backend_report_and_cli():
     assert '"status": "optimal"' in out.stdout
     bad = subprocess.run([cli, "solve", path, "--engine", "simplex"], capture_output=True, text=True)
     assert bad.returncode == 2
+
+
+@pytest.mark.parametrize("fname,status", [("tiny_infeasible", "primal_infeasible"),
+                                          ("tiny_unbounded", "dual_infeasible")])
+def test_certificates_on_the_smoke_instances(fname, status):
+    """benchmarks/smoke/expected.toml marks these infeasible / unbounded.  With
+    detection on, the primitive engine stops with the matching status and a
+    certificate that the oracle's checkers verify; with it off (the
+    reference's behaviour) it runs to the iteration limit."""
+    lp = S.LP.read(os.path.join(ROOT, "tests", "golden", "smoke", fname + ".mps"))
+    M = O.Model(lp.arrays())
+    r = S.solve(lp, backend="host", engine_kind="primitive", detect_infeasibility=True, max_iterations=200000)
+    assert r.status == status, r.message
+    if status == "primal_infeasible":
+        assert O.farkas_certifies(M, r.farkas)
+    else:
+        assert O.primal_ray_certifies(M, r.ray)
+    off = S.solve(lp, backend="host", engine_kind="primitive", max_iterations=4000)
+    assert off.status == "iteration_limit"
+
+
+def test_detection_never_fires_on_a_feasible_lp(golden):
+    for name in ("smoke_tiny_lp", "smoke_tiny_degenerate"):
+        lp = lp_of(golden["lps"][name]["model"])
+        r = S.solve(lp, backend="host", engine_kind="primitive", detect_infeasibility=True, tolerance=1e-8)
+        assert r.status == "optimal"
+    lp = S.LP.random(200, 500, 5, 3)
+    r = S.solve(lp, backend="host", engine_kind="primitive", detect_infeasibility=True, tolerance=1e-6,
+                termination="relative", restarts="adaptive")
+    assert r.status == "optimal"
+
+
+def test_certificates_on_generated_lps():
+    bad, unb = infeasible_and_unbounded()
+    for lp, status in ((bad, "primal_infeasible"), (unb, "dual_infeasible")):
+        r = S.solve(lp, backend="host", engine_kind="primitive", detect_infeasibility=True, max_iterations=400000,
+                    restarts="adaptive", termination="relative", tolerance=1e-6)
+        assert r.status == status, r.message
+        M = O.Model(lp.arrays())
+        assert O.farkas_certifies(M, r.farkas) if status == "primal_infeasible" else O.primal_ray_certifies(M, r.ray)

@@ -207,6 +207,28 @@ def test_farkas_and_cs_hand_cases():
     assert O.cs_violation(Mo, x * 0.5, y, rc, 1e-7) > 0.1
 
 
+def test_primal_ray_hand_cases():
+    # min -x s.t. x - y <= 5, x, y >= 0 (benchmarks/smoke/tiny_unbounded.mps): d = (1, 1) is a ray
+    M = O.Model(dict(m=1, n=2, row_ptr=[0, 2], col_idx=[0, 1], val=[1.0, -1.0], c=[-1.0, 0.0], col_lo=[0.0, 0.0],
+                     col_hi=[INF, INF], row_lo=[-INF], row_hi=[5.0]))
+    assert O.primal_ray_certifies(M, [1.0, 1.0])
+    assert O.primal_ray_certifies(M, [3.0, 7.0])          # any scale, A d <= 0 with room
+    assert not O.primal_ray_certifies(M, [1.0, 0.0])      # A d = 1 leaves the row's recession cone
+    assert not O.primal_ray_certifies(M, [0.0, 1.0])      # c'd = 0: not improving
+    assert not O.primal_ray_certifies(M, [-1.0, -1.0])    # leaves x >= 0
+    assert not O.primal_ray_certifies(M, [0.0, 0.0])
+    assert not O.primal_ray_certifies(M, [np.nan, 1.0])
+    # the sense flips the improving direction: max x has the same ray
+    Mx = O.Model(dict(m=1, n=2, row_ptr=[0, 2], col_idx=[0, 1], val=[1.0, -1.0], c=[1.0, 0.0], col_lo=[0.0, 0.0],
+                      col_hi=[INF, INF], row_lo=[-INF], row_hi=[5.0], maximize=True))
+    assert O.primal_ray_certifies(Mx, [1.0, 1.0])
+    # a bounded LP has none: min x + y, x + y >= 1, x, y >= 0
+    Mb = O.Model(dict(m=1, n=2, row_ptr=[0, 2], col_idx=[0, 1], val=[1.0, 1.0], c=[1.0, 1.0], col_lo=[0.0, 0.0],
+                      col_hi=[INF, INF], row_lo=[1.0], row_hi=[INF]))
+    rng = np.random.default_rng(0)
+    assert not any(O.primal_ray_certifies(Mb, rng.standard_normal(2)) for _ in range(200))
+
+
 def check_same(res, g):
     assert res["status"] == g["status"]
     assert res["iterations"] == g["iterations"]
