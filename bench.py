@@ -38,7 +38,7 @@ class DeviceProblem:
     """The scaled LP and the FP32 iterate state on the device (the same calls
     the fused engine makes, src/l1/pdhg.cpp solve_fused)."""
 
-    def __init__(self, ctx, a, prec):
+    def __init__(self, ctx, a, prec, segments=None):
         from sovereign_amd import device as D
         self.D, self.ctx, L = D, ctx, ctx.L
         m, n = a["m"], a["n"]
@@ -68,6 +68,12 @@ class DeviceProblem:
         if prec == 1:
             D.check(L.sov_hip_csr_make_f32(ctx.h, self.A.h))
         D.check(L.sov_hip_csr_tune(ctx.h, self.A.h, prec))
+        if segments is not None:   # pin the column-segment counts (profiling runs)
+            for orient, k in enumerate(segments):
+                cur = C.c_int()
+                D.check(L.sov_hip_csr_get_segments(self.A.h, orient, C.byref(cur)))
+                if cur.value != k:     # rebuilt parts start from the default lanes
+                    D.check(L.sov_hip_csr_set_segments(ctx.h, self.A.h, orient, k))
         st = D.PdhgState()
         st.prec, st.m, st.n = prec, m, n
         st.c, st.col_lo, st.col_hi = (ctx.upload(v[pc].astype(dt)) for v in (c_s, clo, chi))
@@ -176,6 +182,25 @@ def spmv_bytes(m, n, nnz, sv, sx):
     return primal, dual
 
 
+def committed_traffic(m, n, nnz, dtype, seg_a, seg_t):
+    """HBM bytes per step of the two step kernels from the committed PMC
+    summary (profiles/pmc_traffic_zipf1m_f32.md: FETCH_SIZE and WRITE_SIZE
+    per dispatch, scaled by the factors a known-size copy kernel reads), when
+    it was taken on this workload and segment counts; else None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_zipf1m_f32.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"traffic_per_step primal_step=(\d+) dual_step=(\d+) workload=(\d+)x(\d+)x(\d+) "
+                     r"dtype=(\w+) segments=(\d+),(\d+)")
+    for line in open(path):
+        g = pat.search(line)
+        if g and tuple(int(v) for v in g.group(3, 4, 5)) == (m, n, nnz) and g.group(6) == dtype \
+                and (int(g.group(7)), int(g.group(8))) == (seg_a, seg_t):
+            return {"primal_step": int(g.group(1)), "dual_step": int(g.group(2))}
+    return None
+
+
 def cpu_baseline(a, budget_iters):
     """CPU oracle (single thread) on the same instance: per-iteration time of
     the reference loop measured as (t(setup + k iterations) - t(setup)) / k."""
@@ -203,6 +228,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--segments", default=None,
+                    help="A,T column-segment counts to use instead of the autotuner's (profiling runs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with 3 launches of the f64->f32 convert kernel over 32 Mi elements (reads 256 MiB, "
                          "writes 128 MiB): the known-size reference for HBM byte counters")
@@ -224,7 +251,8 @@ def main():
     prec = 1 if args.precision == "f32" else 0
     ctx = D.Ctx(0)
     t0 = time.perf_counter()
-    P = DeviceProblem(ctx, a, prec)
+    segs = tuple(int(k) for k in args.segments.split(",")) if args.segments else None
+    P = DeviceProblem(ctx, a, prec, segs)
     log(f"setup: {time.perf_counter() - t0:.2f}s  op_norm={P.op_norm:.6g}")
     lanes_a, lanes_t = (C.c_int32 * 8)(), (C.c_int32 * 8)()
     ctx.L.sov_hip_csr_get_lanes(P.A.h, 0, lanes_a)
@@ -263,7 +291,8 @@ def main():
     bp, bd = spmv_bytes(a["m"], a["n"], a["nnz"], sv, sx)
     log(f"primal step {tp * 1e3:.1f} us ({bp / tp / 1e6:.0f} GB/s), dual step {td * 1e3:.1f} us "
         f"({bd / td / 1e6:.0f} GB/s)")
-    dom_name, dom_t, dom_b = ("primal_step (A^T y fused)", tp, bp) if tp >= td else ("dual_step (A x_bar fused)", td, bd)
+    dom_name, dom_t, dom_b = ("primal_step", tp, bp) if tp >= td else ("dual_step", td, bd)
+    traffic = committed_traffic(a["m"], a["n"], a["nnz"], "f32" if prec == 1 else "f64", seg_a.value, seg_t.value)
     achieved = dom_b / dom_t / 1e6
     value = (bp + bd) * its / 1e9
     chk = P.last_check
@@ -299,8 +328,10 @@ def main():
                    "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
                    "kkt_scaled_at_last_check": chk.kkt_cur if chk else None},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(dom_b)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic[dom_name] if traffic else None,
+                     "algorithmic_bytes_per_step": int(dom_b),
+                     "launches_per_step": seg_a.value if dom_name == "dual_step" else seg_t.value},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
