@@ -272,6 +272,55 @@ int sov_hip_pdhg_movement(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, double
 int sov_hip_pdhg_set_point(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double* x_src,
                            const double* y_src, const double* act);
 
+/* ---- batches of LPs ---------------------------------------------------
+ * No reference counterpart (the north star's batched solve).  A batch is a
+ * block-diagonal stack of independent LPs in ONE csr handle: instance b owns
+ * natural rows [row_off[b], row_off[b+1]) and columns [col_off[b],
+ * col_off[b+1]) (count + 1 offsets each, covering the matrix).  All state
+ * vectors are the stacked ones, in the handle's device order.  Each
+ * instance has its own step sizes (tau, sigma), an active flag (inactive
+ * instances are skipped by the step kernels) and `since` = the iterations its
+ * Cesaro sums hold.  One step launch covers every instance; k is the
+ * iteration's index inside a captured interval: an instance starts a new
+ * Cesaro sum at k == 0 when its `since` is 0.  active: 0 = stopped, 1 =
+ * iterating in an FP32 state, 2 = in an FP64 state; a step launch on a state
+ * of one precision skips the rows of the other instances, so a batch can
+ * keep an FP32 and an FP64 state side by side and move instances between
+ * them one at a time (sov_hip_batch_promote). */
+typedef struct sov_hip_batch sov_hip_batch;
+int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* csr, int32_t count, const int32_t* row_off,
+                         const int32_t* col_off, sov_hip_batch** out);
+void sov_hip_batch_destroy(sov_hip_batch* batch);
+/* Upload per-instance parameters from host arrays of `count` (NULL leaves
+ * that parameter as it is).  Defaults: tau = sigma = 1, active = 2, since = 0. */
+int sov_hip_batch_set(sov_hip_ctx* ctx, sov_hip_batch* batch, const double* tau, const double* sigma,
+                      const int32_t* active, const int32_t* since);
+int sov_hip_batch_primal_step(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_batch* batch,
+                              const sov_hip_pdhg_state* st, int k);
+int sov_hip_batch_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_batch* batch,
+                            const sov_hip_pdhg_state* st, int k);
+/* The check of sov_hip_pdhg_check for every instance at once (averages over
+ * each instance's own `since`; a->since_restart is ignored); obj_const is a
+ * host array of `count` (NULL = 0); out receives `count` results. */
+int sov_hip_batch_check(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_batch* batch,
+                        const sov_hip_check_args* a, const double* obj_const, sov_hip_check_out* out);
+/* Restart the instances whose host flag is 1 to their average (as
+ * sov_hip_pdhg_restart, with each instance's `since`). */
+int sov_hip_batch_restart(sov_hip_ctx* ctx, const sov_hip_batch* batch, const sov_hip_pdhg_state* st,
+                          const int32_t* flags, const double* act_avg);
+/* For the instances whose host flag is 1: write their candidate from the
+ * FP32 state (the current point, or the average x_sum/since when use_avg[b])
+ * into the FP64 state's x and x_sum (y and y_sum likewise).  A following
+ * sov_hip_batch_restart of the FP64 state with since = 1 then places them
+ * there, with s from the caller's FP64 activity. */
+int sov_hip_batch_promote(sov_hip_ctx* ctx, const sov_hip_batch* batch, const sov_hip_pdhg_state* st32,
+                          const sov_hip_pdhg_state* st64, const int32_t* flags, const int32_t* use_avg);
+/* Per-instance sum of squares of an FP64 vector of the row (which = 0) or
+ * column (1) dimension into host out[count]; and v *= factors[instance]. */
+int sov_hip_batch_sumsq(sov_hip_ctx* ctx, const sov_hip_batch* batch, int which, const double* v, double* out);
+int sov_hip_batch_scale(sov_hip_ctx* ctx, const sov_hip_batch* batch, int which, const double* factors,
+                        double* v);
+
 /* ---- launch graphs ----------------------------------------------------
  * Capture the context stream's launches between begin and end into a graph
  * and replay it.  Only the step kernels above may be captured (no host

@@ -170,4 +170,34 @@ private:
     PdhgOptions options_;
 };
 
+/// Additive (no reference counterpart): many independent LPs solved together
+/// on a Hip backend.  The instances are stacked block-diagonally into one
+/// device matrix, so each PDHG step is one launch for the whole batch; every
+/// instance keeps its own scaling, step size, Cesaro average, restarts and
+/// convergence test, and stops iterating (its rows are skipped) once it
+/// converges.  Check intervals are captured as HIP graphs.  Uses from
+/// PdhgOptions: tolerance, max_iterations, time_limit_seconds,
+/// restart_check_period, step_size_safety, power_iterations, termination,
+/// precision, fp64_polish, fp32_switch_factor, fp32_stall_checks,
+/// restart_scheme and its betas, use_graphs, autotune.  The primal weight
+/// stays 1; all instances must share one objective sense.
+struct PdhgBatchResult {
+    std::vector<PdhgResult> results;   ///< one per instance, in input order
+    long long iterations = 0;          ///< iterations the batch ran
+    double seconds = 0.0;
+    double setup_seconds = 0.0;
+    double loop_seconds = 0.0;
+    std::string message;
+};
+
+class PdhgBatch {
+public:
+    explicit PdhgBatch(PdhgOptions options = {}) : options_(std::move(options)) {}
+    PdhgBatchResult solve(const std::vector<const Problem*>& problems, Backend& backend) const;
+    const PdhgOptions& options() const noexcept { return options_; }
+
+private:
+    PdhgOptions options_;
+};
+
 }  // namespace sov

@@ -111,6 +111,15 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o);
 int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, double* x, double* y, double* rc,
                    sov_pdhg_result* res, char* msg, int32_t msg_len);
 
+/* Batched solve on the Hip backend (sov::PdhgBatch; no reference
+ * counterpart): count models solved together, each with its own scaling,
+ * step sizes, restarts and convergence.  results[b] per model; xs[b] / ys[b]
+ * (NULL or num_cols / num_rows doubles) receive the points; timing (NULL or
+ * 4 doubles) receives total, setup and loop seconds and the iterations the
+ * batch ran.  All models must share one objective sense. */
+int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* opts,
+                         sov_pdhg_result* results, double* const* xs, double* const* ys, double* timing);
+
 /* Backend report line (available backends, default, index width). */
 int sov_backend_report(char* buf, int32_t len);
 

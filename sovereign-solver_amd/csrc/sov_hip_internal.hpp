@@ -165,6 +165,37 @@ struct sov_hip_csr {
     int device = 0;
 };
 
+// A block-diagonal batch of LPs stacked in one sov_hip_csr: instance b owns
+// natural rows [row_off[b], row_off[b+1]) and columns [col_off[b],
+// col_off[b+1]).  Device order groups rows by length class stably, so inside
+// each class range the rows of one instance stay contiguous: seg holds, per
+// instance, dimension (0 rows, 1 columns) and class, the (start, length) of
+// that run, which makes every per-instance reduction a sum over 8 contiguous
+// ranges.
+struct sov_hip_batch {
+    int32_t count = 0, m = 0, n = 0;
+    int device = 0;
+    int32_t* row_inst = nullptr;    // m, device order
+    int32_t* col_inst = nullptr;    // n, device order
+    int32_t* seg = nullptr;         // count * 2 * kClasses * 2
+    double* tau = nullptr;          // count
+    double* sigma = nullptr;        // count
+    int32_t* active = nullptr;      // count
+    int32_t* since = nullptr;       // count
+    int32_t* flags = nullptr;       // count
+    int32_t* use_avg = nullptr;     // count
+    double* fac = nullptr;          // count
+    double* terms = nullptr;        // 11 * (m + n): per-row check terms
+    // per step-kernel block of each part of A (blk[0]) and A^T (blk[1]):
+    // bit 0 = holds a row of an instance iterating in FP32, bit 1 = in FP64;
+    // blocks with neither exit at once.  Recomputed whenever `active` is set.
+    std::vector<int32_t*> blk[2];
+    std::vector<int64_t> blk_cap[2];
+    const sov_hip_csr* csr = nullptr;
+    double* red = nullptr;          // count * 22
+    double* h_red = nullptr;        // pinned host copy, count * 22
+};
+
 struct sov_hip_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

@@ -28,9 +28,27 @@ struct DeviceGuard3 {
 // step kernels
 // ---------------------------------------------------------------------------
 
+// Per-instance parameters of a block-diagonal batch of LPs, read by the
+// batched (B = true) variants of the kernels: the instance of each
+// device-order row, its step sizes, where it iterates (0 = stopped, 1 = in
+// the FP32 state, 2 = in the FP64 state: a step launch on a state of one
+// precision skips the rows of instances living in the other), and how many
+// iterations its Cesaro sums hold.  k is the iteration's index inside the
+// captured interval (a launch argument, fixed per graph node).
+struct BatchView {
+    const int32_t* inst = nullptr;
+    const double* tau = nullptr;
+    const double* sigma = nullptr;
+    const int32_t* active = nullptr;
+    const int32_t* since = nullptr;
+    int k = 0;
+    int want = 0;   // the active value this launch's state serves
+    const int32_t* blk = nullptr;   // per-block activity bits of this part (want's bit set = work here)
+};
+
 // Per column j (a row of A^T): g = c_j - (A^T y)_j; x+ = clamp(x - tau g);
 // x_bar = 2 x+ - x; x = x+; x_sum = first ? x+ : x_sum + x+.
-template <typename T>
+template <typename T, bool B = false>
 struct PrimalOp {
     const int32_t* ptr;
     const int32_t* idx;
@@ -47,10 +65,27 @@ struct PrimalOp {
     int first;
     int mode;
     Sched s;
+    BatchView bv;
     template <int G>
     __device__ void run(const ClassView& v) {
+        if constexpr (B) {
+            if ((bv.blk[blockIdx.x] & bv.want) == 0) return;   // uniform per block
+        }
         int lane;
-        const int row = row_of<G>(v, lane);
+        int row = row_of<G>(v, lane);
+        T tau_r = tau;
+        int first_r = first;
+        if constexpr (B) {
+            if (row >= 0) {
+                const int b = bv.inst[row];
+                if (bv.active[b] != bv.want) {
+                    row = -1;
+                } else {
+                    tau_r = T(bv.tau[b]);
+                    first_r = (bv.k == 0 && bv.since[b] == 0) ? 1 : 0;
+                }
+            }
+        }
         T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
         if (row >= 0 && lane == 0) {
             if (mode == kFirst) { acc[row] = aty; return; }
@@ -58,20 +93,20 @@ struct PrimalOp {
             if (mode == kLast) aty = acc[row] + aty;
             const T g = c[row] - aty;
             const T xo = x[row];
-            T xn = xo + (-tau) * g;
+            T xn = xo + (-tau_r) * g;
             const T l = lo[row], h = hi[row];
             if (xn < l) xn = l;
             if (xn > h) xn = h;
             xbar[row] = xn * T(2) - xo;
             x[row] = xn;
-            xsum[row] = first ? xn : xsum[row] + xn;
+            xsum[row] = first_r ? xn : xsum[row] + xn;
         }
     }
 };
 
 // Per row i: s+ = clamp(s - tau y); s_bar = 2 s+ - s;
 // y+ = y + sigma ((-(A x_bar)_i) + s_bar); y_sum = first ? y+ : y_sum + y+.
-template <typename T>
+template <typename T, bool B = false>
 struct DualOp {
     const int32_t* ptr;
     const int32_t* idx;
@@ -87,25 +122,43 @@ struct DualOp {
     int first;
     int mode;
     Sched s;
+    BatchView bv;
     template <int G>
     __device__ void run(const ClassView& v) {
+        if constexpr (B) {
+            if ((bv.blk[blockIdx.x] & bv.want) == 0) return;   // uniform per block
+        }
         int lane;
-        const int row = row_of<G>(v, lane);
+        int row = row_of<G>(v, lane);
+        T tau_r = tau, sigma_r = sigma;
+        int first_r = first;
+        if constexpr (B) {
+            if (row >= 0) {
+                const int b = bv.inst[row];
+                if (bv.active[b] != bv.want) {
+                    row = -1;
+                } else {
+                    tau_r = T(bv.tau[b]);
+                    sigma_r = T(bv.sigma[b]);
+                    first_r = (bv.k == 0 && bv.since[b] == 0) ? 1 : 0;
+                }
+            }
+        }
         T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
         if (row >= 0 && lane == 0) {
             if (mode == kFirst) { acc[row] = ax; return; }
             if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
             if (mode == kLast) ax = acc[row] + ax;
             const T so = s_[row], yo = y[row];
-            T sn = so + (-tau) * yo;
+            T sn = so + (-tau_r) * yo;
             const T l = lo[row], h = hi[row];
             if (sn < l) sn = l;
             if (sn > h) sn = h;
             const T sb = sn * T(2) - so;
-            const T yn = yo + sigma * ((-ax) + sb);
+            const T yn = yo + sigma_r * ((-ax) + sb);
             s_[row] = sn;
             y[row] = yn;
-            ysum[row] = first ? yn : ysum[row] + yn;
+            ysum[row] = first_r ? yn : ysum[row] + yn;
         }
     }
 };
@@ -170,7 +223,37 @@ __device__ __forceinline__ bool combine_parts(int mode, double* pair, double& d1
     return true;
 }
 
-template <typename T>
+// The Cesaro divisor of a row: 1/since of the run (single LP) or of the
+// row's instance (batch).
+template <bool B>
+__device__ __forceinline__ double row_inv(const BatchView& bv, double inv, int row) {
+    if constexpr (B) {
+        if (row >= 0) {
+            const int32_t k = bv.since[bv.inst[row]];
+            return 1.0 / double(k > 0 ? k : 1);
+        }
+    }
+    return inv;
+}
+
+// A row's 7 summed and 4 maximised check terms go either into the thread's
+// running accumulators (single LP: block partials follow) or, for a batch,
+// to per-row arrays (slot-major, stride `stride`) that a segmented reduction
+// sums per instance.
+__device__ __forceinline__ void emit_terms(const double (&t)[11], double* rowout, int64_t stride, int64_t row,
+                                           double (&sm)[7], double (&mx)[4]) {
+    if (rowout != nullptr) {
+#pragma unroll
+        for (int k = 0; k < 11; ++k) rowout[k * stride + row] = t[k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) sm[k] += t[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mx[k] = nmax(mx[k], t[7 + k]);
+}
+
+template <typename T, bool B = false>
 struct CheckRowOp {
     const int32_t* ptr;
     const int32_t* idx;
@@ -187,46 +270,53 @@ struct CheckRowOp {
     double* act_avg;
     double* partial;
     double* acc;      // 2 per row: running (current, average) dots across parts
+    double* rowout;   // batch: 11 x rows per-row terms
     double inv, tol_s, tol_o;
     int mode;
     Sched s;
+    BatchView bv;
     // sums: pv2_c, pv2_a, bn2, yres2_c, yres2_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
         int lane;
         const int i = row_of<G>(v, lane);
+        const double inv_r = row_inv<B>(bv, inv, i);
         double ac, aa;
-        row_dot2<G>(ptr, idx, val, x, xsum, inv, i, lane, ac, aa);
+        row_dot2<G>(ptr, idx, val, x, xsum, inv_r, i, lane, ac, aa);
         if (i >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)i, ac, aa)) return;
             if (act_avg != nullptr) act_avg[i] = aa;
-            const double yc = double(y[i]), ya = double(ysum[i]) * inv;
-            mx[0] = nmax(mx[0], row_res(ac, double(lo_s[i]), double(hi_s[i]), yc, tol_s));
-            mx[1] = nmax(mx[1], row_res(aa, double(lo_s[i]), double(hi_s[i]), ya, tol_s));
+            double t[11];
+            const double yc = double(y[i]), ya = double(ysum[i]) * inv_r;
+            t[7] = row_res(ac, double(lo_s[i]), double(hi_s[i]), yc, tol_s);
+            t[8] = row_res(aa, double(lo_s[i]), double(hi_s[i]), ya, tol_s);
             const double r = rs[i], lo = lo_o[i], hi = hi_o[i];
             const double aoc = ac / r, aoa = aa / r;
             const double yic = yc * r, yia = ya * r;
-            mx[2] = nmax(mx[2], row_res(aoc, lo, hi, yic, tol_o));
-            mx[3] = nmax(mx[3], row_res(aoa, lo, hi, yia, tol_o));
+            t[9] = row_res(aoc, lo, hi, yic, tol_o);
+            t[10] = row_res(aoa, lo, hi, yia, tol_o);
             double vc = 0.0, va = 0.0;
             if (finite_b(lo) && aoc < lo) vc = lo - aoc;
             if (finite_b(hi) && aoc > hi) vc = aoc - hi;
             if (finite_b(lo) && aoa < lo) va = lo - aoa;
             if (finite_b(hi) && aoa > hi) va = aoa - hi;
-            sm[0] += vc * vc;
-            sm[1] += va * va;
-            if (finite_b(lo)) sm[2] += lo * lo;
-            if (finite_b(hi) && !(finite_b(lo) && lo == hi)) sm[2] += hi * hi;
+            t[0] = vc * vc;
+            t[1] = va * va;
+            t[2] = 0.0;
+            if (finite_b(lo)) t[2] += lo * lo;
+            if (finite_b(hi) && !(finite_b(lo) && lo == hi)) t[2] += hi * hi;
             const double pc = proj_sign(yic, lo, hi), pa = proj_sign(yia, lo, hi);
-            sm[3] += (yic - pc) * (yic - pc);
-            sm[4] += (yia - pa) * (yia - pa);
-            sm[5] += box_inf(pc, lo, hi);
-            sm[6] += box_inf(pa, lo, hi);
+            t[3] = (yic - pc) * (yic - pc);
+            t[4] = (yia - pa) * (yia - pa);
+            t[5] = box_inf(pc, lo, hi);
+            t[6] = box_inf(pa, lo, hi);
+            emit_terms(t, rowout, stride, i, sm, mx);
         }
     }
+    int64_t stride;   // rows of the orientation (batch per-row arrays)
 };
 
-template <typename T>
+template <typename T, bool B = false>
 struct CheckColOp {
     const int32_t* ptr;
     const int32_t* idx;
@@ -244,38 +334,44 @@ struct CheckColOp {
     const double* cs;
     double* partial;
     double* acc;
+    double* rowout;
     double inv, tol_s, tol_o, sign;
     int mode;
     Sched s;
+    BatchView bv;
     // sums: dres2_c, dres2_a, cn2, pobj_c, pobj_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
         int lane;
         const int j = row_of<G>(v, lane);
+        const double inv_r = row_inv<B>(bv, inv, j);
         double tc, ta;
-        row_dot2<G>(ptr, idx, val, y, ysum, inv, j, lane, tc, ta);
+        row_dot2<G>(ptr, idx, val, y, ysum, inv_r, j, lane, tc, ta);
         if (j >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)j, tc, ta)) return;
-            const double xc = double(x[j]), xa = double(xsum[j]) * inv;
+            double t[11];
+            const double xc = double(x[j]), xa = double(xsum[j]) * inv_r;
             const double cj = double(c_s[j]);
-            mx[0] = nmax(mx[0], dual_class(cj - tc, double(lo_s[j]), double(hi_s[j]), xc, tol_s));
-            mx[1] = nmax(mx[1], dual_class(cj - ta, double(lo_s[j]), double(hi_s[j]), xa, tol_s));
+            t[7] = dual_class(cj - tc, double(lo_s[j]), double(hi_s[j]), xc, tol_s);
+            t[8] = dual_class(cj - ta, double(lo_s[j]), double(hi_s[j]), xa, tol_s);
             const double sc = cs[j], lo = lo_o[j], hi = hi_o[j];
             const double ci = sign * c_o[j];
             const double xoc = xc * sc, xoa = xa * sc;
             const double rc = ci - tc / sc, ra = ci - ta / sc;
-            mx[2] = nmax(mx[2], dual_class(rc, lo, hi, xoc, tol_o));
-            mx[3] = nmax(mx[3], dual_class(ra, lo, hi, xoa, tol_o));
+            t[9] = dual_class(rc, lo, hi, xoc, tol_o);
+            t[10] = dual_class(ra, lo, hi, xoa, tol_o);
             const double lc = proj_sign(rc, lo, hi), la = proj_sign(ra, lo, hi);
-            sm[0] += (rc - lc) * (rc - lc);
-            sm[1] += (ra - la) * (ra - la);
-            sm[2] += ci * ci;
-            sm[3] += ci * xoc;
-            sm[4] += ci * xoa;
-            sm[5] += box_inf(lc, lo, hi);
-            sm[6] += box_inf(la, lo, hi);
+            t[0] = (rc - lc) * (rc - lc);
+            t[1] = (ra - la) * (ra - la);
+            t[2] = ci * ci;
+            t[3] = ci * xoc;
+            t[4] = ci * xoa;
+            t[5] = box_inf(lc, lo, hi);
+            t[6] = box_inf(la, lo, hi);
+            emit_terms(t, rowout, stride, j, sm, mx);
         }
     }
+    int64_t stride;
 };
 
 template <class Op>
@@ -300,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_check(Op op) {
     CheckRunner<Op> r{op, sm, mx};
     dispatch_G(v, r);
     // mode is uniform over the launch: only a launch that completes rows reduces
-    if (op.mode == kOnly || op.mode == kLast)
+    if ((op.mode == kOnly || op.mode == kLast) && op.rowout == nullptr)
         block_reduce_write<7, 4>(sm, mx, op.partial + (int64_t)blockIdx.x * kCheckSlots);
 }
 
@@ -415,6 +511,30 @@ constexpr int kRestartBlocks = 512;
 
 static inline double nmax_host(double a, double b) { return (a > b || a != a) ? a : b; }
 
+// The check results from the 22 reduced values: [0..10] over rows (7 sums,
+// 4 maxima), [11..21] over columns.
+static void finish_check(const double* red, double sign, double obj_const, sov_hip_check_out* out) {
+    const double* r = red;
+    const double* q = red + 11;
+    out->kkt_cur = nmax_host(r[7], q[7]);
+    out->kkt_avg = nmax_host(r[8], q[8]);
+    out->orig_cur = nmax_host(r[9], q[9]);
+    out->orig_avg = nmax_host(r[10], q[10]);
+    const double bn = std::sqrt(r[2]), cn = std::sqrt(q[2]);
+    for (int w = 0; w < 2; ++w) {
+        const double pv2 = r[0 + w], yres2 = r[3 + w], dobj_r = r[5 + w];
+        const double dres2 = q[0 + w], pobj = q[3 + w], dobj_c = q[5 + w];
+        const double p = sign * pobj + obj_const;
+        const double d = sign * (dobj_r + dobj_c) + obj_const;
+        double* rel = w == 0 ? out->rel_cur : out->rel_avg;
+        rel[0] = std::sqrt(pv2) / (1.0 + bn);
+        rel[1] = std::sqrt(yres2 + dres2) / (1.0 + cn);
+        rel[2] = std::fabs(p - d) / (1.0 + std::fabs(p) + std::fabs(d));
+        if (w == 0) { out->pobj_cur = p; out->dobj_cur = d; }
+        else { out->pobj_avg = p; out->dobj_avg = d; }
+    }
+}
+
 template <typename T>
 static const T* vals_of(const Part& p) {
     return std::is_same<T, double>::value ? reinterpret_cast<const T*>(p.val64) : reinterpret_cast<const T*>(p.val32);
@@ -473,26 +593,242 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
     SOV_HIP_TRY(check_launch());
     SOV_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, ctx->d_red, 22 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const double* r = ctx->h_pinned;        // [0..10] rows, [11..21] columns
-    const double* q = ctx->h_pinned + 11;
-    out->kkt_cur = nmax_host(r[7], q[7]);
-    out->kkt_avg = nmax_host(r[8], q[8]);
-    out->orig_cur = nmax_host(r[9], q[9]);
-    out->orig_avg = nmax_host(r[10], q[10]);
-    const double bn = std::sqrt(r[2]), cn = std::sqrt(q[2]);
-    const double sg = a->obj_sign;
-    for (int w = 0; w < 2; ++w) {
-        const double pv2 = r[0 + w], yres2 = r[3 + w], dobj_r = r[5 + w];
-        const double dres2 = q[0 + w], pobj = q[3 + w], dobj_c = q[5 + w];
-        const double p = sg * pobj + a->obj_const;
-        const double d = sg * (dobj_r + dobj_c) + a->obj_const;
-        double* rel = w == 0 ? out->rel_cur : out->rel_avg;
-        rel[0] = std::sqrt(pv2) / (1.0 + bn);
-        rel[1] = std::sqrt(yres2 + dres2) / (1.0 + cn);
-        rel[2] = std::fabs(p - d) / (1.0 + std::fabs(p) + std::fabs(d));
-        if (w == 0) { out->pobj_cur = p; out->dobj_cur = d; }
-        else { out->pobj_avg = p; out->dobj_avg = d; }
+    finish_check(ctx->h_pinned, a->obj_sign, a->obj_const, out);
+    return SOV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches of LPs (block-diagonal in one CSR)
+// ---------------------------------------------------------------------------
+
+// (start, length) of instance b's rows of class c in dimension dim.
+__device__ __forceinline__ int2 batch_range(const int32_t* seg, int b, int dim, int c) {
+    const int32_t* p = seg + (((int64_t)b * 2 + dim) * kClasses + c) * 2;
+    return make_int2(p[0], p[1]);
+}
+
+// Block b reduces instance b's 11 per-row terms of both dimensions: slots
+// 0..6 summed, 7..10 maximised; out[b * 22 + dim * 11 + slot].
+__global__ void __launch_bounds__(kBlock) k_batch_reduce(const int32_t* __restrict__ seg,
+                                                         const double* __restrict__ rows_t, int64_t m,
+                                                         const double* __restrict__ cols_t, int64_t n,
+                                                         double* __restrict__ out) {
+    const int b = blockIdx.x;
+    for (int dim = 0; dim < 2; ++dim) {
+        const double* t = dim == 0 ? rows_t : cols_t;
+        const int64_t stride = dim == 0 ? m : n;
+        for (int slot = 0; slot < 11; ++slot) {
+            const bool is_max = slot >= 7;
+            double acc = 0.0;
+            for (int c = 0; c < kClasses; ++c) {
+                const int2 r = batch_range(seg, b, dim, c);
+                for (int i = threadIdx.x; i < r.y; i += kBlock) {
+                    const double v = t[slot * stride + r.x + i];
+                    acc = is_max ? nmax(acc, v) : acc + v;
+                }
+            }
+            double sm[1] = {acc}, mx[1] = {acc};
+            double* dst = out + (int64_t)b * 22 + dim * 11 + slot;
+            if (is_max) block_reduce_write<0, 1>(sm, mx, dst);
+            else block_reduce_write<1, 0>(sm, mx, dst);
+            __syncthreads();   // the reduction's LDS is reused by the next slot
+        }
     }
+}
+
+// out[b] = sum of v^2 over instance b's entries of dimension dim.
+__global__ void __launch_bounds__(kBlock) k_batch_sumsq(const int32_t* __restrict__ seg, int dim,
+                                                        const double* __restrict__ v, double* __restrict__ out) {
+    const int b = blockIdx.x;
+    double acc = 0.0;
+    for (int c = 0; c < kClasses; ++c) {
+        const int2 r = batch_range(seg, b, dim, c);
+        for (int i = threadIdx.x; i < r.y; i += kBlock) {
+            const double t = v[r.x + i];
+            acc += t * t;
+        }
+    }
+    double sm[1] = {acc}, mx[1] = {0.0};
+    block_reduce_write<1, 0>(sm, mx, out + b);
+}
+
+__global__ void k_batch_scale(int32_t len, const int32_t* __restrict__ inst, const double* __restrict__ fac,
+                              double* __restrict__ v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] *= fac[inst[i]];
+}
+
+template <typename T>
+__global__ void k_batch_restart_x(int32_t n, const int32_t* __restrict__ inst, const int32_t* __restrict__ flags,
+                                  const int32_t* __restrict__ since, const T* __restrict__ xsum,
+                                  const T* __restrict__ lo, const T* __restrict__ hi, T* __restrict__ x) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const int b = inst[j];
+        if (!flags[b]) continue;
+        const int32_t k = since[b];
+        double v = double(xsum[j]) * (1.0 / double(k > 0 ? k : 1));
+        const double l = double(lo[j]), h = double(hi[j]);
+        if (v < l) v = l;
+        if (v > h) v = h;
+        x[j] = T(v);
+    }
+}
+
+template <typename T>
+__global__ void k_batch_restart_y(int32_t m, const int32_t* __restrict__ inst, const int32_t* __restrict__ flags,
+                                  const int32_t* __restrict__ since, const T* __restrict__ ysum,
+                                  const double* __restrict__ act, const T* __restrict__ lo,
+                                  const T* __restrict__ hi, T* __restrict__ y, T* __restrict__ s) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = inst[i];
+        if (!flags[b]) continue;
+        const int32_t k = since[b];
+        y[i] = T(double(ysum[i]) * (1.0 / double(k > 0 ? k : 1)));
+        double a = act[i];
+        const double l = double(lo[i]), h = double(hi[i]);
+        if (a < l) a = l;
+        if (a > h) a = h;
+        s[i] = T(a);
+    }
+}
+
+// Activity bits of every block of one part's step launch (see sov_hip_batch::blk).
+__global__ void k_block_flags(Sched s, int32_t blocks, const int32_t* __restrict__ inst,
+                              const int32_t* __restrict__ active, int32_t* __restrict__ flags) {
+    for (int64_t blk = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; blk < blocks;
+         blk += (int64_t)gridDim.x * blockDim.x) {
+        int c = 0;
+#pragma unroll
+        for (int k = 1; k < kClasses; ++k)
+            if (blk >= pick(s.block_begin, k)) c = k;
+        const int G = pick(s.G, c), rpb = kBlock / G;
+        const int64_t local0 = (blk - pick(s.block_begin, c)) * (int64_t)rpb;
+        const int64_t cnt = min((int64_t)rpb, (int64_t)pick(s.count, c) - local0);
+        const int64_t r0 = pick(s.offset, c) + local0;
+        int f = 0;
+        for (int64_t r = 0; r < cnt; ++r) {
+            const int a = active[inst[r0 + r]];
+            f |= a;   // 1 -> bit 0, 2 -> bit 1
+        }
+        flags[blk] = f;
+    }
+}
+
+// (Re)compute the block activity bits of every part of both orientations.
+static int batch_block_flags(sov_hip_ctx* ctx, sov_hip_batch* bt) {
+    const sov_hip_csr* h = bt->csr;
+    for (int o = 0; o < 2; ++o) {
+        const Orient& orient = o == 0 ? h->a : h->t;
+        const int np = orient.nparts();
+        if ((int)bt->blk[o].size() < np) {
+            bt->blk[o].resize(np, nullptr);
+            bt->blk_cap[o].resize(np, 0);
+        }
+        for (int q = 0; q < np; ++q) {
+            const Sched sch = orient.part(q).sched();
+            const int32_t blocks = sch.block_begin[kClasses];
+            if (blocks > bt->blk_cap[o][q]) {
+                (void)hipFree(bt->blk[o][q]);
+                bt->blk[o][q] = nullptr;
+                // room for every lane choice: at most one block per 4 rows plus one per class
+                const int64_t cap = std::max<int64_t>(blocks, (int64_t)orient.nat.rows / 4 + kClasses + 1);
+                SOV_HIP_TRY(hipMalloc(&bt->blk[o][q], sizeof(int32_t) * cap));
+                bt->blk_cap[o][q] = cap;
+            }
+            if (blocks > 0)
+                k_block_flags<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(
+                    sch, blocks, o == 0 ? bt->row_inst : bt->col_inst, bt->active, bt->blk[o][q]);
+        }
+    }
+    SOV_HIP_TRY(check_launch());
+    return SOV_OK;
+}
+
+template <typename T>
+static BatchView batch_view(const sov_hip_batch* bt, bool rows, int k, int part = -1) {
+    BatchView v;
+    v.inst = rows ? bt->row_inst : bt->col_inst;
+    v.tau = bt->tau;
+    v.sigma = bt->sigma;
+    v.active = bt->active;
+    v.since = bt->since;
+    v.k = k;
+    v.want = std::is_same<T, float>::value ? 1 : 2;
+    if (part >= 0) v.blk = bt->blk[rows ? 0 : 1][part];
+    return v;
+}
+
+// Move flagged instances from an FP32 state to the FP64 one: their candidate
+// (the current point, or the Cesaro average when use_avg) becomes both the
+// FP64 point and its running sum (a restart with since = 1 then lands on it).
+template <bool Rows>
+__global__ void k_batch_promote(int32_t len, const int32_t* __restrict__ inst, const int32_t* __restrict__ flags,
+                                const int32_t* __restrict__ use_avg, const int32_t* __restrict__ since,
+                                const float* __restrict__ cur32, const float* __restrict__ sum32,
+                                double* __restrict__ cur64, double* __restrict__ sum64) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = inst[i];
+        if (!flags[b]) continue;
+        const int32_t k = since[b];
+        const double v = use_avg[b] ? double(sum32[i]) * (1.0 / double(k > 0 ? k : 1)) : double(cur32[i]);
+        cur64[i] = v;
+        sum64[i] = v;
+    }
+}
+
+template <typename T>
+static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,
+                            const sov_hip_check_args* a, const double* obj_const, sov_hip_check_out* out) {
+    const sov_hip_pdhg_state* st = a->st;
+    const Orient& OA = h->a;
+    const Orient& OT = h->t;
+    const int npa = OA.nparts(), npt = OT.nparts();
+    for (const Orient* o : {&OA, &OT})
+        for (const Part& p : o->parts)
+            if (p.nnz > 0 && vals_of<T>(p) == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch check: value image missing");
+    double* rows_t = bt->terms;
+    double* cols_t = bt->terms + (int64_t)11 * bt->m;
+    for (int k = 0; k < npa; ++k) {
+        const Part& p = OA.part(k);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
+        CheckRowOp<T, true> op{};
+        op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
+        op.x = static_cast<const T*>(st->x); op.xsum = static_cast<const T*>(st->x_sum);
+        op.y = static_cast<const T*>(st->y); op.ysum = static_cast<const T*>(st->y_sum);
+        op.lo_s = static_cast<const T*>(st->row_lo); op.hi_s = static_cast<const T*>(st->row_hi);
+        op.lo_o = a->orig_row_lo; op.hi_o = a->orig_row_hi; op.rs = a->row_scale;
+        op.act_avg = a->act_avg; op.acc = OA.chk_acc; op.rowout = rows_t; op.stride = bt->m;
+        op.inv = 1.0; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.mode = seg_mode(k, npa); op.s = s;
+        op.bv = batch_view<T>(bt, true, 0);
+        k_check<CheckRowOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    }
+    for (int k = 0; k < npt; ++k) {
+        const Part& p = OT.part(k);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
+        CheckColOp<T, true> op{};
+        op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
+        op.y = static_cast<const T*>(st->y); op.ysum = static_cast<const T*>(st->y_sum);
+        op.x = static_cast<const T*>(st->x); op.xsum = static_cast<const T*>(st->x_sum);
+        op.c_s = static_cast<const T*>(st->c);
+        op.lo_s = static_cast<const T*>(st->col_lo); op.hi_s = static_cast<const T*>(st->col_hi);
+        op.lo_o = a->orig_col_lo; op.hi_o = a->orig_col_hi; op.c_o = a->orig_c; op.cs = a->col_scale;
+        op.acc = OT.chk_acc; op.rowout = cols_t; op.stride = bt->n;
+        op.inv = 1.0; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign;
+        op.mode = seg_mode(k, npt); op.s = s;
+        op.bv = batch_view<T>(bt, false, 0);
+        k_check<CheckColOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    }
+    k_batch_reduce<<<bt->count, kBlock, 0, ctx->stream>>>(bt->seg, rows_t, bt->m, cols_t, bt->n, bt->red);
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipMemcpyAsync(bt->h_red, bt->red, sizeof(double) * 22 * bt->count, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < bt->count; ++b)
+        finish_check(bt->h_red + (int64_t)b * 22, a->obj_sign, obj_const ? obj_const[b] : 0.0, out + b);
     return SOV_OK;
 }
 
@@ -623,9 +959,336 @@ int set_point_impl(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double*
     return SOV_OK;
 }
 
+template <typename T>
+int batch_primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
+                      int k) {
+    const Orient& o = h->t;
+    const int np = o.nparts();
+    for (int q = 0; q < np; ++q) {
+        const Part& p = o.part(q);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
+        const T* val = vals_of<T>(p);
+        if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "batch primal_step: value image missing");
+        PrimalOp<T, true> op{p.ptr, p.idx, val, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
+                             static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi),
+                             static_cast<T*>(st->x), static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum),
+                             orient_acc<T>(o), T(0), 0, seg_mode(q, np), s, batch_view<T>(bt, false, k, q)};
+        k_step<PrimalOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    }
+    SOV_HIP_TRY(check_launch());
+    return SOV_OK;
+}
+
+template <typename T>
+int batch_dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
+                    int k) {
+    const Orient& o = h->a;
+    const int np = o.nparts();
+    for (int q = 0; q < np; ++q) {
+        const Part& p = o.part(q);
+        const Sched s = p.sched();
+        const int blocks = s.block_begin[kClasses];
+        if (blocks == 0) continue;
+        const T* val = vals_of<T>(p);
+        if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "batch dual_step: value image missing");
+        DualOp<T, true> op{p.ptr, p.idx, val, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
+                           static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
+                           static_cast<T*>(st->y_sum), orient_acc<T>(o), T(0), T(0), 0, seg_mode(q, np), s,
+                           batch_view<T>(bt, true, k, q)};
+        k_step<DualOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+    }
+    SOV_HIP_TRY(check_launch());
+    return SOV_OK;
+}
+
+int batch_ok(const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st) {
+    if (bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch is null");
+    if (h != nullptr && h != bt->csr) return fail(SOV_INVALID_ARGUMENT, "batch belongs to another csr handle");
+    if (h != nullptr)
+        for (int o = 0; o < 2; ++o) {
+            const Orient& orient = o == 0 ? h->a : h->t;
+            if ((int)bt->blk[o].size() < orient.nparts())
+                return fail(SOV_INVALID_ARGUMENT, "batch: segments changed since the last sov_hip_batch_set(active)");
+            for (int q = 0; q < orient.nparts(); ++q)
+                if (orient.part(q).blocks() > bt->blk_cap[o][q])
+                    return fail(SOV_INVALID_ARGUMENT, "batch: lanes changed since the last sov_hip_batch_set(active)");
+        }
+    if (h != nullptr && (bt->m != h->m || bt->n != h->n)) return fail(SOV_DIMENSION_MISMATCH, "batch dims vs csr");
+    if (st != nullptr && (st->m != bt->m || st->n != bt->n)) return fail(SOV_DIMENSION_MISMATCH, "batch dims vs state");
+    return SOV_OK;
+}
+
+void free_batch(sov_hip_batch* bt) {
+    for (void* p : {(void*)bt->row_inst, (void*)bt->col_inst, (void*)bt->seg, (void*)bt->tau, (void*)bt->sigma,
+                    (void*)bt->active, (void*)bt->since, (void*)bt->flags, (void*)bt->use_avg, (void*)bt->fac,
+                    (void*)bt->terms,
+                    (void*)bt->red})
+        (void)hipFree(p);
+    if (bt->h_red) (void)hipHostFree(bt->h_red);
+    for (int o = 0; o < 2; ++o)
+        for (int32_t* p : bt->blk[o]) (void)hipFree(p);
+    delete bt;
+}
+
+// Instance of each device-order row of one dimension, and the per-class runs.
+int batch_layout(const std::vector<int32_t>& perm, const std::vector<int32_t>& off, const Orient& o, int dim,
+                 std::vector<int32_t>& inst_dev, std::vector<int32_t>& seg, int count) {
+    const int32_t rows = static_cast<int32_t>(perm.size());
+    std::vector<int32_t> inst_nat(static_cast<size_t>(rows));
+    for (int b = 0; b < count; ++b)
+        for (int32_t r = off[b]; r < off[b + 1]; ++r) inst_nat[static_cast<size_t>(r)] = b;
+    inst_dev.resize(static_cast<size_t>(rows));
+    for (int32_t p = 0; p < rows; ++p) inst_dev[static_cast<size_t>(p)] = inst_nat[static_cast<size_t>(perm[p])];
+    for (int c = 0; c < kClasses; ++c) {
+        const int32_t lo = o.class_offset[c], hi = lo + o.class_count[c];
+        int32_t prev = -1;
+        for (int32_t p = lo; p < hi; ++p) {
+            const int32_t b = inst_dev[static_cast<size_t>(p)];
+            if (b < prev) return fail(SOV_INVALID_ARGUMENT, "batch: instance rows are not contiguous per class");
+            int32_t* e = &seg[((static_cast<size_t>(b) * 2 + dim) * kClasses + c) * 2];
+            if (b != prev) e[0] = p;
+            ++e[1];
+            prev = b;
+        }
+    }
+    return SOV_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t count, const int32_t* row_off,
+                         const int32_t* col_off, sov_hip_batch** out) {
+    if (ctx == nullptr || h == nullptr || out == nullptr || count < 1 || row_off == nullptr || col_off == nullptr)
+        return fail(SOV_INVALID_ARGUMENT, "batch_create args");
+    *out = nullptr;
+    if (row_off[0] != 0 || col_off[0] != 0 || row_off[count] != h->m || col_off[count] != h->n)
+        return fail(SOV_DIMENSION_MISMATCH, "batch_create: offsets do not cover the matrix");
+    for (int b = 0; b < count; ++b)
+        if (row_off[b + 1] < row_off[b] || col_off[b + 1] < col_off[b])
+            return fail(SOV_INVALID_ARGUMENT, "batch_create: offsets must be non-decreasing");
+    DeviceGuard3 g(ctx->device);
+    std::vector<int32_t> pr(static_cast<size_t>(h->m)), pc(static_cast<size_t>(h->n));
+    int rc = sov_hip_csr_order(ctx, h, 0, pr.data());
+    if (rc == SOV_OK) rc = sov_hip_csr_order(ctx, h, 1, pc.data());
+    if (rc != SOV_OK) return rc;
+    std::vector<int32_t> ro(row_off, row_off + count + 1), co(col_off, col_off + count + 1);
+    std::vector<int32_t> rinst, cinst, seg(static_cast<size_t>(count) * 2 * kClasses * 2, 0);
+    rc = batch_layout(pr, ro, h->a, 0, rinst, seg, count);
+    if (rc == SOV_OK) rc = batch_layout(pc, co, h->t, 1, cinst, seg, count);
+    if (rc != SOV_OK) return rc;
+    auto* bt = new (std::nothrow) sov_hip_batch();
+    if (bt == nullptr) return fail(SOV_OUT_OF_MEMORY, "batch");
+    bt->count = count;
+    bt->m = h->m;
+    bt->n = h->n;
+    bt->device = ctx->device;
+    bt->csr = h;
+    const size_t m1 = std::max<int32_t>(h->m, 1), n1 = std::max<int32_t>(h->n, 1);
+    hipError_t e = hipMalloc(&bt->row_inst, sizeof(int32_t) * m1);
+    if (e == hipSuccess) e = hipMalloc(&bt->col_inst, sizeof(int32_t) * n1);
+    if (e == hipSuccess) e = hipMalloc(&bt->seg, sizeof(int32_t) * seg.size());
+    if (e == hipSuccess) e = hipMalloc(&bt->tau, sizeof(double) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->sigma, sizeof(double) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->active, sizeof(int32_t) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->since, sizeof(int32_t) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->flags, sizeof(int32_t) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->use_avg, sizeof(int32_t) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->fac, sizeof(double) * count);
+    if (e == hipSuccess) e = hipMalloc(&bt->terms, sizeof(double) * 11 * (m1 + n1));
+    if (e == hipSuccess) e = hipMalloc(&bt->red, sizeof(double) * 22 * count);
+    if (e == hipSuccess) e = hipHostMalloc(&bt->h_red, sizeof(double) * 22 * count, hipHostMallocDefault);
+    if (e == hipSuccess && h->m > 0)
+        e = hipMemcpy(bt->row_inst, rinst.data(), sizeof(int32_t) * h->m, hipMemcpyHostToDevice);
+    if (e == hipSuccess && h->n > 0)
+        e = hipMemcpy(bt->col_inst, cinst.data(), sizeof(int32_t) * h->n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->seg, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice);
+    std::vector<int32_t> zero(static_cast<size_t>(count), 0), one(static_cast<size_t>(count), 2);
+    std::vector<double> unit(static_cast<size_t>(count), 1.0);
+    if (e == hipSuccess) e = hipMemcpy(bt->tau, unit.data(), sizeof(double) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->sigma, unit.data(), sizeof(double) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->active, one.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->since, zero.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(bt->flags, zero.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_batch(bt);
+        return hip_fail(e, "batch_create");
+    }
+    rc = batch_block_flags(ctx, bt);
+    if (rc == SOV_OK) {
+        const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e2 != hipSuccess) rc = hip_fail(e2, "batch_create flags");
+    }
+    if (rc != SOV_OK) {
+        free_batch(bt);
+        return rc;
+    }
+    *out = bt;
+    return SOV_OK;
+}
+
+void sov_hip_batch_destroy(sov_hip_batch* bt) {
+    if (bt == nullptr) return;
+    DeviceGuard3 g(bt->device);
+    (void)hipDeviceSynchronize();
+    free_batch(bt);
+}
+
+int sov_hip_batch_set(sov_hip_ctx* ctx, sov_hip_batch* bt, const double* tau, const double* sigma,
+                      const int32_t* active, const int32_t* since) {
+    if (ctx == nullptr || bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch_set args");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch_set during capture");
+    DeviceGuard3 g(ctx->device);
+    const size_t c = static_cast<size_t>(bt->count);
+    if (tau) SOV_HIP_TRY(hipMemcpyAsync(bt->tau, tau, sizeof(double) * c, hipMemcpyHostToDevice, ctx->stream));
+    if (sigma) SOV_HIP_TRY(hipMemcpyAsync(bt->sigma, sigma, sizeof(double) * c, hipMemcpyHostToDevice, ctx->stream));
+    if (active) {
+        SOV_HIP_TRY(hipMemcpyAsync(bt->active, active, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
+        const int rc = batch_block_flags(ctx, bt);
+        if (rc != SOV_OK) return rc;
+    }
+    if (since) SOV_HIP_TRY(hipMemcpyAsync(bt->since, since, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));   // the host arrays may be reused at once
+    return SOV_OK;
+}
+
+int sov_hip_batch_primal_step(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,
+                              const sov_hip_pdhg_state* st, int k) {
+    if (ctx == nullptr || h == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch primal_step args");
+    int rc = state_ok(h, st);
+    if (rc == SOV_OK) rc = batch_ok(h, bt, st);
+    if (rc != SOV_OK) return rc;
+    DeviceGuard3 g(ctx->device);
+    return st->prec == SOV_PREC_F64 ? batch_primal_impl<double>(ctx, h, bt, st, k)
+                                    : batch_primal_impl<float>(ctx, h, bt, st, k);
+}
+
+int sov_hip_batch_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,
+                            const sov_hip_pdhg_state* st, int k) {
+    if (ctx == nullptr || h == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch dual_step args");
+    int rc = state_ok(h, st);
+    if (rc == SOV_OK) rc = batch_ok(h, bt, st);
+    if (rc != SOV_OK) return rc;
+    DeviceGuard3 g(ctx->device);
+    return st->prec == SOV_PREC_F64 ? batch_dual_impl<double>(ctx, h, bt, st, k)
+                                    : batch_dual_impl<float>(ctx, h, bt, st, k);
+}
+
+int sov_hip_batch_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,
+                        const sov_hip_check_args* a, const double* obj_const, sov_hip_check_out* out) {
+    if (ctx == nullptr || h == nullptr || a == nullptr || out == nullptr)
+        return fail(SOV_INVALID_ARGUMENT, "batch check args");
+    int rc = state_ok(h, a->st);
+    if (rc == SOV_OK) rc = batch_ok(h, bt, a->st);
+    if (rc != SOV_OK) return rc;
+    if ((h->m > 0 && (!a->row_scale || !a->orig_row_lo || !a->orig_row_hi)) ||
+        (h->n > 0 && (!a->col_scale || !a->orig_col_lo || !a->orig_col_hi || !a->orig_c)))
+        return fail(SOV_INVALID_ARGUMENT, "batch check: null original-space array");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch check during capture");
+    DeviceGuard3 g(ctx->device);
+    return a->st->prec == SOV_PREC_F64 ? batch_check_impl<double>(ctx, h, bt, a, obj_const, out)
+                                       : batch_check_impl<float>(ctx, h, bt, a, obj_const, out);
+}
+
+int sov_hip_batch_restart(sov_hip_ctx* ctx, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
+                          const int32_t* flags, const double* act) {
+    if (ctx == nullptr || flags == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch restart args");
+    int rc = state_ok(nullptr, st);
+    if (rc == SOV_OK) rc = batch_ok(nullptr, bt, st);
+    if (rc != SOV_OK) return rc;
+    if (st->m > 0 && act == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch restart: act is null");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch restart during capture");
+    DeviceGuard3 g(ctx->device);
+    SOV_HIP_TRY(hipMemcpyAsync(bt->flags, flags, sizeof(int32_t) * bt->count, hipMemcpyHostToDevice, ctx->stream));
+    if (st->prec == SOV_PREC_F64) {
+        if (st->n > 0)
+            k_batch_restart_x<double><<<launch_blocks(st->n), kBlock, 0, ctx->stream>>>(
+                st->n, bt->col_inst, bt->flags, bt->since, static_cast<const double*>(st->x_sum),
+                static_cast<const double*>(st->col_lo), static_cast<const double*>(st->col_hi),
+                static_cast<double*>(st->x));
+        if (st->m > 0)
+            k_batch_restart_y<double><<<launch_blocks(st->m), kBlock, 0, ctx->stream>>>(
+                st->m, bt->row_inst, bt->flags, bt->since, static_cast<const double*>(st->y_sum), act,
+                static_cast<const double*>(st->row_lo), static_cast<const double*>(st->row_hi),
+                static_cast<double*>(st->y), static_cast<double*>(st->s));
+    } else {
+        if (st->n > 0)
+            k_batch_restart_x<float><<<launch_blocks(st->n), kBlock, 0, ctx->stream>>>(
+                st->n, bt->col_inst, bt->flags, bt->since, static_cast<const float*>(st->x_sum),
+                static_cast<const float*>(st->col_lo), static_cast<const float*>(st->col_hi),
+                static_cast<float*>(st->x));
+        if (st->m > 0)
+            k_batch_restart_y<float><<<launch_blocks(st->m), kBlock, 0, ctx->stream>>>(
+                st->m, bt->row_inst, bt->flags, bt->since, static_cast<const float*>(st->y_sum), act,
+                static_cast<const float*>(st->row_lo), static_cast<const float*>(st->row_hi),
+                static_cast<float*>(st->y), static_cast<float*>(st->s));
+    }
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));   // flags' host array may be reused at once
+    return SOV_OK;
+}
+
+int sov_hip_batch_sumsq(sov_hip_ctx* ctx, const sov_hip_batch* bt, int which, const double* v, double* out) {
+    if (ctx == nullptr || bt == nullptr || out == nullptr || (which != 0 && which != 1) ||
+        ((which == 0 ? bt->m : bt->n) > 0 && v == nullptr))
+        return fail(SOV_INVALID_ARGUMENT, "batch sumsq args");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch sumsq during capture");
+    DeviceGuard3 g(ctx->device);
+    k_batch_sumsq<<<bt->count, kBlock, 0, ctx->stream>>>(bt->seg, which, v, bt->red);
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipMemcpyAsync(bt->h_red, bt->red, sizeof(double) * bt->count, hipMemcpyDeviceToHost, ctx->stream));
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::copy(bt->h_red, bt->h_red + bt->count, out);
+    return SOV_OK;
+}
+
+int sov_hip_batch_promote(sov_hip_ctx* ctx, const sov_hip_batch* bt, const sov_hip_pdhg_state* st32,
+                          const sov_hip_pdhg_state* st64, const int32_t* flags, const int32_t* use_avg) {
+    if (ctx == nullptr || flags == nullptr || use_avg == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch promote args");
+    int rc = state_ok(nullptr, st32);
+    if (rc == SOV_OK) rc = state_ok(nullptr, st64);
+    if (rc == SOV_OK) rc = batch_ok(nullptr, bt, st32);
+    if (rc == SOV_OK) rc = batch_ok(nullptr, bt, st64);
+    if (rc != SOV_OK) return rc;
+    if (st32->prec != SOV_PREC_F32 || st64->prec != SOV_PREC_F64)
+        return fail(SOV_INVALID_ARGUMENT, "batch promote: needs an FP32 and an FP64 state");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch promote during capture");
+    DeviceGuard3 g(ctx->device);
+    const size_t c = static_cast<size_t>(bt->count);
+    SOV_HIP_TRY(hipMemcpyAsync(bt->flags, flags, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
+    SOV_HIP_TRY(hipMemcpyAsync(bt->use_avg, use_avg, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
+    if (st32->n > 0)
+        k_batch_promote<false><<<launch_blocks(st32->n), kBlock, 0, ctx->stream>>>(
+            st32->n, bt->col_inst, bt->flags, bt->use_avg, bt->since, static_cast<const float*>(st32->x),
+            static_cast<const float*>(st32->x_sum), static_cast<double*>(st64->x), static_cast<double*>(st64->x_sum));
+    if (st32->m > 0)
+        k_batch_promote<true><<<launch_blocks(st32->m), kBlock, 0, ctx->stream>>>(
+            st32->m, bt->row_inst, bt->flags, bt->use_avg, bt->since, static_cast<const float*>(st32->y),
+            static_cast<const float*>(st32->y_sum), static_cast<double*>(st64->y), static_cast<double*>(st64->y_sum));
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SOV_OK;
+}
+
+int sov_hip_batch_scale(sov_hip_ctx* ctx, const sov_hip_batch* bt, int which, const double* factors, double* v) {
+    if (ctx == nullptr || bt == nullptr || factors == nullptr || (which != 0 && which != 1))
+        return fail(SOV_INVALID_ARGUMENT, "batch scale args");
+    const int32_t len = which == 0 ? bt->m : bt->n;
+    if (len > 0 && v == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch scale: v is null");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch scale during capture");
+    DeviceGuard3 g(ctx->device);
+    SOV_HIP_TRY(hipMemcpyAsync(bt->fac, factors, sizeof(double) * bt->count, hipMemcpyHostToDevice, ctx->stream));
+    if (len > 0)
+        k_batch_scale<<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, which == 0 ? bt->row_inst : bt->col_inst,
+                                                                      bt->fac, v);
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SOV_OK;
+}
+
 
 int sov_hip_pdhg_primal_step(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau,
                              int first) {

@@ -301,3 +301,54 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
                   int(r.fp64_polish_iterations), x[:n], y[:m], rc[:n], msg.value.decode(), tr[:nt], ti[:nt],
                   farkas[:m] if STATUS[r.status] == "primal_infeasible" else None,
                   ray[:n] if STATUS[r.status] == "dual_infeasible" else None)
+
+
+def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, restart_check_period=40,
+                termination="relative", precision="f32", fp64_polish=True, use_graphs=True, autotune=True,
+                restarts="adaptive", fp32_switch_factor=None, fp32_stall_checks=None, verbose=False):
+    """PdhgBatch(options).solve(problems, hip backend) through
+    sov_pdhg_solve_batch: every model solved in one stacked device batch.
+    Returns (list of Result, dict of batch timings)."""
+    L = engine()
+    o = PdhgOpts()
+    L.sov_pdhg_default_opts(C.byref(o))
+    o.tolerance = tolerance
+    o.max_iterations = max_iterations
+    o.time_limit_seconds = time_limit
+    o.restart_check_period = restart_check_period
+    o.termination = 1 if termination == "relative" else 0
+    o.precision = 1 if precision == "f32" else 0
+    o.fp64_polish = int(fp64_polish)
+    o.use_graphs = int(use_graphs)
+    o.autotune = int(autotune)
+    o.restart_scheme = 1 if restarts == "adaptive" else 0
+    o.verbose = int(verbose)
+    if fp32_switch_factor is not None:
+        o.fp32_switch_factor = fp32_switch_factor
+    if fp32_stall_checks is not None:
+        o.fp32_stall_checks = fp32_stall_checks
+    count = len(lps)
+    handles = (C.c_void_p * count)(*[lp.handle for lp in lps])
+    res = (PdhgRes * count)()
+    dims = [lp.dims() for lp in lps]
+    xs = [np.zeros(max(d[1], 1)) for d in dims]
+    ys = [np.zeros(max(d[0], 1)) for d in dims]
+    xp = (_f64p * count)(*[ptr(x, C.c_double) for x in xs])
+    yp = (_f64p * count)(*[ptr(y, C.c_double) for y in ys])
+    timing = np.zeros(4)
+    L.sov_pdhg_solve_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(PdhgOpts), C.POINTER(PdhgRes),
+                                       C.POINTER(_f64p), C.POINTER(_f64p), _f64p]
+    rc_code = L.sov_pdhg_solve_batch(handles, count, C.byref(o), res, xp, yp, ptr(timing, C.c_double))
+    if rc_code != 0:
+        raise RuntimeError(f"sov_pdhg_solve_batch failed ({rc_code}): {last_error()}")
+    out = []
+    for b in range(count):
+        r = res[b]
+        m, n, _ = dims[b]
+        out.append(Result(STATUS[r.status], r.objective, r.primal_infeasibility, r.dual_infeasibility,
+                          r.duality_gap, int(r.iterations), int(r.restarts), r.seconds, r.setup_seconds,
+                          r.loop_seconds, r.operator_norm_estimate, (r.relative_primal, r.relative_dual,
+                                                                     r.relative_gap), r.primal_weight,
+                          int(r.fp64_polish_iterations), xs[b][:n], ys[b][:m], np.zeros(0)))
+    return out, {"seconds": timing[0], "setup_seconds": timing[1], "loop_seconds": timing[2],
+                 "iterations": int(timing[3])}

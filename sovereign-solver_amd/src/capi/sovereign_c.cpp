@@ -158,35 +158,68 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->restart_beta_artificial = d.restart_beta_artificial;
 }
 
+}  // extern "C"
+
+namespace {
+
+sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
+    sov::PdhgOptions p;
+    p.tolerance = o->tolerance;
+    p.max_iterations = o->max_iterations;
+    p.time_limit_seconds = o->time_limit_seconds;
+    p.restart_check_period = o->restart_check_period;
+    p.restart_sufficient_decrease = o->restart_sufficient_decrease;
+    p.step_size_safety = o->step_size_safety;
+    p.power_iterations = o->power_iterations;
+    p.verbose = o->verbose != 0;
+    p.engine = static_cast<sov::PdhgEngine>(o->engine);
+    p.termination = static_cast<sov::PdhgTermination>(o->termination);
+    p.precision = o->precision == 1 ? sov::Precision::F32 : sov::Precision::F64;
+    p.fp64_polish = o->fp64_polish != 0;
+    p.fp32_switch_factor = o->fp32_switch_factor;
+    p.fp32_stall_checks = o->fp32_stall_checks;
+    p.use_primal_weight = o->use_primal_weight != 0;
+    p.initial_primal_weight = o->initial_primal_weight;
+    p.primal_weight_smoothing = o->primal_weight_smoothing;
+    p.detect_infeasibility = o->detect_infeasibility != 0;
+    p.use_graphs = o->use_graphs != 0;
+    p.autotune = o->autotune != 0;
+    p.restart_scheme = o->restart_scheme == 1 ? sov::PdhgRestartScheme::Adaptive : sov::PdhgRestartScheme::Reference;
+    p.restart_beta_sufficient = o->restart_beta_sufficient;
+    p.restart_beta_necessary = o->restart_beta_necessary;
+    p.restart_beta_artificial = o->restart_beta_artificial;
+    return p;
+}
+
+void fill_result(const sov::PdhgResult& r, sov_pdhg_result* res) {
+    std::memset(res, 0, sizeof(*res));
+    res->status = static_cast<int32_t>(r.status);
+    res->objective = r.objective;
+    res->primal_infeasibility = r.primal_infeasibility;
+    res->dual_infeasibility = r.dual_infeasibility;
+    res->duality_gap = r.duality_gap;
+    res->iterations = r.iterations;
+    res->restarts = r.restarts;
+    res->seconds = r.seconds;
+    res->setup_seconds = r.setup_seconds;
+    res->loop_seconds = r.loop_seconds;
+    res->operator_norm_estimate = r.operator_norm_estimate;
+    res->relative_primal = r.relative_primal_residual;
+    res->relative_dual = r.relative_dual_residual;
+    res->relative_gap = r.relative_gap;
+    res->primal_weight = r.primal_weight;
+    res->fp64_polish_iterations = r.fp64_polish_iterations;
+}
+
+}  // namespace
+
+extern "C" {
+
 int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, double* x, double* y, double* rc,
                    sov_pdhg_result* res, char* msg, int32_t msg_len) {
     if (lp == nullptr || o == nullptr || res == nullptr) return set_err(3, "null argument");
     try {
-        sov::PdhgOptions p;
-        p.tolerance = o->tolerance;
-        p.max_iterations = o->max_iterations;
-        p.time_limit_seconds = o->time_limit_seconds;
-        p.restart_check_period = o->restart_check_period;
-        p.restart_sufficient_decrease = o->restart_sufficient_decrease;
-        p.step_size_safety = o->step_size_safety;
-        p.power_iterations = o->power_iterations;
-        p.verbose = o->verbose != 0;
-        p.engine = static_cast<sov::PdhgEngine>(o->engine);
-        p.termination = static_cast<sov::PdhgTermination>(o->termination);
-        p.precision = o->precision == 1 ? sov::Precision::F32 : sov::Precision::F64;
-        p.fp64_polish = o->fp64_polish != 0;
-        p.fp32_switch_factor = o->fp32_switch_factor;
-        p.fp32_stall_checks = o->fp32_stall_checks;
-        p.use_primal_weight = o->use_primal_weight != 0;
-        p.initial_primal_weight = o->initial_primal_weight;
-        p.primal_weight_smoothing = o->primal_weight_smoothing;
-        p.detect_infeasibility = o->detect_infeasibility != 0;
-        p.use_graphs = o->use_graphs != 0;
-        p.autotune = o->autotune != 0;
-        p.restart_scheme = o->restart_scheme == 1 ? sov::PdhgRestartScheme::Adaptive : sov::PdhgRestartScheme::Reference;
-        p.restart_beta_sufficient = o->restart_beta_sufficient;
-        p.restart_beta_necessary = o->restart_beta_necessary;
-        p.restart_beta_artificial = o->restart_beta_artificial;
+        sov::PdhgOptions p = to_options(o);
         if (o->warm_start_dual != nullptr)
             p.warm_start_dual.assign(o->warm_start_dual, o->warm_start_dual + lp->p.num_rows());
         int64_t nt = 0;
@@ -202,23 +235,7 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
         const sov::BackendKind kind = backend == SOV_BACKEND_HIP ? sov::BackendKind::Hip : sov::BackendKind::Host;
         std::unique_ptr<sov::Backend> be = sov::make_backend(kind);
         const sov::PdhgResult r = sov::Pdhg(p).solve(lp->p, *be);
-        std::memset(res, 0, sizeof(*res));
-        res->status = static_cast<int32_t>(r.status);
-        res->objective = r.objective;
-        res->primal_infeasibility = r.primal_infeasibility;
-        res->dual_infeasibility = r.dual_infeasibility;
-        res->duality_gap = r.duality_gap;
-        res->iterations = r.iterations;
-        res->restarts = r.restarts;
-        res->seconds = r.seconds;
-        res->setup_seconds = r.setup_seconds;
-        res->loop_seconds = r.loop_seconds;
-        res->operator_norm_estimate = r.operator_norm_estimate;
-        res->relative_primal = r.relative_primal_residual;
-        res->relative_dual = r.relative_dual_residual;
-        res->relative_gap = r.relative_gap;
-        res->primal_weight = r.primal_weight;
-        res->fp64_polish_iterations = r.fp64_polish_iterations;
+        fill_result(r, res);
         res->trace_len = nt;
         if (o->farkas_out && !r.certificate.row_multipliers.empty())
             std::memcpy(o->farkas_out, r.certificate.row_multipliers.data(), r.certificate.row_multipliers.size() * sizeof(double));
@@ -231,6 +248,38 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
             std::strncpy(msg, r.message.c_str(), static_cast<std::size_t>(msg_len - 1));
             msg[msg_len - 1] = '\0';
         }
+        return 0;
+    } catch (const sov::Error& e) {
+        return set_err(static_cast<int>(e.status()), e.what());
+    } catch (const std::exception& e) {
+        return set_err(6, e.what());
+    }
+}
+
+int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* o, sov_pdhg_result* results,
+                         double* const* xs, double* const* ys, double* timing) {
+    if (lps == nullptr || o == nullptr || results == nullptr || count < 0) return set_err(3, "null argument");
+    try {
+        std::vector<const sov::Problem*> probs(static_cast<std::size_t>(count));
+        for (int32_t b = 0; b < count; ++b) {
+            if (lps[b] == nullptr) return set_err(3, "null model in batch");
+            probs[static_cast<std::size_t>(b)] = &lps[b]->p;
+        }
+        std::unique_ptr<sov::Backend> be = sov::make_backend(sov::BackendKind::Hip);
+        const sov::PdhgBatchResult br = sov::PdhgBatch(to_options(o)).solve(probs, *be);
+        for (int32_t b = 0; b < count; ++b) {
+            const sov::PdhgResult& r = br.results[static_cast<std::size_t>(b)];
+            fill_result(r, &results[b]);
+            if (xs && xs[b] && !r.primal.empty()) std::memcpy(xs[b], r.primal.data(), r.primal.size() * sizeof(double));
+            if (ys && ys[b] && !r.dual.empty()) std::memcpy(ys[b], r.dual.data(), r.dual.size() * sizeof(double));
+        }
+        if (timing) {
+            timing[0] = br.seconds;
+            timing[1] = br.setup_seconds;
+            timing[2] = br.loop_seconds;
+            timing[3] = static_cast<double>(br.iterations);
+        }
+        if (!br.message.empty()) g_err = br.message;
         return 0;
     } catch (const sov::Error& e) {
         return set_err(static_cast<int>(e.status()), e.what());

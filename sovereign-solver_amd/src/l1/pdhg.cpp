@@ -1033,4 +1033,474 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
     return result;
 }
 
+// ===========================================================================
+// batched engine (additive)
+// ===========================================================================
+
+namespace {
+
+struct BatchHandle {
+    sov_hip_batch* b = nullptr;
+    ~BatchHandle() { sov_hip_batch_destroy(b); }
+};
+
+}  // namespace
+
+PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Backend& backend) const {
+    PdhgBatchResult out;
+    const auto t0 = Clock::now();
+    const PdhgOptions& o = options_;
+    out.results.resize(problems.size());
+    for (PdhgResult& r : out.results) r.engine = "batched";
+    auto* ctx = static_cast<sov_hip_ctx*>(backend.native_context());
+    if (backend.kind() != BackendKind::Hip || ctx == nullptr) {
+        out.message = "the batched engine needs a Hip backend";
+        for (PdhgResult& r : out.results) {
+            r.status = PdhgStatus::NumericalFailure;
+            r.message = out.message;
+        }
+        return out;
+    }
+    // instances that pass validation are stacked; the others keep prevalidate's status
+    std::vector<std::size_t> idx;
+    for (std::size_t b = 0; b < problems.size(); ++b)
+        if (problems[b] != nullptr && prevalidate(*problems[b], out.results[b])) idx.push_back(b);
+    const std::size_t B = idx.size();
+    if (B == 0) return out;
+    const ObjSense sense = problems[idx[0]]->sense();
+    for (std::size_t k : idx)
+        if (problems[k]->sense() != sense) throw Error(Status::InvalidArgument, "PdhgBatch: instances differ in sense");
+    const Real sign = sense == ObjSense::Maximize ? -1.0 : 1.0;
+    try {
+        // -- stack the instances block-diagonally (natural order)
+        std::vector<int32_t> row_off{0}, col_off{0};
+        std::size_t nnz = 0;
+        for (std::size_t k : idx) {
+            const Problem& p = *problems[k];
+            row_off.push_back(row_off.back() + p.num_rows());
+            col_off.push_back(col_off.back() + p.num_cols());
+            nnz += static_cast<std::size_t>(p.matrix().nnz());
+        }
+        const auto m = static_cast<std::size_t>(row_off.back()), n = static_cast<std::size_t>(col_off.back());
+        std::vector<int32_t> rp(m + 1, 0), ci(std::max<std::size_t>(nnz, 1));
+        std::vector<double> vals(std::max<std::size_t>(nnz, 1));
+        std::vector<Real> c_nat(n), clo(n), chi(n), rlo(m), rhi(m), oconst(B);
+        {
+            std::size_t q = 0;
+            for (std::size_t b = 0; b < B; ++b) {
+                const Problem& p = *problems[idx[b]];
+                const CsrMatrix& A = p.matrix();
+                const auto r0 = static_cast<std::size_t>(row_off[b]), c0 = static_cast<std::size_t>(col_off[b]);
+                for (Idx r = 0; r < A.rows(); ++r) {
+                    for (Idx k = A.row_ptr()[r]; k < A.row_ptr()[r + 1]; ++k, ++q) {
+                        ci[q] = static_cast<int32_t>(c0) + A.col_idx()[k];
+                        vals[q] = A.values()[k];
+                    }
+                    rp[r0 + static_cast<std::size_t>(r) + 1] = static_cast<int32_t>(q);
+                }
+                for (Idx j = 0; j < p.num_cols(); ++j) {
+                    c_nat[c0 + j] = p.objective()[j];
+                    clo[c0 + j] = p.col_lower()[j];
+                    chi[c0 + j] = p.col_upper()[j];
+                }
+                for (Idx i = 0; i < p.num_rows(); ++i) {
+                    rlo[r0 + i] = p.row_lower()[i];
+                    rhi[r0 + i] = p.row_upper()[i];
+                }
+                oconst[b] = p.objective_constant();
+            }
+        }
+        DevMem d_rp(ctx, (m + 1) * sizeof(int32_t)), d_ci(ctx, ci.size() * sizeof(int32_t)),
+            d_v(ctx, vals.size() * sizeof(double));
+        hip_ok(sov_hip_memcpy_h2d(ctx, d_rp.p, rp.data(), (m + 1) * sizeof(int32_t)), "upload");
+        hip_ok(sov_hip_memcpy_h2d(ctx, d_ci.p, ci.data(), ci.size() * sizeof(int32_t)), "upload");
+        hip_ok(sov_hip_memcpy_h2d(ctx, d_v.p, vals.data(), vals.size() * sizeof(double)), "upload");
+        CsrHandle csr;
+        hip_ok(sov_hip_csr_create(ctx, static_cast<int32_t>(m), static_cast<int32_t>(n), static_cast<int32_t>(nnz),
+                                  d_rp.as<int32_t>(), d_ci.as<int32_t>(), d_v.as<double>(), SOV_CSR_OWN_VALUES, &csr.h),
+               "csr_create");
+        d_v.reset();
+
+        // -- Ruiz + Pock-Chambolle on the stack: row and column norms never
+        //    cross instances, so these are each instance's own factors (the
+        //    Ruiz early stop is taken over the whole batch)
+        DevMem d_rs(ctx, std::max<std::size_t>(m, 1) * 8), d_cs(ctx, std::max<std::size_t>(n, 1) * 8);
+        {
+            const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, d_rs.p, ones_m.data(), m * 8), "upload");
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, d_cs.p, ones_n.data(), n * 8), "upload");
+            const ScalingOptions so;
+            if (so.apply_ruiz)
+                for (int sweep = 0; sweep < so.ruiz_iterations; ++sweep) {
+                    double worst = 0.0;
+                    hip_ok(sov_hip_scale_ruiz_sweep(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>(), &worst), "ruiz");
+                    if (worst < so.ruiz_tolerance) break;
+                }
+            if (so.apply_pock_chambolle)
+                hip_ok(sov_hip_scale_pock_chambolle(ctx, csr.h, so.pock_chambolle_alpha, d_rs.as<double>(),
+                                                    d_cs.as<double>()),
+                       "pock_chambolle");
+        }
+        std::vector<Real> rs(m), cs(n);
+        if (m) hip_ok(sov_hip_memcpy_d2h(ctx, rs.data(), d_rs.p, m * 8), "download scaling");
+        if (n) hip_ok(sov_hip_memcpy_d2h(ctx, cs.data(), d_cs.p, n * 8), "download scaling");
+        hip_ok(sov_hip_csr_scale(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>()), "csr_scale");
+        DeviceOrder ord;
+        ord.row.resize(m);
+        ord.col.resize(n);
+        hip_ok(sov_hip_csr_order(ctx, csr.h, 0, ord.row.data()), "order");
+        hip_ok(sov_hip_csr_order(ctx, csr.h, 1, ord.col.data()), "order");
+        BatchHandle bh;
+        hip_ok(sov_hip_batch_create(ctx, csr.h, static_cast<int32_t>(B), row_off.data(), col_off.data(), &bh.b),
+               "batch_create");
+
+        // scaled vectors (device order)
+        std::vector<Real> c_s(n), clo_s(n), chi_s(n), rlo_s(m), rhi_s(m);
+        for (std::size_t j = 0; j < n; ++j) {
+            c_s[j] = (c_nat[j] * cs[j]) * sign;
+            clo_s[j] = is_finite_bound(clo[j]) ? clo[j] / cs[j] : clo[j];
+            chi_s[j] = is_finite_bound(chi[j]) ? chi[j] / cs[j] : chi[j];
+        }
+        for (std::size_t i = 0; i < m; ++i) {
+            rlo_s[i] = is_finite_bound(rlo[i]) ? rlo[i] * rs[i] : rlo[i];
+            rhi_s[i] = is_finite_bound(rhi[i]) ? rhi[i] * rs[i] : rhi[i];
+        }
+        c_s = DeviceOrder::to_dev(c_s, ord.col);
+        clo_s = DeviceOrder::to_dev(clo_s, ord.col);
+        chi_s = DeviceOrder::to_dev(chi_s, ord.col);
+        rlo_s = DeviceOrder::to_dev(rlo_s, ord.row);
+        rhi_s = DeviceOrder::to_dev(rhi_s, ord.row);
+        DevMem d_olo_r(ctx, std::max<std::size_t>(m, 1) * 8), d_ohi_r(ctx, std::max<std::size_t>(m, 1) * 8),
+            d_olo_c(ctx, std::max<std::size_t>(n, 1) * 8), d_ohi_c(ctx, std::max<std::size_t>(n, 1) * 8),
+            d_oc(ctx, std::max<std::size_t>(n, 1) * 8), d_act(ctx, std::max<std::size_t>(m, 1) * 8),
+            d_rs_dev(ctx, std::max<std::size_t>(m, 1) * 8), d_cs_dev(ctx, std::max<std::size_t>(n, 1) * 8);
+        auto upload_dev = [&](const DevMem& d, std::span<const Real> v, const std::vector<int32_t>& perm) {
+            if (v.empty()) return;
+            const std::vector<Real> pv = DeviceOrder::to_dev(v, perm);
+            hip_ok(sov_hip_memcpy_h2d(ctx, d.p, pv.data(), pv.size() * 8), "upload");
+        };
+        upload_dev(d_olo_r, rlo, ord.row);
+        upload_dev(d_ohi_r, rhi, ord.row);
+        upload_dev(d_rs_dev, rs, ord.row);
+        upload_dev(d_olo_c, clo, ord.col);
+        upload_dev(d_ohi_c, chi, ord.col);
+        upload_dev(d_oc, c_nat, ord.col);
+        upload_dev(d_cs_dev, cs, ord.col);
+
+        // -- per-instance power iteration (reference pdhg.cpp:48-79 on each block)
+        std::vector<Real> est(B, 0.0);
+        {
+            DevMem u(ctx, std::max<std::size_t>(n, 1) * 8), v(ctx, std::max<std::size_t>(m, 1) * 8),
+                w(ctx, std::max<std::size_t>(m, 1) * 8);
+            const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, u.p, ones_n.data(), n * 8), "upload");
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, v.p, ones_m.data(), m * 8), "upload");
+            std::vector<char> live(B, 1);
+            std::vector<double> sq(B), sq2(B), fac(B);
+            for (int it = 0; it < o.power_iterations; ++it) {
+                hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, -1.0, u.p, 0.0, w.p), "power spmv");
+                hip_ok(sov_hip_axpy_f64(ctx, static_cast<int64_t>(m), 1.0, v.as<double>(), w.as<double>()), "power axpy");
+                hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 0, w.as<double>(), sq.data()), "power norms");
+                for (std::size_t b = 0; b < B; ++b) {
+                    const double wn = std::sqrt(sq[b]);
+                    if (live[b] && wn < 1e-300) live[b] = 0;
+                    fac[b] = live[b] ? 1.0 / wn : 1.0;
+                }
+                hip_ok(sov_hip_batch_scale(ctx, bh.b, 0, fac.data(), w.as<double>()), "power scale");
+                hip_ok(sov_hip_spmv_device(ctx, csr.h, 1, SOV_PREC_F64, -1.0, w.p, 0.0, u.p), "power spmv_t");
+                if (m) hip_ok(sov_hip_memcpy_d2d(ctx, v.p, w.p, m * 8), "power copy");
+                hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 1, u.as<double>(), sq.data()), "power norms");
+                hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 0, v.as<double>(), sq2.data()), "power norms");
+                for (std::size_t b = 0; b < B; ++b) {
+                    if (live[b]) {
+                        est[b] = std::sqrt(sq[b] + sq2[b]);
+                        if (est[b] < 1e-300) live[b] = 0;
+                    }
+                    fac[b] = live[b] ? 1.0 / est[b] : 1.0;
+                }
+                hip_ok(sov_hip_batch_scale(ctx, bh.b, 1, fac.data(), u.as<double>()), "power scale");
+                hip_ok(sov_hip_batch_scale(ctx, bh.b, 0, fac.data(), v.as<double>()), "power scale");
+            }
+        }
+        std::vector<double> tau(B), sigma(B);
+        for (std::size_t b = 0; b < B; ++b) {
+            tau[b] = sigma[b] = est[b] > 1e-9 ? o.step_size_safety / est[b] : o.step_size_safety;
+            out.results[idx[b]].operator_norm_estimate = est[b];
+        }
+
+        // -- iterate states: FP64 always (the only one for an FP64 run, the
+        //    polish state of an FP32 run); FP32 for an FP32 run.  Each
+        //    instance lives in one of them (active = 1 or 2) and moves to
+        //    FP64 on its own once it is near tolerance or stalled.
+        const bool f32 = o.precision == Precision::F32;
+        if (f32) hip_ok(sov_hip_csr_make_f32(ctx, csr.h), "make_f32");
+        DeviceState S32, S64;
+        auto init_state = [&](DeviceState& S, int p) {
+            S.allocate(ctx, p, m, n);
+            S.upload(ctx, S.c, c_s);
+            S.upload(ctx, S.clo, clo_s);
+            S.upload(ctx, S.chi, chi_s);
+            S.upload(ctx, S.rlo, rlo_s);
+            S.upload(ctx, S.rhi, rhi_s);
+            const std::vector<Real> zm(m, 0.0), zn(n, 0.0);
+            DevMem zx(ctx, std::max<std::size_t>(n, 1) * 8), zs(ctx, std::max<std::size_t>(m, 1) * 8);
+            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, zx.p, zn.data(), n * 8), "upload");
+            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, zs.p, zm.data(), m * 8), "upload");
+            hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), nullptr, zs.as<double>()), "set_point");
+            hip_ok(sov_hip_sync(ctx), "sync");
+        };
+        if (f32) init_state(S32, SOV_PREC_F32);
+        if (!f32 || o.fp64_polish) init_state(S64, SOV_PREC_F64);
+        DevMem d_act32(ctx, std::max<std::size_t>(m, 1) * 8);
+        if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, f32 ? SOV_PREC_F32 : SOV_PREC_F64), "tune");
+
+        std::vector<int32_t> active(B, f32 ? 1 : 2), since(B, 0), flags32(B, 0), flags64(B, 0), since_stop(B, 0);
+        std::vector<int32_t> stop_state(B, 0), promote(B, 0), use_avg(B, 0);
+        std::vector<char> done(B, 0), cand_avg(B, 0), failed(B, 0);
+        std::vector<long long> its(B, 0), promoted_at(B, -1), restarts(B, 0);
+        std::vector<Real> res_at_restart(B, kInfinity), prev_cand(B, kInfinity), best_rel(B, kInfinity);
+        std::vector<int> stall(B, 0);
+        hip_ok(sov_hip_batch_set(ctx, bh.b, tau.data(), sigma.data(), active.data(), since.data()), "batch_set");
+        std::vector<int32_t> active_sent = active;
+
+        sov_hip_check_args ca{};
+        ca.tol_scaled = o.tolerance;
+        ca.tol_orig = std::max(o.tolerance, Real{1e-6});
+        ca.obj_sign = sign;
+        ca.obj_const = 0.0;
+        ca.row_scale = d_rs_dev.as<double>();
+        ca.col_scale = d_cs_dev.as<double>();
+        ca.orig_row_lo = d_olo_r.as<double>();
+        ca.orig_row_hi = d_ohi_r.as<double>();
+        ca.orig_col_lo = d_olo_c.as<double>();
+        ca.orig_col_hi = d_ohi_c.as<double>();
+        ca.orig_c = d_oc.as<double>();
+        std::vector<sov_hip_check_out> co32(B), co64(B);
+
+        const long long period = std::max<long long>(o.restart_check_period, 1);
+        GraphHandle graph;
+        int graph_key = -1;
+        bool has32 = f32, has64 = !f32;
+        auto run_steps = [&](long long count) {
+            for (long long k = 0; k < count; ++k)
+                for (DeviceState* S : {&S32, &S64}) {
+                    if ((S == &S32 && !has32) || (S == &S64 && !has64)) continue;
+                    hip_ok(sov_hip_batch_primal_step(ctx, csr.h, bh.b, &S->st, static_cast<int>(k)), "batch primal_step");
+                    hip_ok(sov_hip_batch_dual_step(ctx, csr.h, bh.b, &S->st, static_cast<int>(k)), "batch dual_step");
+                }
+        };
+        const bool adaptive = o.restart_scheme == PdhgRestartScheme::Adaptive;
+        std::size_t n_active = B;
+        long long it = 0;
+        const auto t_loop = Clock::now();
+        out.setup_seconds = seconds_since(t0);
+        bool timed_out = false;
+        while (n_active > 0 && it < o.max_iterations) {
+            if (o.time_limit_seconds > 0.0 && seconds_since(t0) > o.time_limit_seconds) {
+                timed_out = true;
+                break;
+            }
+            has32 = has64 = false;
+            for (std::size_t b = 0; b < B; ++b) {
+                has32 = has32 || active[b] == 1;
+                has64 = has64 || active[b] == 2;
+            }
+            const long long count = std::min<long long>(period, o.max_iterations - it);
+            if (o.use_graphs && count == period) {
+                const int key = (has32 ? 1 : 0) | (has64 ? 2 : 0);
+                if (graph.g == nullptr || graph_key != key) {
+                    graph.reset();
+                    hip_ok(sov_hip_graph_begin(ctx), "graph_begin");
+                    run_steps(count);
+                    hip_ok(sov_hip_graph_end(ctx, &graph.g), "graph_end");
+                    graph_key = key;
+                }
+                hip_ok(sov_hip_graph_launch(ctx, graph.g), "graph_launch");
+            } else {
+                run_steps(count);
+            }
+            it += count;
+            for (std::size_t b = 0; b < B; ++b)
+                if (active[b]) since[b] += static_cast<int32_t>(count);
+            hip_ok(sov_hip_batch_set(ctx, bh.b, nullptr, nullptr, nullptr, since.data()), "batch_set");
+            if (count != period) break;   // iteration limit inside the interval
+
+            if (has32) {
+                ca.st = &S32.st;
+                ca.act_avg = d_act32.as<double>();
+                hip_ok(sov_hip_batch_check(ctx, csr.h, bh.b, &ca, oconst.data(), co32.data()), "batch check");
+            }
+            if (has64) {
+                ca.st = &S64.st;
+                ca.act_avg = d_act.as<double>();
+                hip_ok(sov_hip_batch_check(ctx, csr.h, bh.b, &ca, oconst.data(), co64.data()), "batch check");
+            }
+            bool any32 = false, any64 = false, any_promote = false;
+            Real worst = 0.0;
+            for (std::size_t b = 0; b < B; ++b) {
+                if (!active[b]) continue;
+                const sov_hip_check_out& q = active[b] == 1 ? co32[b] : co64[b];
+                const Real rel_c = std::max({q.rel_cur[0], q.rel_cur[1], q.rel_cur[2]});
+                const Real rel_a = std::max({q.rel_avg[0], q.rel_avg[1], q.rel_avg[2]});
+                const bool ab = adaptive ? rel_a <= rel_c : q.kkt_avg <= q.kkt_cur;
+                const Real ores = ab ? q.orig_avg : q.orig_cur;
+                const Real rel_worst = ab ? rel_a : rel_c;
+                worst = std::max(worst, rel_worst);
+                if (!std::isfinite(q.kkt_cur) || !std::isfinite(q.kkt_avg) || !std::isfinite(ores)) {
+                    failed[b] = 1;
+                    stop_state[b] = active[b];
+                    active[b] = 0;
+                    its[b] = it;
+                    --n_active;
+                    continue;
+                }
+                const bool stop = o.termination == PdhgTermination::Reference ? (ores < o.tolerance)
+                                                                              : (rel_worst <= o.tolerance);
+                if (stop) {
+                    done[b] = 1;
+                    cand_avg[b] = ab;
+                    since_stop[b] = since[b];
+                    stop_state[b] = active[b];
+                    its[b] = it;
+                    active[b] = 0;
+                    --n_active;
+                    continue;
+                }
+                if (rel_worst < best_rel[b] * 0.99) {
+                    best_rel[b] = rel_worst;
+                    stall[b] = 0;
+                } else {
+                    ++stall[b];
+                }
+                if (active[b] == 1 && o.fp64_polish) {
+                    const Real gauge = o.termination == PdhgTermination::Reference ? ores : rel_worst;
+                    if (gauge <= o.fp32_switch_factor * o.tolerance || stall[b] >= o.fp32_stall_checks) {
+                        promote[b] = 1;
+                        use_avg[b] = ab;
+                        any_promote = true;
+                        continue;
+                    }
+                }
+                bool do_restart;
+                if (!adaptive) {
+                    if (!std::isfinite(res_at_restart[b])) res_at_restart[b] = q.kkt_cur;
+                    do_restart = q.kkt_avg <= o.restart_sufficient_decrease * res_at_restart[b];
+                } else {
+                    if (!std::isfinite(res_at_restart[b])) res_at_restart[b] = rel_c;
+                    do_restart = rel_worst <= o.restart_beta_sufficient * res_at_restart[b] ||
+                                 (rel_worst <= o.restart_beta_necessary * res_at_restart[b] && rel_worst > prev_cand[b]) ||
+                                 static_cast<Real>(since[b]) >= o.restart_beta_artificial * static_cast<Real>(it);
+                    prev_cand[b] = rel_worst;
+                }
+                if (do_restart) {
+                    const int32_t f = (!adaptive || ab) ? 1 : 0;
+                    (active[b] == 1 ? flags32 : flags64)[b] = f;
+                    any32 = any32 || (f && active[b] == 1);
+                    any64 = any64 || (f && active[b] == 2);
+                    res_at_restart[b] = adaptive ? rel_worst : q.kkt_avg;
+                    prev_cand[b] = kInfinity;
+                    ++restarts[b];
+                    since[b] = -1;   // reset once the restart kernels have read it
+                }
+            }
+            if (any32) hip_ok(sov_hip_batch_restart(ctx, bh.b, &S32.st, flags32.data(), d_act32.as<double>()), "restart");
+            if (any64) hip_ok(sov_hip_batch_restart(ctx, bh.b, &S64.st, flags64.data(), d_act.as<double>()), "restart");
+            if (any_promote) {
+                // candidate -> FP64 state; a restart with since = 1 places it, s from its FP64 activity
+                hip_ok(sov_hip_batch_promote(ctx, bh.b, &S32.st, &S64.st, promote.data(), use_avg.data()), "promote");
+                std::vector<int32_t> since1 = since;
+                for (std::size_t b = 0; b < B; ++b)
+                    if (promote[b]) since1[b] = 1;
+                hip_ok(sov_hip_batch_set(ctx, bh.b, nullptr, nullptr, nullptr, since1.data()), "batch_set");
+                hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, 1.0, S64.x.p, 0.0, d_act.p), "polish activity");
+                hip_ok(sov_hip_batch_restart(ctx, bh.b, &S64.st, promote.data(), d_act.as<double>()), "polish place");
+                for (std::size_t b = 0; b < B; ++b) {
+                    if (!promote[b]) continue;
+                    active[b] = 2;
+                    since[b] = 0;
+                    promoted_at[b] = it;
+                    res_at_restart[b] = kInfinity;
+                    prev_cand[b] = kInfinity;
+                    promote[b] = 0;
+                }
+                if (o.verbose)
+                    std::fprintf(stderr, "  [pdhg batch] iter %lld: instances moved to FP64\n", it);
+            }
+            for (std::size_t b = 0; b < B; ++b) {
+                flags32[b] = flags64[b] = 0;
+                if (since[b] < 0) since[b] = 0;
+            }
+            // the block activity bits are recomputed only when `active` changed
+            const bool changed = active != active_sent;
+            hip_ok(sov_hip_batch_set(ctx, bh.b, nullptr, nullptr, changed ? active.data() : nullptr, since.data()),
+                   "batch_set");
+            if (changed) active_sent = active;
+            if (o.verbose && (it / period) % 100 == 0)
+                std::fprintf(stderr, "  [pdhg batch] iter %lld  active %zu/%zu  worst rel kkt %.3g  %.1fs\n", it,
+                             n_active, B, static_cast<double>(worst), seconds_since(t0));
+        }
+        hip_ok(sov_hip_sync(ctx), "sync");
+        out.loop_seconds = seconds_since(t_loop);
+        out.iterations = it;
+
+        // -- final points: each instance's candidate from the state it ended
+        //    in, natural order, unscaled
+        struct Host {
+            std::vector<Real> x, y, xs, ys;
+        };
+        auto fetch = [&](DeviceState& S) {
+            Host h;
+            h.x = DeviceOrder::to_nat(S.download(ctx, S.x, n), ord.col);
+            h.y = DeviceOrder::to_nat(S.download(ctx, S.y, m), ord.row);
+            h.xs = DeviceOrder::to_nat(S.download(ctx, S.xsum, n), ord.col);
+            h.ys = DeviceOrder::to_nat(S.download(ctx, S.ysum, m), ord.row);
+            return h;
+        };
+        Host h32, h64;
+        if (f32) h32 = fetch(S32);
+        if (!f32 || o.fp64_polish) h64 = fetch(S64);
+        for (std::size_t b = 0; b < B; ++b) {
+            PdhgResult& r = out.results[idx[b]];
+            const Problem& p = *problems[idx[b]];
+            const int where = (done[b] || failed[b]) ? stop_state[b] : active[b];
+            const Host& hv = where == 1 ? h32 : h64;
+            const auto r0 = static_cast<std::size_t>(row_off[b]), c0 = static_cast<std::size_t>(col_off[b]);
+            const std::size_t mb = static_cast<std::size_t>(p.num_rows()), nb = static_cast<std::size_t>(p.num_cols());
+            const bool ua = done[b] && cand_avg[b];
+            const Real inv = 1.0 / static_cast<Real>(since_stop[b] > 0 ? since_stop[b] : 1);
+            std::vector<Real> x(nb), y(mb);
+            for (std::size_t j = 0; j < nb; ++j) x[j] = ua ? hv.xs[c0 + j] * inv : hv.x[c0 + j];
+            for (std::size_t i = 0; i < mb; ++i) y[i] = ua ? hv.ys[r0 + i] * inv : hv.y[r0 + i];
+            const Scaling sc = Scaling::from_factors(std::vector<Real>(rs.begin() + static_cast<std::ptrdiff_t>(r0),
+                                                                       rs.begin() + static_cast<std::ptrdiff_t>(r0 + mb)),
+                                                     std::vector<Real>(cs.begin() + static_cast<std::ptrdiff_t>(c0),
+                                                                       cs.begin() + static_cast<std::ptrdiff_t>(c0 + nb)));
+            auto [ux, uy] = unscale_point(sign, sc, x, y);
+            r.iterations = done[b] || failed[b] ? its[b] : it;
+            r.restarts = restarts[b];
+            if (promoted_at[b] >= 0) r.fp64_polish_iterations = r.iterations - promoted_at[b];
+            if (failed[b]) {
+                r.status = PdhgStatus::NumericalFailure;
+                r.message = "non-finite KKT residual";
+            } else if (done[b]) {
+                r.status = PdhgStatus::Optimal;
+            } else {
+                r.status = timed_out ? PdhgStatus::TimeLimit : PdhgStatus::IterationLimit;
+            }
+            finish_result(p, o, std::move(ux), std::move(uy), r);
+            r.seconds = seconds_since(t0);
+            r.setup_seconds = out.setup_seconds;
+            r.loop_seconds = out.loop_seconds;
+        }
+    } catch (const Error& e) {
+        out.message = std::string("device failure: ") + e.what();
+        for (std::size_t k : idx) {
+            out.results[k].status = PdhgStatus::NumericalFailure;
+            out.results[k].message = out.message;
+        }
+    }
+    out.seconds = seconds_since(t0);
+    return out;
+}
+
 }  // namespace sov
