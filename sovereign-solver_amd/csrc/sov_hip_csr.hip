@@ -141,7 +141,7 @@ __global__ void k_scan_add(int64_t len, int32_t* __restrict__ a, const int32_t* 
     }
 }
 
-static int inclusive_scan(sov_hip_ctx* ctx, int32_t* a, int64_t len) {
+int inclusive_scan(sov_hip_ctx* ctx, int32_t* a, int64_t len) {
     const int64_t tiles = (len + kScanTile - 1) / kScanTile;
     int32_t* sums = nullptr;
     SOV_HIP_TRY(hipMalloc(&sums, sizeof(int32_t) * (tiles + 1)));

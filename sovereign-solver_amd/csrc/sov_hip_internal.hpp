@@ -188,9 +188,16 @@ struct sov_hip_batch {
     double* terms = nullptr;        // 11 * (m + n): per-row check terms
     // per step-kernel block of each part of A (blk[0]) and A^T (blk[1]):
     // bit 0 = holds a row of an instance iterating in FP32, bit 1 = in FP64;
-    // blocks with neither exit at once.  Recomputed whenever `active` is set.
+    // and per part the compacted lists of the blocks with bit 0 (list[.][2q])
+    // and bit 1 (list[.][2q+1]) set, which the batched step launches run.
+    // Recomputed whenever `active` is set.
     std::vector<int32_t*> blk[2];
+    std::vector<int32_t*> list[2];
+    std::vector<int32_t> nlist[2];
     std::vector<int64_t> blk_cap[2];
+    int32_t* scan = nullptr;        // staging for the compaction
+    int64_t scan_cap = 0;
+    int64_t version = 0;            // bumped when the block lists change
     const sov_hip_csr* csr = nullptr;
     double* red = nullptr;          // count * 22
     double* h_red = nullptr;        // pinned host copy, count * 22
@@ -264,12 +271,12 @@ __device__ __forceinline__ void block_reduce_write(const double* s, const double
     }
 }
 
-// The class a workgroup belongs to (uniform per block).
-__device__ __forceinline__ int block_class(const Sched& s) {
+// The class a schedule block belongs to (uniform per workgroup).
+__device__ __forceinline__ int block_class(const Sched& s, int blk) {
     int c = 0;
 #pragma unroll
     for (int k = 1; k < kClasses; ++k)
-        if ((int)blockIdx.x >= s.block_begin[k]) c = k;
+        if (blk >= s.block_begin[k]) c = k;
     return c;
 }
 
@@ -283,14 +290,17 @@ __device__ __forceinline__ int pick(const int32_t (&a)[N], int c) {
     return v;
 }
 
-// Where a workgroup's rows come from: resolved once per block.
+// Where a workgroup's rows come from: resolved once per block.  blk is the
+// schedule block the workgroup runs: blockIdx.x, or an entry of a compacted
+// block list (batches skip the blocks of stopped instances).
 struct ClassView {
-    int cls, G, begin, count, offset;
+    int cls, G, begin, count, offset, blk;
 };
 
-__device__ __forceinline__ ClassView class_view(const Sched& s) {
+__device__ __forceinline__ ClassView class_view(const Sched& s, int blk = -1) {
     ClassView v;
-    v.cls = block_class(s);
+    v.blk = blk >= 0 ? blk : (int)blockIdx.x;
+    v.cls = block_class(s, v.blk);
     v.G = pick(s.G, v.cls);
     v.begin = pick(s.block_begin, v.cls);
     v.count = pick(s.count, v.cls);
@@ -303,7 +313,7 @@ __device__ __forceinline__ ClassView class_view(const Sched& s) {
 template <int G>
 __device__ __forceinline__ int row_of(const ClassView& v, int& lane) {
     constexpr int RPB = kBlock / G;
-    const int local = ((int)blockIdx.x - v.begin) * RPB + (int)threadIdx.x / G;
+    const int local = (v.blk - v.begin) * RPB + (int)threadIdx.x / G;
     lane = (int)threadIdx.x % G;
     return (local < v.count) ? v.offset + local : -1;
 }
@@ -364,5 +374,6 @@ int launch_blocks(int64_t n);      // blocks for an elementwise kernel over n
 hipError_t check_launch();         // hipGetLastError wrapper
 int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32);
 int ensure_partials(sov_hip_csr* csr);
+int inclusive_scan(sov_hip_ctx* ctx, int32_t* a, int64_t len);   // in place, synchronous
 
 }  // namespace sovhip

@@ -43,7 +43,7 @@ struct BatchView {
     const int32_t* since = nullptr;
     int k = 0;
     int want = 0;   // the active value this launch's state serves
-    const int32_t* blk = nullptr;   // per-block activity bits of this part (want's bit set = work here)
+    const int32_t* list = nullptr;  // the blocks this launch runs (compacted), or null for all
 };
 
 // Per column j (a row of A^T): g = c_j - (A^T y)_j; x+ = clamp(x - tau g);
@@ -66,11 +66,9 @@ struct PrimalOp {
     int mode;
     Sched s;
     BatchView bv;
+    __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     template <int G>
     __device__ void run(const ClassView& v) {
-        if constexpr (B) {
-            if ((bv.blk[blockIdx.x] & bv.want) == 0) return;   // uniform per block
-        }
         int lane;
         int row = row_of<G>(v, lane);
         T tau_r = tau;
@@ -123,11 +121,9 @@ struct DualOp {
     int mode;
     Sched s;
     BatchView bv;
+    __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     template <int G>
     __device__ void run(const ClassView& v) {
-        if constexpr (B) {
-            if ((bv.blk[blockIdx.x] & bv.want) == 0) return;   // uniform per block
-        }
         int lane;
         int row = row_of<G>(v, lane);
         T tau_r = tau, sigma_r = sigma;
@@ -165,7 +161,7 @@ struct DualOp {
 
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_step(Op op) {
-    const ClassView v = class_view(op.s);
+    const ClassView v = class_view(op.s, op.block());
     dispatch_G(v, op);
 }
 
@@ -275,6 +271,7 @@ struct CheckRowOp {
     int mode;
     Sched s;
     BatchView bv;
+    __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     // sums: pv2_c, pv2_a, bn2, yres2_c, yres2_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
@@ -339,6 +336,7 @@ struct CheckColOp {
     int mode;
     Sched s;
     BatchView bv;
+    __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     // sums: dres2_c, dres2_a, cn2, pobj_c, pobj_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
     __device__ void run(const ClassView& v, double (&sm)[7], double (&mx)[4]) const {
@@ -392,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_check(Op op) {
     for (int i = 0; i < 7; ++i) sm[i] = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) mx[i] = 0.0;
-    const ClassView v = class_view(op.s);
+    const ClassView v = class_view(op.s, op.block());
     CheckRunner<Op> r{op, sm, mx};
     dispatch_G(v, r);
     // mode is uniform over the launch: only a launch that completes rows reduces
@@ -608,12 +606,15 @@ __device__ __forceinline__ int2 batch_range(const int32_t* seg, int b, int dim, 
 }
 
 // Block b reduces instance b's 11 per-row terms of both dimensions: slots
-// 0..6 summed, 7..10 maximised; out[b * 22 + dim * 11 + slot].
+// 0..6 summed, 7..10 maximised; out[b * 22 + dim * 11 + slot].  Instances
+// not iterating in the checked state (active[b] != want) are skipped.
 __global__ void __launch_bounds__(kBlock) k_batch_reduce(const int32_t* __restrict__ seg,
                                                          const double* __restrict__ rows_t, int64_t m,
                                                          const double* __restrict__ cols_t, int64_t n,
+                                                         const int32_t* __restrict__ active, int want,
                                                          double* __restrict__ out) {
     const int b = blockIdx.x;
+    if (active[b] != want) return;   // uniform per block
     for (int dim = 0; dim < 2; ++dim) {
         const double* t = dim == 0 ? rows_t : cols_t;
         const int64_t stride = dim == 0 ? m : n;
@@ -714,6 +715,19 @@ __global__ void k_block_flags(Sched s, int32_t blocks, const int32_t* __restrict
     }
 }
 
+// keep[i] = 1 when bit `bit` of flags[i] is set.
+__global__ void k_flag_bit(int32_t n, const int32_t* __restrict__ flags, int bit, int32_t* __restrict__ keep) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        keep[i + 1] = (flags[i] >> bit) & 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) keep[0] = 0;
+}
+
+// list[pos[i]] = i for the kept blocks (pos = exclusive prefix sum, scan[i]).
+__global__ void k_compact(int32_t n, const int32_t* __restrict__ scan, int32_t* __restrict__ list) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (scan[i + 1] != scan[i]) list[scan[i]] = (int32_t)i;
+}
+
 // (Re)compute the block activity bits of every part of both orientations.
 static int batch_block_flags(sov_hip_ctx* ctx, sov_hip_batch* bt) {
     const sov_hip_csr* h = bt->csr;
@@ -722,25 +736,52 @@ static int batch_block_flags(sov_hip_ctx* ctx, sov_hip_batch* bt) {
         const int np = orient.nparts();
         if ((int)bt->blk[o].size() < np) {
             bt->blk[o].resize(np, nullptr);
+            bt->list[o].resize(2 * np, nullptr);
+            bt->nlist[o].resize(2 * np, 0);
             bt->blk_cap[o].resize(np, 0);
         }
         for (int q = 0; q < np; ++q) {
             const Sched sch = orient.part(q).sched();
             const int32_t blocks = sch.block_begin[kClasses];
             if (blocks > bt->blk_cap[o][q]) {
-                (void)hipFree(bt->blk[o][q]);
-                bt->blk[o][q] = nullptr;
+                for (int32_t** p : {&bt->blk[o][q], &bt->list[o][2 * q], &bt->list[o][2 * q + 1]}) {
+                    (void)hipFree(*p);
+                    *p = nullptr;
+                }
                 // room for every lane choice: at most one block per 4 rows plus one per class
                 const int64_t cap = std::max<int64_t>(blocks, (int64_t)orient.nat.rows / 4 + kClasses + 1);
                 SOV_HIP_TRY(hipMalloc(&bt->blk[o][q], sizeof(int32_t) * cap));
+                SOV_HIP_TRY(hipMalloc(&bt->list[o][2 * q], sizeof(int32_t) * cap));
+                SOV_HIP_TRY(hipMalloc(&bt->list[o][2 * q + 1], sizeof(int32_t) * cap));
                 bt->blk_cap[o][q] = cap;
             }
-            if (blocks > 0)
-                k_block_flags<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(
-                    sch, blocks, o == 0 ? bt->row_inst : bt->col_inst, bt->active, bt->blk[o][q]);
+            if (blocks + 1 > bt->scan_cap) {
+                (void)hipFree(bt->scan);
+                bt->scan = nullptr;
+                bt->scan_cap = 0;
+                SOV_HIP_TRY(hipMalloc(&bt->scan, sizeof(int32_t) * (blocks + 1)));
+                bt->scan_cap = blocks + 1;
+            }
+            bt->nlist[o][2 * q] = bt->nlist[o][2 * q + 1] = 0;
+            if (blocks == 0) continue;
+            k_block_flags<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(
+                sch, blocks, o == 0 ? bt->row_inst : bt->col_inst, bt->active, bt->blk[o][q]);
+            for (int bit = 0; bit < 2; ++bit) {
+                k_flag_bit<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(blocks, bt->blk[o][q], bit, bt->scan);
+                SOV_HIP_TRY(check_launch());
+                const int rc = inclusive_scan(ctx, bt->scan, (int64_t)blocks + 1);
+                if (rc != SOV_OK) return rc;
+                k_compact<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(blocks, bt->scan, bt->list[o][2 * q + bit]);
+                SOV_HIP_TRY(check_launch());
+                int32_t cnt = 0;
+                SOV_HIP_TRY(hipMemcpyAsync(&cnt, bt->scan + blocks, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+                SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+                bt->nlist[o][2 * q + bit] = cnt;
+            }
         }
     }
     SOV_HIP_TRY(check_launch());
+    ++bt->version;
     return SOV_OK;
 }
 
@@ -754,8 +795,13 @@ static BatchView batch_view(const sov_hip_batch* bt, bool rows, int k, int part 
     v.since = bt->since;
     v.k = k;
     v.want = std::is_same<T, float>::value ? 1 : 2;
-    if (part >= 0) v.blk = bt->blk[rows ? 0 : 1][part];
+    if (part >= 0) v.list = bt->list[rows ? 0 : 1][2 * part + (v.want - 1)];
     return v;
+}
+
+template <typename T>
+static int batch_blocks(const sov_hip_batch* bt, bool rows, int part) {
+    return bt->nlist[rows ? 0 : 1][2 * part + (std::is_same<T, float>::value ? 0 : 1)];
 }
 
 // Move flagged instances from an FP32 state to the FP64 one: their candidate
@@ -791,7 +837,7 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
     for (int k = 0; k < npa; ++k) {
         const Part& p = OA.part(k);
         const Sched s = p.sched();
-        const int blocks = s.block_begin[kClasses];
+        const int blocks = batch_blocks<T>(bt, true, k);
         if (blocks == 0) continue;
         CheckRowOp<T, true> op{};
         op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
@@ -801,13 +847,13 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
         op.lo_o = a->orig_row_lo; op.hi_o = a->orig_row_hi; op.rs = a->row_scale;
         op.act_avg = a->act_avg; op.acc = OA.chk_acc; op.rowout = rows_t; op.stride = bt->m;
         op.inv = 1.0; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.mode = seg_mode(k, npa); op.s = s;
-        op.bv = batch_view<T>(bt, true, 0);
+        op.bv = batch_view<T>(bt, true, 0, k);
         k_check<CheckRowOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     for (int k = 0; k < npt; ++k) {
         const Part& p = OT.part(k);
         const Sched s = p.sched();
-        const int blocks = s.block_begin[kClasses];
+        const int blocks = batch_blocks<T>(bt, false, k);
         if (blocks == 0) continue;
         CheckColOp<T, true> op{};
         op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
@@ -819,10 +865,11 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
         op.acc = OT.chk_acc; op.rowout = cols_t; op.stride = bt->n;
         op.inv = 1.0; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign;
         op.mode = seg_mode(k, npt); op.s = s;
-        op.bv = batch_view<T>(bt, false, 0);
+        op.bv = batch_view<T>(bt, false, 0, k);
         k_check<CheckColOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
-    k_batch_reduce<<<bt->count, kBlock, 0, ctx->stream>>>(bt->seg, rows_t, bt->m, cols_t, bt->n, bt->red);
+    k_batch_reduce<<<bt->count, kBlock, 0, ctx->stream>>>(bt->seg, rows_t, bt->m, cols_t, bt->n, bt->active,
+                                                          std::is_same<T, float>::value ? 1 : 2, bt->red);
     SOV_HIP_TRY(check_launch());
     SOV_HIP_TRY(hipMemcpyAsync(bt->h_red, bt->red, sizeof(double) * 22 * bt->count, hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -967,7 +1014,7 @@ int batch_primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batc
     for (int q = 0; q < np; ++q) {
         const Part& p = o.part(q);
         const Sched s = p.sched();
-        const int blocks = s.block_begin[kClasses];
+        const int blocks = batch_blocks<T>(bt, false, q);
         if (blocks == 0) continue;
         const T* val = vals_of<T>(p);
         if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "batch primal_step: value image missing");
@@ -989,7 +1036,7 @@ int batch_dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch*
     for (int q = 0; q < np; ++q) {
         const Part& p = o.part(q);
         const Sched s = p.sched();
-        const int blocks = s.block_begin[kClasses];
+        const int blocks = batch_blocks<T>(bt, true, q);
         if (blocks == 0) continue;
         const T* val = vals_of<T>(p);
         if (val == nullptr && p.nnz > 0) return fail(SOV_INVALID_ARGUMENT, "batch dual_step: value image missing");
@@ -1027,8 +1074,11 @@ void free_batch(sov_hip_batch* bt) {
                     (void*)bt->red})
         (void)hipFree(p);
     if (bt->h_red) (void)hipHostFree(bt->h_red);
-    for (int o = 0; o < 2; ++o)
+    for (int o = 0; o < 2; ++o) {
         for (int32_t* p : bt->blk[o]) (void)hipFree(p);
+        for (int32_t* p : bt->list[o]) (void)hipFree(p);
+    }
+    (void)hipFree(bt->scan);
     delete bt;
 }
 

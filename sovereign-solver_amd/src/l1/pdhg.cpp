@@ -1279,7 +1279,7 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
 
         const long long period = std::max<long long>(o.restart_check_period, 1);
         GraphHandle graph;
-        int graph_key = -1;
+        long long graph_key = -1, active_version = 0;
         bool has32 = f32, has64 = !f32;
         auto run_steps = [&](long long count) {
             for (long long k = 0; k < count; ++k)
@@ -1307,7 +1307,8 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
             }
             const long long count = std::min<long long>(period, o.max_iterations - it);
             if (o.use_graphs && count == period) {
-                const int key = (has32 ? 1 : 0) | (has64 ? 2 : 0);
+                // the launches' grids follow the active blocks: re-capture when they change
+                const long long key = active_version * 4 + (has32 ? 1 : 0) + (has64 ? 2 : 0);
                 if (graph.g == nullptr || graph_key != key) {
                     graph.reset();
                     hip_ok(sov_hip_graph_begin(ctx), "graph_begin");
@@ -1434,7 +1435,10 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
             const bool changed = active != active_sent;
             hip_ok(sov_hip_batch_set(ctx, bh.b, nullptr, nullptr, changed ? active.data() : nullptr, since.data()),
                    "batch_set");
-            if (changed) active_sent = active;
+            if (changed) {
+                active_sent = active;
+                ++active_version;
+            }
             if (o.verbose && (it / period) % 100 == 0)
                 std::fprintf(stderr, "  [pdhg batch] iter %lld  active %zu/%zu  worst rel kkt %.3g  %.1fs\n", it,
                              n_active, B, static_cast<double>(worst), seconds_since(t0));
