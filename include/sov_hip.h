@@ -321,6 +321,36 @@ int sov_hip_batch_sumsq(sov_hip_ctx* ctx, const sov_hip_batch* batch, int which,
 int sov_hip_batch_scale(sov_hip_ctx* ctx, const sov_hip_batch* batch, int which, const double* factors,
                         double* v);
 
+/* ---- the warm-start network (BF16 MFMA MLP) -----------------------------
+ * No reference counterpart.  A fully connected network with `layers` layers
+ * of widths widths[0] (input) .. widths[layers] (output), GELU after every
+ * layer but the last, BF16 weights and activations on the matrix cores with
+ * FP32 accumulation, FP32 master parameters trained with Adam.  Parameters
+ * are one flat FP32 array: for each layer W (out x in, row-major) then b.
+ * Batches are row-major [batch x features] FP32 device arrays. */
+typedef struct sov_hip_mlp sov_hip_mlp;
+int sov_hip_mlp_create(sov_hip_ctx* ctx, int32_t layers, const int32_t* widths, int32_t max_batch,
+                       sov_hip_mlp** out);
+void sov_hip_mlp_destroy(sov_hip_mlp* mlp);
+int64_t sov_hip_mlp_num_params(const sov_hip_mlp* mlp);
+/* Host FP32 parameters in (and the Adam moments reset) / out; grads (host,
+ * optional) receives the last backward pass's gradients. */
+int sov_hip_mlp_set_params(sov_hip_ctx* ctx, sov_hip_mlp* mlp, const float* params);
+int sov_hip_mlp_get_params(sov_hip_ctx* ctx, const sov_hip_mlp* mlp, float* params, float* grads);
+/* y = net(x) for `batch` rows (y may be NULL: the activations are kept for
+ * the backward pass either way). */
+int sov_hip_mlp_forward(sov_hip_ctx* ctx, sov_hip_mlp* mlp, int32_t batch, const float* d_x, float* d_y);
+/* Gradients of a loss with dL/dy = d_dy (device, batch x out) for the last
+ * forward pass, into the network's gradient array. */
+int sov_hip_mlp_backward(sov_hip_ctx* ctx, sov_hip_mlp* mlp, const float* d_dy);
+/* One Adam step with the current gradients (step counts from 1). */
+int sov_hip_mlp_adam(sov_hip_ctx* ctx, sov_hip_mlp* mlp, float lr, float beta1, float beta2, float eps,
+                     int32_t step);
+/* Weighted mean squared error L = sum_ij w_j (y_ij - t_ij)^2 / (batch out)
+ * (d_weight NULL = all ones): dL/dy into d_dy, L into *loss (host, optional). */
+int sov_hip_mlp_mse(sov_hip_ctx* ctx, sov_hip_mlp* mlp, int32_t batch, const float* d_y, const float* d_target,
+                    const float* d_weight, float* d_dy, double* loss);
+
 /* ---- launch graphs ----------------------------------------------------
  * Capture the context stream's launches between begin and end into a graph
  * and replay it.  Only the step kernels above may be captured (no host

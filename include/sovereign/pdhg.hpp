@@ -114,6 +114,9 @@ struct PdhgOptions {
     bool detect_infeasibility = false;
     /// Optional warm start: original-space row duals (length num_rows).
     std::vector<Real> warm_start_dual;
+    /// Optional warm start: original-space primal point (length num_cols);
+    /// projected onto the column box, with the slack set from its activity.
+    std::vector<Real> warm_start_primal;
     /// Capture each check interval as a HIP graph and replay it.
     bool use_graphs = true;
     /// Time SpMV lane counts per row-length class and column-segment counts

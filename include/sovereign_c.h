@@ -81,6 +81,7 @@ typedef struct sov_pdhg_opts {
     double restart_beta_artificial;
     double* farkas_out;              /* NULL or num_rows: the Farkas certificate when status is primal_infeasible */
     double* ray_out;                 /* NULL or num_cols: the primal ray when status is dual_infeasible */
+    const double* warm_start_primal; /* NULL or num_cols values (original units) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {
@@ -110,6 +111,17 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o);
  * PdhgStatus), else a Status code (e.g. the backend is unavailable). */
 int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, double* x, double* y, double* rc,
                    sov_pdhg_result* res, char* msg, int32_t msg_len);
+
+/* The parametric transportation family of the learned warm start
+ * (sov::gen::TransportFamily): a fixed S x D transportation structure, base
+ * supplies/demands and costs c0, and a fixed (S*D) x P matrix U.  Instance
+ * `index` draws theta ~ N(0, I_P) and demand factors lognormal(sigma = 0.2);
+ * its costs are c0 (1 + 0.2 tanh(U theta)).  features (NULL or S + D + P
+ * doubles) receives (row right-hand sides, theta), the network's input. */
+typedef struct sov_family sov_family;
+sov_family* sov_family_make(int32_t sources, int32_t sinks, int32_t params, uint64_t seed);
+void sov_family_destroy(sov_family* f);
+sov_lp* sov_family_instance(const sov_family* f, uint64_t seed, uint64_t index, double* features);
 
 /* Batched solve on the Hip backend (sov::PdhgBatch; no reference
  * counterpart): count models solved together, each with its own scaling,

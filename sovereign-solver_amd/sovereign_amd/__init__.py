@@ -40,7 +40,7 @@ class PdhgOpts(C.Structure):
         ("trace_cap", C.c_int64), ("trace_res", _f64p), ("trace_it", _i64p),
         ("restart_scheme", C.c_int32), ("autotune", C.c_int32), ("restart_beta_sufficient", C.c_double),
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
-        ("farkas_out", _f64p), ("ray_out", _f64p),
+        ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p),
     ]
 
 
@@ -209,6 +209,40 @@ class LP:
         return rs[:m], cs[:n]
 
 
+class Family:
+    """The parametric transportation family (sov_family_* of sovereign_c.h)."""
+
+    def __init__(self, sources=100, sinks=200, params=16, seed=0):
+        L = engine()
+        L.sov_family_make.restype = C.c_void_p
+        L.sov_family_make.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
+        L.sov_family_destroy.argtypes = [C.c_void_p]
+        L.sov_family_instance.restype = C.c_void_p
+        L.sov_family_instance.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, _f64p]
+        self.S, self.D, self.P = sources, sinks, params
+        self.h = L.sov_family_make(sources, sinks, params, seed)
+        if not self.h:
+            raise RuntimeError("family: " + last_error())
+
+    @property
+    def features(self):
+        return self.S + self.D + self.P
+
+    def instance(self, seed, index):
+        """(LP, features) of instance `index` of draw stream `seed`."""
+        f = np.zeros(self.features)
+        h = engine().sov_family_instance(self.h, seed, index, ptr(f, C.c_double))
+        if not h:
+            raise RuntimeError("family instance: " + last_error())
+        return LP(h), f
+
+    def __del__(self):
+        try:
+            engine().sov_family_destroy(self.h)
+        except Exception:
+            pass
+
+
 def batch_sizes(count, lo, hi, seed):
     out = np.zeros(count, np.int32)
     engine().sov_gen_batch_sizes(count, lo, hi, seed, ptr(out, C.c_int32))
@@ -248,7 +282,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           restart_check_period=40, engine_kind="auto", termination="reference", precision="f64",
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
-          restarts="reference", autotune=True, detect_infeasibility=False) -> Result:
+          restarts="reference", autotune=True, detect_infeasibility=False, primal_weight_smoothing=None,
+          warm_start_primal=None) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point."""
     L = engine()
     o = PdhgOpts()
@@ -268,6 +303,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     o.restart_scheme = 1 if restarts == "adaptive" else 0
     o.autotune = int(autotune)
     o.detect_infeasibility = int(detect_infeasibility)
+    if primal_weight_smoothing is not None:
+        o.primal_weight_smoothing = primal_weight_smoothing
     if fp32_switch_factor is not None:
         o.fp32_switch_factor = fp32_switch_factor
     if fp32_stall_checks is not None:
@@ -279,6 +316,11 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
         assert w.shape == (m,)
         keep.append(w)
         o.warm_start_dual = ptr(w, C.c_double)
+    if warm_start_primal is not None:
+        wp = np.ascontiguousarray(warm_start_primal, np.float64)
+        assert wp.shape == (n,)
+        keep.append(wp)
+        o.warm_start_primal = ptr(wp, C.c_double)
     tr = np.zeros(max(trace_cap, 1))
     ti = np.zeros(max(trace_cap, 1), np.int64)
     if trace_cap > 0:

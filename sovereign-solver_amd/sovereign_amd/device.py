@@ -68,6 +68,17 @@ def lib():
     L.sov_hip_graph_launch.argtypes = [_v, _v]
     L.sov_hip_graph_destroy.argtypes = [_v]
     L.sov_hip_ctx_props.argtypes = [_v, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    f32p = C.POINTER(C.c_float)
+    L.sov_hip_mlp_create.argtypes = [_v, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(_v)]
+    L.sov_hip_mlp_destroy.argtypes = [_v]
+    L.sov_hip_mlp_num_params.argtypes = [_v]
+    L.sov_hip_mlp_num_params.restype = C.c_int64
+    L.sov_hip_mlp_set_params.argtypes = [_v, _v, _v]
+    L.sov_hip_mlp_get_params.argtypes = [_v, _v, _v, _v]
+    L.sov_hip_mlp_forward.argtypes = [_v, _v, C.c_int32, _v, _v]
+    L.sov_hip_mlp_backward.argtypes = [_v, _v, _v]
+    L.sov_hip_mlp_adam.argtypes = [_v, _v, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32]
+    L.sov_hip_mlp_mse.argtypes = [_v, _v, C.c_int32, _v, _v, _v, _v, C.POINTER(d)]
     L._typed = True
     return L
 

@@ -1,0 +1,169 @@
+"""The learned warm start: a BF16 MLP on the GPU (sov_hip_mlp_* of
+include/sov_hip.h, hand-written MFMA kernels) that maps an instance of a
+parametric LP family to a dual warm start and a primal weight for PDHG.
+
+Pipeline (no reference counterpart):
+  dataset()   solves family instances cold with the batched GPU engine and
+              records (features, optimal dual, primal weight target);
+  train()     fits the network on the GPU (forward, backward and Adam all run
+              in this project's kernels);
+  predict()   runs the network on new instances;
+  warm vs cold solves are then compared by the caller.
+The primal-weight target is log(||y*_s|| / ||x*_s||), the ratio of the
+scaled optimal dual and primal norms, which is where PDLP's primal weight
+settles when PDHG starts from zero."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import device as D
+from . import solve_batch
+
+
+class Mlp:
+    """A network on the device: widths[0] inputs ... widths[-1] outputs."""
+
+    def __init__(self, ctx: D.Ctx, widths, max_batch):
+        self.ctx, self.L = ctx, ctx.L
+        self.widths = [int(w) for w in widths]
+        self.max_batch = int(max_batch)
+        self.h = C.c_void_p()
+        wa = (C.c_int32 * len(self.widths))(*self.widths)
+        D.check(self.L.sov_hip_mlp_create(ctx.h, len(self.widths) - 1, wa, self.max_batch, C.byref(self.h)))
+        self.nparams = int(self.L.sov_hip_mlp_num_params(self.h))
+        self._x = ctx.alloc(4 * self.max_batch * self.widths[0])
+        self._y = ctx.alloc(4 * self.max_batch * self.widths[-1])
+        self._t = ctx.alloc(4 * self.max_batch * self.widths[-1])
+        self._dy = ctx.alloc(4 * self.max_batch * self.widths[-1])
+        self._w = ctx.alloc(4 * self.widths[-1])
+        self.step = 0
+
+    def close(self):
+        self.L.sov_hip_mlp_destroy(self.h)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (self.nparams,)
+        D.check(self.L.sov_hip_mlp_set_params(self.ctx.h, self.h, p.ctypes.data))
+        self.step = 0
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        D.check(self.L.sov_hip_mlp_get_params(self.ctx.h, self.h, p.ctypes.data, g.ctypes.data))
+        return p, g
+
+    def _put(self, buf, a):
+        a = np.ascontiguousarray(a, np.float32)
+        D.check(self.L.sov_hip_memcpy_h2d(self.ctx.h, buf, a.ctypes.data, a.nbytes))
+
+    def forward(self, X):
+        X = np.asarray(X, np.float32)
+        B = X.shape[0]
+        assert 1 <= B <= self.max_batch and X.shape[1] == self.widths[0]
+        self._put(self._x, X)
+        D.check(self.L.sov_hip_mlp_forward(self.ctx.h, self.h, B, self._x, self._y))
+        return self.ctx.download(self._y, B * self.widths[-1], np.float32).reshape(B, self.widths[-1])
+
+    def backward(self, dY):
+        """Gradients of a loss with dL/dy = dY for the last forward pass."""
+        self._put(self._dy, dY)
+        D.check(self.L.sov_hip_mlp_backward(self.ctx.h, self.h, self._dy))
+        return self.params()[1]
+
+    def train_step(self, X, T, lr, weight=None, betas=(0.9, 0.999), eps=1e-8):
+        """One Adam step on the weighted mean squared error; returns the loss."""
+        X = np.asarray(X, np.float32)
+        B = X.shape[0]
+        self._put(self._x, X)
+        self._put(self._t, T)
+        if weight is not None:
+            self._put(self._w, weight)
+        D.check(self.L.sov_hip_mlp_forward(self.ctx.h, self.h, B, self._x, self._y))
+        loss = C.c_double()
+        D.check(self.L.sov_hip_mlp_mse(self.ctx.h, self.h, B, self._y, self._t, self._w if weight is not None else None,
+                                       self._dy, C.byref(loss)))
+        D.check(self.L.sov_hip_mlp_backward(self.ctx.h, self.h, self._dy))
+        self.step += 1
+        D.check(self.L.sov_hip_mlp_adam(self.ctx.h, self.h, lr, betas[0], betas[1], eps, self.step))
+        return loss.value
+
+    def predict(self, X, chunk=None):
+        X = np.asarray(X, np.float32)
+        chunk = chunk or self.max_batch
+        return np.concatenate([self.forward(X[i:i + chunk]) for i in range(0, len(X), chunk)])
+
+
+def init_params(widths, rng):
+    """He initialisation W ~ N(0, 2 / fan_in) for the GELU layers, N(0, 1 / fan_in) for the last; b = 0."""
+    parts = []
+    for l in range(1, len(widths)):
+        gain = 2.0 if l < len(widths) - 1 else 1.0
+        parts.append(rng.standard_normal(widths[l] * widths[l - 1]) * np.sqrt(gain / widths[l - 1]))
+        parts.append(np.zeros(widths[l]))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def dataset(family, seed, count, chunk=500, tolerance=1e-6, precision="f32", verbose=False):
+    """Solve `count` instances (indices 0..count-1 of draw stream `seed`)
+    cold with the batched engine.  Returns features (count x F), optimal
+    duals (count x m), log primal-weight targets (count), cold iterations."""
+    feats, duals, logw, its = [], [], [], []
+    for start in range(0, count, chunk):
+        idx = range(start, min(count, start + chunk))
+        lps, fs = zip(*(family.instance(seed, i) for i in idx))
+        res, tim = solve_batch(list(lps), tolerance=tolerance, precision=precision)
+        for lp, f, r in zip(lps, fs, res):
+            if r.status != "optimal":
+                continue
+            rs, cs = lp.scaling()
+            feats.append(f)
+            duals.append(r.y)
+            logw.append(np.log(np.linalg.norm(r.y / rs) / max(np.linalg.norm(r.x / cs), 1e-300)))
+            its.append(r.iterations)
+        if verbose:
+            print(f"  dataset: {min(count, start + chunk)}/{count} solved, batch {tim['seconds']:.1f}s", flush=True)
+    return np.array(feats), np.array(duals), np.array(logw), np.array(its)
+
+
+class Normalizer:
+    """Per-column affine maps of the network's inputs and outputs."""
+
+    def __init__(self, a):
+        self.mu = a.mean(axis=0)
+        self.sd = a.std(axis=0) + 1e-8
+
+    def fwd(self, a):
+        return (a - self.mu) / self.sd
+
+    def inv(self, a):
+        return a * self.sd + self.mu
+
+
+def train(ctx, X, Y, hidden=(512, 512, 512), epochs=60, batch=256, lr=1e-3, seed=0, verbose=False):
+    """Fit the network to (X, Y) on the GPU; returns (mlp, x_norm, y_norm, losses)."""
+    rng = np.random.default_rng(seed)
+    xn, yn = Normalizer(X), Normalizer(Y)
+    Xs, Ys = xn.fwd(X).astype(np.float32), yn.fwd(Y).astype(np.float32)
+    widths = [X.shape[1], *hidden, Y.shape[1]]
+    net = Mlp(ctx, widths, max(batch, 2048))
+    net.set_params(init_params(widths, rng))
+    losses = []
+    n = len(X)
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        lr_ep = lr * 0.5 * (1 + np.cos(np.pi * ep / epochs))    # cosine decay
+        tot = 0.0
+        for i in range(0, n - batch + 1, batch):
+            sel = order[i:i + batch]
+            tot += net.train_step(Xs[sel], Ys[sel], lr_ep) * len(sel)
+        losses.append(tot / max(1, (n // batch) * batch))
+        if verbose and (ep % 10 == 0 or ep == epochs - 1):
+            print(f"  epoch {ep}: loss {losses[-1]:.4g}", flush=True)
+    return net, xn, yn, losses
+
+
+def predict(net, xn, yn, X):
+    return yn.inv(net.predict(xn.fwd(X).astype(np.float32)).astype(np.float64))

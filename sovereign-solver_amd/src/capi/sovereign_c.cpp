@@ -15,6 +15,10 @@ struct sov_lp {
     sov::Problem p;
 };
 
+struct sov_family {
+    sov::gen::TransportFamily f;
+};
+
 namespace {
 thread_local std::string g_err;
 
@@ -108,6 +112,38 @@ sov_lp* sov_gen_random_lp_zipf(int32_t m, int32_t n, double a, int32_t kmin, int
 
 sov_lp* sov_gen_transportation(int32_t s, int32_t d, uint64_t seed) {
     return make_lp([&] { return sov::gen::transportation(s, d, seed).to_problem(); });
+}
+
+sov_family* sov_family_make(int32_t sources, int32_t sinks, int32_t params, uint64_t seed) {
+    if (sources <= 0 || sinks <= 0 || params <= 0) {
+        g_err = "sov_family_make: bad sizes";
+        return nullptr;
+    }
+    try {
+        return new sov_family{sov::gen::TransportFamily::make(sources, sinks, params, seed)};
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void sov_family_destroy(sov_family* f) { delete f; }
+
+sov_lp* sov_family_instance(const sov_family* f, uint64_t seed, uint64_t index, double* features) {
+    if (f == nullptr) {
+        g_err = "null family";
+        return nullptr;
+    }
+    return make_lp([&] {
+        std::vector<double> theta, z;
+        f->f.draw(seed, index, theta, z);
+        sov::gen::LpArrays a = f->f.instance(theta, z);
+        if (features != nullptr) {
+            std::copy(a.row_lo.begin(), a.row_lo.end(), features);
+            std::copy(theta.begin(), theta.end(), features + a.row_lo.size());
+        }
+        return a.to_problem();
+    });
 }
 
 int sov_gen_batch_sizes(int32_t count, int32_t lo, int32_t hi, uint64_t seed, int32_t* out) {
@@ -222,6 +258,8 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
         sov::PdhgOptions p = to_options(o);
         if (o->warm_start_dual != nullptr)
             p.warm_start_dual.assign(o->warm_start_dual, o->warm_start_dual + lp->p.num_rows());
+        if (o->warm_start_primal != nullptr)
+            p.warm_start_primal.assign(o->warm_start_primal, o->warm_start_primal + lp->p.num_cols());
         int64_t nt = 0;
         if (o->trace_res != nullptr && o->trace_cap > 0) {
             p.checkpoint = [&](long long it, const std::vector<double>&, const std::vector<double>&, double r) {

@@ -372,6 +372,18 @@ PdhgResult Pdhg::solve_primitive(const Problem& problem, Backend& be, const Canc
             for (std::size_t i = 0; i < mu; ++i) ys[i] = sign * o.warm_start_dual[i] / scaling.row_scale()[i];
             y = DeviceBuffer::upload(be, ys);
         }
+        if (!o.warm_start_primal.empty()) {
+            // x = Proj_X(x0 / c^s), s = Proj_S(A_s x) (additive)
+            if (o.warm_start_primal.size() != nu) throw Error(Status::DimensionMismatch, "warm_start_primal");
+            std::vector<Real> xs(nu), act(mu, 0.0);
+            for (std::size_t j = 0; j < nu; ++j) xs[j] = o.warm_start_primal[j] / scaling.col_scale()[j];
+            x = DeviceBuffer::upload(be, xs);
+            ok(be.project_box(n, col_lo.data(), col_hi.data(), x.data()), "project_box");
+            const std::vector<Real> xp = x.to_host();
+            scaled.matrix().multiply(xp, act);
+            s = DeviceBuffer::upload(be, act);
+            ok(be.project_box(m, row_lo.data(), row_hi.data(), s.data()), "project_box");
+        }
         DeviceBuffer x_new(be, nu), s_new(be, mu), x_bar(be, nu), s_bar(be, mu), grad(be, nu), kb(be, mu);
         DeviceBuffer x_sum(be, nu), y_sum(be, mu);
         std::vector<Real> x_last = x.to_host(), y_last = y.to_host();
@@ -805,6 +817,21 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 wy = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
                 if (m) hip_ok(sov_hip_memcpy_h2d(ctx, wy.p, ys.data(), m * 8), "upload warm start");
                 ysrc = wy.as<double>();
+            }
+            if (!o.warm_start_primal.empty()) {
+                // x = Proj_X(x0 / c^s), s = Proj_S(A_s x) (additive)
+                if (o.warm_start_primal.size() != n) throw Error(Status::DimensionMismatch, "warm_start_primal length");
+                std::vector<Real> xs(n);
+                for (std::size_t j = 0; j < n; ++j) {
+                    const Real lo = problem.col_lower()[j], hi = problem.col_upper()[j];
+                    Real v = o.warm_start_primal[j] / cs[j];
+                    if (is_finite_bound(lo)) v = std::max(v, lo / cs[j]);
+                    if (is_finite_bound(hi)) v = std::min(v, hi / cs[j]);
+                    xs[j] = v;
+                }
+                xs = DeviceOrder::to_dev(xs, ord.col);
+                if (n) hip_ok(sov_hip_memcpy_h2d(ctx, zx.p, xs.data(), n * 8), "upload warm start");
+                hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, 1.0, zx.p, 0.0, zs.p), "warm start activity");
             }
             hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), ysrc, zs.as<double>()), "set_point");
             hip_ok(sov_hip_sync(ctx), "sync");
