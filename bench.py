@@ -221,6 +221,29 @@ def batched(S, count=256, time_limit=150.0):
             "sum_m": int(sum(sizes)), "sum_nnz": int(sum(lp.dims()[2] for lp in lps))}
 
 
+MODEL = os.path.join(ROOT, "sovereign-solver_amd", "sovereign_amd", "models", "transport_100x200.npz")
+
+
+def learned_warm_start(ctx, count=200, time_subset=10):
+    """Warm vs cold on `count` held-out instances of the 100 x 200
+    transportation family with the committed network (trained on the GPU by
+    `python -m sovereign_amd.learned train`): median iterations and wall time
+    to 1e-6 relative KKT, cold start vs the network's dual + primal weight
+    (and a constant primal weight, no network, for reference)."""
+    if not os.path.exists(MODEL):
+        return None
+    import json as _json
+    import sovereign_amd as S
+    from sovereign_amd import learned as W
+    net, xn, yn = W.load(ctx, MODEL)
+    meta = _json.loads(str(np.load(MODEL, allow_pickle=False)["meta"]))
+    rep = W.evaluate(ctx, S.Family(100, 200, 16, 0), net, xn, yn, count, seed=2,
+                     const_logw=meta.get("mean_log_weight"), chunk=count, time_subset=time_subset)
+    rep["trained_on"] = meta.get("train")
+    net.close()
+    return rep
+
+
 def cpu_baseline(a, budget_iters):
     """CPU oracle (single thread) on the same instance: per-iteration time of
     the reference loop measured as (t(setup + k iterations) - t(setup)) / k."""
@@ -249,6 +272,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-LP batched solve")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
     ap.add_argument("--segments", default=None,
                     help="A,T column-segment counts to use instead of the autotuner's (profiling runs)")
     ap.add_argument("--calibrate", action="store_true",
@@ -334,6 +358,14 @@ def main():
         log(f"batched: {bat['instances']} LPs, {bat['optimal']} optimal in {bat['wall_s']} s -> "
             f"{bat['lps_per_s']} LPs/s (median {bat['iterations_median']}, max {bat['iterations_max']} iterations)")
 
+    lws = None
+    if not args.no_learned:
+        lws = learned_warm_start(ctx)
+        if lws:
+            log(f"learned warm start: median iterations cold {lws['cold']['median_iterations']:.0f} vs learned "
+                f"{lws['learned']['median_iterations']:.0f}; single-solve median "
+                f"{lws['single_solve_median_seconds']['cold']:.3f}s vs {lws['single_solve_median_seconds']['learned']:.3f}s")
+
     out = {
         "metric": "sec to 1e-6 rel KKT (cold vs learned warm start); batched LPs/s; SpMV HBM GB/s",
         "value": round(value, 2),
@@ -354,7 +386,8 @@ def main():
                    "column_segments": {"A": seg_a.value, "A^T": seg_t.value},
                    "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
                    "kkt_scaled_at_last_check": chk.kkt_cur if chk else None,
-                   "batched": bat},
+                   "batched": bat,
+                   "learned_warm_start": lws},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic[dom_name] if traffic else None,

@@ -182,8 +182,9 @@ private:
 /// PdhgOptions: tolerance, max_iterations, time_limit_seconds,
 /// restart_check_period, step_size_safety, power_iterations, termination,
 /// precision, fp64_polish, fp32_switch_factor, fp32_stall_checks,
-/// restart_scheme and its betas, use_graphs, autotune.  The primal weight
-/// stays 1; all instances must share one objective sense.
+/// restart_scheme and its betas, use_graphs, autotune.  The primal weight is
+/// fixed per instance (1 unless given); all instances must share one
+/// objective sense.
 struct PdhgBatchResult {
     std::vector<PdhgResult> results;   ///< one per instance, in input order
     long long iterations = 0;          ///< iterations the batch ran
@@ -196,7 +197,12 @@ struct PdhgBatchResult {
 class PdhgBatch {
 public:
     explicit PdhgBatch(PdhgOptions options = {}) : options_(std::move(options)) {}
-    PdhgBatchResult solve(const std::vector<const Problem*>& problems, Backend& backend) const;
+    /// warm_dual (optional): per instance, original-space row duals (empty =
+    /// cold); primal_weight (optional): per instance, a fixed primal weight
+    /// omega (tau = eta / omega, sigma = eta omega; default 1).
+    PdhgBatchResult solve(const std::vector<const Problem*>& problems, Backend& backend,
+                          const std::vector<std::vector<Real>>* warm_dual = nullptr,
+                          const std::vector<Real>* primal_weight = nullptr) const;
     const PdhgOptions& options() const noexcept { return options_; }
 
 private:

@@ -125,12 +125,15 @@ sov_lp* sov_family_instance(const sov_family* f, uint64_t seed, uint64_t index, 
 
 /* Batched solve on the Hip backend (sov::PdhgBatch; no reference
  * counterpart): count models solved together, each with its own scaling,
- * step sizes, restarts and convergence.  results[b] per model; xs[b] / ys[b]
+ * step sizes, restarts and convergence.  warm_ys (NULL, or per model NULL or
+ * num_rows duals) and primal_weights (NULL or count fixed weights) warm-start
+ * the models.  results[b] per model; xs[b] / ys[b]
  * (NULL or num_cols / num_rows doubles) receive the points; timing (NULL or
  * 4 doubles) receives total, setup and loop seconds and the iterations the
  * batch ran.  All models must share one objective sense. */
 int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* opts,
-                         sov_pdhg_result* results, double* const* xs, double* const* ys, double* timing);
+                         const double* const* warm_ys, const double* primal_weights, sov_pdhg_result* results,
+                         double* const* xs, double* const* ys, double* timing);
 
 /* Backend report line (available backends, default, index width). */
 int sov_backend_report(char* buf, int32_t len);

@@ -318,8 +318,14 @@ __device__ __forceinline__ int row_of(const ClassView& v, int& lane) {
     return (local < v.count) ? v.offset + local : -1;
 }
 
-// Dot product of row r with a gathered vector, G lanes per
-// row.  Every lane of the group returns the full sum.  Accumulation type Acc.
+// Entries per lane that one chunk of a row product loads before it gathers:
+// all index/value loads of a chunk are issued together, then all gathers,
+// so a lane waits for two memory round trips per chunk of kChunk entries.
+constexpr int kChunk = 8;
+
+// Dot product of row r with a gathered vector, G lanes per row.  Every lane
+// of the group returns the full sum.  Accumulation type Acc.  Each lane adds
+// its entries in ascending position (chunking does not change the order).
 template <int G, typename Acc, typename V, typename X>
 __device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
                                        const int32_t* __restrict__ idx,
@@ -328,8 +334,29 @@ __device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
     Acc acc = Acc(0);
     if (r >= 0) {
         const int b = ptr[r], e = ptr[r + 1];
-#pragma unroll 4
-        for (int k = b + lane; k < e; k += G) acc += Acc(val[k]) * Acc(x[idx[k]]);
+        for (int base = b + lane; base < e; base += kChunk * G) {
+            int j[kChunk];
+            V w[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int k = base + u * G;
+                j[u] = -1;
+                w[u] = V(0);
+                if (k < e) {
+                    j[u] = idx[k];
+                    w[u] = val[k];
+                }
+            }
+            X xv[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                xv[u] = X(0);
+                if (j[u] >= 0) xv[u] = x[j[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                if (j[u] >= 0) acc += Acc(w[u]) * Acc(xv[u]);
+        }
     }
     return group_sum<G>(acc);
 }
@@ -344,11 +371,36 @@ __device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
     double a1 = 0.0, a2 = 0.0;
     if (r >= 0) {
         const int b = ptr[r], e = ptr[r + 1];
-        for (int k = b + lane; k < e; k += G) {
-            const double v = double(val[k]);
-            const int j = idx[k];
-            a1 += v * double(x1[j]);
-            a2 += v * (double(x2[j]) * s2);
+        for (int base = b + lane; base < e; base += kChunk * G) {
+            int j[kChunk];
+            V w[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int k = base + u * G;
+                j[u] = -1;
+                w[u] = V(0);
+                if (k < e) {
+                    j[u] = idx[k];
+                    w[u] = val[k];
+                }
+            }
+            X v1[kChunk], v2[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                v1[u] = X(0);
+                v2[u] = X(0);
+                if (j[u] >= 0) {
+                    v1[u] = x1[j[u]];
+                    v2[u] = x2[j[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+                if (j[u] >= 0) {
+                    const double v = double(w[u]);
+                    a1 += v * double(v1[u]);
+                    a2 += v * (double(v2[u]) * s2);
+                }
         }
     }
     d1 = group_sum<G>(a1);

@@ -347,7 +347,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
 
 def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, restart_check_period=40,
                 termination="relative", precision="f32", fp64_polish=True, use_graphs=True, autotune=True,
-                restarts="adaptive", fp32_switch_factor=None, fp32_stall_checks=None, verbose=False):
+                restarts="adaptive", fp32_switch_factor=None, fp32_stall_checks=None, verbose=False,
+                warm_duals=None, primal_weights=None):
     """PdhgBatch(options).solve(problems, hip backend) through
     sov_pdhg_solve_batch: every model solved in one stacked device batch.
     Returns (list of Result, dict of batch timings)."""
@@ -378,9 +379,20 @@ def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, res
     xp = (_f64p * count)(*[ptr(x, C.c_double) for x in xs])
     yp = (_f64p * count)(*[ptr(y, C.c_double) for y in ys])
     timing = np.zeros(4)
-    L.sov_pdhg_solve_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(PdhgOpts), C.POINTER(PdhgRes),
-                                       C.POINTER(_f64p), C.POINTER(_f64p), _f64p]
-    rc_code = L.sov_pdhg_solve_batch(handles, count, C.byref(o), res, xp, yp, ptr(timing, C.c_double))
+    keep = []
+    wyp = None
+    if warm_duals is not None:
+        arrs = [None if w is None else np.ascontiguousarray(w, np.float64) for w in warm_duals]
+        keep += [a for a in arrs if a is not None]
+        wyp = (_f64p * count)(*[ptr(a, C.c_double) if a is not None else _f64p() for a in arrs])
+    pwp = None
+    if primal_weights is not None:
+        pwa = np.ascontiguousarray(primal_weights, np.float64)
+        keep.append(pwa)
+        pwp = ptr(pwa, C.c_double)
+    L.sov_pdhg_solve_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(PdhgOpts), C.POINTER(_f64p),
+                                       _f64p, C.POINTER(PdhgRes), C.POINTER(_f64p), C.POINTER(_f64p), _f64p]
+    rc_code = L.sov_pdhg_solve_batch(handles, count, C.byref(o), wyp, pwp, res, xp, yp, ptr(timing, C.c_double))
     if rc_code != 0:
         raise RuntimeError(f"sov_pdhg_solve_batch failed ({rc_code}): {last_error()}")
     out = []

@@ -167,3 +167,131 @@ def train(ctx, X, Y, hidden=(512, 512, 512), epochs=60, batch=256, lr=1e-3, seed
 
 def predict(net, xn, yn, X):
     return yn.inv(net.predict(xn.fwd(X).astype(np.float32)).astype(np.float64))
+
+
+# ---------------------------------------------------------------------------
+# saved models and the warm-vs-cold evaluation
+# ---------------------------------------------------------------------------
+
+def save(path, net, xn, yn, meta):
+    p, _ = net.params()
+    import json
+    np.savez(path, params=p, widths=np.array(net.widths, np.int32), x_mu=xn.mu, x_sd=xn.sd, y_mu=yn.mu,
+             y_sd=yn.sd, meta=np.array(json.dumps(meta)))
+
+
+def load(ctx, path, max_batch=2048):
+    z = np.load(path, allow_pickle=False)
+    net = Mlp(ctx, [int(w) for w in z["widths"]], max_batch)
+    net.set_params(z["params"])
+    xn, yn = Normalizer.__new__(Normalizer), Normalizer.__new__(Normalizer)
+    xn.mu, xn.sd, yn.mu, yn.sd = z["x_mu"], z["x_sd"], z["y_mu"], z["y_sd"]
+    return net, xn, yn
+
+
+def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500, time_subset=0):
+    """Warm vs cold on `count` held-out instances (draw stream `seed`),
+    each solved to 1e-6 relative KKT by the batched GPU engine:
+      cold     y0 = 0, omega = 1 (the engine's default cold start)
+      constant y0 = 0, omega = exp(const_logw) (the training set's mean; no network)
+      learned  y0 = predicted dual, omega = exp(predicted log weight)
+    Returns per-variant iterations, statuses, objectives and batch times; with
+    time_subset > 0 also single-instance wall times (fused engine) cold vs
+    learned on that many instances."""
+    import time
+    from . import solve
+    m = family.S + family.D
+    out = {k: {"iterations": [], "optimal": 0, "objective": [], "batch_seconds": 0.0}
+           for k in ("cold", "constant", "learned")}
+    pred_err = []
+    lps_all, preds = [], []
+    for start in range(0, count, chunk):
+        idx = range(start, min(count, start + chunk))
+        lps, feats = zip(*(family.instance(seed, i) for i in idx))
+        pr = predict(net, xn, yn, np.array(feats))
+        yw, lw = pr[:, :m], pr[:, m]
+        variants = {"cold": {}, "learned": {"warm_duals": list(yw), "primal_weights": np.exp(lw)}}
+        if const_logw is not None:
+            variants["constant"] = {"primal_weights": np.full(len(lps), np.exp(const_logw))}
+        cold_y = None
+        for name, kw in variants.items():
+            res, tim = solve_batch(list(lps), tolerance=1e-6, precision="f32", **kw)
+            o = out[name]
+            o["iterations"] += [r.iterations for r in res]
+            o["optimal"] += sum(r.status == "optimal" for r in res)
+            o["objective"] += [r.objective for r in res]
+            o["batch_seconds"] += tim["seconds"]
+            if name == "cold":
+                cold_y = [r.y for r in res]
+        pred_err += [np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30) for a, b in zip(yw, cold_y)]
+        lps_all += list(lps)
+        preds += list(zip(yw, lw))
+    summary = {"instances": count, "median_dual_rel_err": float(np.median(pred_err))}
+    for name, o in out.items():
+        if not o["iterations"]:
+            continue
+        its = np.array(o["iterations"])
+        summary[name] = {"median_iterations": float(np.median(its)), "mean_iterations": float(its.mean()),
+                         "optimal": int(o["optimal"]), "batch_seconds": round(o["batch_seconds"], 2)}
+    c, l = np.array(out["cold"]["iterations"]), np.array(out["learned"]["iterations"])
+    summary["median_iteration_ratio_cold_over_learned"] = float(np.median(c / np.maximum(l, 1)))
+    obj_c, obj_l = np.array(out["cold"]["objective"]), np.array(out["learned"]["objective"])
+    summary["max_objective_rel_diff"] = float(np.max(np.abs(obj_l - obj_c) / np.maximum(1.0, np.abs(obj_c))))
+    if time_subset > 0:
+        kw = dict(backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative", restarts="adaptive",
+                  precision="f32")
+        tc, tl = [], []
+        for lp, (yw, lw) in list(zip(lps_all, preds))[:time_subset]:
+            t0 = time.perf_counter()
+            S_cold = solve(lp, **kw)
+            tc.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            S_warm = solve(lp, warm_start_dual=yw, primal_weight=True, initial_primal_weight=float(np.exp(lw)),
+                           primal_weight_smoothing=0.0, **kw)
+            tl.append(time.perf_counter() - t0)
+            assert S_cold.status == "optimal" and S_warm.status == "optimal"
+        summary["single_solve_median_seconds"] = {"cold": float(np.median(tc)), "learned": float(np.median(tl)),
+                                                  "instances": time_subset,
+                                                  "speedup": float(np.median(tc) / np.median(tl))}
+    return summary
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import time
+    from . import Family
+    ap = argparse.ArgumentParser(description="learned PDHG warm start on the 100 x 200 transportation family")
+    ap.add_argument("command", choices=["train", "evaluate"])
+    ap.add_argument("--model", required=True, help="model file (.npz) to write (train) or read (evaluate)")
+    ap.add_argument("--train", type=int, default=20000)
+    ap.add_argument("--test", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--chunk", type=int, default=500)
+    ap.add_argument("--time-subset", type=int, default=20)
+    args = ap.parse_args(argv)
+    ctx = D.Ctx(0)
+    fam = Family(100, 200, 16, 0)
+    rep = {}
+    if args.command == "train":
+        t0 = time.perf_counter()
+        X, Y, LW, its = dataset(fam, 1, args.train, chunk=args.chunk, verbose=True)
+        rep["dataset"] = {"instances": int(len(X)), "seconds": round(time.perf_counter() - t0, 1),
+                          "median_cold_iterations": float(np.median(its))}
+        t0 = time.perf_counter()
+        net, xn, yn, losses = train(ctx, X, np.column_stack([Y, LW]), epochs=args.epochs, verbose=True)
+        rep["training"] = {"seconds": round(time.perf_counter() - t0, 1), "epochs": args.epochs,
+                           "final_loss": losses[-1], "first_loss": losses[0], "params": net.nparams}
+        save(args.model, net, xn, yn, {"train": int(len(X)), "epochs": args.epochs,
+                                       "mean_log_weight": float(LW.mean())})
+        rep["mean_log_weight"] = float(LW.mean())
+    else:
+        net, xn, yn = load(ctx, args.model)
+        meta = json.loads(str(np.load(args.model, allow_pickle=False)["meta"]))
+        rep = evaluate(ctx, fam, net, xn, yn, args.test, const_logw=meta.get("mean_log_weight"), chunk=args.chunk,
+                       time_subset=args.time_subset)
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()

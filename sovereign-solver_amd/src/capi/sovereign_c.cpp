@@ -294,7 +294,8 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
     }
 }
 
-int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* o, sov_pdhg_result* results,
+int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* o,
+                         const double* const* warm_ys, const double* primal_weights, sov_pdhg_result* results,
                          double* const* xs, double* const* ys, double* timing) {
     if (lps == nullptr || o == nullptr || results == nullptr || count < 0) return set_err(3, "null argument");
     try {
@@ -303,8 +304,18 @@ int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg
             if (lps[b] == nullptr) return set_err(3, "null model in batch");
             probs[static_cast<std::size_t>(b)] = &lps[b]->p;
         }
+        std::vector<std::vector<double>> wy;
+        if (warm_ys != nullptr) {
+            wy.resize(static_cast<std::size_t>(count));
+            for (int32_t b = 0; b < count; ++b)
+                if (warm_ys[b] != nullptr)
+                    wy[static_cast<std::size_t>(b)].assign(warm_ys[b], warm_ys[b] + lps[b]->p.num_rows());
+        }
+        std::vector<double> pw;
+        if (primal_weights != nullptr) pw.assign(primal_weights, primal_weights + count);
         std::unique_ptr<sov::Backend> be = sov::make_backend(sov::BackendKind::Hip);
-        const sov::PdhgBatchResult br = sov::PdhgBatch(to_options(o)).solve(probs, *be);
+        const sov::PdhgBatchResult br = sov::PdhgBatch(to_options(o)).solve(probs, *be, warm_ys ? &wy : nullptr,
+                                                                           primal_weights ? &pw : nullptr);
         for (int32_t b = 0; b < count; ++b) {
             const sov::PdhgResult& r = br.results[static_cast<std::size_t>(b)];
             fill_result(r, &results[b]);

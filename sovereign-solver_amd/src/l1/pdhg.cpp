@@ -1073,7 +1073,9 @@ struct BatchHandle {
 
 }  // namespace
 
-PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Backend& backend) const {
+PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Backend& backend,
+                                  const std::vector<std::vector<Real>>* warm_dual,
+                                  const std::vector<Real>* primal_weight) const {
     PdhgBatchResult out;
     const auto t0 = Clock::now();
     const PdhgOptions& o = options_;
@@ -1251,8 +1253,16 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
         }
         std::vector<double> tau(B), sigma(B);
         for (std::size_t b = 0; b < B; ++b) {
-            tau[b] = sigma[b] = est[b] > 1e-9 ? o.step_size_safety / est[b] : o.step_size_safety;
+            const double eta = est[b] > 1e-9 ? o.step_size_safety / est[b] : o.step_size_safety;
+            double w = 1.0;
+            if (primal_weight != nullptr && idx[b] < primal_weight->size()) {
+                w = (*primal_weight)[idx[b]];
+                if (!(w > 0.0) || !std::isfinite(w)) w = 1.0;
+            }
+            tau[b] = eta / w;
+            sigma[b] = eta * w;
             out.results[idx[b]].operator_norm_estimate = est[b];
+            out.results[idx[b]].primal_weight = w;
         }
 
         // -- iterate states: FP64 always (the only one for an FP64 run, the
@@ -1261,6 +1271,20 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
         //    FP64 on its own once it is near tolerance or stalled.
         const bool f32 = o.precision == Precision::F32;
         if (f32) hip_ok(sov_hip_csr_make_f32(ctx, csr.h), "make_f32");
+        // optional per-instance dual warm starts: original units -> scaled, device order
+        std::vector<Real> y_start;
+        if (warm_dual != nullptr) {
+            y_start.assign(m, 0.0);
+            for (std::size_t b = 0; b < B; ++b) {
+                if (idx[b] >= warm_dual->size() || (*warm_dual)[idx[b]].empty()) continue;
+                const std::vector<Real>& w = (*warm_dual)[idx[b]];
+                const auto r0 = static_cast<std::size_t>(row_off[b]);
+                const std::size_t mb = static_cast<std::size_t>(row_off[b + 1] - row_off[b]);
+                if (w.size() != mb) throw Error(Status::DimensionMismatch, "PdhgBatch: warm_dual length");
+                for (std::size_t i = 0; i < mb; ++i) y_start[r0 + i] = sign * w[i] / rs[r0 + i];
+            }
+            y_start = DeviceOrder::to_dev(y_start, ord.row);
+        }
         DeviceState S32, S64;
         auto init_state = [&](DeviceState& S, int p) {
             S.allocate(ctx, p, m, n);
@@ -1273,7 +1297,14 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
             DevMem zx(ctx, std::max<std::size_t>(n, 1) * 8), zs(ctx, std::max<std::size_t>(m, 1) * 8);
             if (n) hip_ok(sov_hip_memcpy_h2d(ctx, zx.p, zn.data(), n * 8), "upload");
             if (m) hip_ok(sov_hip_memcpy_h2d(ctx, zs.p, zm.data(), m * 8), "upload");
-            hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), nullptr, zs.as<double>()), "set_point");
+            DevMem wy;
+            const double* ysrc = nullptr;
+            if (!y_start.empty()) {
+                wy = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
+                if (m) hip_ok(sov_hip_memcpy_h2d(ctx, wy.p, y_start.data(), m * 8), "upload warm start");
+                ysrc = wy.as<double>();
+            }
+            hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), ysrc, zs.as<double>()), "set_point");
             hip_ok(sov_hip_sync(ctx), "sync");
         };
         if (f32) init_state(S32, SOV_PREC_F32);
