@@ -189,7 +189,7 @@ def load(ctx, path, max_batch=2048):
     return net, xn, yn
 
 
-def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500, time_subset=0):
+def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500, time_subset=0, verbose=False):
     """Warm vs cold on `count` held-out instances (draw stream `seed`),
     each solved to 1e-6 relative KKT by the batched GPU engine:
       cold     y0 = 0, omega = 1 (the engine's default cold start)
@@ -223,6 +223,9 @@ def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500
             o["batch_seconds"] += tim["seconds"]
             if name == "cold":
                 cold_y = [r.y for r in res]
+            if verbose:
+                print(f"  evaluate: {name} on {start}..{start + len(lps)}: median "
+                      f"{np.median([r.iterations for r in res]):.0f} iterations, {tim['seconds']:.1f}s", flush=True)
         pred_err += [np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30) for a, b in zip(yw, cold_y)]
         lps_all += list(lps)
         preds += list(zip(yw, lw))
@@ -250,6 +253,8 @@ def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500
                            primal_weight_smoothing=0.0, **kw)
             tl.append(time.perf_counter() - t0)
             assert S_cold.status == "optimal" and S_warm.status == "optimal"
+            if verbose and len(tc) % 10 == 0:
+                print(f"  evaluate: {len(tc)} single solves timed", flush=True)
         summary["single_solve_median_seconds"] = {"cold": float(np.median(tc)), "learned": float(np.median(tl)),
                                                   "instances": time_subset,
                                                   "speedup": float(np.median(tc) / np.median(tl))}
@@ -289,7 +294,7 @@ def main(argv=None):
         net, xn, yn = load(ctx, args.model)
         meta = json.loads(str(np.load(args.model, allow_pickle=False)["meta"]))
         rep = evaluate(ctx, fam, net, xn, yn, args.test, const_logw=meta.get("mean_log_weight"), chunk=args.chunk,
-                       time_subset=args.time_subset)
+                       time_subset=args.time_subset, verbose=True)
     print(json.dumps(rep), flush=True)
 
 
