@@ -38,7 +38,7 @@ class DeviceProblem:
     """The scaled LP and the FP32 iterate state on the device (the same calls
     the fused engine makes, src/l1/pdhg.cpp solve_fused)."""
 
-    def __init__(self, ctx, a, prec, segments=None):
+    def __init__(self, ctx, a, prec, segments=None, tiles=None):
         from sovereign_amd import device as D
         self.D, self.ctx, L = D, ctx, ctx.L
         m, n = a["m"], a["n"]
@@ -74,6 +74,9 @@ class DeviceProblem:
                 D.check(L.sov_hip_csr_get_segments(self.A.h, orient, C.byref(cur)))
                 if cur.value != k:     # rebuilt parts start from the default lanes
                     D.check(L.sov_hip_csr_set_segments(ctx.h, self.A.h, orient, k))
+        if tiles is not None:      # pin the step kernels: 0 = CSR parts, k = LDS-staged tiles in k launches
+            for orient, k in enumerate(tiles):
+                D.check(L.sov_hip_csr_set_tiles(ctx.h, self.A.h, orient, k))
         st = D.PdhgState()
         st.prec, st.m, st.n = prec, m, n
         st.c, st.col_lo, st.col_hi = (ctx.upload(v[pc].astype(dt)) for v in (c_s, clo, chi))
@@ -182,21 +185,22 @@ def spmv_bytes(m, n, nnz, sv, sx):
     return primal, dual
 
 
-def committed_traffic(m, n, nnz, dtype, seg_a, seg_t):
+def committed_traffic(m, n, nnz, dtype, layout):
     """HBM bytes per step of the two step kernels from the committed PMC
     summary (profiles/pmc_traffic_zipf1m_f32.md: FETCH_SIZE and WRITE_SIZE
     per dispatch, scaled by the factors a known-size copy kernel reads), when
-    it was taken on this workload and segment counts; else None."""
+    it was taken on this workload and step-kernel layout
+    ("tiles=a,t segments=a,t"); else None."""
     import re
     path = os.path.join(ROOT, "profiles", "pmc_traffic_zipf1m_f32.md")
     if not os.path.exists(path):
         return None
     pat = re.compile(r"traffic_per_step primal_step=(\d+) dual_step=(\d+) workload=(\d+)x(\d+)x(\d+) "
-                     r"dtype=(\w+) segments=(\d+),(\d+)")
+                     r"dtype=(\w+) layout=(tiles=\d+,\d+ segments=\d+,\d+)")
     for line in open(path):
         g = pat.search(line)
         if g and tuple(int(v) for v in g.group(3, 4, 5)) == (m, n, nnz) and g.group(6) == dtype \
-                and (int(g.group(7)), int(g.group(8))) == (seg_a, seg_t):
+                and g.group(7) == layout:
             return {"primal_step": int(g.group(1)), "dual_step": int(g.group(2))}
     return None
 
@@ -275,6 +279,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
     ap.add_argument("--segments", default=None,
                     help="A,T column-segment counts to use instead of the autotuner's (profiling runs)")
+    ap.add_argument("--tiles", default=None,
+                    help="A,T step-kernel layouts instead of the autotuner's: 0 = CSR parts, k = tiles in k launches")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with 3 launches of the f64->f32 convert kernel over 32 Mi elements (reads 256 MiB, "
                          "writes 128 MiB): the known-size reference for HBM byte counters")
@@ -297,7 +303,8 @@ def main():
     ctx = D.Ctx(0)
     t0 = time.perf_counter()
     segs = tuple(int(k) for k in args.segments.split(",")) if args.segments else None
-    P = DeviceProblem(ctx, a, prec, segs)
+    tls = tuple(int(k) for k in args.tiles.split(",")) if args.tiles else None
+    P = DeviceProblem(ctx, a, prec, segs, tls)
     log(f"setup: {time.perf_counter() - t0:.2f}s  op_norm={P.op_norm:.6g}")
     lanes_a, lanes_t = (C.c_int32 * 8)(), (C.c_int32 * 8)()
     ctx.L.sov_hip_csr_get_lanes(P.A.h, 0, lanes_a)
@@ -305,7 +312,17 @@ def main():
     seg_a, seg_t = C.c_int(), C.c_int()
     ctx.L.sov_hip_csr_get_segments(P.A.h, 0, C.byref(seg_a))
     ctx.L.sov_hip_csr_get_segments(P.A.h, 1, C.byref(seg_t))
-    log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}; column segments A={seg_a.value} A^T={seg_t.value}")
+    til_a, til_t = C.c_int(), C.c_int()
+    ctx.L.sov_hip_csr_get_tiles(P.A.h, 0, C.byref(til_a))
+    ctx.L.sov_hip_csr_get_tiles(P.A.h, 1, C.byref(til_t))
+    log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}; column segments A={seg_a.value} A^T={seg_t.value}; "
+        f"tiles A={til_a.value} A^T={til_t.value}")
+    layout = f"tiles={til_a.value},{til_t.value} segments={seg_a.value},{seg_t.value}"
+    kern = {"primal_step": f"k_tile_step<{'float' if prec else 'double'}, PrimalOp>" if til_t.value
+            else f"k_step<PrimalOp<{'float' if prec else 'double'}>>",
+            "dual_step": f"k_tile_step<{'float' if prec else 'double'}, DualOp>" if til_a.value
+            else f"k_step<DualOp<{'float' if prec else 'double'}>>"}
+    launches = {"primal_step": til_t.value or seg_t.value, "dual_step": til_a.value or seg_a.value}
 
     P.iterate(args.warmup)
     D.check(ctx.L.sov_hip_sync(ctx.h))
@@ -337,7 +354,7 @@ def main():
     log(f"primal step {tp * 1e3:.1f} us ({bp / tp / 1e6:.0f} GB/s), dual step {td * 1e3:.1f} us "
         f"({bd / td / 1e6:.0f} GB/s)")
     dom_name, dom_t, dom_b = ("primal_step", tp, bp) if tp >= td else ("dual_step", td, bd)
-    traffic = committed_traffic(a["m"], a["n"], a["nnz"], "f32" if prec == 1 else "f64", seg_a.value, seg_t.value)
+    traffic = committed_traffic(a["m"], a["n"], a["nnz"], "f32" if prec == 1 else "f64", layout)
     achieved = dom_b / dom_t / 1e6
     value = (bp + bd) * its / 1e9
     chk = P.last_check
@@ -384,15 +401,17 @@ def main():
                                f"{'FP32' if prec == 1 else 'FP64'} iterates, FP64 checks every {PERIOD}",
                    "iterations_per_s": round(its, 2), "restarts": P.restarts,
                    "column_segments": {"A": seg_a.value, "A^T": seg_t.value},
+                   "tiles": {"A": til_a.value, "A^T": til_t.value},
                    "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
                    "kkt_scaled_at_last_check": chk.kkt_cur if chk else None,
                    "batched": bat,
                    "learned_warm_start": lws},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": f"{dom_name}: {kern[dom_name]}", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic[dom_name] if traffic else None,
                      "algorithmic_bytes_per_step": int(dom_b),
-                     "launches_per_step": seg_a.value if dom_name == "dual_step" else seg_t.value},
+                     "launches_per_step": launches[dom_name]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -142,10 +142,42 @@ int sov_hip_csr_get_lanes(const sov_hip_csr* csr, int orient, int32_t lanes[8]);
  * natural CSR. */
 int sov_hip_csr_set_segments(sov_hip_ctx* ctx, sov_hip_csr* csr, int orient, int count);
 int sov_hip_csr_get_segments(const sov_hip_csr* csr, int orient, int* count);
+/* LDS-staged tiles (no reference counterpart: a layout of the same
+ * products).  The rows of an orientation are dealt in 64-row chunks to
+ * row groups (one workgroup each, about one per CU); the gathered vector is
+ * cut into slices of 16384 elements.  Slices dense enough in entries per
+ * group are copied into LDS by each workgroup with 16-byte loads and their
+ * entries gather from LDS; runs of sparser slices gather from memory, as
+ * the CSR kernels do.  parts >= 1 builds
+ * the layout (FP64 and, after make_f32, FP32 values) and runs the
+ * orientation's SpMV and PDHG steps on it, in `parts` launches over ranges
+ * of slices joined by per-row partial sums; parts = 0 frees it and returns
+ * to the CSR parts.  FP64 launches need 160 KiB of LDS per workgroup and
+ * otherwise use the CSR parts.  The summation order is fixed (results are
+ * reproducible run to run) but differs from the CSR kernels'. */
+int sov_hip_csr_set_tiles(sov_hip_ctx* ctx, sov_hip_csr* csr, int orient, int parts);
+int sov_hip_csr_get_tiles(const sov_hip_csr* csr, int orient, int* parts);
+/* *usable = 1 when the orientation's products of precision prec run on the
+ * tiles (tiles set, value image present, LDS of the device large enough). */
+int sov_hip_csr_tiles_usable(const sov_hip_ctx* ctx, const sov_hip_csr* csr, int orient, int prec, int* usable);
+/* The tiled layout of a host CSR (row pointers, column indices in
+ * [0, cols), and a source position per entry), as sov_hip_csr_set_tiles
+ * builds it for a device with `cus` compute units.  shape = {groups,
+ * slices, slice width, local rows per group, units, sub-ranges per tile};
+ * *entries = stored entries including padding.  keys / src_out (entries
+ * each), wstart (groups * units * sub-ranges + 1) and units (4 * units + 1: first slice, staged,
+ * buffer, next staged unit; then the slice count) may be null (first call
+ * for the sizes).  Host only: no device is touched. */
+int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
+                        const int32_t* col_idx, const int32_t* src, int32_t cus, int32_t shape[6],
+                        int64_t* entries, uint32_t* keys, int32_t* src_out, int32_t* wstart,
+                        int32_t* units);
 /* Autotuner: for each orientation, try 1, 2, 4 and 8 column segments (slices
  * of at least 64k elements); for each, time every G in {1,2,4,8,16,32,64} per
  * length class of every segment on the SpMV of precision prec; keep the
- * fastest segment count with its lanes. */
+ * fastest segment count with its lanes.  Orientations with at least 2^20
+ * entries then also time the tiles (1, 2 and 4 launches) and keep them when
+ * they beat the CSR parts. */
 int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* csr, int prec);
 
 /* ---- sparse level 2 ---------------------------------------------------

@@ -124,6 +124,10 @@ struct PdhgOptions {
     /// changes summation order, so runs with it on are not bit-reproducible
     /// across processes; off = the fixed default schedule.
     bool autotune = true;
+    /// LDS-staged tiles for the step kernels (sov_hip_csr_set_tiles): -1 =
+    /// the autotuner decides (off without autotune), 0 = never, k >= 1 =
+    /// always, in k launches per product.
+    int tiles = -1;
     /// Device ordinal for a fused solve that creates its own backend context.
     int device = 0;
 };

@@ -82,6 +82,7 @@ typedef struct sov_pdhg_opts {
     double* farkas_out;              /* NULL or num_rows: the Farkas certificate when status is primal_infeasible */
     double* ray_out;                 /* NULL or num_cols: the primal ray when status is dual_infeasible */
     const double* warm_start_primal; /* NULL or num_cols values (original units) */
+    int32_t tiles;                   /* LDS-staged step kernels: -1 autotuner decides, 0 never, k >= 1 always (k launches) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

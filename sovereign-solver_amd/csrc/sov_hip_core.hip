@@ -194,6 +194,7 @@ int sov_hip_ctx_create(int device, sov_hip_ctx** out) {
                     std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
         ctx->cus = prop.multiProcessorCount;
         ctx->l2_bytes = prop.l2CacheSize;
+        ctx->lds_max = static_cast<int64_t>(prop.sharedMemPerBlock);
     } else {
         ctx->name = "HIP device " + std::to_string(device);
     }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "sov_hip_internal.hpp"
+#include "sov_hip_tiles.hpp"
 
 namespace sovhip {
 
@@ -474,6 +475,10 @@ static int refresh_values(sov_hip_ctx* ctx, sov_hip_csr* h) {
             k_gather_f64_to_f32<<<launch_blocks(h->nnz), kBlock, 0, ctx->stream>>>(h->nnz, o->p_src, av, o->p_val32);
     }
     SOV_HIP_TRY(check_launch());
+    for (Orient* o : {&h->a, &h->t}) {
+        const int rc = refresh_tile_values(ctx, h, *o);
+        if (rc != SOV_OK) return rc;
+    }
     return SOV_OK;
 }
 
@@ -525,6 +530,7 @@ static int launch_spmv_part(sov_hip_ctx* ctx, const Part& p, const V* val, int m
 
 static int spmv_orient(sov_hip_ctx* ctx, const Orient& o, int prec, double alpha, const void* x, double beta,
                        void* y) {
+    if (tiles_usable(ctx, o.tiles, prec == SOV_PREC_F64, x)) return spmv_tiles(ctx, o, prec, alpha, x, beta, y);
     const int np = o.nparts();
     for (int k = 0; k < np; ++k) {
         const Part& p = o.part(k);
@@ -836,6 +842,8 @@ void sov_hip_csr_destroy(sov_hip_csr* h) {
     if (h == nullptr) return;
     DeviceGuard2 g(h->device);
     (void)hipDeviceSynchronize();
+    h->a.tiles.free();
+    h->t.tiles.free();
     h->a.free_parts();
     h->t.free_parts();
     h->a.free_order();
@@ -873,6 +881,9 @@ int sov_hip_csr_make_f32(sov_hip_ctx* ctx, sov_hip_csr* h) {
             off += p.nnz;
         }
     }
+    for (Orient* o : {&h->a, &h->t})
+        if (o->tiles.built() && o->tiles.val32 == nullptr)
+            SOV_HIP_TRY(hipMalloc(&o->tiles.val32, sizeof(float) * std::max<int64_t>(o->tiles.E, kTileLane)));
     h->f32 = true;
     return refresh_values(ctx, h);
 }
@@ -1010,6 +1021,7 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
     if (e != hipSuccess) rc = hip_fail(e, "tune fill");
     for (int orient = 0; orient < 2 && rc == SOV_OK; ++orient) {
         Orient& o = orient == 0 ? h->a : h->t;
+        o.tiles.parts = 0;   // the CSR timings below must not run on the tiles
         // segment counts worth trying: keep each gathered slice >= 64k elements
         const int64_t cols = o.nat.cols;
         int best_K = 1;
@@ -1034,6 +1046,26 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
                 for (int k = 0; k < o.nparts(); ++k)
                     std::copy(best_lanes[static_cast<size_t>(k)].begin(), best_lanes[static_cast<size_t>(k)].end(),
                               o.part(k).lanes);
+        }
+        // the LDS-staged tiles, for products large enough to amortise their
+        // per-group staging (kept only when faster than the best CSR parts)
+        if (rc == SOV_OK && o.nat.nnz >= kTileTuneMinNnz && o.nat.rows >= 64) {
+            if (!o.tiles.built() || (h->f32 && o.tiles.val32 == nullptr)) rc = build_tiles(ctx, h, o, h->f32);
+            int best_p = 0;
+            for (int P : {1, 2, 4}) {
+                if (rc != SOV_OK || P > std::max(o.tiles.S, 1)) break;
+                o.tiles.parts = P;
+                if (!tiles_usable(ctx, o.tiles, prec == SOV_PREC_F64, xv)) break;
+                const float ms = time_spmv(ctx, o, prec, xv, yv, 4);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_p = P;
+                }
+            }
+            if (rc == SOV_OK) {
+                if (best_p > 0) o.tiles.parts = best_p;
+                else o.tiles.free();
+            }
         }
     }
     (void)hipStreamSynchronize(ctx->stream);

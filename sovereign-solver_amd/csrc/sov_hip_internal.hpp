@@ -86,11 +86,62 @@ struct Natural {
     const int32_t* src = nullptr;   // null for A itself
 };
 
+// Tiled layout of an orientation for the LDS-staged step kernels.
+//
+// Rows (device order) are cut into 64-row chunks; chunk c belongs to group
+// c % nb, where it holds local rows (c / nb) * 64 .. + 63, so every group
+// mixes all length classes and the per-row work is coalesced per chunk.
+// One workgroup runs one group and keeps its row sums in LDS.
+//
+// The gathered dimension is cut into slices of W columns, and consecutive
+// slices form units.  A staged unit is one slice whose entries are dense
+// enough per group (on average at least kTileStageMin per group) to repay
+// copying the vector slice into LDS with 16-byte loads; its entries then
+// gather from LDS.  A gather unit is a run of up to kTileGatherSlices
+// sparser slices whose entries gather straight from the vector in memory
+// (staging them would move more bytes than it saves).
+//
+// Entries are stored group-major, then unit, then wave: each (group, unit)
+// tile is split at row boundaries into kTileWaves contiguous sub-ranges of
+// about equal size, one per entry wave, each padded to a multiple of kTileLane
+// entries with key kTileSentinel.  key = (local row << 20) | (column -
+// first column of the unit).  wstart[(b U + u) kTileWaves + w] is the first
+// entry of wave w of tile (b, u); the array ends with the total count E.
+// units[4 u .. 4 u + 3] = {first slice, staged, LDS buffer of a staged
+// unit, next staged unit (U if none)}, units[4 U] = S.
+constexpr int kTileThreads = 1024;                    // workgroup of a group
+constexpr int kTileStagers = 4;                       // waves that copy vector slices into LDS
+constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 12 waves that add the entries
+constexpr int kTileLane = 4;                          // entries per lane per pass
+constexpr int kTileWidth = 16384;                     // columns per slice
+constexpr int kTileRowCap = 4096;                     // local rows per group (12 key bits)
+constexpr int kTileColBits = 20;                      // key bits of the column within a unit
+constexpr int kTileGatherSlices = 63;                 // slices per gather unit (columns < 2^20 - 1)
+constexpr int kTileStageMin = 384;                    // entries per group that make a slice staged
+constexpr uint32_t kTileSentinel = 0xFFFFFFFFu;       // never a real key: columns stay below 2^20 - 1
+
+struct Tiles {
+    int32_t nb = 0, S = 0, W = kTileWidth, rmax = 0;  // groups, slices, slice width, local rows per group
+    int32_t U = 0;                                    // units
+    int64_t E = 0;                                    // stored entries (with padding)
+    uint32_t* keys = nullptr;                         // E
+    int32_t* src = nullptr;                           // E: A natural position, -1 for padding
+    int32_t* wstart = nullptr;                        // nb * U * kTileWaves + 1
+    int32_t* units = nullptr;                         // 4 U + 1
+    std::vector<int32_t> h_units;                     // host copy of units
+    float* val32 = nullptr;                           // E (optional)
+    double* val64 = nullptr;                          // E
+    int parts = 0;                                    // launches per product; 0 = tiles not in use
+    bool built() const { return wstart != nullptr; }
+    void free();
+};
+
 // An orientation (A or A^T): its natural CSR, its device order, the
 // device-order CSR the parts are cut from, and the parts (one, or one per
 // column segment when the gathered vector is larger than a comfortable share
 // of an XCD's L2).
 struct Orient {
+    Tiles tiles;
     Natural nat;
     int32_t* perm = nullptr;        // rows: natural row at each device position
     int32_t* inv = nullptr;         // rows: device position of each natural row
@@ -139,6 +190,7 @@ struct sov_hip_ctx {
     int cus = 0;
     int64_t l2_bytes = 0;
     int64_t mall_bytes = 0;
+    int64_t lds_max = 65536;   // LDS bytes one workgroup may allocate
     // FP64 scratch for two-stage reductions (device) and pinned host staging
     double* d_red = nullptr;
     size_t d_red_count = 0;

@@ -10,6 +10,7 @@
 #include <type_traits>
 
 #include "sov_hip_internal.hpp"
+#include "sov_hip_tiles.hpp"
 
 namespace sovhip {
 
@@ -84,22 +85,25 @@ struct PrimalOp {
                 }
             }
         }
-        T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
-        if (row >= 0 && lane == 0) {
-            if (mode == kFirst) { acc[row] = aty; return; }
-            if (mode == kMiddle) { acc[row] = acc[row] + aty; return; }
-            if (mode == kLast) aty = acc[row] + aty;
-            const T g = c[row] - aty;
-            const T xo = x[row];
-            T xn = xo + (-tau_r) * g;
-            const T l = lo[row], h = hi[row];
-            if (xn < l) xn = l;
-            if (xn > h) xn = h;
-            xbar[row] = xn * T(2) - xo;
-            x[row] = xn;
-            xsum[row] = first_r ? xn : xsum[row] + xn;
-        }
+        const T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
+        if (row >= 0 && lane == 0) apply(row, aty, tau_r, first_r);
     }
+    // the row's full product (A^T y)_row, or a segment's part of it
+    __device__ __forceinline__ void apply(int row, T aty, T tau_r, int first_r) {
+        if (mode == kFirst) { acc[row] = aty; return; }
+        if (mode == kMiddle) { acc[row] = acc[row] + aty; return; }
+        if (mode == kLast) aty = acc[row] + aty;
+        const T g = c[row] - aty;
+        const T xo = x[row];
+        T xn = xo + (-tau_r) * g;
+        const T l = lo[row], h = hi[row];
+        if (xn < l) xn = l;
+        if (xn > h) xn = h;
+        xbar[row] = xn * T(2) - xo;
+        x[row] = xn;
+        xsum[row] = first_r ? xn : xsum[row] + xn;
+    }
+    __device__ __forceinline__ void finish(int row, T aty) { apply(row, aty, tau, first); }
 };
 
 // Per row i: s+ = clamp(s - tau y); s_bar = 2 s+ - s;
@@ -140,29 +144,39 @@ struct DualOp {
                 }
             }
         }
-        T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
-        if (row >= 0 && lane == 0) {
-            if (mode == kFirst) { acc[row] = ax; return; }
-            if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
-            if (mode == kLast) ax = acc[row] + ax;
-            const T so = s_[row], yo = y[row];
-            T sn = so + (-tau_r) * yo;
-            const T l = lo[row], h = hi[row];
-            if (sn < l) sn = l;
-            if (sn > h) sn = h;
-            const T sb = sn * T(2) - so;
-            const T yn = yo + sigma_r * ((-ax) + sb);
-            s_[row] = sn;
-            y[row] = yn;
-            ysum[row] = first_r ? yn : ysum[row] + yn;
-        }
+        const T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
+        if (row >= 0 && lane == 0) apply(row, ax, tau_r, sigma_r, first_r);
     }
+    // the row's full product (A x_bar)_row, or a segment's part of it
+    __device__ __forceinline__ void apply(int row, T ax, T tau_r, T sigma_r, int first_r) {
+        if (mode == kFirst) { acc[row] = ax; return; }
+        if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
+        if (mode == kLast) ax = acc[row] + ax;
+        const T so = s_[row], yo = y[row];
+        T sn = so + (-tau_r) * yo;
+        const T l = lo[row], h = hi[row];
+        if (sn < l) sn = l;
+        if (sn > h) sn = h;
+        const T sb = sn * T(2) - so;
+        const T yn = yo + sigma_r * ((-ax) + sb);
+        s_[row] = sn;
+        y[row] = yn;
+        ysum[row] = first_r ? yn : ysum[row] + yn;
+    }
+    __device__ __forceinline__ void finish(int row, T ax) { apply(row, ax, tau, sigma, first); }
 };
 
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_step(Op op) {
     const ClassView v = class_view(op.s, op.block());
     dispatch_G(v, op);
+}
+
+// The same steps over the tiled layout: x is the gathered vector (y for the
+// primal step, x_bar for the dual step).
+template <typename T, class Op>
+__global__ void __launch_bounds__(kTileThreads) k_tile_step(Op op, TileArgs ta, const T* x) {
+    tile_rows<T>(ta, x, op);
 }
 
 // ---------------------------------------------------------------------------
@@ -890,9 +904,30 @@ T* orient_acc(const Orient& o) {
     return std::is_same<T, double>::value ? reinterpret_cast<T*>(o.acc64) : reinterpret_cast<T*>(o.acc32);
 }
 
+template <typename T, class Op>
+int launch_tile_step(sov_hip_ctx* ctx, const Orient& o, Op op, const T* x) {
+    const Tiles& t = o.tiles;
+    SOV_HIP_TRY(tile_lds_attr(ctx, reinterpret_cast<const void*>(&k_tile_step<T, Op>), sizeof(T)));
+    const size_t lds = tile_lds_bytes(t.W, t.rmax, sizeof(T));
+    for (int k = 0; k < t.parts; ++k) {
+        op.mode = seg_mode(k, t.parts);
+        const TileArgs ta = tile_args(t, o.nat.rows, o.nat.cols, sizeof(T) == 8, k, t.parts);
+        k_tile_step<T, Op><<<t.nb, kTileThreads, lds, ctx->stream>>>(op, ta, x);
+    }
+    SOV_HIP_TRY(check_launch());
+    return SOV_OK;
+}
+
 template <typename T>
 int primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau, int first) {
     const Orient& o = h->t;
+    if (tiles_usable(ctx, o.tiles, sizeof(T) == 8, st->y)) {
+        PrimalOp<T> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
+                       static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi), static_cast<T*>(st->x),
+                       static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum), orient_acc<T>(o), T(tau), first,
+                       kOnly, Sched{}};
+        return launch_tile_step<T>(ctx, o, op, static_cast<const T*>(st->y));
+    }
     const int np = o.nparts();
     for (int k = 0; k < np; ++k) {
         const Part& p = o.part(k);
@@ -915,6 +950,12 @@ template <typename T>
 int dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau, double sigma,
               int first) {
     const Orient& o = h->a;
+    if (tiles_usable(ctx, o.tiles, sizeof(T) == 8, st->x_bar)) {
+        DualOp<T> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
+                     static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
+                     static_cast<T*>(st->y_sum), orient_acc<T>(o), T(tau), T(sigma), first, kOnly, Sched{}};
+        return launch_tile_step<T>(ctx, o, op, static_cast<const T*>(st->x_bar));
+    }
     const int np = o.nparts();
     for (int k = 0; k < np; ++k) {
         const Part& p = o.part(k);

@@ -1,0 +1,446 @@
+// LDS-staged row products over the tiled layout (struct Tiles): the device
+// routine shared by the tiled SpMV and the tiled PDHG step kernels.
+//
+// One workgroup of kTileThreads runs one row group.  For every unit u of
+// the launch's range, each wave adds the products of its row-aligned
+// sub-range of tile (b, u) into the group's LDS row sums acc[local row],
+// gathering the vector from LDS (staged unit: the slice was copied there
+// beforehand, two buffers when they fit) or from memory (gather unit).
+// After the last unit the epilogue functor sees every row of the group
+// once, with its full sum.
+//
+// Summation order (fixed, so results are run-to-run identical): inside a
+// tile a row's entries are summed per lane in storage order, lanes combine
+// by a segmented Hillis-Steele scan over the wave; the tile sums of a row
+// are added to acc in ascending unit order.
+#pragma once
+
+#include "sov_hip_internal.hpp"
+
+namespace sovhip {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Read-only tables addressed through the constant address space: the
+// compiler treats them as invariant, so uniform reads become scalar loads
+// (which never wait behind a wave's vector loads) wherever they occur.
+typedef const int32_t __attribute__((address_space(4))) cint32;
+__device__ __forceinline__ const cint32* const_table(const int32_t* p) {
+    return (const cint32*)(p);
+}
+
+// What one tiled launch reads: units [u0, u1) of every group's tiles.
+struct TileArgs {
+    const uint32_t* keys;
+    const void* vals;         // T
+    const int32_t* wstart;
+    const int32_t* units;     // 4 U + 1
+    int32_t nb, U, W, rmax, rows, cols;
+    int32_t u0, u1;
+};
+
+// Device row of local row l of group b.
+__host__ __device__ __forceinline__ int tile_row(int b, int nb, int l) { return ((l >> 6) * nb + b) * 64 + (l & 63); }
+
+// Slice buffers that fit next to the row sums (2: double-buffered, 1: single).
+__host__ __device__ constexpr int tile_buffers(int tsize) { return tsize == 4 ? 2 : 1; }
+
+__host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize) {
+    return (size_t)tile_buffers(tsize) * W * tsize + (size_t)rmax * tsize;
+}
+
+// The four values of one lane's entries (T = float: one 16-byte load;
+// double: two).
+template <typename T>
+struct Quad {
+    T v[4];
+};
+
+template <typename T>
+__device__ __forceinline__ Quad<T> load_quad(const T* __restrict__ vals, int64_t e);
+
+template <>
+__device__ __forceinline__ Quad<float> load_quad<float>(const float* __restrict__ vals, int64_t e) {
+    const f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(vals + e));
+    return Quad<float>{{a.x, a.y, a.z, a.w}};
+}
+
+template <>
+__device__ __forceinline__ Quad<double> load_quad<double>(const double* __restrict__ vals, int64_t e) {
+    const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals + e));
+    const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals + e + 2));
+    return Quad<double>{{a.x, a.y, b.x, b.y}};
+}
+
+// One lane's four entries of a pass as loaded; `in` = the lane has entries
+// (else its registers hold whatever entry 0 holds and count as sentinels).
+template <typename T>
+struct PassData {
+    u32x4 k;
+    Quad<T> v;
+    bool in;
+};
+
+template <typename T>
+__device__ __forceinline__ PassData<T> empty_pass() {
+    PassData<T> d;
+    d.k = u32x4{kTileSentinel, kTileSentinel, kTileSentinel, kTileSentinel};
+    d.v = Quad<T>{{T(0), T(0), T(0), T(0)}};
+    d.in = false;
+    return d;
+}
+
+// The lane's four entries at e (valid if e < end).  The loads are issued
+// either way (from entry 0 when there is nothing to load; the arrays hold at
+// least kTileLane entries) and nothing reads the loaded registers here, so
+// every pass refill issues the same loads on every path, the compiler can
+// count exactly how many loads follow each queue slot, and the refill never
+// waits for its own loads.
+template <typename T>
+__device__ __forceinline__ PassData<T> load_pass(const TileArgs& ta, int64_t e, int64_t end) {
+    PassData<T> d;
+    d.in = e < end;
+    const int64_t at = d.in ? e : 0;
+    d.k = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ta.keys + at));
+    d.v = load_quad<T>(static_cast<const T*>(ta.vals), at);
+    return d;
+}
+
+// The keys of a pass with the sentinel where the lane has no entries.
+template <typename T>
+__device__ __forceinline__ void pass_keys(const PassData<T>& d, uint32_t (&kk)[4]) {
+    kk[0] = d.in ? d.k.x : kTileSentinel;
+    kk[1] = d.in ? d.k.y : kTileSentinel;
+    kk[2] = d.in ? d.k.z : kTileSentinel;
+    kk[3] = d.in ? d.k.w : kTileSentinel;
+}
+
+// The next pass of a wave's stream of passes (its sub-ranges of units u0,
+// u0 + 1, ..., in order, kWave * kTileLane entries per pass).  ws points at
+// the wave's first sub-range start of its group (ws[u * kTileWaves] and
+// ws[u * kTileWaves + 1] bound its sub-range of unit u).  Wave-uniform.
+struct PassCursor {
+    int u;            // unit of the next pass; u1 when the stream is exhausted
+    int64_t b, e;     // what is left of the wave's sub-range of unit u
+
+    __device__ __forceinline__ void settle(const cint32* ws, int u1) {
+        while (u < u1 && b >= e) {
+            ++u;
+            if (u < u1) {
+                b = ws[u * kTileWaves];
+                e = ws[u * kTileWaves + 1];
+            }
+        }
+    }
+    template <typename T>
+    __device__ __forceinline__ PassData<T> next(const TileArgs& ta, const cint32* ws, int lane) {
+        const bool more = u < ta.u1;
+        const PassData<T> d = load_pass<T>(ta, more ? b + lane * kTileLane : 0, more ? e : 0);
+        if (more) {
+            b += kWave * kTileLane;
+            settle(ws, ta.u1);
+        }
+        return d;
+    }
+};
+
+// Add one pass (64 lanes x kTileLane entries, rows ascending, sentinels
+// last; xv = the gathered vector values of the lane's entries) into acc.
+// carry_row / carry_val: the last row of the previous pass of this
+// sub-range, not yet written (-1: none); wave-uniform.
+template <typename T>
+__device__ __forceinline__ void tile_pass(const uint32_t (&kk)[4], const Quad<T>& vals, const T (&xv)[4],
+                                          T* __restrict__ acc, int lane, int& carry_row, T& carry_val) {
+    int r[4];
+    T p[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        ok[u] = kk[u] != kTileSentinel;
+        r[u] = ok[u] ? static_cast<int>(kk[u] >> kTileColBits) : -1;
+        p[u] = ok[u] ? vals.v[u] * xv[u] : T(0);
+    }
+    // runs of equal rows inside the lane: the first (fv), interior ones
+    // (complete here: written at once) and the last (lv)
+    const bool has = ok[0];
+    const int fr = r[0];
+    int cr = fr, nruns = 1;
+    T cur = p[0], fv = T(0);
+#pragma unroll
+    for (int u = 1; u < 4; ++u) {
+        if (ok[u]) {
+            if (r[u] != cr) {
+                if (nruns == 1) fv = cur;
+                else acc[cr] = acc[cr] + cur;
+                ++nruns;
+                cr = r[u];
+                cur = p[u];
+            } else {
+                cur = cur + p[u];
+            }
+        }
+    }
+    const bool single = nruns == 1;
+    if (single) fv = cur;
+    const T lv = cur;
+    const int lr = cr;
+    // does this lane's first row continue the previous lane's last row?
+    const int prev_lr = __shfl_up(lr, 1, kWave);
+    const int prev_has = __shfl_up(has ? 1 : 0, 1, kWave);
+    const int lane0_fr = __shfl(fr, 0, kWave);
+    const int lane0_has = __shfl(has ? 1 : 0, 0, kWave);
+    const bool carried = carry_row >= 0 && lane0_has && lane0_fr == carry_row;
+    const bool cont = has && (lane > 0 ? (prev_has && prev_lr == fr) : carried);
+    // segmented inclusive scan of the open row's partial sums
+    T S = has ? (single ? fv : lv) : T(0);
+    int F = (!has || !single || !cont) ? 1 : 0;
+#pragma unroll
+    for (int dd = 1; dd < kWave; dd <<= 1) {
+        const T Su = __shfl_up(S, dd, kWave);
+        const int Fu = __shfl_up(F, dd, kWave);
+        if (lane >= dd && !F) {
+            S = Su + S;
+            F = Fu;
+        }
+    }
+    if (carried && !F) S = carry_val + S;
+    if (carry_row >= 0 && !carried && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
+    T Sprev = __shfl_up(S, 1, kWave);
+    if (lane == 0) Sprev = carried ? carry_val : T(0);
+    if (has && !single) {
+        const T cin = cont ? Sprev : T(0);
+        acc[fr] = acc[fr] + (cin + fv);
+    }
+    const int next_fr = __shfl_down(fr, 1, kWave);
+    const int next_has = __shfl_down(has ? 1 : 0, 1, kWave);
+    if (has && lane < kWave - 1 && !(next_has && next_fr == lr)) acc[lr] = acc[lr] + S;
+    const int has63 = __shfl(has ? 1 : 0, kWave - 1, kWave);
+    const int lr63 = __shfl(lr, kWave - 1, kWave);
+    const T s63 = __shfl(S, kWave - 1, kWave);
+    carry_row = has63 ? lr63 : -1;
+    carry_val = has63 ? s63 : T(0);
+}
+
+// x itself, through a DPP move (identity lane pattern).  A gathered value
+// passed through it is consumed where it is loaded, so the wait for the
+// gather stays on the gather path instead of following the value to where
+// the paths join (where it would also wait for the entry queue).
+__device__ __forceinline__ float lane_identity(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xE4, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_identity(double x) {
+    const long long v = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0xE4, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(v >> 32), 0xE4, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Workgroup barrier for LDS traffic only.  The kernels below share nothing
+// between waves through global memory, so the barrier fences the LDS
+// address space alone and leaves a wave's global loads in flight
+// (a full __syncthreads would wait for all of them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Copy vector slice s (its first min(W, cols - s W) values) into an LDS
+// buffer: run by the kTileStagers staging waves, thread st in [0, 64
+// kTileStagers), kChunkLoads 16-byte loads in flight per thread.
+template <typename T>
+__device__ __forceinline__ void stage_slice(const T* __restrict__ x, const TileArgs& ta, int s, T* __restrict__ buf,
+                                            int st) {
+    constexpr int NV = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int kStageThreads = kTileStagers * kWave;
+    constexpr int kChunkLoads = 16;
+    constexpr int kPer = kTileWidth / (kStageThreads * NV);         // 16-byte loads per thread and slice
+    static_assert(kPer % kChunkLoads == 0, "slice must split into whole chunks");
+    const int64_t base = (int64_t)s * ta.W;
+    const int w = min(ta.W, ta.cols - (int)base);
+#pragma unroll 1
+    for (int c0 = 0; c0 < kPer; c0 += kChunkLoads) {
+        vec r[kChunkLoads];
+#pragma unroll
+        for (int k = 0; k < kChunkLoads; ++k) {
+            const int j = ((c0 + k) * kStageThreads + st) * NV;
+            if (j + NV <= w) {
+                r[k] = *reinterpret_cast<const vec*>(x + base + j);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) r[k][i] = (j + i < w) ? x[base + j + i] : T(0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kChunkLoads; ++k) {
+            const int j = ((c0 + k) * kStageThreads + st) * NV;
+            *reinterpret_cast<vec*>(buf + j) = r[k];
+        }
+    }
+}
+
+// The group's row sums over units [u0, u1) into acc (LDS, rmax entries),
+// then op.finish(device row, sum) for each row of the group.  x: the
+// gathered vector (device order, 16-byte aligned).  All threads call it.
+//
+// Waves 0 .. kTileWaves-1 add the entries.  Each walks its stream of
+// passes (its sub-ranges of the units, in order) with the loads of the
+// next kRing passes in flight: a queue of kRing register slots, each
+// consumed in place and refilled with the pass kRing further on, so passes
+// are used in the order their loads were issued (a wave's vector memory
+// counter completes in that order) and no slot is copied while its load is
+// pending.  The last kTileStagers waves copy vector slices into LDS: while
+// the entry waves consume staged unit u from one buffer they fill the other
+// with the next staged slice (FP32; FP64 has one buffer, filled between
+// units while the entry waves wait).  Staging runs in waves of its own so
+// its loads never sit in an entry wave's queue.  Every unit ends in a
+// barrier of all waves: the next unit's sub-ranges give rows to other waves.
+constexpr int kRing = 4;
+
+template <typename T, class Op>
+__device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restrict__ x, Op& op) {
+    constexpr int NBUF = tile_buffers(sizeof(T));
+    extern __shared__ __align__(16) unsigned char tile_lds[];
+    T* xs = reinterpret_cast<T*>(tile_lds);
+    T* acc = xs + (size_t)NBUF * ta.W;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1);
+    // wave-uniform, so the unit table and sub-range bounds are scalar loads
+    // (their own counter: they never wait for the vector loads in flight)
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const cint32* un = const_table(ta.units);
+    for (int l = tid; l < ta.rmax; l += kTileThreads) acc[l] = T(0);
+    if (wave >= kTileWaves) {
+        // ---- staging waves
+        const int st = tid - kTileWaves * kWave;
+        if (ta.u0 < ta.u1) {
+            int fs = ta.u0;
+            if (NBUF == 1) fs = un[4 * ta.u0 + 1] ? ta.u0 : ta.u1;
+            else if (!un[4 * fs + 1]) fs = un[4 * fs + 3];
+            if (fs < ta.u1) stage_slice<T>(x, ta, un[4 * fs], xs + (size_t)(NBUF == 2 ? un[4 * fs + 2] : 0) * ta.W, st);
+        }
+        lds_barrier();
+        for (int u = ta.u0; u < ta.u1; ++u) {
+            if (NBUF == 2) {
+                const int ns = un[4 * u + 3];
+                if (un[4 * u + 1] && ns < ta.u1) stage_slice<T>(x, ta, un[4 * ns], xs + (size_t)un[4 * ns + 2] * ta.W, st);
+                lds_barrier();
+            } else {
+                lds_barrier();
+                if (u + 1 < ta.u1 && un[4 * (u + 1) + 1]) {
+                    stage_slice<T>(x, ta, un[4 * (u + 1)], xs, st);
+                    lds_barrier();
+                }
+            }
+        }
+    } else {
+        // ---- entry waves
+        const cint32* ws = const_table(ta.wstart) + ((int64_t)b * ta.U) * kTileWaves + wave;
+        PassCursor cur{ta.u1, 0, 0};
+        PassData<T> q0 = empty_pass<T>(), q1 = q0, q2 = q0, q3 = q0;
+        if (ta.u0 < ta.u1) {
+            cur = PassCursor{ta.u0, ws[ta.u0 * kTileWaves], ws[ta.u0 * kTileWaves + 1]};
+            cur.settle(ws, ta.u1);
+            q0 = cur.next<T>(ta, ws, lane);
+            q1 = cur.next<T>(ta, ws, lane);
+            q2 = cur.next<T>(ta, ws, lane);
+            q3 = cur.next<T>(ta, ws, lane);
+        }
+        lds_barrier();
+        // consumer state: unit cu, passes of this wave left in it
+        int cu = ta.u0, rem = 0, staged = 0, carry_row = -1;
+        T carry_val = T(0);
+        const T* xl = xs;
+        const T* xg = x;
+        auto open_unit = [&](int u) {
+            staged = un[4 * u + 1];
+            xl = xs + (size_t)(NBUF == 2 ? un[4 * u + 2] : 0) * ta.W;
+            xg = x + (int64_t)un[4 * u] * ta.W;
+            const int64_t e0 = ws[u * kTileWaves], e1 = ws[u * kTileWaves + 1];
+            rem = static_cast<int>((e1 - e0 + kWave * kTileLane - 1) / (kWave * kTileLane));
+            carry_row = -1;
+            carry_val = T(0);
+        };
+        auto close_unit = [&](int u) {
+            if (carry_row >= 0 && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
+            lds_barrier();
+            if (NBUF == 1 && u + 1 < ta.u1 && un[4 * (u + 1) + 1]) lds_barrier();   // staging waves fill the buffer
+        };
+        // close finished units (and units with no passes for this wave),
+        // open the next; false when the range is done
+        auto settle = [&]() -> bool {
+            while (cu < ta.u1 && rem == 0) {
+                close_unit(cu);
+                ++cu;
+                if (cu < ta.u1) open_unit(cu);
+            }
+            return cu < ta.u1;
+        };
+        auto consume_pass = [&](const PassData<T>& slot) {
+            uint32_t kk[4];
+            pass_keys(slot, kk);
+            uint32_t q[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q[k] = kk[k] != kTileSentinel ? (kk[k] & ((1u << kTileColBits) - 1u)) : 0u;
+            T xv[4];
+            if (staged) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) xv[k] = xl[q[k]];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) xv[k] = lane_identity(xg[q[k]]);
+            }
+            tile_pass<T>(kk, slot.v, xv, acc, lane, carry_row, carry_val);
+            --rem;
+        };
+        // one queue slot: close/open units, add the slot's pass if the
+        // wave's stream is not finished, refill the slot (loads are issued
+        // on every path, so each slot is always followed by the same number
+        // of loads, and the loop has its only exit at the top)
+        auto step = [&](PassData<T>& slot) {
+            if (settle()) consume_pass(slot);
+            slot = cur.next<T>(ta, ws, lane);
+        };
+        if (cu < ta.u1) open_unit(cu);
+        while (cu < ta.u1) {
+            step(q0);
+            step(q1);
+            step(q2);
+            step(q3);
+        }
+    }
+    lds_barrier();
+    for (int l = tid; l < ta.rmax; l += kTileThreads) {
+        const int p = tile_row(b, ta.nb, l);
+        if (p < ta.rows) op.finish(p, acc[l]);
+    }
+}
+
+// Products with fewer entries than this are not offered the tiles by the
+// autotuner (their vectors stay in L2 and the CSR kernels win).
+constexpr int64_t kTileTuneMinNnz = 1 << 20;
+
+// Host: the launch arguments of slice range k of `parts` for one precision.
+TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, int parts);
+
+// Host: build the tiled layout of an orientation from its device-order CSR
+// (values from A's natural FP64 values; FP32 image when want_f32).
+int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32);
+
+// Host: re-derive the tiles' value images from A's natural FP64 values.
+int refresh_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o);
+
+// Host: whether a tiled launch of precision f64 is possible on this device
+// for these tiles (LDS) and these vectors (alignment).
+bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* x);
+
+// Host: let `kernel` (a tiled kernel of element size tsize) allocate the LDS
+// its largest launch needs.
+hipError_t tile_lds_attr(const sov_hip_ctx* ctx, const void* kernel, int tsize);
+
+// Host: y = alpha (M x) + beta y over the tiles of an orientation.
+int spmv_tiles(sov_hip_ctx* ctx, const Orient& o, int prec, double alpha, const void* x, double beta, void* y);
+
+}  // namespace sovhip

@@ -40,7 +40,7 @@ class PdhgOpts(C.Structure):
         ("trace_cap", C.c_int64), ("trace_res", _f64p), ("trace_it", _i64p),
         ("restart_scheme", C.c_int32), ("autotune", C.c_int32), ("restart_beta_sufficient", C.c_double),
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
-        ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p),
+        ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p), ("tiles", C.c_int32),
     ]
 
 
@@ -283,8 +283,10 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
           restarts="reference", autotune=True, detect_infeasibility=False, primal_weight_smoothing=None,
-          warm_start_primal=None) -> Result:
-    """Pdhg(options).solve(problem, backend) through the C entry point."""
+          warm_start_primal=None, tiles=None) -> Result:
+    """Pdhg(options).solve(problem, backend) through the C entry point.
+    tiles: None = the autotuner decides, 0 = CSR step kernels only, k >= 1 =
+    the LDS-staged tiled step kernels in k launches per product."""
     L = engine()
     o = PdhgOpts()
     L.sov_pdhg_default_opts(C.byref(o))
@@ -302,6 +304,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     o.verbose = int(verbose)
     o.restart_scheme = 1 if restarts == "adaptive" else 0
     o.autotune = int(autotune)
+    if tiles is not None:
+        o.tiles = int(tiles)
     o.detect_infeasibility = int(detect_infeasibility)
     if primal_weight_smoothing is not None:
         o.primal_weight_smoothing = primal_weight_smoothing

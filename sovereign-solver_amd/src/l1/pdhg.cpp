@@ -837,6 +837,9 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             hip_ok(sov_hip_sync(ctx), "sync");
         }
         if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, prec), "tune");
+        if (o.tiles >= 0 && nnz > 0)
+            for (int orient : {0, 1})
+                if ((orient == 0 ? m : n) > 0) hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, o.tiles), "tiles");
 
         // primal-weight bookkeeping (FP64 copies of the last restart point)
         DevMem d_xlast, d_ylast;
@@ -1311,6 +1314,8 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
         if (!f32 || o.fp64_polish) init_state(S64, SOV_PREC_F64);
         DevMem d_act32(ctx, std::max<std::size_t>(m, 1) * 8);
         if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, f32 ? SOV_PREC_F32 : SOV_PREC_F64), "tune");
+        // the batched step kernels run on the CSR parts only
+        for (int orient : {0, 1}) hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, 0), "tiles");
 
         std::vector<int32_t> active(B, f32 ? 1 : 2), since(B, 0), flags32(B, 0), flags64(B, 0), since_stop(B, 0);
         std::vector<int32_t> stop_state(B, 0), promote(B, 0), use_avg(B, 0);
