@@ -152,9 +152,11 @@ int sov_hip_csr_get_segments(const sov_hip_csr* csr, int orient, int* count);
  * the layout (FP64 and, after make_f32, FP32 values) and runs the
  * orientation's SpMV and PDHG steps on it, in `parts` launches over ranges
  * of slices joined by per-row partial sums; parts = 0 frees it and returns
- * to the CSR parts.  FP64 launches need 160 KiB of LDS per workgroup and
- * otherwise use the CSR parts.  The summation order is fixed (results are
- * reproducible run to run) but differs from the CSR kernels'. */
+ * to the CSR parts; set_tiles applies to both precisions (the autotuner
+ * chooses per precision).  FP64 launches need 160 KiB of LDS per workgroup
+ * and otherwise use the CSR parts.  The summation order is fixed (results
+ * are reproducible run to run) but differs from the CSR kernels'.
+ * get_tiles reports the FP32 launch count (0: FP32 products on the CSR). */
 int sov_hip_csr_set_tiles(sov_hip_ctx* ctx, sov_hip_csr* csr, int orient, int parts);
 int sov_hip_csr_get_tiles(const sov_hip_csr* csr, int orient, int* parts);
 /* *usable = 1 when the orientation's products of precision prec run on the
@@ -176,8 +178,9 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
  * of at least 64k elements); for each, time every G in {1,2,4,8,16,32,64} per
  * length class of every segment on the SpMV of precision prec; keep the
  * fastest segment count with its lanes.  Orientations with at least 2^20
- * entries then also time the tiles (1, 2 and 4 launches) and keep them when
- * they beat the CSR parts. */
+ * entries then also time the tiles (1, 2 and 4 launches), in precision prec
+ * and in the other one (FP32 iterates end in an FP64 polish), and keep them
+ * for each precision where they beat the CSR parts. */
 int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* csr, int prec);
 
 /* ---- sparse level 2 ---------------------------------------------------

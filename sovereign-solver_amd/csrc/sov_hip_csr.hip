@@ -1005,23 +1005,25 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "tune during capture");
     if (prec == SOV_PREC_F32 && !h->f32) return fail(SOV_INVALID_ARGUMENT, "tune f32 needs make_f32");
     DeviceGuard2 g(ctx->device);
-    const size_t es = prec == SOV_PREC_F64 ? sizeof(double) : sizeof(float);
     const int64_t len = std::max<int64_t>(std::max(h->m, h->n), 1);
-    void *xv = nullptr, *yv = nullptr;
-    SOV_HIP_TRY(hipMalloc(&xv, es * len));
-    hipError_t e = hipMalloc(&yv, es * len);
+    void *xv = nullptr, *yv = nullptr;   // room for either precision
+    SOV_HIP_TRY(hipMalloc(&xv, sizeof(double) * len));
+    hipError_t e = hipMalloc(&yv, sizeof(double) * len);
     if (e != hipSuccess) { (void)hipFree(xv); return hip_fail(e, "tune buffers"); }
     int rc = SOV_OK;
-    if (es == sizeof(double)) {
-        k_fill_value<double><<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, 1.0, static_cast<double*>(xv));
-    } else {
-        k_fill_value<float><<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, 1.0f, static_cast<float*>(xv));
-    }
-    e = check_launch();
-    if (e != hipSuccess) rc = hip_fail(e, "tune fill");
+    // x = 1 in the element type of precision q
+    auto fill = [&](int q) {
+        if (q == SOV_PREC_F64)
+            k_fill_value<double><<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, 1.0, static_cast<double*>(xv));
+        else
+            k_fill_value<float><<<launch_blocks(len), kBlock, 0, ctx->stream>>>(len, 1.0f, static_cast<float*>(xv));
+        const hipError_t fe = check_launch();
+        return fe == hipSuccess ? SOV_OK : hip_fail(fe, "tune fill");
+    };
+    rc = fill(prec);
     for (int orient = 0; orient < 2 && rc == SOV_OK; ++orient) {
         Orient& o = orient == 0 ? h->a : h->t;
-        o.tiles.parts = 0;   // the CSR timings below must not run on the tiles
+        o.tiles.parts = o.tiles.parts64 = 0;   // the CSR timings below must not run on the tiles
         // segment counts worth trying: keep each gathered slice >= 64k elements
         const int64_t cols = o.nat.cols;
         int best_K = 1;
@@ -1048,24 +1050,35 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
                               o.part(k).lanes);
         }
         // the LDS-staged tiles, for products large enough to amortise their
-        // per-group staging (kept only when faster than the best CSR parts)
+        // per-group staging: timed in both precisions (FP32 iterates end in
+        // an FP64 polish), each kept only where faster than the CSR parts
         if (rc == SOV_OK && o.nat.nnz >= kTileTuneMinNnz && o.nat.rows >= 64) {
             if (!o.tiles.built() || (h->f32 && o.tiles.val32 == nullptr)) rc = build_tiles(ctx, h, o, h->f32);
-            int best_p = 0;
-            for (int P : {1, 2, 4}) {
-                if (rc != SOV_OK || P > std::max(o.tiles.S, 1)) break;
-                o.tiles.parts = P;
-                if (!tiles_usable(ctx, o.tiles, prec == SOV_PREC_F64, xv)) break;
-                const float ms = time_spmv(ctx, o, prec, xv, yv, 4);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best_p = P;
+            int choice[2] = {0, 0};   // by SOV_PREC_*
+            const int other = prec == SOV_PREC_F64 ? static_cast<int>(SOV_PREC_F32) : static_cast<int>(SOV_PREC_F64);
+            for (int q : {prec, other}) {
+                if (rc != SOV_OK || (q == SOV_PREC_F32 && !h->f32)) continue;
+                if (q != prec) rc = fill(q);
+                if (rc != SOV_OK) break;
+                o.tiles.parts = o.tiles.parts64 = 0;
+                float bms = q == prec ? best_ms : time_spmv(ctx, o, q, xv, yv, 4);
+                for (int P : {1, 2, 4}) {
+                    if (P > std::max(o.tiles.S, 1)) break;
+                    (q == SOV_PREC_F64 ? o.tiles.parts64 : o.tiles.parts) = P;
+                    if (!tiles_usable(ctx, o.tiles, q == SOV_PREC_F64, xv)) break;
+                    const float ms = time_spmv(ctx, o, q, xv, yv, 4);
+                    if (ms < bms) {
+                        bms = ms;
+                        choice[q] = P;
+                    }
                 }
             }
             if (rc == SOV_OK) {
-                if (best_p > 0) o.tiles.parts = best_p;
-                else o.tiles.free();
+                o.tiles.parts = choice[SOV_PREC_F32];
+                o.tiles.parts64 = choice[SOV_PREC_F64];
+                if (o.tiles.parts == 0 && o.tiles.parts64 == 0) o.tiles.free();
             }
+            if (rc == SOV_OK) rc = fill(prec);   // the next orientation times precision prec again
         }
     }
     (void)hipStreamSynchronize(ctx->stream);

@@ -131,7 +131,8 @@ struct Tiles {
     std::vector<int32_t> h_units;                     // host copy of units
     float* val32 = nullptr;                           // E (optional)
     double* val64 = nullptr;                          // E
-    int parts = 0;                                    // launches per product; 0 = tiles not in use
+    int parts = 0;                                    // FP32 launches per product; 0 = FP32 products on the CSR parts
+    int parts64 = 0;                                  // the same for FP64 products
     bool built() const { return wstart != nullptr; }
     void free();
 };

@@ -909,9 +909,10 @@ int launch_tile_step(sov_hip_ctx* ctx, const Orient& o, Op op, const T* x) {
     const Tiles& t = o.tiles;
     SOV_HIP_TRY(tile_lds_attr(ctx, reinterpret_cast<const void*>(&k_tile_step<T, Op>), sizeof(T)));
     const size_t lds = tile_lds_bytes(t.W, t.rmax, sizeof(T));
-    for (int k = 0; k < t.parts; ++k) {
-        op.mode = seg_mode(k, t.parts);
-        const TileArgs ta = tile_args(t, o.nat.rows, o.nat.cols, sizeof(T) == 8, k, t.parts);
+    const int np = sizeof(T) == 8 ? t.parts64 : t.parts;
+    for (int k = 0; k < np; ++k) {
+        op.mode = seg_mode(k, np);
+        const TileArgs ta = tile_args(t, o.nat.rows, o.nat.cols, sizeof(T) == 8, k, np);
         k_tile_step<T, Op><<<t.nb, kTileThreads, lds, ctx->stream>>>(op, ta, x);
     }
     SOV_HIP_TRY(check_launch());

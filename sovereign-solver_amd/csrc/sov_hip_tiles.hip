@@ -50,6 +50,7 @@ void Tiles::free() {
     nb = S = rmax = U = 0;
     E = 0;
     parts = 0;
+    parts64 = 0;
 }
 
 // Launch k of `parts` covers the units whose first slice lies in
@@ -77,7 +78,7 @@ TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, 
 }
 
 bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* x) {
-    if (t.parts <= 0 || t.nb <= 0 || !t.built()) return false;
+    if ((f64 ? t.parts64 : t.parts) <= 0 || t.nb <= 0 || !t.built()) return false;
     if (f64 ? (t.val64 == nullptr && t.E > 0) : (t.val32 == nullptr && t.E > 0)) return false;
     if ((reinterpret_cast<uintptr_t>(x) & 15u) != 0) return false;
     return (int64_t)tile_lds_bytes(t.W, t.rmax, f64 ? 8 : 4) <= ctx->lds_max;
@@ -255,7 +256,7 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
 }
 
 int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32) {
-    const int parts = o.tiles.parts;
+    const int parts = o.tiles.parts, parts64 = o.tiles.parts64;
     o.tiles.free();
     Tiles& t = o.tiles;
     const int32_t rows = o.nat.rows, nnz = o.nat.nnz;
@@ -295,6 +296,7 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32
         return hip_fail(e, "tiles upload");
     }
     t.parts = parts;
+    t.parts64 = parts64;
     rc = refresh_tile_values(ctx, h, o);
     if (rc != SOV_OK) {
         t.free();
@@ -341,11 +343,12 @@ static int spmv_tiles_t(sov_hip_ctx* ctx, const Orient& o, double alpha, const v
     const Tiles& t = o.tiles;
     const size_t lds = tile_lds_bytes(t.W, t.rmax, sizeof(T));
     SOV_HIP_TRY(tile_lds_attr(ctx, reinterpret_cast<const void*>(&k_tile_spmv<T>), sizeof(T)));
-    for (int k = 0; k < t.parts; ++k) {
-        const TileArgs ta = tile_args(t, o.nat.rows, o.nat.cols, sizeof(T) == 8, k, t.parts);
+    const int np = sizeof(T) == 8 ? t.parts64 : t.parts;
+    for (int k = 0; k < np; ++k) {
+        const TileArgs ta = tile_args(t, o.nat.rows, o.nat.cols, sizeof(T) == 8, k, np);
         TileSpmvOp<T> op{static_cast<T*>(y),
                          std::is_same<T, double>::value ? reinterpret_cast<T*>(o.acc64) : reinterpret_cast<T*>(o.acc32),
-                         T(alpha), T(beta), seg_mode(k, t.parts)};
+                         T(alpha), T(beta), seg_mode(k, np)};
         k_tile_spmv<T><<<t.nb, kTileThreads, lds, ctx->stream>>>(op, ta, static_cast<const T*>(x));
     }
     SOV_HIP_TRY(check_launch());
@@ -378,7 +381,7 @@ int sov_hip_csr_set_tiles(sov_hip_ctx* ctx, sov_hip_csr* h, int orient, int part
         const int rc = build_tiles(ctx, h, o, h->f32);
         if (rc != SOV_OK) return rc;
     }
-    o.tiles.parts = std::max(1, std::min(parts, std::max(o.tiles.S, 1)));
+    o.tiles.parts = o.tiles.parts64 = std::max(1, std::min(parts, std::max(o.tiles.S, 1)));
     return SOV_OK;
 }
 

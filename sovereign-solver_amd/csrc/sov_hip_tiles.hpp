@@ -146,6 +146,61 @@ struct PassCursor {
     }
 };
 
+// Cross-lane moves on the VALU (DPP, GFX9 controls).  update_dpp with
+// old = 0 and bound_ctrl off: a lane whose source lies outside its row (or
+// whose row is masked off) reads 0.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, dpp<CTRL, ROWS>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = dpp<CTRL, ROWS>(static_cast<int>(b));
+    const int hi = dpp<CTRL, ROWS>(static_cast<int>(b >> 32));
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double readlane(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ float readlane(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// One step of the segmented scan: (S, F) of the lane CTRL reaches combined
+// in front of the lane's own: S = F ? S : S_src + S, F = F | F_src.
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ void seg_step(T& S, int& F) {
+    const T Su = dpp<CTRL, ROWS>(S);
+    const int Fu = dpp<CTRL, ROWS>(F);
+    if (!F) S = Su + S;
+    F = F | Fu;
+}
+
+// Inclusive segmented scan over the 64 lanes: within rows of 16 by shifts
+// of 1, 2, 4, 8, then row 0's total into row 1 (and 2's into 3), then the
+// first half's into the second half.  A fixed tree: run-to-run identical.
+template <typename T>
+__device__ __forceinline__ void seg_scan(T& S, int& F) {
+    seg_step<kDppRowShr1, 0xF>(S, F);
+    seg_step<kDppRowShr2, 0xF>(S, F);
+    seg_step<kDppRowShr4, 0xF>(S, F);
+    seg_step<kDppRowShr8, 0xF>(S, F);
+    seg_step<kDppRowBcast15, 0xA>(S, F);
+    seg_step<kDppRowBcast31, 0xC>(S, F);
+}
+
 // Add one pass (64 lanes x kTileLane entries, rows ascending, sentinels
 // last; xv = the gathered vector values of the lane's entries) into acc.
 // carry_row / carry_val: the last row of the previous pass of this
@@ -186,41 +241,31 @@ __device__ __forceinline__ void tile_pass(const uint32_t (&kk)[4], const Quad<T>
     if (single) fv = cur;
     const T lv = cur;
     const int lr = cr;
-    // does this lane's first row continue the previous lane's last row?
-    const int prev_lr = __shfl_up(lr, 1, kWave);
-    const int prev_has = __shfl_up(has ? 1 : 0, 1, kWave);
-    const int lane0_fr = __shfl(fr, 0, kWave);
-    const int lane0_has = __shfl(has ? 1 : 0, 0, kWave);
-    const bool carried = carry_row >= 0 && lane0_has && lane0_fr == carry_row;
-    const bool cont = has && (lane > 0 ? (prev_has && prev_lr == fr) : carried);
+    // rows are encoded +1 in the lane exchanges so that 0 (what a lane with
+    // no source reads) means "no row"
+    const int lr1 = has ? lr + 1 : 0, fr1 = has ? fr + 1 : 0;
+    const int prev_lr1 = dpp<kDppWaveShr1, 0xF>(lr1);        // lane - 1's last row (+1), 0 for lane 0
+    const int next_fr1 = dpp<kDppWaveShl1, 0xF>(fr1);        // lane + 1's first row (+1), 0 for lane 63
+    const int lane0_fr1 = __builtin_amdgcn_readfirstlane(fr1);
+    const bool carried = carry_row >= 0 && lane0_fr1 == carry_row + 1;
+    const bool cont = has && (lane > 0 ? prev_lr1 == fr1 : carried);
     // segmented inclusive scan of the open row's partial sums
     T S = has ? (single ? fv : lv) : T(0);
     int F = (!has || !single || !cont) ? 1 : 0;
-#pragma unroll
-    for (int dd = 1; dd < kWave; dd <<= 1) {
-        const T Su = __shfl_up(S, dd, kWave);
-        const int Fu = __shfl_up(F, dd, kWave);
-        if (lane >= dd && !F) {
-            S = Su + S;
-            F = Fu;
-        }
-    }
+    seg_scan(S, F);
     if (carried && !F) S = carry_val + S;
     if (carry_row >= 0 && !carried && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
-    T Sprev = __shfl_up(S, 1, kWave);
+    T Sprev = dpp<kDppWaveShr1, 0xF>(S);
     if (lane == 0) Sprev = carried ? carry_val : T(0);
     if (has && !single) {
         const T cin = cont ? Sprev : T(0);
         acc[fr] = acc[fr] + (cin + fv);
     }
-    const int next_fr = __shfl_down(fr, 1, kWave);
-    const int next_has = __shfl_down(has ? 1 : 0, 1, kWave);
-    if (has && lane < kWave - 1 && !(next_has && next_fr == lr)) acc[lr] = acc[lr] + S;
-    const int has63 = __shfl(has ? 1 : 0, kWave - 1, kWave);
-    const int lr63 = __shfl(lr, kWave - 1, kWave);
-    const T s63 = __shfl(S, kWave - 1, kWave);
-    carry_row = has63 ? lr63 : -1;
-    carry_val = has63 ? s63 : T(0);
+    if (has && lane < kWave - 1 && next_fr1 != lr1) acc[lr] = acc[lr] + S;
+    const int lr63 = __builtin_amdgcn_readlane(lr1, kWave - 1);
+    const T s63 = readlane(S, kWave - 1);
+    carry_row = lr63 - 1;
+    carry_val = lr63 ? s63 : T(0);
 }
 
 // x itself, through a DPP move (identity lane pattern).  A gathered value
@@ -282,6 +327,34 @@ __device__ __forceinline__ void stage_slice(const T* __restrict__ x, const TileA
     }
 }
 
+// One staging thread's share of a whole FP32 slice, held in registers
+// between its loads and its store to LDS (so the loads of the next slice
+// are in flight while the entry waves consume the current one).
+struct SliceRegs {
+    static constexpr int kStageThreads = kTileStagers * kWave;
+    static constexpr int kPer = kTileWidth / (kStageThreads * 4);   // 16-byte loads per thread
+    f32x4 r[kPer];
+
+    __device__ __forceinline__ void load(const float* __restrict__ x, const TileArgs& ta, int s, int st) {
+        const int64_t base = (int64_t)s * ta.W;
+        const int w = min(ta.W, ta.cols - (int)base);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int j = (k * kStageThreads + st) * 4;
+            if (j + 4 <= w) {
+                r[k] = *reinterpret_cast<const f32x4*>(x + base + j);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) r[k][i] = (j + i < w) ? x[base + j + i] : 0.f;
+            }
+        }
+    }
+    __device__ __forceinline__ void store(float* __restrict__ buf, int st) const {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) *reinterpret_cast<f32x4*>(buf + (k * kStageThreads + st) * 4) = r[k];
+    }
+};
+
 // The group's row sums over units [u0, u1) into acc (LDS, rmax entries),
 // then op.finish(device row, sum) for each row of the group.  x: the
 // gathered vector (device order, 16-byte aligned).  All threads call it.
@@ -313,21 +386,37 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
     const cint32* un = const_table(ta.units);
     for (int l = tid; l < ta.rmax; l += kTileThreads) acc[l] = T(0);
     if (wave >= kTileWaves) {
-        // ---- staging waves
         const int st = tid - kTileWaves * kWave;
-        if (ta.u0 < ta.u1) {
-            int fs = ta.u0;
-            if (NBUF == 1) fs = un[4 * ta.u0 + 1] ? ta.u0 : ta.u1;
-            else if (!un[4 * fs + 1]) fs = un[4 * fs + 3];
-            if (fs < ta.u1) stage_slice<T>(x, ta, un[4 * fs], xs + (size_t)(NBUF == 2 ? un[4 * fs + 2] : 0) * ta.W, st);
-        }
-        lds_barrier();
-        for (int u = ta.u0; u < ta.u1; ++u) {
-            if (NBUF == 2) {
-                const int ns = un[4 * u + 3];
-                if (un[4 * u + 1] && ns < ta.u1) stage_slice<T>(x, ta, un[4 * ns], xs + (size_t)un[4 * ns + 2] * ta.W, st);
+        if constexpr (NBUF == 2) {
+            // ---- staging waves, FP32: during staged unit u they store the
+            // slice of the next staged unit (loaded into registers one
+            // staged unit earlier) into the free buffer, then load the one
+            // after
+            SliceRegs R;
+            int nxt = ta.u1;   // staged unit whose slice R holds
+            if (ta.u0 < ta.u1) {
+                const int fs = un[4 * ta.u0 + 1] ? ta.u0 : un[4 * ta.u0 + 3];
+                if (fs < ta.u1) {
+                    R.load(x, ta, un[4 * fs], st);
+                    R.store(xs + (size_t)un[4 * fs + 2] * ta.W, st);
+                    nxt = un[4 * fs + 3];
+                    if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
+                }
+            }
+            lds_barrier();
+            for (int u = ta.u0; u < ta.u1; ++u) {
+                if (un[4 * u + 1] && nxt < ta.u1) {
+                    R.store(xs + (size_t)un[4 * nxt + 2] * ta.W, st);
+                    nxt = un[4 * nxt + 3];
+                    if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
+                }
                 lds_barrier();
-            } else {
+            }
+        } else {
+            // ---- staging waves, FP64 (one buffer): between units
+            if (ta.u0 < ta.u1 && un[4 * ta.u0 + 1]) stage_slice<T>(x, ta, un[4 * ta.u0], xs, st);
+            lds_barrier();
+            for (int u = ta.u0; u < ta.u1; ++u) {
                 lds_barrier();
                 if (u + 1 < ta.u1 && un[4 * (u + 1) + 1]) {
                     stage_slice<T>(x, ta, un[4 * (u + 1)], xs, st);
