@@ -174,6 +174,16 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
                         const int32_t* col_idx, const int32_t* src, int32_t cus, int32_t shape[6],
                         int64_t* entries, uint32_t* keys, int32_t* src_out, int32_t* wstart,
                         int32_t* units);
+/* The pass list the tiled kernels read for the same layout: *passes = pass
+ * count; pidx (groups * sub-ranges per tile * (units + 1)) = index of the
+ * first pass of (group, wave) in each unit, then one past its last; pdesc
+ * (2 * passes) = first entry and entry count of each pass (at most 256
+ * entries, inside one sub-range).  pidx / pdesc may be null (first call for
+ * the size).  Host only.  No reference counterpart (layout of the tiled
+ * products that replace Backend::spmv / spmv_transpose). */
+int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
+                        const int32_t* col_idx, const int32_t* src, int32_t cus, int64_t* passes,
+                        int32_t* pidx, int32_t* pdesc);
 /* Autotuner: for each orientation, try 1, 2, 4 and 8 column segments (slices
  * of at least 64k elements); for each, time every G in {1,2,4,8,16,32,64} per
  * length class of every segment on the SpMV of precision prec; keep the

@@ -883,7 +883,7 @@ int sov_hip_csr_make_f32(sov_hip_ctx* ctx, sov_hip_csr* h) {
     }
     for (Orient* o : {&h->a, &h->t})
         if (o->tiles.built() && o->tiles.val32 == nullptr)
-            SOV_HIP_TRY(hipMalloc(&o->tiles.val32, sizeof(float) * std::max<int64_t>(o->tiles.E, kTileLane)));
+            SOV_HIP_TRY(hipMalloc(&o->tiles.val32, sizeof(float) * (o->tiles.E + kTileLane)));
     h->f32 = true;
     return refresh_values(ctx, h);
 }

@@ -104,21 +104,30 @@ struct Natural {
 // Entries are stored group-major, then unit, then wave: each (group, unit)
 // tile is split at row boundaries into kTileWaves contiguous sub-ranges of
 // about equal size, one per entry wave, each padded to a multiple of kTileLane
-// entries with key kTileSentinel.  key = (local row << 20) | (column -
+// entries with key kTileSentinel (value 0); kTileLane more sentinels follow
+// the last entry (what a lane with no entries in a pass loads).  key = (local row << 20) | (column -
 // first column of the unit).  wstart[(b U + u) kTileWaves + w] is the first
 // entry of wave w of tile (b, u); the array ends with the total count E.
 // units[4 u .. 4 u + 3] = {first slice, staged, LDS buffer of a staged
 // unit, next staged unit (U if none)}, units[4 U] = S.
+//
+// What the kernels read instead of wstart: the pass list.  A pass is up to
+// kWave * kTileLane consecutive entries of one sub-range.  The passes of
+// wave w of group b are listed unit by unit: pidx[(b kTileWaves + w) (U + 1)
+// + u] is the index of its first pass in unit u (entry U: one past its
+// last), and pdesc[2 i], pdesc[2 i + 1] = first entry and entry count of
+// pass i.
 constexpr int kTileThreads = 1024;                    // workgroup of a group
 constexpr int kTileStagers = 4;                       // waves that copy vector slices into LDS
 constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 12 waves that add the entries
 constexpr int kTileLane = 4;                          // entries per lane per pass
 constexpr int kTileWidth = 16384;                     // columns per slice
-constexpr int kTileRowCap = 4096;                     // local rows per group (12 key bits)
+constexpr int kTileRowCap = 4032;                     // local rows per group (12 key bits; FP64 LDS)
 constexpr int kTileColBits = 20;                      // key bits of the column within a unit
+constexpr uint32_t kTileColMask = (1u << kTileColBits) - 1u;
 constexpr int kTileGatherSlices = 63;                 // slices per gather unit (columns < 2^20 - 1)
 constexpr int kTileStageMin = 384;                    // entries per group that make a slice staged
-constexpr uint32_t kTileSentinel = 0xFFFFFFFFu;       // never a real key: columns stay below 2^20 - 1
+constexpr uint32_t kTileSentinel = 0xFFF00000u;       // row 4095, column 0: never a real key (rows < kTileRowCap)
 
 struct Tiles {
     int32_t nb = 0, S = 0, W = kTileWidth, rmax = 0;  // groups, slices, slice width, local rows per group
@@ -126,14 +135,16 @@ struct Tiles {
     int64_t E = 0;                                    // stored entries (with padding)
     uint32_t* keys = nullptr;                         // E
     int32_t* src = nullptr;                           // E: A natural position, -1 for padding
-    int32_t* wstart = nullptr;                        // nb * U * kTileWaves + 1
+    int32_t* pidx = nullptr;                          // nb * kTileWaves * (U + 1)
+    int32_t* pdesc = nullptr;                         // 2 * passes
+    int64_t passes = 0;
     int32_t* units = nullptr;                         // 4 U + 1
     std::vector<int32_t> h_units;                     // host copy of units
     float* val32 = nullptr;                           // E (optional)
     double* val64 = nullptr;                          // E
     int parts = 0;                                    // FP32 launches per product; 0 = FP32 products on the CSR parts
     int parts64 = 0;                                  // the same for FP64 products
-    bool built() const { return wstart != nullptr; }
+    bool built() const { return pidx != nullptr; }
     void free();
 };
 

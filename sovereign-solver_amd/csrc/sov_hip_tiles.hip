@@ -36,13 +36,16 @@ struct DeviceGuardT {
 void Tiles::free() {
     (void)hipFree(keys);
     (void)hipFree(src);
-    (void)hipFree(wstart);
+    (void)hipFree(pidx);
+    (void)hipFree(pdesc);
     (void)hipFree(units);
     (void)hipFree(val32);
     (void)hipFree(val64);
     keys = nullptr;
     src = nullptr;
-    wstart = nullptr;
+    pidx = nullptr;
+    pdesc = nullptr;
+    passes = 0;
     units = nullptr;
     val32 = nullptr;
     val64 = nullptr;
@@ -59,7 +62,9 @@ TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, 
     TileArgs a{};
     a.keys = t.keys;
     a.vals = f64 ? static_cast<const void*>(t.val64) : static_cast<const void*>(t.val32);
-    a.wstart = t.wstart;
+    a.pidx = t.pidx;
+    a.pdesc = t.pdesc;
+    a.sent = static_cast<int32_t>(t.E);
     a.units = t.units;
     a.nb = t.nb;
     a.U = t.U;
@@ -173,13 +178,41 @@ __global__ void k_tile_vals(int64_t E, const int32_t* __restrict__ src, const do
 
 template <typename T>
 int gather_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, const Tiles& t, T* out) {
-    const int64_t n = std::max<int64_t>(t.E, kTileLane);   // the arrays' full length (padding reads 0)
+    const int64_t n = t.E + kTileLane;   // the arrays' full length (padding and the sentinel block read 0)
     k_tile_vals<T><<<launch_blocks(n), kBlock, 0, ctx->stream>>>(n, t.src, h->a.nat.val64, out);
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
 }
 
 }  // namespace
+
+// The pass list of a layout (what the kernels read instead of wstart): for
+// each (group, wave), unit by unit, the passes of its sub-range, each up to
+// kWave * kTileLane entries.  Host only.
+static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, std::vector<int32_t>& pidx,
+                       std::vector<int32_t>& pdesc) {
+    const int U = t.U;
+    constexpr int kPass = kWave * kTileLane;
+    pidx.assign(static_cast<size_t>(t.nb) * kTileWaves * (U + 1), 0);
+    pdesc.clear();
+    for (int b = 0; b < t.nb; ++b)
+        for (int w = 0; w < kTileWaves; ++w) {
+            const size_t base = (static_cast<size_t>(b) * kTileWaves + w) * (U + 1);
+            for (int u = 0; u < U; ++u) {
+                pidx[base + u] = static_cast<int32_t>(pdesc.size() / 2);
+                const size_t at = (static_cast<size_t>(b) * U + u) * kTileWaves + w;
+                const int64_t e0 = wstart[at], e1 = wstart[at + 1];
+                for (int64_t e = e0; e < e1; e += kPass) {
+                    pdesc.push_back(static_cast<int32_t>(e));
+                    pdesc.push_back(static_cast<int32_t>(std::min<int64_t>(kPass, e1 - e)));
+                }
+            }
+            if (pdesc.size() / 2 >= static_cast<size_t>(INT32_MAX))
+                return fail(SOV_INVALID_ARGUMENT, "tiles: too many passes");
+            pidx[base + U] = static_cast<int32_t>(pdesc.size() / 2);
+        }
+    return SOV_OK;
+}
 
 // The tiled layout of a device-order CSR (rows x cols, nnz entries with
 // source positions src), for a device with `cus` compute units: fills the
@@ -237,9 +270,9 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     }
     int64_t E = 0;
     for (const GroupOut& g : groups) E += static_cast<int64_t>(g.keys.size());
-    if (E >= INT32_MAX) return fail(SOV_INVALID_ARGUMENT, "tiles: too many entries for 32-bit offsets");
+    if (E + kTileLane >= INT32_MAX) return fail(SOV_INVALID_ARGUMENT, "tiles: too many entries for 32-bit offsets");
     wstart.assign(static_cast<size_t>(nb) * t.U * kTileWaves + 1, 0);
-    keys.assign(static_cast<size_t>(std::max<int64_t>(E, kTileLane)), kTileSentinel);
+    keys.assign(static_cast<size_t>(E + kTileLane), kTileSentinel);   // entries, then the sentinel block
     src.assign(keys.size(), -1);
     int64_t off = 0;
     for (int64_t b = 0; b < nb; ++b) {
@@ -273,15 +306,20 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32
     std::vector<uint32_t> keys;
     std::vector<int32_t> src, wstart;
     int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, t, keys, src, wstart);
+    std::vector<int32_t> pidx, pdesc;
+    if (rc == SOV_OK) rc = pass_tables(t, wstart, pidx, pdesc);
     if (rc != SOV_OK) {
         t.free();
         return rc;
     }
+    t.passes = static_cast<int64_t>(pdesc.size() / 2);
+    if (pdesc.empty()) pdesc.assign(2, 0);
     const int64_t E = t.E;
-    const int64_t e1 = std::max<int64_t>(E, kTileLane);   // the kernels always read one pass at entry 0
+    const int64_t e1 = E + kTileLane;   // entries and the sentinel block
     hipError_t e = hipMalloc(&t.keys, sizeof(uint32_t) * e1);
     if (e == hipSuccess) e = hipMalloc(&t.src, sizeof(int32_t) * e1);
-    if (e == hipSuccess) e = hipMalloc(&t.wstart, sizeof(int32_t) * wstart.size());
+    if (e == hipSuccess) e = hipMalloc(&t.pidx, sizeof(int32_t) * pidx.size());
+    if (e == hipSuccess) e = hipMalloc(&t.pdesc, sizeof(int32_t) * pdesc.size());
     if (e == hipSuccess) e = hipMalloc(&t.units, sizeof(int32_t) * t.h_units.size());
     if (e == hipSuccess)
         e = hipMemcpy(t.units, t.h_units.data(), sizeof(int32_t) * t.h_units.size(), hipMemcpyHostToDevice);
@@ -289,8 +327,8 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32
     if (e == hipSuccess && want_f32) e = hipMalloc(&t.val32, sizeof(float) * e1);
     if (e == hipSuccess) e = hipMemcpy(t.keys, keys.data(), sizeof(uint32_t) * e1, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(t.src, src.data(), sizeof(int32_t) * e1, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(t.wstart, wstart.data(), sizeof(int32_t) * wstart.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.pidx, pidx.data(), sizeof(int32_t) * pidx.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.pdesc, pdesc.data(), sizeof(int32_t) * pdesc.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         t.free();
         return hip_fail(e, "tiles upload");
@@ -414,6 +452,30 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     if (src_out) std::copy(so.begin(), so.begin() + t.E, src_out);
     if (wstart) std::copy(ws.begin(), ws.end(), wstart);
     if (units) std::copy(t.h_units.begin(), t.h_units.end(), units);
+    return SOV_OK;
+}
+
+int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* ptr, const int32_t* idx,
+                        const int32_t* src, int32_t cus, int64_t* passes, int32_t* pidx, int32_t* pdesc) {
+    if (rows <= 0 || cols < 0 || nnz < 0 || ptr == nullptr || passes == nullptr ||
+        (nnz > 0 && (idx == nullptr || src == nullptr)))
+        return fail(SOV_INVALID_ARGUMENT, "tile_passes args");
+    if (ptr[0] != 0 || ptr[rows] != nnz) return fail(SOV_INVALID_ARGUMENT, "tile_passes: row pointers");
+    for (int32_t r = 0; r < rows; ++r)
+        if (ptr[r + 1] < ptr[r]) return fail(SOV_INVALID_ARGUMENT, "tile_passes: row pointers");
+    for (int32_t q = 0; q < nnz; ++q)
+        if (idx[q] < 0 || idx[q] >= cols) return fail(SOV_INVALID_ARGUMENT, "tile_passes: column index");
+    std::vector<int32_t> p(ptr, ptr + rows + 1), ix(idx ? idx : ptr, idx ? idx + nnz : ptr),
+        sv(src ? src : ptr, src ? src + nnz : ptr);
+    Tiles t;
+    std::vector<uint32_t> k;
+    std::vector<int32_t> so, ws, pi, pd;
+    int rc = layout_host(rows, cols, p, ix, sv, cus, t, k, so, ws);
+    if (rc == SOV_OK) rc = pass_tables(t, ws, pi, pd);
+    if (rc != SOV_OK) return rc;
+    *passes = static_cast<int64_t>(pd.size() / 2);
+    if (pidx) std::copy(pi.begin(), pi.end(), pidx);
+    if (pdesc) std::copy(pd.begin(), pd.end(), pdesc);
     return SOV_OK;
 }
 

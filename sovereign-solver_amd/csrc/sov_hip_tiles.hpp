@@ -35,10 +35,12 @@ __device__ __forceinline__ const cint32* const_table(const int32_t* p) {
 struct TileArgs {
     const uint32_t* keys;
     const void* vals;         // T
-    const int32_t* wstart;
+    const int32_t* pidx;      // nb * kTileWaves * (U + 1): first pass of (group, wave) in unit u
+    const int32_t* pdesc;     // 2 per pass: first entry, entries in the pass
     const int32_t* units;     // 4 U + 1
     int32_t nb, U, W, rmax, rows, cols;
     int32_t u0, u1;
+    int32_t sent;             // first entry of the sentinel block
 };
 
 // Device row of local row l of group b.
@@ -47,8 +49,10 @@ __host__ __device__ __forceinline__ int tile_row(int b, int nb, int l) { return 
 // Slice buffers that fit next to the row sums (2: double-buffered, 1: single).
 __host__ __device__ constexpr int tile_buffers(int tsize) { return tsize == 4 ? 2 : 1; }
 
+// LDS of a tiled launch: the slice buffers, the row sums and one slot per
+// lane that takes the writes a lane makes no real write with.
 __host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize) {
-    return (size_t)tile_buffers(tsize) * W * tsize + (size_t)rmax * tsize;
+    return (size_t)tile_buffers(tsize) * W * tsize + (size_t)(rmax + kWave) * tsize;
 }
 
 // The four values of one lane's entries (T = float: one 16-byte load;
@@ -59,92 +63,44 @@ struct Quad {
 };
 
 template <typename T>
-__device__ __forceinline__ Quad<T> load_quad(const T* __restrict__ vals, int64_t e);
+__device__ __forceinline__ Quad<T> load_quad(const T* __restrict__ vals, uint32_t e);
 
 template <>
-__device__ __forceinline__ Quad<float> load_quad<float>(const float* __restrict__ vals, int64_t e) {
+__device__ __forceinline__ Quad<float> load_quad<float>(const float* __restrict__ vals, uint32_t e) {
     const f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(vals + e));
     return Quad<float>{{a.x, a.y, a.z, a.w}};
 }
 
 template <>
-__device__ __forceinline__ Quad<double> load_quad<double>(const double* __restrict__ vals, int64_t e) {
+__device__ __forceinline__ Quad<double> load_quad<double>(const double* __restrict__ vals, uint32_t e) {
     const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals + e));
     const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals + e + 2));
     return Quad<double>{{a.x, a.y, b.x, b.y}};
 }
 
-// One lane's four entries of a pass as loaded; `in` = the lane has entries
-// (else its registers hold whatever entry 0 holds and count as sentinels).
+// One lane's four entries of a pass as loaded (sentinels where it has
+// none).
 template <typename T>
 struct PassData {
     u32x4 k;
     Quad<T> v;
-    bool in;
 };
 
+// The lane's four entries of the pass whose first entry is `start` and
+// which holds `cnt` entries; a lane at or past cnt loads the sentinel block
+// (keys kTileSentinel, values 0), so what a lane holds needs no mask.  The
+// loads are issued on every path and nothing reads the loaded registers
+// here, so every refill issues the same loads, the compiler can count
+// exactly how many loads follow each queue slot, and the refill never waits
+// for its own loads.
 template <typename T>
-__device__ __forceinline__ PassData<T> empty_pass() {
+__device__ __forceinline__ PassData<T> load_pass(const TileArgs& ta, int32_t start, int32_t cnt, int lane) {
     PassData<T> d;
-    d.k = u32x4{kTileSentinel, kTileSentinel, kTileSentinel, kTileSentinel};
-    d.v = Quad<T>{{T(0), T(0), T(0), T(0)}};
-    d.in = false;
-    return d;
-}
-
-// The lane's four entries at e (valid if e < end).  The loads are issued
-// either way (from entry 0 when there is nothing to load; the arrays hold at
-// least kTileLane entries) and nothing reads the loaded registers here, so
-// every pass refill issues the same loads on every path, the compiler can
-// count exactly how many loads follow each queue slot, and the refill never
-// waits for its own loads.
-template <typename T>
-__device__ __forceinline__ PassData<T> load_pass(const TileArgs& ta, int64_t e, int64_t end) {
-    PassData<T> d;
-    d.in = e < end;
-    const int64_t at = d.in ? e : 0;
+    const uint32_t at = static_cast<uint32_t>(lane * kTileLane < cnt ? start + lane * kTileLane : ta.sent);
     d.k = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ta.keys + at));
     d.v = load_quad<T>(static_cast<const T*>(ta.vals), at);
     return d;
 }
-
-// The keys of a pass with the sentinel where the lane has no entries.
-template <typename T>
-__device__ __forceinline__ void pass_keys(const PassData<T>& d, uint32_t (&kk)[4]) {
-    kk[0] = d.in ? d.k.x : kTileSentinel;
-    kk[1] = d.in ? d.k.y : kTileSentinel;
-    kk[2] = d.in ? d.k.z : kTileSentinel;
-    kk[3] = d.in ? d.k.w : kTileSentinel;
-}
-
-// The next pass of a wave's stream of passes (its sub-ranges of units u0,
-// u0 + 1, ..., in order, kWave * kTileLane entries per pass).  ws points at
-// the wave's first sub-range start of its group (ws[u * kTileWaves] and
-// ws[u * kTileWaves + 1] bound its sub-range of unit u).  Wave-uniform.
-struct PassCursor {
-    int u;            // unit of the next pass; u1 when the stream is exhausted
-    int64_t b, e;     // what is left of the wave's sub-range of unit u
-
-    __device__ __forceinline__ void settle(const cint32* ws, int u1) {
-        while (u < u1 && b >= e) {
-            ++u;
-            if (u < u1) {
-                b = ws[u * kTileWaves];
-                e = ws[u * kTileWaves + 1];
-            }
-        }
-    }
-    template <typename T>
-    __device__ __forceinline__ PassData<T> next(const TileArgs& ta, const cint32* ws, int lane) {
-        const bool more = u < ta.u1;
-        const PassData<T> d = load_pass<T>(ta, more ? b + lane * kTileLane : 0, more ? e : 0);
-        if (more) {
-            b += kWave * kTileLane;
-            settle(ws, ta.u1);
-        }
-        return d;
-    }
-};
 
 // Cross-lane moves on the VALU (DPP, GFX9 controls).  update_dpp with
 // old = 0 and bound_ctrl off: a lane whose source lies outside its row (or
@@ -205,67 +161,84 @@ __device__ __forceinline__ void seg_scan(T& S, int& F) {
 // last; xv = the gathered vector values of the lane's entries) into acc.
 // carry_row / carry_val: the last row of the previous pass of this
 // sub-range, not yet written (-1: none); wave-uniform.
+//
+// Each lane sums its runs of equal rows in storage order.  A run that ends
+// inside the lane (the next entry has another row, a sentinel included) is
+// complete, except that the lane's first run may continue a row of earlier
+// lanes, whose part arrives as cin.  The lane's tail run (the one holding
+// its last entry, if real) may continue into later lanes: the tails are
+// joined by a segmented scan over the wave and written by the lane where
+// the row ends (lane 63's goes to the carry).  Sentinels have row 4095 and
+// value 0, so they never join a real run.
+//
+// The writes are one batch: all reads of acc first, then all writes.  The
+// rows written in one pass are distinct (a lane's runs are distinct rows,
+// a run crossing lanes is written once), so the batch reads no value it
+// writes; a write a lane does not make goes to its own slot acc[dummy]
+// (dummy = rmax + lane), so the batch needs no branches.
 template <typename T>
-__device__ __forceinline__ void tile_pass(const uint32_t (&kk)[4], const Quad<T>& vals, const T (&xv)[4],
-                                          T* __restrict__ acc, int lane, int& carry_row, T& carry_val) {
+__device__ __forceinline__ void tile_pass(const u32x4& key, const Quad<T>& vals, const T (&xv)[4],
+                                          T* __restrict__ acc, int dummy, int lane, int& carry_row, T& carry_val) {
+    const uint32_t kk[4] = {key.x, key.y, key.z, key.w};
     int r[4];
-    T p[4];
+    T s[4];
     bool ok[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        ok[u] = kk[u] != kTileSentinel;
-        r[u] = ok[u] ? static_cast<int>(kk[u] >> kTileColBits) : -1;
-        p[u] = ok[u] ? vals.v[u] * xv[u] : T(0);
+        ok[u] = kk[u] < kTileSentinel;
+        r[u] = static_cast<int>(kk[u] >> kTileColBits);
     }
-    // runs of equal rows inside the lane: the first (fv), interior ones
-    // (complete here: written at once) and the last (lv)
-    const bool has = ok[0];
-    const int fr = r[0];
-    int cr = fr, nruns = 1;
-    T cur = p[0], fv = T(0);
+    // running sums of the runs; a run ends inside the lane at u < 3 when
+    // entry u + 1 has another row
+    bool endin[3];
+    s[0] = vals.v[0] * xv[0];
 #pragma unroll
     for (int u = 1; u < 4; ++u) {
-        if (ok[u]) {
-            if (r[u] != cr) {
-                if (nruns == 1) fv = cur;
-                else acc[cr] = acc[cr] + cur;
-                ++nruns;
-                cr = r[u];
-                cur = p[u];
-            } else {
-                cur = cur + p[u];
-            }
-        }
+        const T p = vals.v[u] * xv[u];
+        endin[u - 1] = r[u] != r[u - 1];
+        s[u] = endin[u - 1] ? p : s[u - 1] + p;
     }
-    const bool single = nruns == 1;
-    if (single) fv = cur;
-    const T lv = cur;
-    const int lr = cr;
+    const bool single = !(endin[0] | endin[1] | endin[2]);
     // rows are encoded +1 in the lane exchanges so that 0 (what a lane with
-    // no source reads) means "no row"
-    const int lr1 = has ? lr + 1 : 0, fr1 = has ? fr + 1 : 0;
-    const int prev_lr1 = dpp<kDppWaveShr1, 0xF>(lr1);        // lane - 1's last row (+1), 0 for lane 0
+    // no source reads, or a lane whose entry is a sentinel) means "no row"
+    const int fr1 = ok[0] ? r[0] + 1 : 0, tl1 = ok[3] ? r[3] + 1 : 0;
+    const int prev_tl1 = dpp<kDppWaveShr1, 0xF>(tl1);        // lane - 1's tail row (+1), 0 for lane 0
     const int next_fr1 = dpp<kDppWaveShl1, 0xF>(fr1);        // lane + 1's first row (+1), 0 for lane 63
     const int lane0_fr1 = __builtin_amdgcn_readfirstlane(fr1);
-    const bool carried = carry_row >= 0 && lane0_fr1 == carry_row + 1;
-    const bool cont = has && (lane > 0 ? prev_lr1 == fr1 : carried);
-    // segmented inclusive scan of the open row's partial sums
-    T S = has ? (single ? fv : lv) : T(0);
-    int F = (!has || !single || !cont) ? 1 : 0;
+    const bool carried = (carry_row >= 0) & (lane0_fr1 == carry_row + 1);
+    const bool cont = (fr1 != 0) & (lane > 0 ? prev_tl1 == fr1 : carried);
+    // segmented inclusive scan of the tails
+    T S = ok[3] ? s[3] : T(0);
+    int F = (single & cont) ? 0 : 1;
     seg_scan(S, F);
-    if (carried && !F) S = carry_val + S;
-    if (carry_row >= 0 && !carried && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
+    if (carried & !F) S = carry_val + S;
     T Sprev = dpp<kDppWaveShr1, 0xF>(S);
     if (lane == 0) Sprev = carried ? carry_val : T(0);
-    if (has && !single) {
-        const T cin = cont ? Sprev : T(0);
-        acc[fr] = acc[fr] + (cin + fv);
+    const T cin = cont ? Sprev : T(0);
+    // the carried row, when this pass does not continue it (wave-uniform)
+    if ((carry_row >= 0) & !carried) {
+        if (lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
     }
-    if (has && lane < kWave - 1 && next_fr1 != lr1) acc[lr] = acc[lr] + S;
-    const int lr63 = __builtin_amdgcn_readlane(lr1, kWave - 1);
+    // the batch: runs ending at u = 0..2 (the first takes cin) and the tail
+    // when its row ends in this lane
+    T wv[4];
+    bool wm[4];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        wm[u] = endin[u] & ok[u];
+        wv[u] = s[u] + (r[u] == r[0] ? cin : T(0));
+    }
+    wm[3] = (tl1 != 0) & (lane < kWave - 1) & (next_fr1 != tl1);
+    wv[3] = S;
+    T old[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) old[u] = acc[wm[u] ? r[u] : 0];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[wm[u] ? r[u] : dummy] = old[u] + wv[u];
+    const int tl63 = __builtin_amdgcn_readlane(tl1, kWave - 1);
     const T s63 = readlane(S, kWave - 1);
-    carry_row = lr63 - 1;
-    carry_val = lr63 ? s63 : T(0);
+    carry_row = tl63 - 1;
+    carry_val = tl63 ? s63 : T(0);
 }
 
 // x itself, through a DPP move (identity lane pattern).  A gathered value
@@ -426,29 +399,46 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         }
     } else {
         // ---- entry waves
-        const cint32* ws = const_table(ta.wstart) + ((int64_t)b * ta.U) * kTileWaves + wave;
-        PassCursor cur{ta.u1, 0, 0};
-        PassData<T> q0 = empty_pass<T>(), q1 = q0, q2 = q0, q3 = q0;
-        if (ta.u0 < ta.u1) {
-            cur = PassCursor{ta.u0, ws[ta.u0 * kTileWaves], ws[ta.u0 * kTileWaves + 1]};
-            cur.settle(ws, ta.u1);
-            q0 = cur.next<T>(ta, ws, lane);
-            q1 = cur.next<T>(ta, ws, lane);
-            q2 = cur.next<T>(ta, ws, lane);
-            q3 = cur.next<T>(ta, ws, lane);
-        }
+        // The wave's passes of units [u0, u1) are [i, iend) of the pass
+        // list.  Lane l holds the descriptor of pass dbase + l (reloaded
+        // every kWave passes); a refill reads its pass's descriptor with
+        // readlane, so it waits for no memory.
+        const cint32* pi = const_table(ta.pidx) + (int64_t)(b * kTileWaves + wave) * (ta.U + 1);
+        int i = pi[ta.u0];
+        const int iend = pi[ta.u1];
+        int ld = i, dbase = i;
+        auto load_desc = [&]() {
+            const int at = dbase + lane;
+            return at < iend ? *reinterpret_cast<const int2*>(ta.pdesc + 2 * (int64_t)at) : make_int2(0, 0);
+        };
+        int2 dl = load_desc();
+        auto fetch = [&]() -> PassData<T> {
+            if (ld - dbase == kWave) {
+                dbase = ld;
+                dl = load_desc();
+                // consume the reload here, so the wait for it stays on this
+                // (rare) path instead of joining every refill
+                asm volatile("" : : "v"(dl.x), "v"(dl.y));
+            }
+            const int j = ld - dbase;
+            const int32_t s0 = __builtin_amdgcn_readlane(dl.x, j);
+            const int32_t c0 = __builtin_amdgcn_readlane(dl.y, j);
+            ++ld;
+            return load_pass<T>(ta, s0, c0, lane);
+        };
+        PassData<T> q0 = fetch(), q1 = fetch(), q2 = fetch(), q3 = fetch();
         lds_barrier();
-        // consumer state: unit cu, passes of this wave left in it
-        int cu = ta.u0, rem = 0, staged = 0, carry_row = -1;
+        // consumer state: unit cu, its end in the pass list
+        int cu = ta.u0, nxt = i, staged = 0, carry_row = -1;
         T carry_val = T(0);
         const T* xl = xs;
         const T* xg = x;
+        const int dummy = ta.rmax + lane;
         auto open_unit = [&](int u) {
             staged = un[4 * u + 1];
             xl = xs + (size_t)(NBUF == 2 ? un[4 * u + 2] : 0) * ta.W;
             xg = x + (int64_t)un[4 * u] * ta.W;
-            const int64_t e0 = ws[u * kTileWaves], e1 = ws[u * kTileWaves + 1];
-            rem = static_cast<int>((e1 - e0 + kWave * kTileLane - 1) / (kWave * kTileLane));
+            nxt = pi[u + 1];
             carry_row = -1;
             carry_val = T(0);
         };
@@ -457,22 +447,9 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             lds_barrier();
             if (NBUF == 1 && u + 1 < ta.u1 && un[4 * (u + 1) + 1]) lds_barrier();   // staging waves fill the buffer
         };
-        // close finished units (and units with no passes for this wave),
-        // open the next; false when the range is done
-        auto settle = [&]() -> bool {
-            while (cu < ta.u1 && rem == 0) {
-                close_unit(cu);
-                ++cu;
-                if (cu < ta.u1) open_unit(cu);
-            }
-            return cu < ta.u1;
-        };
         auto consume_pass = [&](const PassData<T>& slot) {
-            uint32_t kk[4];
-            pass_keys(slot, kk);
-            uint32_t q[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) q[k] = kk[k] != kTileSentinel ? (kk[k] & ((1u << kTileColBits) - 1u)) : 0u;
+            const uint32_t q[4] = {slot.k.x & kTileColMask, slot.k.y & kTileColMask, slot.k.z & kTileColMask,
+                                   slot.k.w & kTileColMask};
             T xv[4];
             if (staged) {
 #pragma unroll
@@ -481,16 +458,24 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
 #pragma unroll
                 for (int k = 0; k < 4; ++k) xv[k] = lane_identity(xg[q[k]]);
             }
-            tile_pass<T>(kk, slot.v, xv, acc, lane, carry_row, carry_val);
-            --rem;
+            tile_pass<T>(slot.k, slot.v, xv, acc, dummy, lane, carry_row, carry_val);
         };
-        // one queue slot: close/open units, add the slot's pass if the
-        // wave's stream is not finished, refill the slot (loads are issued
-        // on every path, so each slot is always followed by the same number
-        // of loads, and the loop has its only exit at the top)
+        // one queue slot: close the units this wave has finished (and those
+        // it has no passes in), add the slot's pass (pass i) if the range
+        // is not done, refill the slot (loads are issued on every path, so
+        // each slot is always followed by the same number of loads, and the
+        // loop has its only exit at the top)
         auto step = [&](PassData<T>& slot) {
-            if (settle()) consume_pass(slot);
-            slot = cur.next<T>(ta, ws, lane);
+            while (cu < ta.u1 && i == nxt) {
+                close_unit(cu);
+                ++cu;
+                if (cu < ta.u1) open_unit(cu);
+            }
+            if (cu < ta.u1) {
+                consume_pass(slot);
+                ++i;
+            }
+            slot = fetch();
         };
         if (cu < ta.u1) open_unit(cu);
         while (cu < ta.u1) {

@@ -5,10 +5,12 @@ decodable from (group, unit, key); units that cover the slices in order
 (a staged unit is one slice with at least 384 entries per group, a gather
 unit a run of up to 63 sparser slices); each (group, unit) tile split into 12
 row-aligned sub-ranges (one per entry wave), each a multiple of 4 entries with padding only
-at its end; rows dealt to groups in 64-row chunks; and the product the
-layout encodes equal to the CSR product.  Cases cover empty rows, a row
+at its end; rows dealt to groups in 64-row chunks; the pass list the
+kernels read (every sub-range cut into passes of at most 256 entries,
+listed per group and wave, unit by unit); and the product the layout
+encodes equal to the CSR product.  Cases cover empty rows, a row
 denser than one wave pass, several slices (ragged last slice), and more
-row groups than compute units when a group would exceed 4096 rows."""
+row groups than compute units when a group would exceed 4032 rows."""
 import ctypes as C
 
 import numpy as np
@@ -16,7 +18,7 @@ import pytest
 
 from sovereign_amd.device import lib
 
-SENT = 0xFFFFFFFF
+SENT = 0xFFF00000          # row 4095, column 0
 
 
 def layout(ptr, idx, src, rows, cols, cus):
@@ -34,6 +36,19 @@ def layout(ptr, idx, src, rows, cols, cus):
     units = np.zeros(4 * U + 1, np.int32)
     assert L.sov_hip_tile_layout(*args, keys.ctypes.data, so.ctypes.data, ws.ctypes.data, units.ctypes.data) == 0
     return nb, S, W, rmax, U, waves, keys[: E[0]], so[: E[0]], ws, units
+
+
+def passes(ptr, idx, src, rows, cols, cus, nb, U, waves):
+    L = lib()
+    P = np.zeros(1, np.int64)
+    nnz = len(idx)
+    args = (rows, cols, nnz, ptr.ctypes.data, idx.ctypes.data if nnz else None, src.ctypes.data if nnz else None,
+            cus, P.ctypes.data)
+    assert L.sov_hip_tile_passes(*args, None, None) == 0
+    pidx = np.zeros(nb * waves * (U + 1), np.int32)
+    pdesc = np.zeros(2 * max(int(P[0]), 1), np.int32)
+    assert L.sov_hip_tile_passes(*args, pidx.ctypes.data, pdesc.ctypes.data) == 0
+    return pidx, pdesc[: 2 * int(P[0])].reshape(-1, 2)
 
 
 def random_csr(rows, cols, lens, rng):
@@ -78,8 +93,8 @@ def test_layout_stores_the_csr(name):
     assert WAVES == 12
     nchunks = -(-rows // 64)
     assert W == 16384 and S == -(-cols // W)
-    assert rmax <= 4096 and nb * rmax >= rows and rmax % 64 == 0
-    assert nb == max(min(cus, nchunks), -(-nchunks // 64))
+    assert rmax <= 4032 and nb * rmax >= rows and rmax % 64 == 0
+    assert nb == max(min(cus, nchunks), -(-nchunks // 63))
     # units: contiguous slice ranges; staged = one slice with >= 384 entries per group
     first = units[0:4 * U:4]
     staged = units[1:4 * U:4]
@@ -132,6 +147,20 @@ def test_layout_stores_the_csr(name):
             for v in range(w + 1, WAVES):
                 assert not (tile_rows[w] & tile_rows[v])
     assert seen.all()
+    # the pass list: per (group, wave), unit by unit, each sub-range cut into
+    # passes of at most 256 entries
+    pidx, pdesc = passes(ptr, idx, src, rows, cols, cus, nb, U, WAVES)
+    want = []
+    for b in range(nb):
+        for w in range(WAVES):
+            base = (b * WAVES + w) * (U + 1)
+            for u in range(U):
+                assert pidx[base + u] == len(want)
+                t = b * U + u
+                e0, e1 = ws[t * WAVES + w], ws[t * WAVES + w + 1]
+                want += [(e, min(256, e1 - e)) for e in range(e0, e1, 256)]
+            assert pidx[base + U] == len(want)
+    assert pdesc.tolist() == [list(p) for p in want]
     # the product the layout encodes
     rng2 = np.random.default_rng(1)
     val = rng2.standard_normal(max(nnz, 1))
