@@ -313,8 +313,8 @@ def main():
     ctx.L.sov_hip_csr_get_segments(P.A.h, 0, C.byref(seg_a))
     ctx.L.sov_hip_csr_get_segments(P.A.h, 1, C.byref(seg_t))
     til_a, til_t = C.c_int(), C.c_int()
-    ctx.L.sov_hip_csr_get_tiles(P.A.h, 0, C.byref(til_a))
-    ctx.L.sov_hip_csr_get_tiles(P.A.h, 1, C.byref(til_t))
+    ctx.L.sov_hip_csr_get_tiles_prec(P.A.h, 0, prec, C.byref(til_a))
+    ctx.L.sov_hip_csr_get_tiles_prec(P.A.h, 1, prec, C.byref(til_t))
     log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}; column segments A={seg_a.value} A^T={seg_t.value}; "
         f"tiles A={til_a.value} A^T={til_t.value}")
     layout = f"tiles={til_a.value},{til_t.value} segments={seg_a.value},{seg_t.value}"

@@ -145,35 +145,42 @@ int sov_hip_csr_get_segments(const sov_hip_csr* csr, int orient, int* count);
 /* LDS-staged tiles (no reference counterpart: a layout of the same
  * products).  The rows of an orientation are dealt in 64-row chunks to
  * row groups (one workgroup each, about one per CU); the gathered vector is
- * cut into slices of 16384 elements.  Slices dense enough in entries per
- * group are copied into LDS by each workgroup with 16-byte loads and their
+ * cut into 64 KiB slices (16384 FP32 or 8192 FP64 elements).  Slices dense
+ * enough in entries per group are copied into LDS by each workgroup with
+ * 16-byte loads, one slice while the previous one is consumed, and their
  * entries gather from LDS; runs of sparser slices gather from memory, as
- * the CSR kernels do.  parts >= 1 builds
- * the layout (FP64 and, after make_f32, FP32 values) and runs the
+ * the CSR kernels do.  Each precision has its own layout.  parts >= 1
+ * builds the FP64 layout (and, after make_f32, the FP32 one) and runs the
  * orientation's SpMV and PDHG steps on it, in `parts` launches over ranges
- * of slices joined by per-row partial sums; parts = 0 frees it and returns
- * to the CSR parts; set_tiles applies to both precisions (the autotuner
- * chooses per precision).  FP64 launches need 160 KiB of LDS per workgroup
- * and otherwise use the CSR parts.  The summation order is fixed (results
- * are reproducible run to run) but differs from the CSR kernels'.
- * get_tiles reports the FP32 launch count (0: FP32 products on the CSR). */
+ * of slices joined by per-row partial sums; parts = 0 frees both and
+ * returns to the CSR parts (the autotuner chooses per precision).  Launches
+ * need 160 KiB of LDS per workgroup and otherwise use the CSR parts.  The
+ * summation order is fixed (results are reproducible run to run) but
+ * differs from the CSR kernels'.  get_tiles reports the FP32 launch count
+ * (0: FP32 products on the CSR). */
 int sov_hip_csr_set_tiles(sov_hip_ctx* ctx, sov_hip_csr* csr, int orient, int parts);
 int sov_hip_csr_get_tiles(const sov_hip_csr* csr, int orient, int* parts);
+/* The launches per product of the tiles of precision prec (0: that
+ * precision's products run on the CSR parts).  sov_hip_csr_get_tiles gives
+ * the FP32 value.  No reference counterpart. */
+int sov_hip_csr_get_tiles_prec(const sov_hip_csr* csr, int orient, int prec, int* parts);
 /* *usable = 1 when the orientation's products of precision prec run on the
  * tiles (tiles set, value image present, LDS of the device large enough). */
 int sov_hip_csr_tiles_usable(const sov_hip_ctx* ctx, const sov_hip_csr* csr, int orient, int prec, int* usable);
 /* The tiled layout of a host CSR (row pointers, column indices in
  * [0, cols), and a source position per entry), as sov_hip_csr_set_tiles
- * builds it for a device with `cus` compute units.  shape = {groups,
- * slices, slice width, local rows per group, units, sub-ranges per tile};
- * *entries = stored entries including padding.  keys / src_out (entries
- * each), wstart (groups * units * sub-ranges + 1) and units (4 * units + 1: first slice, staged,
- * buffer, next staged unit; then the slice count) may be null (first call
- * for the sizes).  Host only: no device is touched. */
+ * builds it for a device with `cus` compute units and the products of
+ * element size tsize (4: FP32, slices of 16384 columns; 8: FP64, slices of
+ * 8192).  shape = {groups, slices, slice width, local rows per group,
+ * units, sub-ranges per tile}; *entries = stored entries including
+ * padding.  keys / src_out (entries each), wstart (groups * units *
+ * sub-ranges + 1) and units (4 * units + 1: first slice, staged, buffer,
+ * next staged unit; then the slice count) may be null (first call for the
+ * sizes).  Host only: no device is touched. */
 int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
-                        const int32_t* col_idx, const int32_t* src, int32_t cus, int32_t shape[6],
-                        int64_t* entries, uint32_t* keys, int32_t* src_out, int32_t* wstart,
-                        int32_t* units);
+                        const int32_t* col_idx, const int32_t* src, int32_t cus, int32_t tsize,
+                        int32_t shape[6], int64_t* entries, uint32_t* keys, int32_t* src_out,
+                        int32_t* wstart, int32_t* units);
 /* The pass list the tiled kernels read for the same layout: *passes = pass
  * count; pidx (groups * sub-ranges per tile * (units + 1)) = index of the
  * first pass of (group, wave) in each unit, then one past its last; pdesc
@@ -182,8 +189,8 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
  * the size).  Host only.  No reference counterpart (layout of the tiled
  * products that replace Backend::spmv / spmv_transpose). */
 int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
-                        const int32_t* col_idx, const int32_t* src, int32_t cus, int64_t* passes,
-                        int32_t* pidx, int32_t* pdesc);
+                        const int32_t* col_idx, const int32_t* src, int32_t cus, int32_t tsize,
+                        int64_t* passes, int32_t* pidx, int32_t* pdesc);
 /* Autotuner: for each orientation, try 1, 2, 4 and 8 column segments (slices
  * of at least 64k elements); for each, time every G in {1,2,4,8,16,32,64} per
  * length class of every segment on the SpMV of precision prec; keep the

@@ -530,7 +530,8 @@ static int launch_spmv_part(sov_hip_ctx* ctx, const Part& p, const V* val, int m
 
 static int spmv_orient(sov_hip_ctx* ctx, const Orient& o, int prec, double alpha, const void* x, double beta,
                        void* y) {
-    if (tiles_usable(ctx, o.tiles, prec == SOV_PREC_F64, x)) return spmv_tiles(ctx, o, prec, alpha, x, beta, y);
+    if (tiles_usable(ctx, o.tiles_of(prec == SOV_PREC_F64), prec == SOV_PREC_F64, x))
+        return spmv_tiles(ctx, o, prec, alpha, x, beta, y);
     const int np = o.nparts();
     for (int k = 0; k < np; ++k) {
         const Part& p = o.part(k);
@@ -842,8 +843,10 @@ void sov_hip_csr_destroy(sov_hip_csr* h) {
     if (h == nullptr) return;
     DeviceGuard2 g(h->device);
     (void)hipDeviceSynchronize();
-    h->a.tiles.free();
-    h->t.tiles.free();
+    for (Orient* o : {&h->a, &h->t}) {
+        o->tiles.free();
+        o->tiles64.free();
+    }
     h->a.free_parts();
     h->t.free_parts();
     h->a.free_order();
@@ -881,9 +884,6 @@ int sov_hip_csr_make_f32(sov_hip_ctx* ctx, sov_hip_csr* h) {
             off += p.nnz;
         }
     }
-    for (Orient* o : {&h->a, &h->t})
-        if (o->tiles.built() && o->tiles.val32 == nullptr)
-            SOV_HIP_TRY(hipMalloc(&o->tiles.val32, sizeof(float) * (o->tiles.E + kTileLane)));
     h->f32 = true;
     return refresh_values(ctx, h);
 }
@@ -1023,7 +1023,7 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
     rc = fill(prec);
     for (int orient = 0; orient < 2 && rc == SOV_OK; ++orient) {
         Orient& o = orient == 0 ? h->a : h->t;
-        o.tiles.parts = o.tiles.parts64 = 0;   // the CSR timings below must not run on the tiles
+        o.tiles.parts = o.tiles64.parts64 = 0;   // the CSR timings below must not run on the tiles
         // segment counts worth trying: keep each gathered slice >= 64k elements
         const int64_t cols = o.nat.cols;
         int best_K = 1;
@@ -1051,21 +1051,28 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
         }
         // the LDS-staged tiles, for products large enough to amortise their
         // per-group staging: timed in both precisions (FP32 iterates end in
-        // an FP64 polish), each kept only where faster than the CSR parts
+        // an FP64 polish), each on its own layout, each kept where its SpMV
+        // is within kTileTuneMargin of the CSR parts' (the fused steps add a
+        // per-row epilogue, which the tiles run on coalesced 64-row chunks
+        // from LDS and the CSR parts per row: on the 1M-column Zipf LP in
+        // FP64 the A^T step takes 145 us on the tiles and 194 us on the CSR
+        // parts, yet an SpMV-only comparison kept the CSR parts)
         if (rc == SOV_OK && o.nat.nnz >= kTileTuneMinNnz && o.nat.rows >= 64) {
-            if (!o.tiles.built() || (h->f32 && o.tiles.val32 == nullptr)) rc = build_tiles(ctx, h, o, h->f32);
             int choice[2] = {0, 0};   // by SOV_PREC_*
             const int other = prec == SOV_PREC_F64 ? static_cast<int>(SOV_PREC_F32) : static_cast<int>(SOV_PREC_F64);
             for (int q : {prec, other}) {
                 if (rc != SOV_OK || (q == SOV_PREC_F32 && !h->f32)) continue;
-                if (q != prec) rc = fill(q);
+                const bool f64 = q == SOV_PREC_F64;
+                Tiles& t = o.tiles_of(f64);
+                if (!t.built()) rc = build_tiles(ctx, h, o, f64);
+                if (rc == SOV_OK && q != prec) rc = fill(q);
                 if (rc != SOV_OK) break;
-                o.tiles.parts = o.tiles.parts64 = 0;
-                float bms = q == prec ? best_ms : time_spmv(ctx, o, q, xv, yv, 4);
+                (f64 ? t.parts64 : t.parts) = 0;
+                float bms = (q == prec ? best_ms : time_spmv(ctx, o, q, xv, yv, 4)) * kTileTuneMargin;
                 for (int P : {1, 2, 4}) {
-                    if (P > std::max(o.tiles.S, 1)) break;
-                    (q == SOV_PREC_F64 ? o.tiles.parts64 : o.tiles.parts) = P;
-                    if (!tiles_usable(ctx, o.tiles, q == SOV_PREC_F64, xv)) break;
+                    if (P > std::max(t.S, 1)) break;
+                    (f64 ? t.parts64 : t.parts) = P;
+                    if (!tiles_usable(ctx, t, f64, xv)) break;
                     const float ms = time_spmv(ctx, o, q, xv, yv, 4);
                     if (ms < bms) {
                         bms = ms;
@@ -1075,8 +1082,9 @@ int sov_hip_csr_tune(sov_hip_ctx* ctx, sov_hip_csr* h, int prec) {
             }
             if (rc == SOV_OK) {
                 o.tiles.parts = choice[SOV_PREC_F32];
-                o.tiles.parts64 = choice[SOV_PREC_F64];
-                if (o.tiles.parts == 0 && o.tiles.parts64 == 0) o.tiles.free();
+                o.tiles64.parts64 = choice[SOV_PREC_F64];
+                if (o.tiles.parts == 0) o.tiles.free();
+                if (o.tiles64.parts64 == 0) o.tiles64.free();
             }
             if (rc == SOV_OK) rc = fill(prec);   // the next orientation times precision prec again
         }

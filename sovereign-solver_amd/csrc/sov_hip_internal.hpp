@@ -93,8 +93,10 @@ struct Natural {
 // mixes all length classes and the per-row work is coalesced per chunk.
 // One workgroup runs one group and keeps its row sums in LDS.
 //
-// The gathered dimension is cut into slices of W columns, and consecutive
-// slices form units.  A staged unit is one slice whose entries are dense
+// The gathered dimension is cut into slices of W = kTileSliceBytes / (element
+// size) columns, and consecutive slices form units: a layout serves the
+// products of one precision (an orientation holds one per precision), so a
+// slice fills one 64 KB LDS buffer in both and every launch double-buffers.  A staged unit is one slice whose entries are dense
 // enough per group (on average at least kTileStageMin per group) to repay
 // copying the vector slice into LDS with 16-byte loads; its entries then
 // gather from LDS.  A gather unit is a run of up to kTileGatherSlices
@@ -121,7 +123,7 @@ constexpr int kTileThreads = 1024;                    // workgroup of a group
 constexpr int kTileStagers = 4;                       // waves that copy vector slices into LDS
 constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 12 waves that add the entries
 constexpr int kTileLane = 4;                          // entries per lane per pass
-constexpr int kTileWidth = 16384;                     // columns per slice
+constexpr int kTileSliceBytes = 65536;                // bytes of a slice: 16384 FP32 or 8192 FP64 columns
 constexpr int kTileRowCap = 4032;                     // local rows per group (12 key bits; FP64 LDS)
 constexpr int kTileColBits = 20;                      // key bits of the column within a unit
 constexpr uint32_t kTileColMask = (1u << kTileColBits) - 1u;
@@ -130,7 +132,7 @@ constexpr int kTileStageMin = 384;                    // entries per group that 
 constexpr uint32_t kTileSentinel = 0xFFF00000u;       // row 4095, column 0: never a real key (rows < kTileRowCap)
 
 struct Tiles {
-    int32_t nb = 0, S = 0, W = kTileWidth, rmax = 0;  // groups, slices, slice width, local rows per group
+    int32_t nb = 0, S = 0, W = 0, rmax = 0;           // groups, slices, slice width, local rows per group
     int32_t U = 0;                                    // units
     int64_t E = 0;                                    // stored entries (with padding)
     uint32_t* keys = nullptr;                         // E
@@ -140,10 +142,10 @@ struct Tiles {
     int64_t passes = 0;
     int32_t* units = nullptr;                         // 4 U + 1
     std::vector<int32_t> h_units;                     // host copy of units
-    float* val32 = nullptr;                           // E (optional)
-    double* val64 = nullptr;                          // E
-    int parts = 0;                                    // FP32 launches per product; 0 = FP32 products on the CSR parts
-    int parts64 = 0;                                  // the same for FP64 products
+    float* val32 = nullptr;                           // E + kTileLane: the FP32 layout's values
+    double* val64 = nullptr;                          // E + kTileLane: the FP64 layout's values
+    int parts = 0;                                    // FP32 layout: launches per product (0 = on the CSR parts)
+    int parts64 = 0;                                  // FP64 layout: the same
     bool built() const { return pidx != nullptr; }
     void free();
 };
@@ -153,7 +155,8 @@ struct Tiles {
 // column segment when the gathered vector is larger than a comfortable share
 // of an XCD's L2).
 struct Orient {
-    Tiles tiles;
+    Tiles tiles;                    // FP32 products (W = 16384)
+    Tiles tiles64;                  // FP64 products (W = 8192)
     Natural nat;
     int32_t* perm = nullptr;        // rows: natural row at each device position
     int32_t* inv = nullptr;         // rows: device position of each natural row
@@ -173,6 +176,8 @@ struct Orient {
     double* acc64 = nullptr;        // rows: running partial sums across segments
     float* acc32 = nullptr;         // rows
     double* chk_acc = nullptr;      // 2 * rows: the checks' two running dots
+    Tiles& tiles_of(bool f64) { return f64 ? tiles64 : tiles; }
+    const Tiles& tiles_of(bool f64) const { return f64 ? tiles64 : tiles; }
     int nparts() const { return static_cast<int>(parts.size()); }
     const Part& part(int k) const { return parts[static_cast<size_t>(k)]; }
     Part& part(int k) { return parts[static_cast<size_t>(k)]; }

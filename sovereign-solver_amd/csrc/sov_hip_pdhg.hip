@@ -906,7 +906,7 @@ T* orient_acc(const Orient& o) {
 
 template <typename T, class Op>
 int launch_tile_step(sov_hip_ctx* ctx, const Orient& o, Op op, const T* x) {
-    const Tiles& t = o.tiles;
+    const Tiles& t = o.tiles_of(sizeof(T) == 8);
     SOV_HIP_TRY(tile_lds_attr(ctx, reinterpret_cast<const void*>(&k_tile_step<T, Op>), sizeof(T)));
     const size_t lds = tile_lds_bytes(t.W, t.rmax, sizeof(T));
     const int np = sizeof(T) == 8 ? t.parts64 : t.parts;
@@ -922,7 +922,7 @@ int launch_tile_step(sov_hip_ctx* ctx, const Orient& o, Op op, const T* x) {
 template <typename T>
 int primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau, int first) {
     const Orient& o = h->t;
-    if (tiles_usable(ctx, o.tiles, sizeof(T) == 8, st->y)) {
+    if (tiles_usable(ctx, o.tiles_of(sizeof(T) == 8), sizeof(T) == 8, st->y)) {
         PrimalOp<T> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
                        static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi), static_cast<T*>(st->x),
                        static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum), orient_acc<T>(o), T(tau), first,
@@ -951,7 +951,7 @@ template <typename T>
 int dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* st, double tau, double sigma,
               int first) {
     const Orient& o = h->a;
-    if (tiles_usable(ctx, o.tiles, sizeof(T) == 8, st->x_bar)) {
+    if (tiles_usable(ctx, o.tiles_of(sizeof(T) == 8), sizeof(T) == 8, st->x_bar)) {
         DualOp<T> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
                      static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
                      static_cast<T*>(st->y_sum), orient_acc<T>(o), T(tau), T(sigma), first, kOnly, Sched{}};

@@ -90,7 +90,8 @@ bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* 
 }
 
 hipError_t tile_lds_attr(const sov_hip_ctx* ctx, const void* kernel, int tsize) {
-    const int64_t want = std::min<int64_t>(ctx->lds_max, (int64_t)tile_lds_bytes(kTileWidth, kTileRowCap, tsize));
+    const int64_t want =
+        std::min<int64_t>(ctx->lds_max, (int64_t)tile_lds_bytes(kTileSliceBytes / tsize, kTileRowCap, tsize));
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(want));
 }
 
@@ -218,7 +219,7 @@ static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, std::
 // source positions src), for a device with `cus` compute units: fills the
 // shape fields and unit table of t and the host arrays.  Host only.
 static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& ptr, const std::vector<int32_t>& idx,
-                       const std::vector<int32_t>& srcv, int cus, Tiles& t, std::vector<uint32_t>& keys,
+                       const std::vector<int32_t>& srcv, int cus, int W, Tiles& t, std::vector<uint32_t>& keys,
                        std::vector<int32_t>& src, std::vector<int32_t>& wstart) {
     if (rows <= 0) return fail(SOV_INVALID_ARGUMENT, "tiles: no rows");
     const int64_t nchunks = (rows + 63) / 64;
@@ -227,7 +228,7 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     nb = std::max<int64_t>(nb, (nchunks + cap - 1) / cap);
     t.nb = static_cast<int32_t>(nb);
     t.rmax = static_cast<int32_t>((nchunks + nb - 1) / nb * 64);
-    t.W = kTileWidth;
+    t.W = W;
     t.S = static_cast<int32_t>((static_cast<int64_t>(cols) + t.W - 1) / t.W);
     // units: a slice with at least kTileStageMin entries per group on
     // average is staged; runs of the others become gather units
@@ -288,10 +289,10 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     return SOV_OK;
 }
 
-int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32) {
-    const int parts = o.tiles.parts, parts64 = o.tiles.parts64;
-    o.tiles.free();
-    Tiles& t = o.tiles;
+int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
+    Tiles& t = o.tiles_of(f64);
+    const int parts = t.parts, parts64 = t.parts64;
+    t.free();
     const int32_t rows = o.nat.rows, nnz = o.nat.nnz;
     if (rows == 0) return fail(SOV_INVALID_ARGUMENT, "tiles: no rows");
     // the device-order CSR on the host
@@ -305,7 +306,8 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32
     SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
     std::vector<uint32_t> keys;
     std::vector<int32_t> src, wstart;
-    int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, t, keys, src, wstart);
+    int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, kTileSliceBytes / (f64 ? 8 : 4), t, keys, src,
+                         wstart);
     std::vector<int32_t> pidx, pdesc;
     if (rc == SOV_OK) rc = pass_tables(t, wstart, pidx, pdesc);
     if (rc != SOV_OK) {
@@ -323,8 +325,8 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32
     if (e == hipSuccess) e = hipMalloc(&t.units, sizeof(int32_t) * t.h_units.size());
     if (e == hipSuccess)
         e = hipMemcpy(t.units, t.h_units.data(), sizeof(int32_t) * t.h_units.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&t.val64, sizeof(double) * e1);
-    if (e == hipSuccess && want_f32) e = hipMalloc(&t.val32, sizeof(float) * e1);
+    if (e == hipSuccess && f64) e = hipMalloc(&t.val64, sizeof(double) * e1);
+    if (e == hipSuccess && !f64) e = hipMalloc(&t.val32, sizeof(float) * e1);
     if (e == hipSuccess) e = hipMemcpy(t.keys, keys.data(), sizeof(uint32_t) * e1, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(t.src, src.data(), sizeof(int32_t) * e1, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(t.pidx, pidx.data(), sizeof(int32_t) * pidx.size(), hipMemcpyHostToDevice);
@@ -345,11 +347,12 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32
 }
 
 int refresh_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o) {
-    const Tiles& t = o.tiles;
-    if (!t.built()) return SOV_OK;
     int rc = SOV_OK;
-    if (t.val64) rc = gather_tile_values<double>(ctx, h, t, t.val64);
-    if (rc == SOV_OK && t.val32) rc = gather_tile_values<float>(ctx, h, t, t.val32);
+    for (const Tiles* t : {&o.tiles, &o.tiles64}) {
+        if (!t->built()) continue;
+        if (rc == SOV_OK && t->val64) rc = gather_tile_values<double>(ctx, h, *t, t->val64);
+        if (rc == SOV_OK && t->val32) rc = gather_tile_values<float>(ctx, h, *t, t->val32);
+    }
     return rc;
 }
 
@@ -378,7 +381,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_spmv(TileSpmvOp<T> op, Ti
 
 template <typename T>
 static int spmv_tiles_t(sov_hip_ctx* ctx, const Orient& o, double alpha, const void* x, double beta, void* y) {
-    const Tiles& t = o.tiles;
+    const Tiles& t = o.tiles_of(sizeof(T) == 8);
     const size_t lds = tile_lds_bytes(t.W, t.rmax, sizeof(T));
     SOV_HIP_TRY(tile_lds_attr(ctx, reinterpret_cast<const void*>(&k_tile_spmv<T>), sizeof(T)));
     const int np = sizeof(T) == 8 ? t.parts64 : t.parts;
@@ -412,22 +415,29 @@ int sov_hip_csr_set_tiles(sov_hip_ctx* ctx, sov_hip_csr* h, int orient, int part
     Orient& o = orient == 0 ? h->a : h->t;
     if (parts == 0) {
         o.tiles.free();
+        o.tiles64.free();
         return SOV_OK;
     }
     if (o.nat.rows == 0) return fail(SOV_INVALID_ARGUMENT, "set_tiles: orientation has no rows");
-    if (!o.tiles.built() || (h->f32 && o.tiles.val32 == nullptr && o.tiles.E > 0)) {
-        const int rc = build_tiles(ctx, h, o, h->f32);
-        if (rc != SOV_OK) return rc;
+    // FP64 products always (FP64 runs and the FP64 polish); FP32 ones when
+    // the handle has its FP32 image
+    for (const bool f64 : {true, false}) {
+        if (!f64 && !h->f32) continue;
+        Tiles& t = o.tiles_of(f64);
+        if (!t.built()) {
+            const int rc = build_tiles(ctx, h, o, f64);
+            if (rc != SOV_OK) return rc;
+        }
+        (f64 ? t.parts64 : t.parts) = std::max(1, std::min(parts, std::max(t.S, 1)));
     }
-    o.tiles.parts = o.tiles.parts64 = std::max(1, std::min(parts, std::max(o.tiles.S, 1)));
     return SOV_OK;
 }
 
 int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* ptr, const int32_t* idx,
-                        const int32_t* src, int32_t cus, int32_t shape[6], int64_t* entries, uint32_t* keys,
-                        int32_t* src_out, int32_t* wstart, int32_t* units) {
+                        const int32_t* src, int32_t cus, int32_t tsize, int32_t shape[6], int64_t* entries,
+                        uint32_t* keys, int32_t* src_out, int32_t* wstart, int32_t* units) {
     if (rows <= 0 || cols < 0 || nnz < 0 || ptr == nullptr || shape == nullptr || entries == nullptr ||
-        (nnz > 0 && (idx == nullptr || src == nullptr)))
+        (nnz > 0 && (idx == nullptr || src == nullptr)) || (tsize != 4 && tsize != 8))
         return fail(SOV_INVALID_ARGUMENT, "tile_layout args");
     if (ptr[0] != 0 || ptr[rows] != nnz) return fail(SOV_INVALID_ARGUMENT, "tile_layout: row pointers");
     for (int32_t r = 0; r < rows; ++r)
@@ -439,7 +449,7 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     Tiles t;
     std::vector<uint32_t> k;
     std::vector<int32_t> so, ws;
-    const int rc = layout_host(rows, cols, p, ix, sv, cus, t, k, so, ws);
+    const int rc = layout_host(rows, cols, p, ix, sv, cus, kTileSliceBytes / tsize, t, k, so, ws);
     if (rc != SOV_OK) return rc;
     shape[0] = t.nb;
     shape[1] = t.S;
@@ -456,9 +466,10 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
 }
 
 int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* ptr, const int32_t* idx,
-                        const int32_t* src, int32_t cus, int64_t* passes, int32_t* pidx, int32_t* pdesc) {
+                        const int32_t* src, int32_t cus, int32_t tsize, int64_t* passes, int32_t* pidx,
+                        int32_t* pdesc) {
     if (rows <= 0 || cols < 0 || nnz < 0 || ptr == nullptr || passes == nullptr ||
-        (nnz > 0 && (idx == nullptr || src == nullptr)))
+        (nnz > 0 && (idx == nullptr || src == nullptr)) || (tsize != 4 && tsize != 8))
         return fail(SOV_INVALID_ARGUMENT, "tile_passes args");
     if (ptr[0] != 0 || ptr[rows] != nnz) return fail(SOV_INVALID_ARGUMENT, "tile_passes: row pointers");
     for (int32_t r = 0; r < rows; ++r)
@@ -470,7 +481,7 @@ int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     Tiles t;
     std::vector<uint32_t> k;
     std::vector<int32_t> so, ws, pi, pd;
-    int rc = layout_host(rows, cols, p, ix, sv, cus, t, k, so, ws);
+    int rc = layout_host(rows, cols, p, ix, sv, cus, kTileSliceBytes / tsize, t, k, so, ws);
     if (rc == SOV_OK) rc = pass_tables(t, ws, pi, pd);
     if (rc != SOV_OK) return rc;
     *passes = static_cast<int64_t>(pd.size() / 2);
@@ -485,7 +496,8 @@ int sov_hip_csr_tiles_usable(const sov_hip_ctx* ctx, const sov_hip_csr* h, int o
         return fail(SOV_INVALID_ARGUMENT, "tiles_usable args");
     const Orient& o = orient == 0 ? h->a : h->t;
     // a 256-byte aligned stand-in for a device vector (hipMalloc alignment)
-    *usable = tiles_usable(ctx, o.tiles, prec == SOV_PREC_F64, reinterpret_cast<const void*>(uintptr_t{256})) ? 1 : 0;
+    const bool f64 = prec == SOV_PREC_F64;
+    *usable = tiles_usable(ctx, o.tiles_of(f64), f64, reinterpret_cast<const void*>(uintptr_t{256})) ? 1 : 0;
     return SOV_OK;
 }
 
@@ -493,6 +505,15 @@ int sov_hip_csr_get_tiles(const sov_hip_csr* h, int orient, int* parts) {
     if (h == nullptr || parts == nullptr || (orient != 0 && orient != 1))
         return fail(SOV_INVALID_ARGUMENT, "get_tiles args");
     *parts = (orient == 0 ? h->a : h->t).tiles.parts;
+    return SOV_OK;
+}
+
+int sov_hip_csr_get_tiles_prec(const sov_hip_csr* h, int orient, int prec, int* parts) {
+    if (h == nullptr || parts == nullptr || (orient != 0 && orient != 1) ||
+        (prec != SOV_PREC_F64 && prec != SOV_PREC_F32))
+        return fail(SOV_INVALID_ARGUMENT, "get_tiles_prec args");
+    const Orient& o = orient == 0 ? h->a : h->t;
+    *parts = prec == SOV_PREC_F64 ? o.tiles64.parts64 : o.tiles.parts;
     return SOV_OK;
 }
 

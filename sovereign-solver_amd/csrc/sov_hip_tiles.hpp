@@ -46,13 +46,12 @@ struct TileArgs {
 // Device row of local row l of group b.
 __host__ __device__ __forceinline__ int tile_row(int b, int nb, int l) { return ((l >> 6) * nb + b) * 64 + (l & 63); }
 
-// Slice buffers that fit next to the row sums (2: double-buffered, 1: single).
-__host__ __device__ constexpr int tile_buffers(int tsize) { return tsize == 4 ? 2 : 1; }
-
-// LDS of a tiled launch: the slice buffers, the row sums and one slot per
-// lane that takes the writes a lane makes no real write with.
+// LDS of a tiled launch: two slice buffers (one is consumed while the
+// other is filled), the row sums and one slot per lane that takes the
+// writes a lane makes no real write with.  A slice is kTileSliceBytes
+// whatever the element type (W = kTileSliceBytes / tsize columns).
 __host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize) {
-    return (size_t)tile_buffers(tsize) * W * tsize + (size_t)(rmax + kWave) * tsize;
+    return (size_t)2 * W * tsize + (size_t)(rmax + kWave) * tsize;
 }
 
 // The four values of one lane's entries (T = float: one 16-byte load;
@@ -265,26 +264,23 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Copy vector slice s (its first min(W, cols - s W) values) into an LDS
-// buffer: run by the kTileStagers staging waves, thread st in [0, 64
-// kTileStagers), kChunkLoads 16-byte loads in flight per thread.
+// One staging thread's share of a whole slice (kTileSliceBytes), held in
+// registers between its loads and its store to LDS, so the loads of the
+// next slice are in flight while the entry waves consume the current one.
 template <typename T>
-__device__ __forceinline__ void stage_slice(const T* __restrict__ x, const TileArgs& ta, int s, T* __restrict__ buf,
-                                            int st) {
-    constexpr int NV = 16 / sizeof(T);
+struct SliceRegs {
+    static constexpr int NV = 16 / sizeof(T);
     typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
-    constexpr int kStageThreads = kTileStagers * kWave;
-    constexpr int kChunkLoads = 16;
-    constexpr int kPer = kTileWidth / (kStageThreads * NV);         // 16-byte loads per thread and slice
-    static_assert(kPer % kChunkLoads == 0, "slice must split into whole chunks");
-    const int64_t base = (int64_t)s * ta.W;
-    const int w = min(ta.W, ta.cols - (int)base);
-#pragma unroll 1
-    for (int c0 = 0; c0 < kPer; c0 += kChunkLoads) {
-        vec r[kChunkLoads];
+    static constexpr int kStageThreads = kTileStagers * kWave;
+    static constexpr int kPer = kTileSliceBytes / (kStageThreads * 16);   // 16-byte loads per thread
+    vec r[kPer];
+
+    __device__ __forceinline__ void load(const T* __restrict__ x, const TileArgs& ta, int s, int st) {
+        const int64_t base = (int64_t)s * ta.W;
+        const int w = min(ta.W, ta.cols - (int)base);
 #pragma unroll
-        for (int k = 0; k < kChunkLoads; ++k) {
-            const int j = ((c0 + k) * kStageThreads + st) * NV;
+        for (int k = 0; k < kPer; ++k) {
+            const int j = (k * kStageThreads + st) * NV;
             if (j + NV <= w) {
                 r[k] = *reinterpret_cast<const vec*>(x + base + j);
             } else {
@@ -292,39 +288,10 @@ __device__ __forceinline__ void stage_slice(const T* __restrict__ x, const TileA
                 for (int i = 0; i < NV; ++i) r[k][i] = (j + i < w) ? x[base + j + i] : T(0);
             }
         }
-#pragma unroll
-        for (int k = 0; k < kChunkLoads; ++k) {
-            const int j = ((c0 + k) * kStageThreads + st) * NV;
-            *reinterpret_cast<vec*>(buf + j) = r[k];
-        }
     }
-}
-
-// One staging thread's share of a whole FP32 slice, held in registers
-// between its loads and its store to LDS (so the loads of the next slice
-// are in flight while the entry waves consume the current one).
-struct SliceRegs {
-    static constexpr int kStageThreads = kTileStagers * kWave;
-    static constexpr int kPer = kTileWidth / (kStageThreads * 4);   // 16-byte loads per thread
-    f32x4 r[kPer];
-
-    __device__ __forceinline__ void load(const float* __restrict__ x, const TileArgs& ta, int s, int st) {
-        const int64_t base = (int64_t)s * ta.W;
-        const int w = min(ta.W, ta.cols - (int)base);
+    __device__ __forceinline__ void store(T* __restrict__ buf, int st) const {
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int j = (k * kStageThreads + st) * 4;
-            if (j + 4 <= w) {
-                r[k] = *reinterpret_cast<const f32x4*>(x + base + j);
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) r[k][i] = (j + i < w) ? x[base + j + i] : 0.f;
-            }
-        }
-    }
-    __device__ __forceinline__ void store(float* __restrict__ buf, int st) const {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) *reinterpret_cast<f32x4*>(buf + (k * kStageThreads + st) * 4) = r[k];
+        for (int k = 0; k < kPer; ++k) *reinterpret_cast<vec*>(buf + (k * kStageThreads + st) * NV) = r[k];
     }
 };
 
@@ -340,18 +307,16 @@ struct SliceRegs {
 // counter completes in that order) and no slot is copied while its load is
 // pending.  The last kTileStagers waves copy vector slices into LDS: while
 // the entry waves consume staged unit u from one buffer they fill the other
-// with the next staged slice (FP32; FP64 has one buffer, filled between
-// units while the entry waves wait).  Staging runs in waves of its own so
-// its loads never sit in an entry wave's queue.  Every unit ends in a
+// with the next staged slice.  Staging runs in waves of its own so its
+// loads never sit in an entry wave's queue.  Every unit ends in a
 // barrier of all waves: the next unit's sub-ranges give rows to other waves.
 constexpr int kRing = 4;
 
 template <typename T, class Op>
 __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restrict__ x, Op& op) {
-    constexpr int NBUF = tile_buffers(sizeof(T));
     extern __shared__ __align__(16) unsigned char tile_lds[];
     T* xs = reinterpret_cast<T*>(tile_lds);
-    T* acc = xs + (size_t)NBUF * ta.W;
+    T* acc = xs + (size_t)2 * ta.W;
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1);
     // wave-uniform, so the unit table and sub-range bounds are scalar loads
     // (their own counter: they never wait for the vector loads in flight)
@@ -360,42 +325,28 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
     for (int l = tid; l < ta.rmax; l += kTileThreads) acc[l] = T(0);
     if (wave >= kTileWaves) {
         const int st = tid - kTileWaves * kWave;
-        if constexpr (NBUF == 2) {
-            // ---- staging waves, FP32: during staged unit u they store the
-            // slice of the next staged unit (loaded into registers one
-            // staged unit earlier) into the free buffer, then load the one
-            // after
-            SliceRegs R;
-            int nxt = ta.u1;   // staged unit whose slice R holds
-            if (ta.u0 < ta.u1) {
-                const int fs = un[4 * ta.u0 + 1] ? ta.u0 : un[4 * ta.u0 + 3];
-                if (fs < ta.u1) {
-                    R.load(x, ta, un[4 * fs], st);
-                    R.store(xs + (size_t)un[4 * fs + 2] * ta.W, st);
-                    nxt = un[4 * fs + 3];
-                    if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
-                }
+        // ---- staging waves: during staged unit u they store the slice of
+        // the next staged unit (loaded into registers one staged unit
+        // earlier) into the free buffer, then load the one after
+        SliceRegs<T> R;
+        int nxt = ta.u1;   // staged unit whose slice R holds
+        if (ta.u0 < ta.u1) {
+            const int fs = un[4 * ta.u0 + 1] ? ta.u0 : un[4 * ta.u0 + 3];
+            if (fs < ta.u1) {
+                R.load(x, ta, un[4 * fs], st);
+                R.store(xs + (size_t)un[4 * fs + 2] * ta.W, st);
+                nxt = un[4 * fs + 3];
+                if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
+            }
+        }
+        lds_barrier();
+        for (int u = ta.u0; u < ta.u1; ++u) {
+            if (un[4 * u + 1] && nxt < ta.u1) {
+                R.store(xs + (size_t)un[4 * nxt + 2] * ta.W, st);
+                nxt = un[4 * nxt + 3];
+                if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
             }
             lds_barrier();
-            for (int u = ta.u0; u < ta.u1; ++u) {
-                if (un[4 * u + 1] && nxt < ta.u1) {
-                    R.store(xs + (size_t)un[4 * nxt + 2] * ta.W, st);
-                    nxt = un[4 * nxt + 3];
-                    if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
-                }
-                lds_barrier();
-            }
-        } else {
-            // ---- staging waves, FP64 (one buffer): between units
-            if (ta.u0 < ta.u1 && un[4 * ta.u0 + 1]) stage_slice<T>(x, ta, un[4 * ta.u0], xs, st);
-            lds_barrier();
-            for (int u = ta.u0; u < ta.u1; ++u) {
-                lds_barrier();
-                if (u + 1 < ta.u1 && un[4 * (u + 1) + 1]) {
-                    stage_slice<T>(x, ta, un[4 * (u + 1)], xs, st);
-                    lds_barrier();
-                }
-            }
         }
     } else {
         // ---- entry waves
@@ -436,7 +387,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         const int dummy = ta.rmax + lane;
         auto open_unit = [&](int u) {
             staged = un[4 * u + 1];
-            xl = xs + (size_t)(NBUF == 2 ? un[4 * u + 2] : 0) * ta.W;
+            xl = xs + (size_t)un[4 * u + 2] * ta.W;
             xg = x + (int64_t)un[4 * u] * ta.W;
             nxt = pi[u + 1];
             carry_row = -1;
@@ -445,7 +396,6 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         auto close_unit = [&](int u) {
             if (carry_row >= 0 && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
             lds_barrier();
-            if (NBUF == 1 && u + 1 < ta.u1 && un[4 * (u + 1) + 1]) lds_barrier();   // staging waves fill the buffer
         };
         auto consume_pass = [&](const PassData<T>& slot) {
             const uint32_t q[4] = {slot.k.x & kTileColMask, slot.k.y & kTileColMask, slot.k.z & kTileColMask,
@@ -495,15 +445,19 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
 // Products with fewer entries than this are not offered the tiles by the
 // autotuner (their vectors stay in L2 and the CSR kernels win).
 constexpr int64_t kTileTuneMinNnz = 1 << 20;
+// The autotuner keeps the tiles when their SpMV takes at most this factor
+// of the CSR parts' time.
+constexpr float kTileTuneMargin = 1.1f;
 
 // Host: the launch arguments of slice range k of `parts` for one precision.
 TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, int parts);
 
-// Host: build the tiled layout of an orientation from its device-order CSR
-// (values from A's natural FP64 values; FP32 image when want_f32).
-int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool want_f32);
+// Host: build the tiled layout of an orientation's products of one
+// precision (o.tiles64 when f64, else o.tiles) from its device-order CSR,
+// with that precision's image of A's natural FP64 values.
+int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64);
 
-// Host: re-derive the tiles' value images from A's natural FP64 values.
+// Host: re-derive both layouts' value images from A's natural FP64 values.
 int refresh_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o);
 
 // Host: whether a tiled launch of precision f64 is possible on this device

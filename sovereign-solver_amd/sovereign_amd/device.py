@@ -49,8 +49,10 @@ def lib():
     L.sov_hip_csr_set_tiles.argtypes = [_v, _v, C.c_int, C.c_int]
     L.sov_hip_csr_get_tiles.argtypes = [_v, C.c_int, C.POINTER(C.c_int)]
     L.sov_hip_csr_tiles_usable.argtypes = [_v, _v, C.c_int, C.c_int, C.POINTER(C.c_int)]
-    L.sov_hip_tile_layout.argtypes = [C.c_int32, C.c_int32, C.c_int32, _v, _v, _v, C.c_int32, _v, _v, _v, _v, _v, _v]
-    L.sov_hip_tile_passes.argtypes = [C.c_int32, C.c_int32, C.c_int32, _v, _v, _v, C.c_int32, _v, _v, _v]
+    L.sov_hip_csr_get_tiles_prec.argtypes = [_v, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.sov_hip_tile_layout.argtypes = [C.c_int32, C.c_int32, C.c_int32, _v, _v, _v, C.c_int32, C.c_int32, _v, _v, _v, _v,
+                                      _v, _v]
+    L.sov_hip_tile_passes.argtypes = [C.c_int32, C.c_int32, C.c_int32, _v, _v, _v, C.c_int32, C.c_int32, _v, _v, _v]
     L.sov_hip_spmv.argtypes = [_v, _v, C.c_int, C.c_int, d, _v, d, _v]
     L.sov_hip_spmv_device.argtypes = [_v, _v, C.c_int, C.c_int, d, _v, d, _v]
     L.sov_hip_csr_order.argtypes = [_v, _v, C.c_int, _v]

@@ -64,7 +64,7 @@ def set_tiles(ctx, A, parts, prec):
     for orient in (0, 1):
         check(ctx.L.sov_hip_csr_set_tiles(ctx.h, A.h, orient, parts))
         got, ok = C.c_int(), C.c_int()
-        check(ctx.L.sov_hip_csr_get_tiles(A.h, orient, C.byref(got)))
+        check(ctx.L.sov_hip_csr_get_tiles_prec(A.h, orient, prec, C.byref(got)))
         check(ctx.L.sov_hip_csr_tiles_usable(ctx.h, A.h, orient, prec, C.byref(ok)))
         assert got.value >= 1 and ok.value == 1, (orient, got.value, ok.value)
 

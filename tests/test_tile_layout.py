@@ -21,13 +21,13 @@ from sovereign_amd.device import lib
 SENT = 0xFFF00000          # row 4095, column 0
 
 
-def layout(ptr, idx, src, rows, cols, cus):
+def layout(ptr, idx, src, rows, cols, cus, tsize):
     L = lib()
     shape = np.zeros(6, np.int32)
     E = np.zeros(1, np.int64)
     nnz = len(idx)
     args = (rows, cols, nnz, ptr.ctypes.data, idx.ctypes.data if nnz else None, src.ctypes.data if nnz else None,
-            cus, shape.ctypes.data, E.ctypes.data)
+            cus, tsize, shape.ctypes.data, E.ctypes.data)
     assert L.sov_hip_tile_layout(*args, None, None, None, None) == 0
     nb, S, W, rmax, U, waves = (int(v) for v in shape)
     keys = np.zeros(max(int(E[0]), 1), np.uint32)
@@ -38,12 +38,12 @@ def layout(ptr, idx, src, rows, cols, cus):
     return nb, S, W, rmax, U, waves, keys[: E[0]], so[: E[0]], ws, units
 
 
-def passes(ptr, idx, src, rows, cols, cus, nb, U, waves):
+def passes(ptr, idx, src, rows, cols, cus, tsize, nb, U, waves):
     L = lib()
     P = np.zeros(1, np.int64)
     nnz = len(idx)
     args = (rows, cols, nnz, ptr.ctypes.data, idx.ctypes.data if nnz else None, src.ctypes.data if nnz else None,
-            cus, P.ctypes.data)
+            cus, tsize, P.ctypes.data)
     assert L.sov_hip_tile_passes(*args, None, None) == 0
     pidx = np.zeros(nb * waves * (U + 1), np.int32)
     pdesc = np.zeros(2 * max(int(P[0]), 1), np.int32)
@@ -72,8 +72,9 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("tsize", [4, 8])
 @pytest.mark.parametrize("name", list(CASES))
-def test_layout_stores_the_csr(name):
+def test_layout_stores_the_csr(name, tsize):
     rows, cols, lens, cus = CASES[name]
     rng = np.random.default_rng(len(name))
     if lens == "mixed":
@@ -89,10 +90,10 @@ def test_layout_stores_the_csr(name):
         ptr, idx = random_csr(rows, cols, lens, rng)
     nnz = int(ptr[-1])
     src = rng.permutation(nnz).astype(np.int32)      # arbitrary source positions
-    nb, S, W, rmax, U, WAVES, keys, so, ws, units = layout(ptr, idx, src, rows, cols, cus)
+    nb, S, W, rmax, U, WAVES, keys, so, ws, units = layout(ptr, idx, src, rows, cols, cus, tsize)
     assert WAVES == 12
     nchunks = -(-rows // 64)
-    assert W == 16384 and S == -(-cols // W)
+    assert W == 65536 // tsize and S == -(-cols // W)       # a slice is 64 KiB in either precision
     assert rmax <= 4032 and nb * rmax >= rows and rmax % 64 == 0
     assert nb == max(min(cus, nchunks), -(-nchunks // 63))
     # units: contiguous slice ranges; staged = one slice with >= 384 entries per group
@@ -113,7 +114,7 @@ def test_layout_stores_the_csr(name):
     for u in range(U):
         later = sidx[sidx > u]
         assert units[4 * u + 3] == (later[0] if len(later) else U)
-    if name == "staged_and_gather":
+    if name == "staged_and_gather" and tsize == 4:
         assert list(staged) == [1, 1, 0] and list(first) == [0, 1, 2]
     assert len(ws) == nb * U * WAVES + 1 and ws[-1] == len(keys)
     assert np.all(np.diff(ws) >= 0) and np.all(ws % 4 == 0)
@@ -149,7 +150,7 @@ def test_layout_stores_the_csr(name):
     assert seen.all()
     # the pass list: per (group, wave), unit by unit, each sub-range cut into
     # passes of at most 256 entries
-    pidx, pdesc = passes(ptr, idx, src, rows, cols, cus, nb, U, WAVES)
+    pidx, pdesc = passes(ptr, idx, src, rows, cols, cus, tsize, nb, U, WAVES)
     want = []
     for b in range(nb):
         for w in range(WAVES):
@@ -184,10 +185,14 @@ def test_layout_rejects_bad_input():
     ptr = np.array([0, 2], np.int32)
     idx = np.array([0, 7], np.int32)      # column 7 out of range for cols = 5
     src = np.array([0, 1], np.int32)
-    assert L.sov_hip_tile_layout(1, 5, 2, ptr.ctypes.data, idx.ctypes.data, src.ctypes.data, 4, shape.ctypes.data,
+    assert L.sov_hip_tile_layout(1, 5, 2, ptr.ctypes.data, idx.ctypes.data, src.ctypes.data, 4, 4, shape.ctypes.data,
                                  E.ctypes.data, None, None, None, None) != 0
     bad_ptr = np.array([0, 3], np.int32)  # row pointers disagree with nnz
-    assert L.sov_hip_tile_layout(1, 5, 2, bad_ptr.ctypes.data, idx.ctypes.data, src.ctypes.data, 4,
+    assert L.sov_hip_tile_layout(1, 5, 2, bad_ptr.ctypes.data, idx.ctypes.data, src.ctypes.data, 4, 4,
                                  shape.ctypes.data, E.ctypes.data, None, None, None, None) != 0
-    assert L.sov_hip_tile_layout(0, 5, 0, ptr.ctypes.data, None, None, 4, shape.ctypes.data, E.ctypes.data, None,
+    assert L.sov_hip_tile_layout(0, 5, 0, ptr.ctypes.data, None, None, 4, 4, shape.ctypes.data, E.ctypes.data, None,
                                  None, None, None) != 0
+    good = np.array([0, 2], np.int32)
+    idx_ok = np.array([0, 3], np.int32)
+    assert L.sov_hip_tile_layout(1, 5, 2, good.ctypes.data, idx_ok.ctypes.data, src.ctypes.data, 4, 6,
+                                 shape.ctypes.data, E.ctypes.data, None, None, None, None) != 0   # element size
