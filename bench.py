@@ -13,6 +13,10 @@ Prints one JSON line.  value = algorithmic HBM bytes of the SpMV kernels per
 iteration x iterations/s (GB/s); the roofline object gives the dominant
 kernel's achieved bandwidth measured with HIP events on the engine stream;
 cpu_baseline times the CPU oracle (single thread) on a bounded sample.
+config also carries the seconds to 1e-6 relative KKT of three solves (the
+2,000 x 5,000 LP, the 2,000 x 2,000 transportation LP and the 1M-column Zipf
+LP), the batched workload's LPs/s and the learned warm start's cold vs warm
+time to 1e-6.
 """
 import argparse
 import ctypes as C
@@ -225,6 +229,30 @@ def batched(S, count=256, time_limit=150.0):
             "sum_m": int(sum(sizes)), "sum_nnz": int(sum(lp.dims()[2] for lp in lps))}
 
 
+def time_to_tolerance(S):
+    """Seconds to 1e-6 relative KKT through the engine's own entry point
+    (Pdhg::solve, fused HIP engine, adaptive restarts, relative-KKT stop) on
+    three workloads: the 2,000 x 5,000 LP (5 nonzeros per column, seed 0) in
+    FP64; the 2,000 x 2,000 transportation LP (4M columns, seed 0) and the
+    1M-column Zipf LP (seed 0), each with FP32 iterates and the FP64 polish.
+    `seconds` is the whole solve (setup included: scaling, layouts,
+    autotuning, power iteration); `loop_seconds` the iterations alone."""
+    cases = [("lp_2000x5000_f64", lambda: S.LP.random(2000, 5000, 5, 0), "f64"),
+             ("transport_2000x2000_f32", lambda: S.LP.transportation(2000, 2000, 0), "f32"),
+             ("zipf_500000x1000000_f32", lambda: S.LP.random_zipf(500000, 1000000, 0), "f32")]
+    out = {}
+    for name, make, prec in cases:
+        lp = make()
+        r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
+                    restarts="adaptive", precision=prec, max_iterations=2000000, time_limit=240)
+        out[name] = {"status": r.status, "seconds": round(r.seconds, 3), "loop_seconds": round(r.loop_seconds, 3),
+                     "iterations": r.iterations, "fp64_polish_iterations": r.fp64_polish_iterations,
+                     "relative_kkt": max(r.relative_kkt), "objective": r.objective}
+        log(f"time to 1e-6: {name}: {r.status} in {r.seconds:.2f} s ({r.iterations} iterations, "
+            f"{r.fp64_polish_iterations} FP64 polish)")
+    return out
+
+
 MODEL = os.path.join(ROOT, "sovereign-solver_amd", "sovereign_amd", "models", "transport_100x200.npz")
 
 
@@ -277,6 +305,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-LP batched solve")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
+    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-1e-6 solves")
     ap.add_argument("--segments", default=None,
                     help="A,T column-segment counts to use instead of the autotuner's (profiling runs)")
     ap.add_argument("--tiles", default=None,
@@ -369,6 +398,10 @@ def main():
                          f"algorithmic bytes per iteration as the GPU line"}
         log(f"cpu oracle: {per_it * 1e3:.1f} ms/iteration")
 
+    ttt = None
+    if not args.no_solve:
+        ttt = time_to_tolerance(S)
+
     bat = None
     if not args.no_batched:
         bat = batched(S)
@@ -404,6 +437,7 @@ def main():
                    "tiles": {"A": til_a.value, "A^T": til_t.value},
                    "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
                    "kkt_scaled_at_last_check": chk.kkt_cur if chk else None,
+                   "time_to_1e-6_relative_kkt": ttt,
                    "batched": bat,
                    "learned_warm_start": lws},
         "roofline": {"bound": "hbm", "kernel": f"{dom_name}: {kern[dom_name]}", "achieved": round(achieved, 1),
