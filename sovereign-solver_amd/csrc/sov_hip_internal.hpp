@@ -120,8 +120,8 @@ struct Natural {
 // last), and pdesc[2 i], pdesc[2 i + 1] = first entry and entry count of
 // pass i.
 constexpr int kTileThreads = 1024;                    // workgroup of a group
-constexpr int kTileStagers = 4;                       // waves that copy vector slices into LDS
-constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 12 waves that add the entries
+constexpr int kTileStagers = 8;                       // waves that copy vector slices into LDS
+constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 8 waves that add the entries
 constexpr int kTileLane = 4;                          // entries per lane per pass
 constexpr int kTileSliceBytes = 65536;                // bytes of a slice: 16384 FP32 or 8192 FP64 columns
 constexpr int kTileRowCap = 4032;                     // local rows per group (12 key bits; FP64 LDS)
@@ -134,6 +134,7 @@ constexpr uint32_t kTileSentinel = 0xFFF00000u;       // row 4095, column 0: nev
 struct Tiles {
     int32_t nb = 0, S = 0, W = 0, rmax = 0;           // groups, slices, slice width, local rows per group
     int32_t U = 0;                                    // units
+    int32_t cols = 0;                                 // length of the gathered vector
     int64_t E = 0;                                    // stored entries (with padding)
     uint32_t* keys = nullptr;                         // E
     int32_t* src = nullptr;                           // E: A natural position, -1 for padding

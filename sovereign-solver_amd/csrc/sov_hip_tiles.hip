@@ -50,7 +50,7 @@ void Tiles::free() {
     val32 = nullptr;
     val64 = nullptr;
     h_units.clear();
-    nb = S = rmax = U = 0;
+    nb = S = rmax = U = cols = 0;
     E = 0;
     parts = 0;
     parts64 = 0;
@@ -79,6 +79,8 @@ TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, 
     while (u1 < t.U && t.h_units[4 * static_cast<size_t>(u1)] < hi) ++u1;
     a.u0 = u0;
     a.u1 = u1;
+    a.staged = 0;
+    for (int u = u0; u < u1; ++u) a.staged += t.h_units[4 * static_cast<size_t>(u) + 1] != 0;
     return a;
 }
 
@@ -86,6 +88,10 @@ bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* 
     if ((f64 ? t.parts64 : t.parts) <= 0 || t.nb <= 0 || !t.built()) return false;
     if (f64 ? (t.val64 == nullptr && t.E > 0) : (t.val32 == nullptr && t.E > 0)) return false;
     if ((reinterpret_cast<uintptr_t>(x) & 15u) != 0) return false;
+    // the staging waves load whole 16-byte vectors of the gathered vector,
+    // through a buffer resource whose byte offsets are 32-bit
+    if (t.cols % (16 / (f64 ? 8 : 4)) != 0) return false;
+    if ((int64_t)t.cols * (f64 ? 8 : 4) > INT32_MAX) return false;
     return (int64_t)tile_lds_bytes(t.W, t.rmax, f64 ? 8 : 4) <= ctx->lds_max;
 }
 
@@ -229,6 +235,7 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     t.nb = static_cast<int32_t>(nb);
     t.rmax = static_cast<int32_t>((nchunks + nb - 1) / nb * 64);
     t.W = W;
+    t.cols = cols;
     t.S = static_cast<int32_t>((static_cast<int64_t>(cols) + t.W - 1) / t.W);
     // units: a slice with at least kTileStageMin entries per group on
     // average is staged; runs of the others become gather units

@@ -40,6 +40,7 @@ struct TileArgs {
     const int32_t* units;     // 4 U + 1
     int32_t nb, U, W, rmax, rows, cols;
     int32_t u0, u1;
+    int32_t staged;           // staged units in [u0, u1)
     int32_t sent;             // first entry of the sentinel block
 };
 
@@ -275,25 +276,33 @@ struct SliceRegs {
     static constexpr int kPer = kTileSliceBytes / (kStageThreads * 16);   // 16-byte loads per thread
     vec r[kPer];
 
-    __device__ __forceinline__ void load(const T* __restrict__ x, const TileArgs& ta, int s, int st) {
-        const int64_t base = (int64_t)s * ta.W;
-        const int w = min(ta.W, ta.cols - (int)base);
+    // Buffer loads: the vector is one buffer resource (its length in bytes
+    // bounds every access, and the hardware returns zeros past it, which is
+    // what a short last slice needs), each thread's offset inside a chunk is
+    // one register, and the chunk's offset a scalar.  So the loads need no
+    // address registers, every register set takes exactly kPer loads on
+    // every path, and the compiler counts the loads in flight exactly.
+    // The gathered length is a multiple of NV (checked by tiles_usable), so
+    // a 16-byte load is either inside the vector or past it.
+    __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, const TileArgs& ta, int s, int st) {
+        const int base = s * ta.W * static_cast<int>(sizeof(T));
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int j = (k * kStageThreads + st) * NV;
-            if (j + NV <= w) {
-                r[k] = *reinterpret_cast<const vec*>(x + base + j);
-            } else {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) r[k][i] = (j + i < w) ? x[base + j + i] : T(0);
-            }
-        }
+        for (int k = 0; k < kPer; ++k)
+            r[k] = __builtin_bit_cast(vec, __builtin_amdgcn_raw_buffer_load_b128(rs, st * 16, base + k * kStageThreads * 16, 0));
     }
     __device__ __forceinline__ void store(T* __restrict__ buf, int st) const {
 #pragma unroll
         for (int k = 0; k < kPer; ++k) *reinterpret_cast<vec*>(buf + (k * kStageThreads + st) * NV) = r[k];
     }
 };
+
+// The gathered vector as a buffer resource of cols elements (raw: no
+// stride, no format conversion; 32-bit data format as the raw loads need).
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int cols) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(x), static_cast<short>(0), cols * static_cast<int>(sizeof(T)),
+                                             0x00020000);
+}
 
 // The group's row sums over units [u0, u1) into acc (LDS, rmax entries),
 // then op.finish(device row, sum) for each row of the group.  x: the
@@ -325,29 +334,78 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
     for (int l = tid; l < ta.rmax; l += kTileThreads) acc[l] = T(0);
     if (wave >= kTileWaves) {
         const int st = tid - kTileWaves * kWave;
-        // ---- staging waves: during staged unit u they store the slice of
-        // the next staged unit (loaded into registers one staged unit
-        // earlier) into the free buffer, then load the one after
-        SliceRegs<T> R;
-        int nxt = ta.u1;   // staged unit whose slice R holds
-        if (ta.u0 < ta.u1) {
-            const int fs = un[4 * ta.u0 + 1] ? ta.u0 : un[4 * ta.u0 + 3];
-            if (fs < ta.u1) {
-                R.load(x, ta, un[4 * fs], st);
-                R.store(xs + (size_t)un[4 * fs + 2] * ta.W, st);
-                nxt = un[4 * fs + 3];
-                if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
-            }
+        // The compiled code joins the two branches' control flow at this
+        // point, so the compiler's count of loads in flight here includes
+        // the entry waves' loads, which it would then take as newer than the
+        // register sets' own.  Nothing is in flight on this path: an
+        // explicit vmcnt(0) clears that count at no cost.
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        // ---- staging waves: two register sets; with s_0, s_1, ... the
+        // staged units of the range, R0 holds the slices s_j of even j and
+        // R1 those of odd j.  During s_i they store s_{i+1} (loaded two
+        // staged units earlier) into its buffer, freed by the barrier that
+        // ended s_{i-1}, and load s_{i+3} into the set just stored: every
+        // slice's loads have two units' time to arrive.  The loop takes two
+        // staged units per turn, so each set stays in its own registers.
+        SliceRegs<T> R0, R1;
+        const __amdgpu_buffer_rsrc_t rs = vector_rsrc(x, ta.cols);
+        auto next_staged = [&](int v) { return v < ta.u1 ? un[4 * v + 3] : ta.u1; };
+        auto buffer = [&](int v) { return xs + (size_t)(un[4 * v + 2] & 1) * ta.W; };
+        int q0 = ta.u1;   // s_i; then s_{i+1}, s_{i+2}, s_{i+3}
+        if (ta.u0 < ta.u1) q0 = un[4 * ta.u0 + 1] ? ta.u0 : un[4 * ta.u0 + 3];
+        int q1 = next_staged(q0), q2 = next_staged(q1), q3 = next_staged(q2);
+        // The order of the loads matters: at the loop's head R1's loads are
+        // the older set, as after every turn (scheduling barriers keep the
+        // compiler from reordering the prologue).  Every step issues its
+        // set's loads, from slice s_i when no slice is left to load, so the
+        // compiler counts the loads in flight exactly on every path.
+        auto slice_of = [&](int v, int fallback) { return un[4 * (v < ta.u1 ? v : fallback)]; };
+        // where s_{i+1} goes; past the last staged unit, the buffer s_i does
+        // not use (free since s_{i-1} ended), so the store is unconditional
+        auto store_target = [&](int v, int cur) {
+            return v < ta.u1 ? buffer(v) : xs + (size_t)(1 - (un[4 * cur + 2] & 1)) * ta.W;
+        };
+        if (q0 < ta.u1) {
+            R0.load(rs, ta, un[4 * q0], st);
+            R1.load(rs, ta, slice_of(q1, q0), st);
+            __builtin_amdgcn_sched_barrier(0);
+            R0.store(buffer(q0), st);
+            R0.load(rs, ta, slice_of(q2, q0), st);
+            __builtin_amdgcn_sched_barrier(0);
         }
         lds_barrier();
-        for (int u = ta.u0; u < ta.u1; ++u) {
-            if (un[4 * u + 1] && nxt < ta.u1) {
-                R.store(xs + (size_t)un[4 * nxt + 2] * ta.W, st);
-                nxt = un[4 * nxt + 3];
-                if (nxt < ta.u1) R.load(x, ta, un[4 * nxt], st);
-            }
+        int u = ta.u0;
+        auto shift = [&]() {
+            q0 = q1;
+            q1 = q2;
+            q2 = q3;
+            q3 = next_staged(q3);
+        };
+        // The turns are counted (ta.staged staged units), so the only way
+        // back to the loop's head is the end of an odd step: at the head
+        // R1's loads are always the older set and the even step's stores
+        // wait for them alone.  An odd count ends with one even step.
+        auto even_step = [&]() {
+            for (; u < q0; ++u) lds_barrier();   // the gather units before s_i
+            R1.store(store_target(q1, q0), st);
+            R1.load(rs, ta, slice_of(q3, q0), st);
             lds_barrier();
+            u = q0 + 1;
+            shift();
+        };
+        int n = ta.staged;
+        for (; n >= 2; n -= 2) {
+            even_step();   // i even: s_{i+1} and s_{i+3} are odd
+            for (; u < q0; ++u) lds_barrier();
+            // i odd: s_{i+1} and s_{i+3} are even
+            R0.store(store_target(q1, q0), st);
+            R0.load(rs, ta, slice_of(q3, q0), st);
+            lds_barrier();
+            u = q0 + 1;
+            shift();
         }
+        if (n) even_step();
+        for (; u < ta.u1; ++u) lds_barrier();
     } else {
         // ---- entry waves
         // The wave's passes of units [u0, u1) are [i, iend) of the pass

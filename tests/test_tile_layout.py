@@ -3,7 +3,7 @@ device path uploads), checked on the CPU against the CSR it is built from:
 every entry stored exactly once with its row, column and source position
 decodable from (group, unit, key); units that cover the slices in order
 (a staged unit is one slice with at least 384 entries per group, a gather
-unit a run of up to 63 sparser slices); each (group, unit) tile split into 12
+unit a run of up to 63 sparser slices); each (group, unit) tile split into 8
 row-aligned sub-ranges (one per entry wave), each a multiple of 4 entries with padding only
 at its end; rows dealt to groups in 64-row chunks; the pass list the
 kernels read (every sub-range cut into passes of at most 256 entries,
@@ -91,7 +91,7 @@ def test_layout_stores_the_csr(name, tsize):
     nnz = int(ptr[-1])
     src = rng.permutation(nnz).astype(np.int32)      # arbitrary source positions
     nb, S, W, rmax, U, WAVES, keys, so, ws, units = layout(ptr, idx, src, rows, cols, cus, tsize)
-    assert WAVES == 12
+    assert WAVES == 8
     nchunks = -(-rows // 64)
     assert W == 65536 // tsize and S == -(-cols // W)       # a slice is 64 KiB in either precision
     assert rmax <= 4032 and nb * rmax >= rows and rmax % 64 == 0
