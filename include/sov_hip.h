@@ -147,9 +147,11 @@ int sov_hip_csr_get_segments(const sov_hip_csr* csr, int orient, int* count);
  * row groups (one workgroup each, about one per CU); the gathered vector is
  * cut into 64 KiB slices (16384 FP32 or 8192 FP64 elements).  Slices dense
  * enough in entries per group are copied into LDS by each workgroup with
- * 16-byte loads, one slice while the previous one is consumed, and their
- * entries gather from LDS; runs of sparser slices gather from memory, as
- * the CSR kernels do.  Each precision has its own layout.  parts >= 1
+ * 16-byte loads issued two staged slices ahead, and their entries gather
+ * from LDS; runs of sparser slices gather from memory, as the CSR kernels
+ * do.  Each precision has its own layout.  A product runs on the tiles
+ * only when its gathered vector is 16-byte aligned and shorter than 2 GiB
+ * (else on the CSR parts).  parts >= 1
  * builds the FP64 layout (and, after make_f32, the FP32 one) and runs the
  * orientation's SpMV and PDHG steps on it, in `parts` launches over ranges
  * of slices joined by per-row partial sums; parts = 0 frees both and
@@ -165,7 +167,8 @@ int sov_hip_csr_get_tiles(const sov_hip_csr* csr, int orient, int* parts);
  * the FP32 value.  No reference counterpart. */
 int sov_hip_csr_get_tiles_prec(const sov_hip_csr* csr, int orient, int prec, int* parts);
 /* *usable = 1 when the orientation's products of precision prec run on the
- * tiles (tiles set, value image present, LDS of the device large enough). */
+ * tiles (tiles set, value image present, LDS of the device large enough,
+ * gathered vector shorter than 2 GiB). */
 int sov_hip_csr_tiles_usable(const sov_hip_ctx* ctx, const sov_hip_csr* csr, int orient, int prec, int* usable);
 /* The tiled layout of a host CSR (row pointers, column indices in
  * [0, cols), and a source position per entry), as sov_hip_csr_set_tiles

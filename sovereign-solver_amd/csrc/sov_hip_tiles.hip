@@ -88,9 +88,8 @@ bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* 
     if ((f64 ? t.parts64 : t.parts) <= 0 || t.nb <= 0 || !t.built()) return false;
     if (f64 ? (t.val64 == nullptr && t.E > 0) : (t.val32 == nullptr && t.E > 0)) return false;
     if ((reinterpret_cast<uintptr_t>(x) & 15u) != 0) return false;
-    // the staging waves load whole 16-byte vectors of the gathered vector,
-    // through a buffer resource whose byte offsets are 32-bit
-    if (t.cols % (16 / (f64 ? 8 : 4)) != 0) return false;
+    // the staging waves load the gathered vector through a buffer resource,
+    // whose byte offsets are 32-bit
     if ((int64_t)t.cols * (f64 ? 8 : 4) > INT32_MAX) return false;
     return (int64_t)tile_lds_bytes(t.W, t.rmax, f64 ? 8 : 4) <= ctx->lds_max;
 }

@@ -282,8 +282,8 @@ struct SliceRegs {
     // one register, and the chunk's offset a scalar.  So the loads need no
     // address registers, every register set takes exactly kPer loads on
     // every path, and the compiler counts the loads in flight exactly.
-    // The gathered length is a multiple of NV (checked by tiles_usable), so
-    // a 16-byte load is either inside the vector or past it.
+    // The resource ends at a whole number of 16-byte vectors, so a load is
+    // either inside it or wholly past it (zeros).
     __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, const TileArgs& ta, int s, int st) {
         const int base = s * ta.W * static_cast<int>(sizeof(T));
 #pragma unroll
@@ -316,8 +316,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
 // counter completes in that order) and no slot is copied while its load is
 // pending.  The last kTileStagers waves copy vector slices into LDS: while
 // the entry waves consume staged unit u from one buffer they fill the other
-// with the next staged slice.  Staging runs in waves of its own so its
-// loads never sit in an entry wave's queue.  Every unit ends in a
+// with the next staged slice, whose loads they issued two staged units
+// earlier.  Staging runs in waves of its own so its loads never sit in an
+// entry wave's queue.  Every unit ends in a
 // barrier of all waves: the next unit's sub-ranges give rows to other waves.
 constexpr int kRing = 4;
 
@@ -348,7 +349,17 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // slice's loads have two units' time to arrive.  The loop takes two
         // staged units per turn, so each set stays in its own registers.
         SliceRegs<T> R0, R1;
-        const __amdgpu_buffer_rsrc_t rs = vector_rsrc(x, ta.cols);
+        // A gathered length that is not a whole number of 16-byte vectors:
+        // the resource ends at the last whole vector (the partial one loads
+        // as zeros), and the thread that stores the partial vector then
+        // writes its rem elements, read with scalar loads (x does not
+        // change during the launch).
+        constexpr int NV = SliceRegs<T>::NV;
+        const int rem = ta.cols % NV;
+        const __amdgpu_buffer_rsrc_t rs = vector_rsrc(x, ta.cols - rem);
+        const int last = (ta.cols - 1) / ta.W;              // the slice holding it
+        const int lo = ta.cols - rem - last * ta.W;         // its first element there
+        const bool owner = rem != 0 && st == (lo / NV) % SliceRegs<T>::kStageThreads;
         auto next_staged = [&](int v) { return v < ta.u1 ? un[4 * v + 3] : ta.u1; };
         auto buffer = [&](int v) { return xs + (size_t)(un[4 * v + 2] & 1) * ta.W; };
         int q0 = ta.u1;   // s_i; then s_{i+1}, s_{i+2}, s_{i+3}
@@ -365,11 +376,21 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         auto store_target = [&](int v, int cur) {
             return v < ta.u1 ? buffer(v) : xs + (size_t)(1 - (un[4 * cur + 2] & 1)) * ta.W;
         };
+        // store a set holding the slice of unit v into buf
+        auto store_set = [&](const SliceRegs<T>& R, T* buf, int v) {
+            R.store(buf, st);
+            if (owner && v < ta.u1 && un[4 * v] == last) {
+                typedef const T __attribute__((address_space(4))) ctail;
+                const ctail* xt = (const ctail*)(x + (ta.cols - rem));
+                for (int i = 0; i < NV - 1; ++i)
+                    if (i < rem) buf[lo + i] = xt[i];
+            }
+        };
         if (q0 < ta.u1) {
             R0.load(rs, ta, un[4 * q0], st);
             R1.load(rs, ta, slice_of(q1, q0), st);
             __builtin_amdgcn_sched_barrier(0);
-            R0.store(buffer(q0), st);
+            store_set(R0, buffer(q0), q0);
             R0.load(rs, ta, slice_of(q2, q0), st);
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -387,7 +408,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // wait for them alone.  An odd count ends with one even step.
         auto even_step = [&]() {
             for (; u < q0; ++u) lds_barrier();   // the gather units before s_i
-            R1.store(store_target(q1, q0), st);
+            store_set(R1, store_target(q1, q0), q1);
             R1.load(rs, ta, slice_of(q3, q0), st);
             lds_barrier();
             u = q0 + 1;
@@ -398,7 +419,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             even_step();   // i even: s_{i+1} and s_{i+3} are odd
             for (; u < q0; ++u) lds_barrier();
             // i odd: s_{i+1} and s_{i+3} are even
-            R0.store(store_target(q1, q0), st);
+            store_set(R0, store_target(q1, q0), q1);
             R0.load(rs, ta, slice_of(q3, q0), st);
             lds_barrier();
             u = q0 + 1;

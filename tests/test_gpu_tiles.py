@@ -18,6 +18,7 @@ import oracle_ffi as O
 import sovereign_amd as S
 from sovereign_amd.device import Csr, Ctx, check
 from test_gpu_steps import Device, abs_model, random_point, scaled_problem
+from test_tile_layout import ragged_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -94,6 +95,34 @@ def test_tiled_spmv_matches_oracle(ctx, name, prec, parts):
         print(name, prec, parts, transpose, "max err / bound", float(np.max(np.abs(got - want) / np.maximum(bound, 1e-300))),
               flush=True)
         assert not bad.any(), (np.flatnonzero(bad)[:5], got[bad][:5], want[bad][:5])
+    A.destroy()
+
+
+@pytest.mark.parametrize("prec,parts", [(0, 1), (0, 2), (1, 1), (1, 2)])
+def test_tiled_spmv_with_ragged_lengths(ctx, prec, parts):
+    """Gathered lengths past the last whole 16-byte vector (n = 40003,
+    m = 301), with that partial vector in a staged unit in both orientations
+    (pinned on the CPU by test_tile_layout.py): the staging waves load whole
+    vectors only and write the partial one's elements separately."""
+    m, n, ptr, idx = ragged_csr()
+    rng = np.random.default_rng(5 + parts + 10 * prec)
+    val = rng.uniform(0.5, 2.0, len(idx)) * rng.choice([-1.0, 1.0], len(idx))
+    dt = np.float64 if prec == 0 else np.float32
+    vals = val.astype(dt).astype(np.float64)
+    ones = dict(c=np.ones(n), col_lo=np.zeros(n), col_hi=np.ones(n), row_lo=np.zeros(m), row_hi=np.ones(m))
+    Mr = O.Model(dict(m=m, n=n, row_ptr=ptr, col_idx=idx, val=vals, **ones))
+    Ma = O.Model(dict(m=m, n=n, row_ptr=ptr, col_idx=idx, val=np.abs(vals), **ones))
+    A = Csr(ctx, m, n, ptr, idx, val)
+    set_tiles(ctx, A, parts, prec)
+    for transpose in (False, True):
+        x = rng.standard_normal(m if transpose else n).astype(dt).astype(np.float64)
+        got = A.spmv(x, transpose=transpose, prec=prec).astype(np.float64)
+        want = O.spmv_t(Mr, x) if transpose else O.spmv(Mr, x)
+        ln = np.bincount(idx, minlength=n) if transpose else np.diff(ptr)
+        bound = (ln + 4) * U[prec] * 2 * (O.spmv_t(Ma, np.abs(x)) if transpose else O.spmv(Ma, np.abs(x)))
+        print("ragged", prec, parts, transpose, "max err / bound",
+              float(np.max(np.abs(got - want) / np.maximum(bound, 1e-300))), flush=True)
+        assert np.all(np.abs(got - want) <= bound)
     A.destroy()
 
 

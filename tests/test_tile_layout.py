@@ -196,3 +196,40 @@ def test_layout_rejects_bad_input():
     idx_ok = np.array([0, 3], np.int32)
     assert L.sov_hip_tile_layout(1, 5, 2, good.ctypes.data, idx_ok.ctypes.data, src.ctypes.data, 4, 6,
                                  shape.ctypes.data, E.ctypes.data, None, None, None, None) != 0   # element size
+
+
+def ragged_csr():
+    """A CSR whose gathered lengths are not whole 16-byte vectors in either
+    orientation: n = 40003 columns (3 FP32 / 1 FP64 elements past the last
+    whole vector), m = 301 rows (1 / 1).  Row 7 has an entry in every column,
+    so the last columns are used."""
+    rng = np.random.default_rng(11)
+    m, n = 301, 40003
+    lens = rng.integers(300, 500, m)
+    lens[7] = n
+    ptr, idx = random_csr(m, n, lens, rng)
+    return m, n, ptr, idx
+
+
+def transpose_csr(rows, cols, ptr, idx):
+    order = np.argsort(idx, kind="stable")
+    tptr = np.zeros(cols + 1, np.int32)
+    tptr[1:] = np.cumsum(np.bincount(idx, minlength=cols))
+    tidx = np.repeat(np.arange(rows, dtype=np.int32), np.diff(ptr))[order]
+    return tptr, tidx.astype(np.int32), order.astype(np.int32)
+
+
+@pytest.mark.parametrize("tsize", [4, 8])
+def test_ragged_lengths_stage_the_last_slice(tsize):
+    # The staging waves load whole 16-byte vectors and write the partial last
+    # one's elements separately; this case (the one the GPU tiles test runs)
+    # must put the partial vector in a staged unit in both orientations on a
+    # 256-CU device, or that path goes untested.
+    m, n, ptr, idx = ragged_csr()
+    src = np.arange(len(idx), dtype=np.int32)
+    tptr, tidx, tsrc = transpose_csr(m, n, ptr, idx)
+    for rows, cols, p, i, s in ((m, n, ptr, idx, src), (n, m, tptr, tidx, tsrc)):
+        assert cols % (16 // tsize) != 0
+        nb, S, W, rmax, U, waves, keys, so, ws, units = layout(p, i, s, rows, cols, 256, tsize)
+        last = [u for u in range(U) if units[4 * u] == S - 1]
+        assert len(last) == 1 and units[4 * last[0] + 1] == 1, (rows, cols, S, units)
