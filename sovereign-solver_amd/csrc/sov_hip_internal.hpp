@@ -129,6 +129,12 @@ constexpr int kTileColBits = 20;                      // key bits of the column 
 constexpr uint32_t kTileColMask = (1u << kTileColBits) - 1u;
 constexpr int kTileGatherSlices = 63;                 // slices per gather unit (columns < 2^20 - 1)
 constexpr int kTileStageMin = 384;                    // entries per group that make a slice staged
+// Whether the tiled kernels of element size tsize add passes in pairs (and
+// the pass list evens every (wave, unit) count for them).  Measured on the
+// 1M-column Zipf LP: pairs cut the FP32 primal step from 66 to 60 us; the
+// FP64 tiles hold about one pass per wave and unit, where the padding only
+// halves the load queue's lookahead.
+constexpr bool tile_pairs(int tsize) { return tsize == 4; }
 constexpr uint32_t kTileSentinel = 0xFFF00000u;       // row 4095, column 0: never a real key (rows < kTileRowCap)
 
 struct Tiles {

@@ -195,7 +195,7 @@ int gather_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, const Tiles& t, T
 // The pass list of a layout (what the kernels read instead of wstart): for
 // each (group, wave), unit by unit, the passes of its sub-range, each up to
 // kWave * kTileLane entries.  Host only.
-static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, std::vector<int32_t>& pidx,
+static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, bool pairs, std::vector<int32_t>& pidx,
                        std::vector<int32_t>& pdesc) {
     const int U = t.U;
     constexpr int kPass = kWave * kTileLane;
@@ -207,10 +207,21 @@ static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, std::
             for (int u = 0; u < U; ++u) {
                 pidx[base + u] = static_cast<int32_t>(pdesc.size() / 2);
                 const size_t at = (static_cast<size_t>(b) * U + u) * kTileWaves + w;
+                // passes of kPass entries, the last one short; then, for
+                // kernels that add passes in pairs, an empty pass (its lanes
+                // load only the sentinel block) when the count is odd.  So
+                // only the last real pass of a sub-range is short: a pass
+                // with a successor fills all 64 lanes, and its last row is
+                // lane 63's, which the carry takes to the successor.
                 const int64_t e0 = wstart[at], e1 = wstart[at + 1];
-                for (int64_t e = e0; e < e1; e += kPass) {
+                int64_t np = 0;
+                for (int64_t e = e0; e < e1; e += kPass, ++np) {
                     pdesc.push_back(static_cast<int32_t>(e));
                     pdesc.push_back(static_cast<int32_t>(std::min<int64_t>(kPass, e1 - e)));
+                }
+                if (pairs && (np & 1)) {
+                    pdesc.push_back(static_cast<int32_t>(e1));
+                    pdesc.push_back(0);
                 }
             }
             if (pdesc.size() / 2 >= static_cast<size_t>(INT32_MAX))
@@ -315,7 +326,7 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
     int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, kTileSliceBytes / (f64 ? 8 : 4), t, keys, src,
                          wstart);
     std::vector<int32_t> pidx, pdesc;
-    if (rc == SOV_OK) rc = pass_tables(t, wstart, pidx, pdesc);
+    if (rc == SOV_OK) rc = pass_tables(t, wstart, tile_pairs(f64 ? 8 : 4), pidx, pdesc);
     if (rc != SOV_OK) {
         t.free();
         return rc;
@@ -488,7 +499,7 @@ int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     std::vector<uint32_t> k;
     std::vector<int32_t> so, ws, pi, pd;
     int rc = layout_host(rows, cols, p, ix, sv, cus, kTileSliceBytes / tsize, t, k, so, ws);
-    if (rc == SOV_OK) rc = pass_tables(t, ws, pi, pd);
+    if (rc == SOV_OK) rc = pass_tables(t, ws, tile_pairs(tsize), pi, pd);
     if (rc != SOV_OK) return rc;
     *passes = static_cast<int64_t>(pd.size() / 2);
     if (pidx) std::copy(pi.begin(), pi.end(), pidx);

@@ -84,6 +84,7 @@ template <typename T>
 struct PassData {
     u32x4 k;
     Quad<T> v;
+    int32_t n;   // entries in the pass (wave-uniform; 0: an empty pass)
 };
 
 // The lane's four entries of the pass whose first entry is `start` and
@@ -99,6 +100,7 @@ __device__ __forceinline__ PassData<T> load_pass(const TileArgs& ta, int32_t sta
     const uint32_t at = static_cast<uint32_t>(lane * kTileLane < cnt ? start + lane * kTileLane : ta.sent);
     d.k = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ta.keys + at));
     d.v = load_quad<T>(static_cast<const T*>(ta.vals), at);
+    d.n = cnt;
     return d;
 }
 
@@ -157,6 +159,74 @@ __device__ __forceinline__ void seg_scan(T& S, int& F) {
     seg_step<kDppRowBcast31, 0xC>(S, F);
 }
 
+// What a pass computes from its own entries alone: rows, validity, the
+// lanes' running sums and the row exchanges with the neighbouring lanes.
+template <typename T>
+struct PassLocal {
+    int r[4];
+    T s[4];
+    bool ok[4];
+    bool endin[3];   // a run ends inside the lane at u < 3 (entry u + 1 has another row)
+    bool single;     // the lane's entries are one run
+    // rows are encoded +1 in the lane exchanges so that 0 (what a lane with
+    // no source reads, or a lane whose entry is a sentinel) means "no row"
+    int fr1, tl1, prev_tl1, next_fr1, lane0_fr1;
+};
+
+template <typename T>
+__device__ __forceinline__ PassLocal<T> pass_local(const u32x4& key, const Quad<T>& vals, const T (&xv)[4]) {
+    PassLocal<T> p;
+    const uint32_t kk[4] = {key.x, key.y, key.z, key.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        p.ok[u] = kk[u] < kTileSentinel;
+        p.r[u] = static_cast<int>(kk[u] >> kTileColBits);
+    }
+    p.s[0] = vals.v[0] * xv[0];
+#pragma unroll
+    for (int u = 1; u < 4; ++u) {
+        const T q = vals.v[u] * xv[u];
+        p.endin[u - 1] = p.r[u] != p.r[u - 1];
+        p.s[u] = p.endin[u - 1] ? q : p.s[u - 1] + q;
+    }
+    p.single = !(p.endin[0] | p.endin[1] | p.endin[2]);
+    p.fr1 = p.ok[0] ? p.r[0] + 1 : 0;
+    p.tl1 = p.ok[3] ? p.r[3] + 1 : 0;
+    p.prev_tl1 = dpp<kDppWaveShr1, 0xF>(p.tl1);   // lane - 1's tail row (+1), 0 for lane 0
+    p.next_fr1 = dpp<kDppWaveShl1, 0xF>(p.fr1);   // lane + 1's first row (+1), 0 for lane 63
+    p.lane0_fr1 = __builtin_amdgcn_readfirstlane(p.fr1);
+    return p;
+}
+
+// The segmented inclusive scan of a pass's tails, before the carry is
+// added.  carried: lane 0's first run continues the incoming carry row.
+template <typename T>
+__device__ __forceinline__ void pass_scan(const PassLocal<T>& p, bool carried, int lane, T& S, int& F, bool& cont) {
+    cont = (p.fr1 != 0) & (lane > 0 ? p.prev_tl1 == p.fr1 : carried);
+    S = p.ok[3] ? p.s[3] : T(0);
+    F = (p.single & cont) ? 0 : 1;
+    seg_scan(S, F);
+}
+
+// The pass's writes once its scan has the carry in (cin: what the lane's
+// first run takes from earlier lanes): runs ending at u = 0..2, and the tail
+// when its row ends in this lane (lane 63's goes to the carry instead);
+// wm / wv: 4 entries each.
+template <typename T>
+__device__ __forceinline__ void pass_writes(const PassLocal<T>& p, T S, bool cont, bool carried, T carry_val, int lane,
+                                            bool* wm, T* wv) {
+    T Sprev = dpp<kDppWaveShr1, 0xF>(S);
+    if (lane == 0) Sprev = carried ? carry_val : T(0);
+    const T cin = cont ? Sprev : T(0);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        wm[u] = p.endin[u] & p.ok[u];
+        wv[u] = p.s[u] + (p.r[u] == p.r[0] ? cin : T(0));
+    }
+    wm[3] = (p.tl1 != 0) & (lane < kWave - 1) & (p.next_fr1 != p.tl1);
+    wv[3] = S;
+}
+
 // Add one pass (64 lanes x kTileLane entries, rows ascending, sentinels
 // last; xv = the gathered vector values of the lane's entries) into acc.
 // carry_row / carry_val: the last row of the previous pass of this
@@ -179,64 +249,75 @@ __device__ __forceinline__ void seg_scan(T& S, int& F) {
 template <typename T>
 __device__ __forceinline__ void tile_pass(const u32x4& key, const Quad<T>& vals, const T (&xv)[4],
                                           T* __restrict__ acc, int dummy, int lane, int& carry_row, T& carry_val) {
-    const uint32_t kk[4] = {key.x, key.y, key.z, key.w};
-    int r[4];
-    T s[4];
-    bool ok[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        ok[u] = kk[u] < kTileSentinel;
-        r[u] = static_cast<int>(kk[u] >> kTileColBits);
-    }
-    // running sums of the runs; a run ends inside the lane at u < 3 when
-    // entry u + 1 has another row
-    bool endin[3];
-    s[0] = vals.v[0] * xv[0];
-#pragma unroll
-    for (int u = 1; u < 4; ++u) {
-        const T p = vals.v[u] * xv[u];
-        endin[u - 1] = r[u] != r[u - 1];
-        s[u] = endin[u - 1] ? p : s[u - 1] + p;
-    }
-    const bool single = !(endin[0] | endin[1] | endin[2]);
-    // rows are encoded +1 in the lane exchanges so that 0 (what a lane with
-    // no source reads, or a lane whose entry is a sentinel) means "no row"
-    const int fr1 = ok[0] ? r[0] + 1 : 0, tl1 = ok[3] ? r[3] + 1 : 0;
-    const int prev_tl1 = dpp<kDppWaveShr1, 0xF>(tl1);        // lane - 1's tail row (+1), 0 for lane 0
-    const int next_fr1 = dpp<kDppWaveShl1, 0xF>(fr1);        // lane + 1's first row (+1), 0 for lane 63
-    const int lane0_fr1 = __builtin_amdgcn_readfirstlane(fr1);
-    const bool carried = (carry_row >= 0) & (lane0_fr1 == carry_row + 1);
-    const bool cont = (fr1 != 0) & (lane > 0 ? prev_tl1 == fr1 : carried);
-    // segmented inclusive scan of the tails
-    T S = ok[3] ? s[3] : T(0);
-    int F = (single & cont) ? 0 : 1;
-    seg_scan(S, F);
+    const PassLocal<T> p = pass_local<T>(key, vals, xv);
+    const bool carried = (carry_row >= 0) & (p.lane0_fr1 == carry_row + 1);
+    T S;
+    int F;
+    bool cont;
+    pass_scan(p, carried, lane, S, F, cont);
     if (carried & !F) S = carry_val + S;
-    T Sprev = dpp<kDppWaveShr1, 0xF>(S);
-    if (lane == 0) Sprev = carried ? carry_val : T(0);
-    const T cin = cont ? Sprev : T(0);
+    bool wm[4];
+    T wv[4];
+    pass_writes(p, S, cont, carried, carry_val, lane, wm, wv);
     // the carried row, when this pass does not continue it (wave-uniform)
     if ((carry_row >= 0) & !carried) {
         if (lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
     }
-    // the batch: runs ending at u = 0..2 (the first takes cin) and the tail
-    // when its row ends in this lane
-    T wv[4];
-    bool wm[4];
-#pragma unroll
-    for (int u = 0; u < 3; ++u) {
-        wm[u] = endin[u] & ok[u];
-        wv[u] = s[u] + (r[u] == r[0] ? cin : T(0));
-    }
-    wm[3] = (tl1 != 0) & (lane < kWave - 1) & (next_fr1 != tl1);
-    wv[3] = S;
     T old[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) old[u] = acc[wm[u] ? r[u] : 0];
+    for (int u = 0; u < 4; ++u) old[u] = acc[wm[u] ? p.r[u] : 0];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[wm[u] ? r[u] : dummy] = old[u] + wv[u];
-    const int tl63 = __builtin_amdgcn_readlane(tl1, kWave - 1);
+    for (int u = 0; u < 4; ++u) acc[wm[u] ? p.r[u] : dummy] = old[u] + wv[u];
+    const int tl63 = __builtin_amdgcn_readlane(p.tl1, kWave - 1);
     const T s63 = readlane(S, kWave - 1);
+    carry_row = tl63 - 1;
+    carry_val = tl63 ? s63 : T(0);
+}
+
+// Two consecutive passes A, B of one sub-range, with the result of
+// tile_pass(A) then tile_pass(B), bit for bit.  B's dependence on A is its
+// incoming carry: the carry row is A's lane-63 tail row, known from A's keys
+// alone, so both scans run side by side and only the carry value (A's
+// lane-63 sum) joins them afterwards.  The two batches are one: A's rows,
+// the flushed carries and B's rows are all distinct (rows ascend through
+// the sub-range, and A's tail row is written by B or flushed, never both).
+template <typename T>
+__device__ __forceinline__ void tile_pass2(const u32x4& keyA, const Quad<T>& valsA, const T (&xa)[4],
+                                           const u32x4& keyB, const Quad<T>& valsB, const T (&xb)[4],
+                                           T* __restrict__ acc, int dummy, int lane, int& carry_row, T& carry_val) {
+    const PassLocal<T> a = pass_local<T>(keyA, valsA, xa);
+    const PassLocal<T> b = pass_local<T>(keyB, valsB, xb);
+    const bool carriedA = (carry_row >= 0) & (a.lane0_fr1 == carry_row + 1);
+    const int tlA63 = __builtin_amdgcn_readlane(a.tl1, kWave - 1);
+    const int rowA = tlA63 - 1;   // A's outgoing carry row
+    const bool carriedB = (rowA >= 0) & (b.lane0_fr1 == rowA + 1);
+    T SA, SB;
+    int FA, FB;
+    bool contA, contB;
+    pass_scan(a, carriedA, lane, SA, FA, contA);
+    pass_scan(b, carriedB, lane, SB, FB, contB);
+    if (carriedA & !FA) SA = carry_val + SA;
+    const T sA63 = readlane(SA, kWave - 1);
+    const T valA = tlA63 ? sA63 : T(0);   // A's outgoing carry value
+    if (carriedB & !FB) SB = valA + SB;
+    bool wm[8];
+    T wv[8];
+    pass_writes(a, SA, contA, carriedA, carry_val, lane, wm, wv);
+    pass_writes(b, SB, contB, carriedB, valA, lane, wm + 4, wv + 4);
+    if ((carry_row >= 0) & !carriedA) {
+        if (lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
+    }
+    if ((rowA >= 0) & !carriedB) {
+        if (lane == 0) acc[rowA] = acc[rowA] + valA;
+    }
+    const int rw[8] = {a.r[0], a.r[1], a.r[2], a.r[3], b.r[0], b.r[1], b.r[2], b.r[3]};
+    T old[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) old[u] = acc[wm[u] ? rw[u] : 0];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[wm[u] ? rw[u] : dummy] = old[u] + wv[u];
+    const int tl63 = __builtin_amdgcn_readlane(b.tl1, kWave - 1);
+    const T s63 = readlane(SB, kWave - 1);
     carry_row = tl63 - 1;
     carry_val = tl63 ? s63 : T(0);
 }
@@ -476,10 +557,9 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             if (carry_row >= 0 && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
             lds_barrier();
         };
-        auto consume_pass = [&](const PassData<T>& slot) {
+        auto gather = [&](const PassData<T>& slot, T (&xv)[4]) {
             const uint32_t q[4] = {slot.k.x & kTileColMask, slot.k.y & kTileColMask, slot.k.z & kTileColMask,
                                    slot.k.w & kTileColMask};
-            T xv[4];
             if (staged) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) xv[k] = xl[q[k]];
@@ -487,19 +567,83 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
 #pragma unroll
                 for (int k = 0; k < 4; ++k) xv[k] = lane_identity(xg[q[k]]);
             }
+        };
+        // the gathers of two passes, all issued before any is waited for
+        auto gather2 = [&](const PassData<T>& sa, const PassData<T>& sb, T (&xa)[4], T (&xb)[4]) {
+            const uint32_t qa[4] = {sa.k.x & kTileColMask, sa.k.y & kTileColMask, sa.k.z & kTileColMask,
+                                    sa.k.w & kTileColMask};
+            const uint32_t qb[4] = {sb.k.x & kTileColMask, sb.k.y & kTileColMask, sb.k.z & kTileColMask,
+                                    sb.k.w & kTileColMask};
+            if (staged) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    xa[k] = xl[qa[k]];
+                    xb[k] = xl[qb[k]];
+                }
+            } else {
+                T ga[4], gb[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    ga[k] = xg[qa[k]];
+                    gb[k] = xg[qb[k]];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    xa[k] = lane_identity(ga[k]);
+                    xb[k] = lane_identity(gb[k]);
+                }
+            }
+        };
+        auto consume_pass = [&](const PassData<T>& slot) {
+            T xv[4];
+            gather(slot, xv);
             tile_pass<T>(slot.k, slot.v, xv, acc, dummy, lane, carry_row, carry_val);
         };
-        // one queue slot: close the units this wave has finished (and those
-        // it has no passes in), add the slot's pass (pass i) if the range
-        // is not done, refill the slot (loads are issued on every path, so
-        // each slot is always followed by the same number of loads, and the
-        // loop has its only exit at the top)
-        auto step = [&](PassData<T>& slot) {
+        // close the units this wave has finished (and those it has no
+        // passes in) before pass i
+        auto close_done = [&]() {
             while (cu < ta.u1 && i == nxt) {
                 close_unit(cu);
                 ++cu;
                 if (cu < ta.u1) open_unit(cu);
             }
+        };
+        // two queue slots, passes i and i + 1: when both lie in the current
+        // unit (always, as the pass list gives every (wave, unit) an even
+        // count) they are added together (tile_pass2: their gathers, scans
+        // and row-sum batches overlap), or pass i alone when i + 1 is the
+        // empty pass that evens the count; else one at a time with the unit
+        // change between them.  Then both slots are refilled (loads are
+        // issued on every path, so each slot is always followed by the same
+        // number of loads, and the loop has its only exit at the top).
+        auto step2 = [&](PassData<T>& sa, PassData<T>& sb) {
+            close_done();
+            if (cu < ta.u1) {
+                if (i + 1 < nxt && sb.n == 0) {
+                    consume_pass(sa);
+                    i += 2;
+                } else if (i + 1 < nxt) {
+                    T xa[4], xb[4];
+                    gather2(sa, sb, xa, xb);
+                    tile_pass2<T>(sa.k, sa.v, xa, sb.k, sb.v, xb, acc, dummy, lane, carry_row, carry_val);
+                    i += 2;
+                } else {
+                    consume_pass(sa);
+                    ++i;
+                    close_done();
+                    if (cu < ta.u1) {
+                        consume_pass(sb);
+                        ++i;
+                    }
+                }
+            }
+            sa = fetch();
+            sb = fetch();
+        };
+        // one queue slot, pass i (the pass list of unpaired kernels has no
+        // empty passes)
+        auto step = [&](PassData<T>& slot) {
+            close_done();
             if (cu < ta.u1) {
                 consume_pass(slot);
                 ++i;
@@ -507,11 +651,18 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             slot = fetch();
         };
         if (cu < ta.u1) open_unit(cu);
-        while (cu < ta.u1) {
-            step(q0);
-            step(q1);
-            step(q2);
-            step(q3);
+        if constexpr (tile_pairs(sizeof(T))) {
+            while (cu < ta.u1) {
+                step2(q0, q1);
+                step2(q2, q3);
+            }
+        } else {
+            while (cu < ta.u1) {
+                step(q0);
+                step(q1);
+                step(q2);
+                step(q3);
+            }
         }
     }
     lds_barrier();

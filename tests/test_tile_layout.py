@@ -6,7 +6,7 @@ decodable from (group, unit, key); units that cover the slices in order
 unit a run of up to 63 sparser slices); each (group, unit) tile split into 8
 row-aligned sub-ranges (one per entry wave), each a multiple of 4 entries with padding only
 at its end; rows dealt to groups in 64-row chunks; the pass list the
-kernels read (every sub-range cut into passes of at most 256 entries,
+kernels read (every sub-range cut into passes of at most 256 entries, padded to an even count for FP32,
 listed per group and wave, unit by unit); and the product the layout
 encodes equal to the CSR product.  Cases cover empty rows, a row
 denser than one wave pass, several slices (ragged last slice), and more
@@ -149,7 +149,9 @@ def test_layout_stores_the_csr(name, tsize):
                 assert not (tile_rows[w] & tile_rows[v])
     assert seen.all()
     # the pass list: per (group, wave), unit by unit, each sub-range cut into
-    # passes of at most 256 entries
+    # passes of 256 entries (the last one short), plus, for the FP32 tiles,
+    # an empty pass when that count is odd (their kernels add passes in
+    # pairs, and a pass with a successor must fill all 64 lanes)
     pidx, pdesc = passes(ptr, idx, src, rows, cols, cus, tsize, nb, U, WAVES)
     want = []
     for b in range(nb):
@@ -159,8 +161,11 @@ def test_layout_stores_the_csr(name, tsize):
                 assert pidx[base + u] == len(want)
                 t = b * U + u
                 e0, e1 = ws[t * WAVES + w], ws[t * WAVES + w + 1]
-                want += [(e, min(256, e1 - e)) for e in range(e0, e1, 256)]
+                cut = [(e, min(256, e1 - e)) for e in range(e0, e1, 256)]
+                want += cut + ([(e1, 0)] if len(cut) % 2 and tsize == 4 else [])
             assert pidx[base + U] == len(want)
+            if tsize == 4:
+                assert np.all((pidx[base + 1:base + U + 1] - pidx[base:base + U]) % 2 == 0)
     assert pdesc.tolist() == [list(p) for p in want]
     # the product the layout encodes
     rng2 = np.random.default_rng(1)
