@@ -128,6 +128,12 @@ struct PdhgOptions {
     /// the autotuner decides (off without autotune), 0 = never, k >= 1 =
     /// always, in k launches per product.
     int tiles = -1;
+    /// When the autotuner runs (fused engine): at the first check after the
+    /// iterations have taken this many seconds; 0 = before the first
+    /// iteration.  Tuning costs about as much as a few thousand iterations
+    /// on a large LP, so a short solve finishes on the default schedule
+    /// instead of paying for it.
+    double autotune_after_seconds = 0.5;
     /// Device ordinal for a fused solve that creates its own backend context.
     int device = 0;
 };

@@ -83,6 +83,7 @@ typedef struct sov_pdhg_opts {
     double* ray_out;                 /* NULL or num_cols: the primal ray when status is dual_infeasible */
     const double* warm_start_primal; /* NULL or num_cols values (original units) */
     int32_t tiles;                   /* LDS-staged step kernels: -1 autotuner decides, 0 never, k >= 1 always (k launches) */
+    double autotune_after_seconds;   /* the autotuner runs at the first check after this much loop time (0: before it) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

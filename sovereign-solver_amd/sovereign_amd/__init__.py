@@ -41,6 +41,7 @@ class PdhgOpts(C.Structure):
         ("restart_scheme", C.c_int32), ("autotune", C.c_int32), ("restart_beta_sufficient", C.c_double),
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
         ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p), ("tiles", C.c_int32),
+        ("autotune_after_seconds", C.c_double),
     ]
 
 
@@ -283,10 +284,12 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
           restarts="reference", autotune=True, detect_infeasibility=False, primal_weight_smoothing=None,
-          warm_start_primal=None, tiles=None) -> Result:
+          warm_start_primal=None, tiles=None, autotune_after=None) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point.
     tiles: None = the autotuner decides, 0 = CSR step kernels only, k >= 1 =
-    the LDS-staged tiled step kernels in k launches per product."""
+    the LDS-staged tiled step kernels in k launches per product.
+    autotune_after: seconds of iterations before the autotuner runs (None:
+    the engine's default; 0: before the first iteration)."""
     L = engine()
     o = PdhgOpts()
     L.sov_pdhg_default_opts(C.byref(o))
@@ -306,6 +309,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     o.autotune = int(autotune)
     if tiles is not None:
         o.tiles = int(tiles)
+    if autotune_after is not None:
+        o.autotune_after_seconds = float(autotune_after)
     o.detect_infeasibility = int(detect_infeasibility)
     if primal_weight_smoothing is not None:
         o.primal_weight_smoothing = primal_weight_smoothing

@@ -190,6 +190,7 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->restart_scheme = 0;
     o->autotune = d.autotune ? 1 : 0;
     o->tiles = d.tiles;
+    o->autotune_after_seconds = d.autotune_after_seconds;
     o->restart_beta_sufficient = d.restart_beta_sufficient;
     o->restart_beta_necessary = d.restart_beta_necessary;
     o->restart_beta_artificial = d.restart_beta_artificial;
@@ -222,6 +223,7 @@ sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
     p.use_graphs = o->use_graphs != 0;
     p.autotune = o->autotune != 0;
     p.tiles = o->tiles;
+    p.autotune_after_seconds = o->autotune_after_seconds;
     p.restart_scheme = o->restart_scheme == 1 ? sov::PdhgRestartScheme::Adaptive : sov::PdhgRestartScheme::Reference;
     p.restart_beta_sufficient = o->restart_beta_sufficient;
     p.restart_beta_necessary = o->restart_beta_necessary;

@@ -836,10 +836,21 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), ysrc, zs.as<double>()), "set_point");
             hip_ok(sov_hip_sync(ctx), "sync");
         }
-        if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, prec), "tune");
-        if (o.tiles >= 0 && nnz > 0)
-            for (int orient : {0, 1})
-                if ((orient == 0 ? m : n) > 0) hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, o.tiles), "tiles");
+        // the autotuner (before the loop, or deferred to the first check
+        // after autotune_after_seconds of iterations); forced tiles are set
+        // again after it, since it also chooses tiles
+        bool tuned = !(o.autotune && nnz > 0);
+        auto set_forced_tiles = [&]() {
+            if (o.tiles >= 0 && nnz > 0)
+                for (int orient : {0, 1})
+                    if ((orient == 0 ? m : n) > 0)
+                        hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, o.tiles), "tiles");
+        };
+        if (!tuned && o.autotune_after_seconds <= 0.0) {
+            hip_ok(sov_hip_csr_tune(ctx, csr.h, prec), "tune");
+            tuned = true;
+        }
+        set_forced_tiles();
 
         // primal-weight bookkeeping (FP64 copies of the last restart point)
         DevMem d_xlast, d_ylast;
@@ -922,6 +933,16 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             since += count;
             result.iterations += count;
             if (result.iterations != next_check) break;   // iteration limit inside the interval
+            if (!tuned && seconds_since(t_loop) >= o.autotune_after_seconds) {
+                // deferred autotuning: new schedules and layouts, so the
+                // captured intervals are captured again
+                hip_ok(sov_hip_csr_tune(ctx, csr.h, S.prec), "tune");
+                set_forced_tiles();
+                tuned = true;
+                g_first.reset();
+                g_cont.reset();
+                graph_prec = -1;
+            }
 
             ca.st = &S.st;
             ca.since_restart = since;

@@ -86,13 +86,16 @@ def test_graph_replay_is_deterministic(gpu):
     assert a.iterations == c.iterations and np.array_equal(a.x, c.x) and np.array_equal(a.y, c.y)
 
 
-def test_autotuned_schedule_agrees_with_default(gpu):
-    # the tuned lane counts / segments change only the rounding of the dots
+@pytest.mark.parametrize("after", [0.0, 1e-9])
+def test_autotuned_schedule_agrees_with_default(gpu, after):
+    # the tuned lane counts / segments change only the rounding of the dots;
+    # after = 0 tunes before the loop, 1e-9 at the first check (mid-solve:
+    # the captured intervals are captured again with the new schedule)
     lp = S.LP.random(300, 800, 5, 3)
     a = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
                 restarts="adaptive", autotune=False)
     b = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
-                restarts="adaptive", autotune=True)
+                restarts="adaptive", autotune=True, autotune_after=after)
     assert a.status == b.status == "optimal"
     assert rel(a.objective, b.objective) < OBJ_RTOL
     assert max(b.relative_kkt) <= 1e-6 * 1.0001
