@@ -188,7 +188,9 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
  * count; pidx (groups * sub-ranges per tile * (units + 1)) = index of the
  * first pass of (group, wave) in each unit, then one past its last; pdesc
  * (2 * passes) = first entry and entry count of each pass (at most 256
- * entries, inside one sub-range).  pidx / pdesc may be null (first call for
+ * entries, inside one sub-range, only the sub-range's last one short; for
+ * tsize 4, whose kernels add passes in pairs, an empty pass evens each
+ * (group, wave, unit) count).  pidx / pdesc may be null (first call for
  * the size).  Host only.  No reference counterpart (layout of the tiled
  * products that replace Backend::spmv / spmv_transpose). */
 int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,

@@ -114,11 +114,13 @@ struct Natural {
 // unit, next staged unit (U if none)}, units[4 U] = S.
 //
 // What the kernels read instead of wstart: the pass list.  A pass is up to
-// kWave * kTileLane consecutive entries of one sub-range.  The passes of
-// wave w of group b are listed unit by unit: pidx[(b kTileWaves + w) (U + 1)
-// + u] is the index of its first pass in unit u (entry U: one past its
-// last), and pdesc[2 i], pdesc[2 i + 1] = first entry and entry count of
-// pass i.
+// kWave * kTileLane consecutive entries of one sub-range; only a sub-range's
+// last pass is short, and for kernels that add passes in pairs
+// (tile_pairs) an empty pass (count 0) evens each (wave, unit) count.  The
+// passes of wave w of group b are listed unit by unit: pidx[(b kTileWaves +
+// w) (U + 1) + u] is the index of its first pass in unit u (entry U: one
+// past its last), and pdesc[2 i], pdesc[2 i + 1] = first entry and entry
+// count of pass i.
 constexpr int kTileThreads = 1024;                    // workgroup of a group
 constexpr int kTileStagers = 8;                       // waves that copy vector slices into LDS
 constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 8 waves that add the entries
