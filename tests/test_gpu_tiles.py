@@ -169,11 +169,13 @@ def solve_case():
     return lp, o
 
 
-@pytest.mark.parametrize("precision,tiles", [("f64", 1), ("f32", 2)])
-def test_solve_on_tiles_reaches_the_oracles_objective(gpu, solve_case, precision, tiles):
+@pytest.mark.parametrize("precision,tiles,after", [("f64", 1, None), ("f32", 2, None), ("f32", 1, 1e-9)])
+def test_solve_on_tiles_reaches_the_oracles_objective(gpu, solve_case, precision, tiles, after):
+    # after = 1e-9: the autotuner runs at the first check, mid-solve, and the
+    # forced tiles are set again after it
     lp, o = solve_case
     kw = dict(backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative", restarts="adaptive",
-              precision=precision, tiles=tiles, max_iterations=400000)
+              precision=precision, tiles=tiles, max_iterations=400000, autotune_after=after)
     r = S.solve(lp, **kw)
     print(precision, tiles, r.status, r.iterations, r.objective, o["objective"], flush=True)
     assert r.status == "optimal"
