@@ -629,25 +629,33 @@ __global__ void __launch_bounds__(kBlock) k_batch_reduce(const int32_t* __restri
                                                          double* __restrict__ out) {
     const int b = blockIdx.x;
     if (active[b] != want) return;   // uniform per block
+    // one sweep over the instance's entries per dimension, all 11 terms of
+    // an entry loaded together (7 sums, then 4 maxima); each term keeps its
+    // own accumulation order (classes, then entries, ascending) and
+    // reduction tree, so the sums are those of one sweep per term
     for (int dim = 0; dim < 2; ++dim) {
         const double* t = dim == 0 ? rows_t : cols_t;
         const int64_t stride = dim == 0 ? m : n;
-        for (int slot = 0; slot < 11; ++slot) {
-            const bool is_max = slot >= 7;
-            double acc = 0.0;
-            for (int c = 0; c < kClasses; ++c) {
-                const int2 r = batch_range(seg, b, dim, c);
-                for (int i = threadIdx.x; i < r.y; i += kBlock) {
-                    const double v = t[slot * stride + r.x + i];
-                    acc = is_max ? nmax(acc, v) : acc + v;
-                }
+        double sm[7], mx[4];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) sm[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mx[k] = 0.0;
+        for (int c = 0; c < kClasses; ++c) {
+            const int2 r = batch_range(seg, b, dim, c);
+            for (int i = threadIdx.x; i < r.y; i += kBlock) {
+                const int64_t at = r.x + i;
+                double v[11];
+#pragma unroll
+                for (int k = 0; k < 11; ++k) v[k] = t[k * stride + at];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) sm[k] = sm[k] + v[k];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) mx[k] = nmax(mx[k], v[7 + k]);
             }
-            double sm[1] = {acc}, mx[1] = {acc};
-            double* dst = out + (int64_t)b * 22 + dim * 11 + slot;
-            if (is_max) block_reduce_write<0, 1>(sm, mx, dst);
-            else block_reduce_write<1, 0>(sm, mx, dst);
-            __syncthreads();   // the reduction's LDS is reused by the next slot
         }
+        block_reduce_write<7, 4>(sm, mx, out + (int64_t)b * 22 + dim * 11);
+        __syncthreads();   // the reduction's LDS is reused by the next dimension
     }
 }
 
