@@ -209,7 +209,7 @@ def committed_traffic(m, n, nnz, dtype, layout):
     return None
 
 
-def batched(S, count=256, time_limit=150.0):
+def batched(S, count=256, time_limit=150.0, use_graphs=True):
     """The batched workload: `count` LPs of the 2,000 x 5,000 family (5 nonzeros
     per column) with m log-uniform in [200, 20000] (seed 0), n = 2.5 m, solved
     together to 1e-6 relative KKT in one stacked device batch with FP32
@@ -217,7 +217,8 @@ def batched(S, count=256, time_limit=150.0):
     sizes = S.batch_sizes(count, 200, 20000, 0)
     lps = [S.LP.random(int(m), int(round(2.5 * m)), 5, 1000 + b) for b, m in enumerate(sizes)]
     t0 = time.perf_counter()
-    res, tim = S.solve_batch(lps, tolerance=1e-6, precision="f32", max_iterations=2000000, time_limit=time_limit)
+    res, tim = S.solve_batch(lps, tolerance=1e-6, precision="f32", max_iterations=2000000, time_limit=time_limit,
+                             use_graphs=use_graphs)
     wall = time.perf_counter() - t0
     its = np.array([r.iterations for r in res])
     nopt = sum(r.status == "optimal" for r in res)
@@ -256,7 +257,7 @@ def time_to_tolerance(S):
 MODEL = os.path.join(ROOT, "sovereign-solver_amd", "sovereign_amd", "models", "transport_100x200.npz")
 
 
-def learned_warm_start(ctx, count=200, time_subset=10):
+def learned_warm_start(ctx, count=200, time_subset=24):
     """Warm vs cold on `count` held-out instances of the 100 x 200
     transportation family with the committed network (trained on the GPU by
     `python -m sovereign_amd.learned train`): median iterations and wall time
@@ -304,6 +305,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-LP batched solve")
+    ap.add_argument("--batch-no-graphs", action="store_true",
+                    help="run the batched solve's check intervals as plain launches (kernel-trace runs: this "
+                         "rocprofv3 does not trace graph launches)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-1e-6 solves")
     ap.add_argument("--segments", default=None,
@@ -404,7 +408,7 @@ def main():
 
     bat = None
     if not args.no_batched:
-        bat = batched(S)
+        bat = batched(S, use_graphs=not args.batch_no_graphs)
         log(f"batched: {bat['instances']} LPs, {bat['optimal']} optimal in {bat['wall_s']} s -> "
             f"{bat['lps_per_s']} LPs/s (median {bat['iterations_median']}, max {bat['iterations_max']} iterations)")
 
