@@ -187,3 +187,15 @@ def test_certificates_on_generated_lps():
         assert r.status == status, r.message
         M = O.Model(lp.arrays())
         assert O.farkas_certifies(M, r.farkas) if status == "primal_infeasible" else O.primal_ray_certifies(M, r.ray)
+
+
+def test_default_options_through_the_c_api():
+    # sov_pdhg_default_opts carries PdhgOptions' defaults, the appended fields
+    # included (the Python struct mirrors sov_pdhg_opts field for field)
+    import ctypes as C
+    o = S.PdhgOpts()
+    o.autotune_after_seconds = -1.0
+    o.tiles = 99
+    S.engine().sov_pdhg_default_opts(C.byref(o))
+    assert o.autotune == 1 and o.tiles == -1
+    assert o.autotune_after_seconds == 0.5
