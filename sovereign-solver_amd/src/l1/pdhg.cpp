@@ -106,11 +106,11 @@ struct RelKkt {
 };
 
 /// Relative KKT of an original-space point (definitions in pdhg.hpp).
-RelKkt relative_kkt(const Problem& p, std::span<const Real> x, std::span<const Real> y) {
+/// act = A x and aty = A^T y, as CsrMatrix::multiply / multiply_transpose
+/// give them.
+RelKkt relative_kkt(const Problem& p, std::span<const Real> x, std::span<const Real> y, std::span<const Real> act,
+                    std::span<const Real> aty) {
     const Real sg = p.sense() == ObjSense::Maximize ? -1.0 : 1.0;
-    std::vector<Real> act(static_cast<std::size_t>(p.num_rows())), aty(static_cast<std::size_t>(p.num_cols()));
-    p.matrix().multiply(x, act);
-    p.matrix().multiply_transpose(y, aty);
     Real pv2 = 0, bn2 = 0, dr2 = 0, cn2 = 0, pobj = 0, dobj = 0;
     for (std::size_t i = 0; i < act.size(); ++i) {
         const Real lo = p.row_lower()[i], hi = p.row_upper()[i];
@@ -148,6 +148,13 @@ RelKkt relative_kkt(const Problem& p, std::span<const Real> x, std::span<const R
     r.rd = std::sqrt(dr2) / (1.0 + std::sqrt(cn2));
     r.gap = std::abs(r.pobj - r.dobj) / (1.0 + std::abs(r.pobj) + std::abs(r.dobj));
     return r;
+}
+
+RelKkt relative_kkt(const Problem& p, std::span<const Real> x, std::span<const Real> y) {
+    std::vector<Real> act(static_cast<std::size_t>(p.num_rows())), aty(static_cast<std::size_t>(p.num_cols()));
+    p.matrix().multiply(x, act);
+    p.matrix().multiply_transpose(y, aty);
+    return relative_kkt(p, x, y, act, aty);
 }
 
 /// Unscale a scaled, internal-sense point (reference pdhg.cpp:205-214).
@@ -195,7 +202,7 @@ void finish_result(const Problem& p, const PdhgOptions& o, std::vector<Real> x, 
         else dobj += t;
     }
     r.duality_gap = finite ? std::abs(r.objective - dobj) / (1.0 + std::abs(r.objective) + std::abs(dobj)) : kInfinity;
-    const RelKkt k = relative_kkt(p, r.primal, r.dual);
+    const RelKkt k = relative_kkt(p, r.primal, r.dual, act, aty);   // the products above, not again
     r.relative_primal_residual = k.rp;
     r.relative_dual_residual = k.rd;
     r.relative_gap = k.gap;
@@ -676,6 +683,14 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         const auto m = static_cast<std::size_t>(problem.num_rows()), n = static_cast<std::size_t>(problem.num_cols());
         const auto nnz = static_cast<std::size_t>(A0.nnz());
         const Real sign = problem.sense() == ObjSense::Maximize ? -1.0 : 1.0;
+        // verbose: the setup's phases, each timed to the device's completion
+        auto t_mark = Clock::now();
+        auto mark = [&](const char* what) {
+            if (!o.verbose) return;
+            hip_ok(sov_hip_sync(ctx), "sync");
+            std::fprintf(stderr, "  [pdhg] setup: %s %.1f ms\n", what, 1e3 * seconds_since(t_mark));
+            t_mark = Clock::now();
+        };
 
         // -- the matrix crosses the bus once (3 copies), then everything is device work
         DevMem d_rp(ctx, (m + 1) * sizeof(int32_t)), d_ci(ctx, std::max<std::size_t>(nnz, 1) * sizeof(int32_t)),
@@ -695,6 +710,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         ord.col.resize(n);
         hip_ok(sov_hip_csr_order(ctx, csr.h, 0, ord.row.data()), "order");
         hip_ok(sov_hip_csr_order(ctx, csr.h, 1, ord.col.data()), "order");
+        mark("upload, transpose and device order");
 
         // -- kernel (d): Ruiz + Pock-Chambolle on the device (reference scaling.cpp:19-104)
         DevMem d_rs(ctx, std::max<std::size_t>(m, 1) * 8), d_cs(ctx, std::max<std::size_t>(n, 1) * 8);
@@ -719,6 +735,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         if (n) hip_ok(sov_hip_memcpy_d2h(ctx, cs.data(), d_cs.p, n * 8), "download scaling");
         const Scaling scaling = Scaling::from_factors(rs, cs);
         hip_ok(sov_hip_csr_scale(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>()), "csr_scale");
+        mark("scaling");
 
         // scaled vectors (reference scaling.cpp:172-192, pdhg.cpp:250-252)
         std::vector<Real> c_s(n), clo_s(n), chi_s(n), rlo_s(m), rhi_s(m);
@@ -756,6 +773,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         upload_dev(d_ohi_c, problem.col_upper(), ord.col);
         upload_dev(d_oc, problem.objective(), ord.col);
         upload_dev(d_cs_dev, cs, ord.col);
+        mark("scaled vectors and check arrays");
 
         // -- step size: device power iteration in FP64 (reference pdhg.cpp:48-79)
         Real est = 0.0;
@@ -785,6 +803,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             }
         }
         result.operator_norm_estimate = est;
+        mark("power iteration");
         const Real eta = est > 1e-9 ? o.step_size_safety / est : o.step_size_safety;
         Real omega = o.use_primal_weight ? o.initial_primal_weight : 1.0;
 
@@ -851,6 +870,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             tuned = true;
         }
         set_forced_tiles();
+        mark("iterate state, tuning and layouts");
 
         // primal-weight bookkeeping (FP64 copies of the last restart point)
         DevMem d_xlast, d_ylast;
