@@ -19,7 +19,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <utility>
+#include <vector>
 
 #include "sov_hip.h"
 #include "sovereign/scaling.hpp"
@@ -594,16 +596,38 @@ struct CsrHandle {
 
 /// Host copy of the handle's device order (sov_hip_csr_order): every vector
 /// the fused kernels touch is stored with element p = natural index perm[p].
+/// f(begin, end) over [0, n) in contiguous chunks on up to 16 host threads
+/// (one below 2^18 elements).  For element-wise work: every element is
+/// computed as in a serial loop, so results do not depend on the split.
+template <class F>
+void parallel_for(std::size_t n, F f) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const std::size_t parts = n < (std::size_t{1} << 18) ? 1 : hw;
+    if (parts <= 1) {
+        f(std::size_t{0}, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(parts - 1);
+    for (std::size_t k = 1; k < parts; ++k) pool.emplace_back(f, n * k / parts, n * (k + 1) / parts);
+    f(std::size_t{0}, n / parts);
+    for (auto& t : pool) t.join();
+}
+
 struct DeviceOrder {
     std::vector<int32_t> row, col;
     static std::vector<Real> to_dev(std::span<const Real> v, const std::vector<int32_t>& perm) {
         std::vector<Real> out(v.size());
-        for (std::size_t p = 0; p < out.size(); ++p) out[p] = v[static_cast<std::size_t>(perm[p])];
+        parallel_for(out.size(), [&](std::size_t b, std::size_t e) {
+            for (std::size_t p = b; p < e; ++p) out[p] = v[static_cast<std::size_t>(perm[p])];
+        });
         return out;
     }
     static std::vector<Real> to_nat(std::span<const Real> v, const std::vector<int32_t>& perm) {
         std::vector<Real> out(v.size());
-        for (std::size_t p = 0; p < out.size(); ++p) out[static_cast<std::size_t>(perm[p])] = v[p];
+        parallel_for(out.size(), [&](std::size_t b, std::size_t e) {
+            for (std::size_t p = b; p < e; ++p) out[static_cast<std::size_t>(perm[p])] = v[p];
+        });
         return out;
     }
 };
@@ -661,10 +685,14 @@ struct DeviceState {
         } else {
             std::vector<float> f(len);
             hip_ok(sov_hip_memcpy_d2h(ctx, f.data(), src.p, len * 4), "download");
-            for (std::size_t i = 0; i < len; ++i) out[i] = static_cast<Real>(f[i]);
+            parallel_for(len, [&](std::size_t b, std::size_t e) {
+                for (std::size_t i = b; i < e; ++i) out[i] = static_cast<Real>(f[i]);
+            });
         }
         if (scale != 1.0)
-            for (Real& v : out) v *= scale;
+            parallel_for(len, [&](std::size_t b, std::size_t e) {
+                for (std::size_t i = b; i < e; ++i) out[i] *= scale;
+            });
         return out;
     }
 };
@@ -739,12 +767,14 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
 
         // scaled vectors (reference scaling.cpp:172-192, pdhg.cpp:250-252)
         std::vector<Real> c_s(n), clo_s(n), chi_s(n), rlo_s(m), rhi_s(m);
-        for (std::size_t j = 0; j < n; ++j) {
-            c_s[j] = (problem.objective()[j] * cs[j]) * sign;
-            const Real lo = problem.col_lower()[j], hi = problem.col_upper()[j];
-            clo_s[j] = is_finite_bound(lo) ? lo / cs[j] : lo;
-            chi_s[j] = is_finite_bound(hi) ? hi / cs[j] : hi;
-        }
+        parallel_for(n, [&](std::size_t b, std::size_t e) {
+            for (std::size_t j = b; j < e; ++j) {
+                c_s[j] = (problem.objective()[j] * cs[j]) * sign;
+                const Real lo = problem.col_lower()[j], hi = problem.col_upper()[j];
+                clo_s[j] = is_finite_bound(lo) ? lo / cs[j] : lo;
+                chi_s[j] = is_finite_bound(hi) ? hi / cs[j] : hi;
+            }
+        });
         for (std::size_t i = 0; i < m; ++i) {
             const Real lo = problem.row_lower()[i], hi = problem.row_upper()[i];
             rlo_s[i] = is_finite_bound(lo) ? lo * rs[i] : lo;
