@@ -4,19 +4,25 @@
 Workload: the 1M-column Zipf LP (m = 500,000 rows, n = 1,000,000 columns,
 column nonzeros clip(Zipf(1.5), 2, 200), seed 0; the actual nnz is printed),
 FP32 iterates and matrix values, FP64 checks.  A step is one PDHG iteration
-(the fused primal step over A^T and the fused dual step over A), with the
-KKT/restart check every 40 iterations exactly as the engine runs it.  Setup
-(upload, device Ruiz/Pock-Chambolle scaling, FP32 images, lane autotuning,
-power iteration) happens before the timed region; inputs are resident in HBM.
+(the fused primal step over A^T and the fused dual step over A).  The timed
+region is the engine's own loop (Pdhg::solve, fused HIP engine, through the
+C API): exactly --warmup untimed iterations, then exactly --steps timed ones,
+with the KKT/restart check every 40 iterations and the restarts as the engine
+runs them.  Setup (upload, device Ruiz/Pock-Chambolle scaling, FP32 images,
+autotuning, power iteration) happens before the timed region; inputs are
+resident in HBM.
 
-Prints one JSON line.  value = algorithmic HBM bytes of the SpMV kernels per
-iteration x iterations/s (GB/s); the roofline object gives the dominant
-kernel's achieved bandwidth measured with HIP events on the engine stream;
-cpu_baseline times the CPU oracle (single thread) on a bounded sample.
-config also carries the seconds to 1e-6 relative KKT of three solves (the
-2,000 x 5,000 LP, the 2,000 x 2,000 transportation LP and the 1M-column Zipf
-LP), the batched workload's LPs/s and the learned warm start's cold vs warm
-time to 1e-6.
+Prints one JSON line.  value = algorithmic HBM bytes of the two step kernels
+per iteration x iterations/s (GB/s); the roofline object gives the dominant
+kernel's achieved bandwidth, its duration measured with HIP events around
+each launch on the engine stream (FP32 and FP64 step kernels both reported
+in config); cpu_baseline times the reference's own host PDHG (oracle/_ref,
+compiled from the reference sources) pinned to one core, in a child process
+that runs beside the GPU work.  config also carries the seconds to 1e-6
+relative KKT of four solves (the 2,000 x 5,000 LP in FP64, the 2,000 x 2,000
+transportation LP, the 1M-column Zipf LP with FP32 iterates + FP64 polish and
+in FP64), the batched workload's LPs/s, the learned warm start's cold vs
+warm time to 1e-6 and the host baselines on the 2,000 x 5,000 LP.
 """
 import argparse
 import ctypes as C
@@ -39,8 +45,9 @@ def log(*a):
 
 
 class DeviceProblem:
-    """The scaled LP and the FP32 iterate state on the device (the same calls
-    the fused engine makes, src/l1/pdhg.cpp solve_fused)."""
+    """The scaled LP and an iterate state on the device, set up with the same
+    C-ABI calls the fused engine makes (src/l1/pdhg.cpp solve_fused), for
+    timing the step kernels one launch at a time."""
 
     def __init__(self, ctx, a, prec, segments=None, tiles=None):
         from sovereign_amd import device as D
@@ -117,11 +124,6 @@ class DeviceProblem:
         ca.orig_col_lo, ca.orig_col_hi, ca.orig_c = (ctx.upload(a[k][pc]) for k in ("col_lo", "col_hi", "c"))
         ca.act_avg = ctx.upload(np.zeros(m))
         self.ca = ca
-        self.since = 0
-        self.res_at_restart = float("inf")
-        self.iterations = 0
-        self.restarts = 0
-        self.last_check = None
         D.check(L.sov_hip_sync(ctx.h))
 
     def primal(self, first):
@@ -131,61 +133,21 @@ class DeviceProblem:
         self.D.check(self.ctx.L.sov_hip_pdhg_dual_step(self.ctx.h, self.A.h, C.byref(self.st), self.tau,
                                                        self.sigma, first))
 
-    def _interval_graph(self, fresh):
-        """The PERIOD-iteration interval captured once as a HIP graph (one for
-        an interval that starts a new Cesaro average, one that continues it),
-        replayed like the engine replays it."""
-        if not hasattr(self, "graphs"):
-            self.graphs = {}
-        if fresh not in self.graphs:
-            L = self.ctx.L
-            self.D.check(L.sov_hip_graph_begin(self.ctx.h))
-            for k in range(PERIOD):
-                first = 1 if (fresh and k == 0) else 0
-                self.primal(first)
-                self.dual(first)
-            g = C.c_void_p()
-            self.D.check(L.sov_hip_graph_end(self.ctx.h, C.byref(g)))
-            self.graphs[fresh] = g
-        return self.graphs[fresh]
-
-    def iterate(self, count):
-        """count iterations (a multiple of PERIOD) with the engine's
-        check/restart after every PERIOD."""
-        assert count % PERIOD == 0
-        for _ in range(count // PERIOD):
-            g = self._interval_graph(self.since == 0)
-            self.D.check(self.ctx.L.sov_hip_graph_launch(self.ctx.h, g))
-            self.since += PERIOD
-            self.iterations += PERIOD
-            self.check_and_restart()
-
-    def check_and_restart(self):
-        D, L = self.D, self.ctx.L
-        self.ca.since_restart = self.since
-        out = D.CheckOut()
-        D.check(L.sov_hip_pdhg_check(self.ctx.h, self.A.h, C.byref(self.ca), C.byref(out)))
-        self.last_check = out
-        cur, avg = out.kkt_cur, out.kkt_avg
-        if not np.isfinite(self.res_at_restart):
-            self.res_at_restart = cur
-        if avg <= 0.5 * self.res_at_restart:
-            D.check(L.sov_hip_pdhg_restart(self.ctx.h, C.byref(self.st), self.since, self.ca.act_avg, None, None,
-                                           None))
-            self.res_at_restart = avg
-            self.since = 0
-            self.restarts += 1
+    def close(self):
+        self.A.destroy()
 
 
-def spmv_bytes(m, n, nnz, sv, sx):
+def spmv_bytes(m, n, nnz, sv, sx, primal_tiled=False, dual_tiled=False):
     """Algorithmic bytes of the two fused SpMV kernels (each vector element
     counted once; gather misses not credited):
       primal (A^T y):  nnz (sv + 4) + 4 (n + 1) + 4 n (row list) + sx m (y)
                        + sx n * 8 (x, c, lo, hi, x_sum read; x, x_bar, x_sum written)
       dual   (A x_bar): nnz (sv + 4) + 4 (m + 1) + 4 m + sx n (x_bar)
-                       + sx m * 8 (s, y, lo, hi, y_sum read; s, y, y_sum written)"""
-    primal = nnz * (sv + 4) + 4 * (n + 1) + 4 * n + sx * m + 8 * sx * n
-    dual = nnz * (sv + 4) + 4 * (m + 1) + 4 * m + sx * n + 8 * sx * m
+                       + sx m * 8 (s, y, lo, hi, y_sum read; s, y, y_sum written)
+    The tiled kernels read a 4-byte key per entry and no row pointers, so a
+    tiled product is credited without the two row-pointer terms."""
+    primal = nnz * (sv + 4) + sx * m + 8 * sx * n + (0 if primal_tiled else 4 * (n + 1) + 4 * n)
+    dual = nnz * (sv + 4) + sx * n + 8 * sx * m + (0 if dual_tiled else 4 * (m + 1) + 4 * m)
     return primal, dual
 
 
@@ -233,14 +195,16 @@ def batched(S, count=256, time_limit=150.0, use_graphs=True):
 def time_to_tolerance(S):
     """Seconds to 1e-6 relative KKT through the engine's own entry point
     (Pdhg::solve, fused HIP engine, adaptive restarts, relative-KKT stop) on
-    three workloads: the 2,000 x 5,000 LP (5 nonzeros per column, seed 0) in
+    four workloads: the 2,000 x 5,000 LP (5 nonzeros per column, seed 0) in
     FP64; the 2,000 x 2,000 transportation LP (4M columns, seed 0) and the
-    1M-column Zipf LP (seed 0), each with FP32 iterates and the FP64 polish.
+    1M-column Zipf LP (seed 0), each with FP32 iterates and the FP64 polish;
+    and the 1M-column Zipf LP again in FP64 (the reference's precision).
     `seconds` is the whole solve (setup included: scaling, layouts,
     autotuning, power iteration); `loop_seconds` the iterations alone."""
     cases = [("lp_2000x5000_f64", lambda: S.LP.random(2000, 5000, 5, 0), "f64"),
              ("transport_2000x2000_f32", lambda: S.LP.transportation(2000, 2000, 0), "f32"),
-             ("zipf_500000x1000000_f32", lambda: S.LP.random_zipf(500000, 1000000, 0), "f32")]
+             ("zipf_500000x1000000_f32", lambda: S.LP.random_zipf(500000, 1000000, 0), "f32"),
+             ("zipf_500000x1000000_f64", lambda: S.LP.random_zipf(500000, 1000000, 0), "f64")]
     out = {}
     for name, make, prec in cases:
         lp = make()
@@ -277,130 +241,204 @@ def learned_warm_start(ctx, count=200, time_subset=24):
     return rep
 
 
-def cpu_baseline(a, budget_iters):
-    """CPU oracle (single thread) on the same instance: per-iteration time of
-    the reference loop measured as (t(setup + k iterations) - t(setup)) / k."""
+def cpu_child(out_path):
+    """The host baselines, in a child process pinned to one core (the
+    reference's host backend is single-threaded): the reference's own PDHG
+    (oracle/_ref, compiled from its sources) on
+      * the 1M-column Zipf LP: its loop's iteration rate from two checkpoints
+        40 iterations apart (setup, Problem::Builder included, reported
+        beside it), the solve then cancelled;
+      * the 2,000 x 5,000 LP, seeds 0-4, to the reference's 1e-4 tolerance,
+        and seed 0 to 1e-6 capped at 20 s.
+    Falls back to the oracle port (oracle/oracle.c) when oracle/_ref is not
+    built.  Writes one JSON object to out_path."""
+    import platform
+    import subprocess
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
+    import sovereign_amd as S
+    allowed = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {allowed[0]})
+    model = platform.processor()
+    try:
+        for ln in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    rep = {"cpu_model": model, "nproc": os.cpu_count(), "affinity": [allowed[0]],
+           "kind": "reference" if O.ref_available() else "port"}
+    a = S.LP.random_zipf(500000, 1000000, 0).arrays()
     M = O.Model(a)
     t0 = time.perf_counter()
-    O.pdhg(M, tolerance=1e-12, max_iterations=0)
-    t_setup = time.perf_counter() - t0
+    if O.ref_available():
+        stamps, its, t_build, status = O.ref_rate(M, 2)
+        per_it = (stamps[1] - stamps[0]) / (its[1] - its[0])
+        rep["zipf_500000x1000000"] = {
+            "seconds_per_iteration": per_it, "iterations_per_s": 1.0 / per_it, "builder_seconds": t_build,
+            "seconds_to_first_checkpoint": float(stamps[0]), "checkpoint_iterations": [int(v) for v in its],
+            "status": status, "wall_s": time.perf_counter() - t0,
+            "sample": "reference Pdhg::solve on the Host backend, default options; wall clock between the "
+                      "checkpoints at iterations %d and %d, then cancelled" % (its[0], its[1])}
+    else:
+        O.pdhg(M, tolerance=1e-12, max_iterations=0)
+        t_setup = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        O.pdhg(M, tolerance=1e-12, max_iterations=12)
+        per_it = max(time.perf_counter() - t1 - t_setup, 1e-9) / 12
+        rep["zipf_500000x1000000"] = {"seconds_per_iteration": per_it, "iterations_per_s": 1.0 / per_it,
+                                      "sample": "oracle port, 12 iterations after setup"}
+    del M, a
+    c0 = []
+    solve = O.ref_solve if O.ref_available() else O.pdhg
+    for seed in range(5):
+        Mc = O.Model(S.LP.random(2000, 5000, 5, seed).arrays())
+        t0 = time.perf_counter()
+        r = solve(Mc, tolerance=1e-4)
+        c0.append({"seed": seed, "tolerance": 1e-4, "status": r["status"], "iterations": r["iterations"],
+                   "restarts": r["restarts"], "seconds": round(time.perf_counter() - t0, 3),
+                   "objective": float(r["objective"])})
+    Mc = O.Model(S.LP.random(2000, 5000, 5, 0).arrays())
     t0 = time.perf_counter()
-    O.pdhg(M, tolerance=1e-12, max_iterations=budget_iters)
-    t_run = time.perf_counter() - t0
-    per_it = max(t_run - t_setup, 1e-9) / budget_iters
-    return per_it, t_setup
+    if O.ref_available():
+        r = O.ref_solve(Mc, tolerance=1e-6, time_limit=20.0, trace_cap=100000)
+        last = float(r["trace"][-1]) if len(r["trace"]) else None
+    else:
+        r = O.pdhg(Mc, tolerance=1e-6, max_iterations=150000)
+        last = None
+    c0.append({"seed": 0, "tolerance": 1e-6, "status": r["status"], "iterations": r["iterations"],
+               "restarts": r["restarts"], "seconds": round(time.perf_counter() - t0, 3),
+               "time_limit_s": 20.0, "residual_reached": last, "objective": float(r["objective"])})
+    rep["lp_2000x5000"] = c0
+    with open(out_path, "w") as f:
+        json.dump(rep, f)
+
+
+def kernel_timing(ctx, a, prec, iters=50):
+    """Average duration of each step kernel on the engine stream (HIP events
+    around every launch), at the layout the autotuner picks."""
+    from sovereign_amd import device as D
+    P = DeviceProblem(ctx, a, prec)
+    L = ctx.L
+    lanes_a, lanes_t = (C.c_int32 * 8)(), (C.c_int32 * 8)()
+    L.sov_hip_csr_get_lanes(P.A.h, 0, lanes_a)
+    L.sov_hip_csr_get_lanes(P.A.h, 1, lanes_t)
+    seg_a, seg_t = C.c_int(), C.c_int()
+    L.sov_hip_csr_get_segments(P.A.h, 0, C.byref(seg_a))
+    L.sov_hip_csr_get_segments(P.A.h, 1, C.byref(seg_t))
+    til_a, til_t = C.c_int(), C.c_int()
+    L.sov_hip_csr_get_tiles_prec(P.A.h, 0, prec, C.byref(til_a))
+    L.sov_hip_csr_get_tiles_prec(P.A.h, 1, prec, C.byref(til_t))
+    t = "float" if prec == 1 else "double"
+    kern = {"primal_step": f"k_tile_step<{t}, PrimalOp>" if til_t.value else f"k_step<PrimalOp<{t}>>",
+            "dual_step": f"k_tile_step<{t}, DualOp>" if til_a.value else f"k_step<DualOp<{t}>>"}
+    launches = {"primal_step": til_t.value or seg_t.value, "dual_step": til_a.value or seg_a.value}
+    for k in range(PERIOD):   # warm-up launches
+        P.primal(1 if k == 0 else 0)
+        P.dual(1 if k == 0 else 0)
+    ev = D.Events(ctx, 3 * iters + 1)
+    for k in range(iters):
+        ev.record(3 * k)
+        P.primal(0)
+        ev.record(3 * k + 1)
+        P.dual(0)
+        ev.record(3 * k + 2)
+    D.check(L.sov_hip_sync(ctx.h))
+    tp = sum(ev.elapsed(3 * k, 3 * k + 1) for k in range(iters)) / iters
+    td = sum(ev.elapsed(3 * k + 1, 3 * k + 2) for k in range(iters)) / iters
+    ev.destroy()
+    tiled = {"primal_step": bool(til_t.value), "dual_step": bool(til_a.value)}
+    s = 4 if prec == 1 else 8
+    bp, bd = spmv_bytes(a["m"], a["n"], a["nnz"], s, s, tiled["primal_step"], tiled["dual_step"])
+    layout = f"tiles={til_a.value},{til_t.value} segments={seg_a.value},{seg_t.value}"
+    log(f"{'FP32' if prec == 1 else 'FP64'} lanes A={list(lanes_a)} A^T={list(lanes_t)}; {layout}; primal step "
+        f"{tp * 1e3:.1f} us ({bp / tp / 1e6:.0f} GB/s), dual step {td * 1e3:.1f} us ({bd / td / 1e6:.0f} GB/s)")
+    P.close()
+    return {"primal_us": tp * 1e3, "dual_us": td * 1e3, "primal_bytes": bp, "dual_bytes": bd, "kernels": kern,
+            "launches": launches, "layout": layout, "tiled": tiled}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--m", type=int, default=500_000)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-LP batched solve")
     ap.add_argument("--batch-no-graphs", action="store_true",
-                    help="run the batched solve's check intervals as plain launches (kernel-trace runs: this "
-                         "rocprofv3 does not trace graph launches)")
+                    help="run the batched solve's check intervals as plain launches (kernel-trace runs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-1e-6 solves")
-    ap.add_argument("--segments", default=None,
-                    help="A,T column-segment counts to use instead of the autotuner's (profiling runs)")
-    ap.add_argument("--tiles", default=None,
-                    help="A,T step-kernel layouts instead of the autotuner's: 0 = CSR parts, k = tiles in k launches")
+    ap.add_argument("--no-engine-loop", action="store_true",
+                    help="skip the engine's timed loop (profiling runs of the step kernels alone)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with 3 launches of the f64->f32 convert kernel over 32 Mi elements (reads 256 MiB, "
                          "writes 128 MiB): the known-size reference for HBM byte counters")
     args = ap.parse_args()
-    args.steps = max(PERIOD, (args.steps + PERIOD - 1) // PERIOD * PERIOD)
-    args.warmup = (args.warmup + PERIOD - 1) // PERIOD * PERIOD
+    if args.cpu_child:
+        cpu_child(args.cpu_child)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is described "
               "in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
 
+    import subprocess
+    import tempfile
     import sovereign_amd as S
     from sovereign_amd import device as D
+
+    child, child_out = None, None
+    if not args.no_cpu_baseline:
+        # the host baselines run beside the GPU work, on a core of their own
+        child_out = os.path.join(tempfile.mkdtemp(prefix="sov_bench_"), "cpu.json")
+        child = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", child_out],
+                                 stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+        rest = sorted(os.sched_getaffinity(0))[1:]
+        if rest:
+            os.sched_setaffinity(0, set(rest))
 
     t0 = time.perf_counter()
     lp = S.LP.random_zipf(args.m, args.n, args.seed)
     a = lp.arrays()
     log(f"instance: m={a['m']} n={a['n']} nnz={a['nnz']} ({time.perf_counter() - t0:.1f}s to generate)")
     prec = 1 if args.precision == "f32" else 0
+    pname = "f32" if prec == 1 else "f64"
+
+    # the headline: the engine's own loop (Pdhg::solve, fused HIP engine),
+    # exactly `warmup` untimed iterations then exactly `steps` timed ones
+    eng = None
+    if not args.no_engine_loop:
+        r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=0.0, termination="relative",
+                    restarts="adaptive", precision=pname, fp64_polish=False, autotune_after=0.0,
+                    max_iterations=args.warmup + args.steps, timing_mark=args.warmup)
+        assert r.timed_iterations == args.steps, (r.timed_iterations, args.steps, r.status, r.message)
+        eng = {"iterations_per_s": args.steps / r.timed_seconds, "timed_seconds": r.timed_seconds,
+               "setup_seconds": r.setup_seconds, "restarts": r.restarts, "status": r.status,
+               "relative_kkt_at_end": max(r.relative_kkt)}
+        log(f"engine loop: {args.steps} iterations in {r.timed_seconds:.3f}s after {args.warmup} warm-up -> "
+            f"{eng['iterations_per_s']:.1f} it/s; setup {r.setup_seconds:.2f}s, {r.restarts} restarts")
+
     ctx = D.Ctx(0)
-    t0 = time.perf_counter()
-    segs = tuple(int(k) for k in args.segments.split(",")) if args.segments else None
-    tls = tuple(int(k) for k in args.tiles.split(",")) if args.tiles else None
-    P = DeviceProblem(ctx, a, prec, segs, tls)
-    log(f"setup: {time.perf_counter() - t0:.2f}s  op_norm={P.op_norm:.6g}")
-    lanes_a, lanes_t = (C.c_int32 * 8)(), (C.c_int32 * 8)()
-    ctx.L.sov_hip_csr_get_lanes(P.A.h, 0, lanes_a)
-    ctx.L.sov_hip_csr_get_lanes(P.A.h, 1, lanes_t)
-    seg_a, seg_t = C.c_int(), C.c_int()
-    ctx.L.sov_hip_csr_get_segments(P.A.h, 0, C.byref(seg_a))
-    ctx.L.sov_hip_csr_get_segments(P.A.h, 1, C.byref(seg_t))
-    til_a, til_t = C.c_int(), C.c_int()
-    ctx.L.sov_hip_csr_get_tiles_prec(P.A.h, 0, prec, C.byref(til_a))
-    ctx.L.sov_hip_csr_get_tiles_prec(P.A.h, 1, prec, C.byref(til_t))
-    log(f"tuned lanes A={list(lanes_a)} A^T={list(lanes_t)}; column segments A={seg_a.value} A^T={seg_t.value}; "
-        f"tiles A={til_a.value} A^T={til_t.value}")
-    layout = f"tiles={til_a.value},{til_t.value} segments={seg_a.value},{seg_t.value}"
-    kern = {"primal_step": f"k_tile_step<{'float' if prec else 'double'}, PrimalOp>" if til_t.value
-            else f"k_step<PrimalOp<{'float' if prec else 'double'}>>",
-            "dual_step": f"k_tile_step<{'float' if prec else 'double'}, DualOp>" if til_a.value
-            else f"k_step<DualOp<{'float' if prec else 'double'}>>"}
-    launches = {"primal_step": til_t.value or seg_t.value, "dual_step": til_a.value or seg_a.value}
-
-    P.iterate(args.warmup)
-    D.check(ctx.L.sov_hip_sync(ctx.h))
-    t0 = time.perf_counter()
-    P.iterate(args.steps)
-    D.check(ctx.L.sov_hip_sync(ctx.h))
-    elapsed = time.perf_counter() - t0
-    its = args.steps / elapsed
-    log(f"timed: {args.steps} iterations in {elapsed:.3f}s -> {its:.1f} it/s; restarts so far {P.restarts}")
-
-    # per-kernel durations with events on the engine stream
-    K = 50
-    ev = D.Events(ctx, 3 * K + 1)
-    for k in range(K):
-        first = 1 if P.since == 0 else 0
-        ev.record(3 * k)
-        P.primal(first)
-        ev.record(3 * k + 1)
-        P.dual(first)
-        ev.record(3 * k + 2)
-        P.since += 1
-        P.iterations += 1
-    D.check(ctx.L.sov_hip_sync(ctx.h))
-    tp = sum(ev.elapsed(3 * k, 3 * k + 1) for k in range(K)) / K
-    td = sum(ev.elapsed(3 * k + 1, 3 * k + 2) for k in range(K)) / K
-    ev.destroy()
-    sv = sx = 4 if prec == 1 else 8
-    bp, bd = spmv_bytes(a["m"], a["n"], a["nnz"], sv, sx)
-    log(f"primal step {tp * 1e3:.1f} us ({bp / tp / 1e6:.0f} GB/s), dual step {td * 1e3:.1f} us "
-        f"({bd / td / 1e6:.0f} GB/s)")
+    kt = {pname: kernel_timing(ctx, a, prec)}
+    other = "f64" if prec == 1 else "f32"
+    kt[other] = kernel_timing(ctx, a, 1 - prec)
+    k = kt[pname]
+    tp, td = k["primal_us"] * 1e-3, k["dual_us"] * 1e-3
+    bp, bd = k["primal_bytes"], k["dual_bytes"]
     dom_name, dom_t, dom_b = ("primal_step", tp, bp) if tp >= td else ("dual_step", td, bd)
-    traffic = committed_traffic(a["m"], a["n"], a["nnz"], "f32" if prec == 1 else "f64", layout)
+    traffic = committed_traffic(a["m"], a["n"], a["nnz"], pname, k["layout"])
     achieved = dom_b / dom_t / 1e6
+    its = eng["iterations_per_s"] if eng else 1e3 / (k["primal_us"] + k["dual_us"])
     value = (bp + bd) * its / 1e9
-    chk = P.last_check
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        per_it, t_setup = cpu_baseline(a, args.cpu_iters)
-        cpu = {"value": round((bp + bd) / per_it / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle (oracle/oracle.c, FP64, 1 thread) on the same 1M-column instance: "
-                         f"{args.cpu_iters} reference-loop iterations after setup ({t_setup:.1f}s), "
-                         f"{per_it * 1e3:.1f} ms/iteration = {1.0 / per_it:.3f} it/s; value uses the same "
-                         f"algorithmic bytes per iteration as the GPU line"}
-        log(f"cpu oracle: {per_it * 1e3:.1f} ms/iteration")
 
     ttt = None
     if not args.no_solve:
@@ -417,8 +455,24 @@ def main():
         lws = learned_warm_start(ctx)
         if lws:
             log(f"learned warm start: median iterations cold {lws['cold']['median_iterations']:.0f} vs learned "
-                f"{lws['learned']['median_iterations']:.0f}; single-solve median "
-                f"{lws['single_solve_median_seconds']['cold']:.3f}s vs {lws['single_solve_median_seconds']['learned']:.3f}s")
+                f"{lws['learned']['median_iterations']:.0f}")
+
+    cpu, host = None, None
+    if child is not None:
+        _, err = child.communicate(timeout=900)
+        if child.returncode == 0 and os.path.exists(child_out):
+            host = json.load(open(child_out))
+            z = host["zipf_500000x1000000"]
+            # the CPU computes in FP64: its bytes are the FP64 algorithmic bytes
+            b64 = kt["f64"]["primal_bytes"] + kt["f64"]["dual_bytes"]
+            cpu = {"value": round(b64 / z["seconds_per_iteration"] / 1e9, 4), "unit": "GB/s", "cores": 1,
+                   "kind": host["kind"], "iterations_per_s": round(z["iterations_per_s"], 4),
+                   "cpu_model": host["cpu_model"], "nproc": host["nproc"],
+                   "sample": z["sample"] + "; pinned to one core (sched_setaffinity, as taskset -c); value = FP64 "
+                                           "algorithmic bytes per iteration x iterations/s"}
+            log(f"cpu baseline ({host['kind']}): {z['seconds_per_iteration']:.3f} s/iteration on {host['cpu_model']}")
+        else:
+            log(f"cpu baseline child failed ({child.returncode}): {err[-2000:] if err else ''}")
 
     out = {
         "metric": "sec to 1e-6 rel KKT (cold vs learned warm start); batched LPs/s; SpMV HBM GB/s",
@@ -431,25 +485,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec == 1 else "f64",
+        "dtype": pname,
         "data": "synthetic (seeded generator)",
-        "config": {"workload": f"PDHG iteration on the Zipf LP m={a['m']} n={a['n']} nnz={a['nnz']} "
-                               f"(clip(Zipf(1.5),2,200) nonzeros per column, seed {args.seed}); "
+        "config": {"workload": f"PDHG iteration of the fused engine on the Zipf LP m={a['m']} n={a['n']} "
+                               f"nnz={a['nnz']} (clip(Zipf(1.5),2,200) nonzeros per column, seed {args.seed}); "
                                f"{'FP32' if prec == 1 else 'FP64'} iterates, FP64 checks every {PERIOD}",
-                   "iterations_per_s": round(its, 2), "restarts": P.restarts,
-                   "column_segments": {"A": seg_a.value, "A^T": seg_t.value},
-                   "tiles": {"A": til_a.value, "A^T": til_t.value},
-                   "primal_step_us": round(tp * 1e3, 2), "dual_step_us": round(td * 1e3, 2),
-                   "kkt_scaled_at_last_check": chk.kkt_cur if chk else None,
+                   "engine_loop": eng,
+                   "step_kernels_us": {p: {"primal": round(v["primal_us"], 2), "dual": round(v["dual_us"], 2),
+                                           "layout": v["layout"],
+                                           "primal_frac": round(v["primal_bytes"] / v["primal_us"] / 1e3 /
+                                                                HBM_PEAK_GBS, 4),
+                                           "dual_frac": round(v["dual_bytes"] / v["dual_us"] / 1e3 /
+                                                              HBM_PEAK_GBS, 4)}
+                                       for p, v in kt.items()},
                    "time_to_1e-6_relative_kkt": ttt,
                    "batched": bat,
-                   "learned_warm_start": lws},
-        "roofline": {"bound": "hbm", "kernel": f"{dom_name}: {kern[dom_name]}", "achieved": round(achieved, 1),
+                   "learned_warm_start": lws,
+                   "host_baselines": host},
+        "roofline": {"bound": "hbm", "kernel": f"{dom_name}: {k['kernels'][dom_name]}", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic[dom_name] if traffic else None,
                      "algorithmic_bytes_per_step": int(dom_b),
-                     "launches_per_step": launches[dom_name]},
+                     "launches_per_step": k["launches"][dom_name]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

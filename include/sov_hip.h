@@ -15,6 +15,10 @@
  * Threading: a context owns one HIP stream and its own scratch; use one
  * context per host thread.  Every call is ordered on that stream; calls that
  * return host scalars synchronise that stream only (never the device).
+ * A handle (csr, batch, mlp) remembers the stream of the context that
+ * created it; its destroy waits for that stream only, so it must run before
+ * that context is destroyed.  (The runtime's own hipFree may still wait for
+ * other streams' queued work: that delays the freeing thread, never theirs.)
  *
  * Device pointers are those returned by sov_hip_malloc (or any allocation of
  * the same device).  "Vector" arguments whose element type depends on a

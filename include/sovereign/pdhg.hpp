@@ -136,6 +136,12 @@ struct PdhgOptions {
     double autotune_after_seconds = 0.5;
     /// Device ordinal for a fused solve that creates its own backend context.
     int device = 0;
+    /// Measurement mark (fused engine): when >= 0, the loop waits for the
+    /// device once this many iterations are done (splitting the interval that
+    /// holds the mark into two plain launch sequences) and
+    /// PdhgResult::timed_seconds / timed_iterations then cover the rest of
+    /// the loop, checks and restarts included, up to the device's completion.
+    long long timing_mark_iteration = -1;
 };
 
 struct PdhgResult {
@@ -164,6 +170,8 @@ struct PdhgResult {
     PrimalRay ray;                         ///< set when DualInfeasible
     double setup_seconds = 0.0;            ///< scaling, upload, operator norm
     double loop_seconds = 0.0;             ///< the iteration loop alone
+    double timed_seconds = 0.0;            ///< from PdhgOptions::timing_mark_iteration to the loop's end
+    long long timed_iterations = 0;        ///< iterations inside timed_seconds
 };
 
 class Pdhg {

@@ -84,6 +84,7 @@ typedef struct sov_pdhg_opts {
     const double* warm_start_primal; /* NULL or num_cols values (original units) */
     int32_t tiles;                   /* LDS-staged step kernels: -1 autotuner decides, 0 never, k >= 1 always (k launches) */
     double autotune_after_seconds;   /* the autotuner runs at the first check after this much loop time (0: before it) */
+    int64_t timing_mark_iteration;   /* -1, or the iteration the timed part of the loop starts at (timed_seconds) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {
@@ -105,6 +106,8 @@ typedef struct sov_pdhg_result {
     double primal_weight;
     int64_t fp64_polish_iterations;
     int64_t trace_len;
+    double timed_seconds;            /* PdhgResult::timed_seconds */
+    int64_t timed_iterations;        /* PdhgResult::timed_iterations */
 } sov_pdhg_result;
 
 void sov_pdhg_default_opts(sov_pdhg_opts* o);

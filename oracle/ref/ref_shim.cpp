@@ -7,12 +7,14 @@
 // factors (src/l1/scaling.cpp:19), and the LP/MPS readers
 // (src/io/lp_reader.cpp:498, src/io/mps_reader.cpp:566) so the hand-written
 // test LPs of tests/test_pdhg.cpp can be turned into CSR fixtures.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "sovereign/backend.hpp"
+#include "sovereign/cancellation.hpp"
 #include "sovereign/io.hpp"
 #include "sovereign/pdhg.hpp"
 #include "sovereign/problem.hpp"
@@ -86,6 +88,42 @@ REF_API int ref_pdhg_solve(int32_t m, int32_t n, const int32_t* rp, const int32_
         stats[5] = static_cast<double>(r.restarts);
         stats[6] = r.seconds;
         stats[7] = r.operator_norm_estimate;
+        return static_cast<int>(r.status);
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Iteration rate of the reference's own loop on the Host backend (the CPU
+// baseline): Pdhg::solve with the default options (check period 40), the
+// wall clock stamped at each checkpoint (seconds since the solve started)
+// with its iteration, and the solve cancelled through its CancellationToken
+// once `checks` checkpoints have been stamped.  build_seconds receives the
+// time Problem::Builder took; *stamped the checkpoints recorded.
+REF_API int ref_pdhg_rate(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
+                          const double* c, const double* clo, const double* chi, const double* rlo,
+                          const double* rhi, int32_t maximize, double obj_const, int64_t checks,
+                          double* stamps, int64_t* iters, int64_t* stamped, double* build_seconds) {
+    using clock = std::chrono::steady_clock;
+    try {
+        const auto tb = clock::now();
+        const Problem p = build(m, n, rp, ci, v, c, clo, chi, rlo, rhi, maximize, obj_const);
+        *build_seconds = std::chrono::duration<double>(clock::now() - tb).count();
+        CancellationToken stop;
+        PdhgOptions opt;
+        int64_t k = 0;
+        const auto t0 = clock::now();
+        opt.checkpoint = [&](long long it, const std::vector<Real>&, const std::vector<Real>&, Real) {
+            if (k < checks) {
+                stamps[k] = std::chrono::duration<double>(clock::now() - t0).count();
+                iters[k] = it;
+                ++k;
+            }
+            if (k >= checks) stop.cancel();
+        };
+        auto be = make_backend(BackendKind::Host);
+        const PdhgResult r = Pdhg(opt).solve(p, *be, &stop);
+        *stamped = k;
         return static_cast<int>(r.status);
     } catch (...) {
         return -1;

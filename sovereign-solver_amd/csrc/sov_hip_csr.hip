@@ -767,6 +767,7 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
     h->n = n;
     h->nnz = nnz;
     h->device = ctx->device;
+    h->stream = ctx->stream;
     auto cleanup = [&](int rc) {
         sov_hip_csr_destroy(h);
         return rc;
@@ -842,7 +843,8 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
 void sov_hip_csr_destroy(sov_hip_csr* h) {
     if (h == nullptr) return;
     DeviceGuard2 g(h->device);
-    (void)hipDeviceSynchronize();
+    // Only the creating context's stream can still be using these buffers.
+    if (h->stream != nullptr) (void)hipStreamSynchronize(h->stream);
     for (Orient* o : {&h->a, &h->t}) {
         o->tiles.free();
         o->tiles64.free();

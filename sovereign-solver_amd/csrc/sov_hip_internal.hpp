@@ -241,6 +241,7 @@ struct sov_hip_csr {
     void* vy = nullptr;
     int64_t partial_blocks = 0;
     int device = 0;
+    hipStream_t stream = nullptr;   // the creating context's stream (destroy waits for it only)
 };
 
 // A block-diagonal batch of LPs stacked in one sov_hip_csr: instance b owns
@@ -253,6 +254,7 @@ struct sov_hip_csr {
 struct sov_hip_batch {
     int32_t count = 0, m = 0, n = 0;
     int device = 0;
+    hipStream_t stream = nullptr;   // the creating context's stream (destroy waits for it only)
     int32_t* row_inst = nullptr;    // m, device order
     int32_t* col_inst = nullptr;    // n, device order
     int32_t* seg = nullptr;         // count * 2 * kClasses * 2

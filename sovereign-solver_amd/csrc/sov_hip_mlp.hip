@@ -37,6 +37,7 @@ struct sov_hip_mlp {
     double* red = nullptr;              // loss partials
     int last_batch = 0;                 // batch of the last forward pass
     int device = 0;
+    hipStream_t stream = nullptr;       // the creating context's stream (destroy waits for it only)
 };
 
 namespace sovhip {
@@ -241,6 +242,7 @@ int sov_hip_mlp_create(sov_hip_ctx* ctx, int32_t layers, const int32_t* widths, 
     h->max_batch = max_batch;
     h->max_w = *std::max_element(h->w.begin(), h->w.end());
     h->device = ctx->device;
+    h->stream = ctx->stream;
     h->woff.assign(layers + 1, 0);
     h->boff.assign(layers + 1, 0);
     int64_t off = 0;
@@ -284,8 +286,13 @@ int sov_hip_mlp_create(sov_hip_ctx* ctx, int32_t layers, const int32_t* widths, 
 
 void sov_hip_mlp_destroy(sov_hip_mlp* h) {
     if (h == nullptr) return;
-    (void)hipDeviceSynchronize();
+    int prev = -1;
+    if (hipGetDevice(&prev) == hipSuccess && prev != h->device) (void)hipSetDevice(h->device);
+    else prev = -1;
+    // Only the creating context's stream can still be using these buffers.
+    if (h->stream != nullptr) (void)hipStreamSynchronize(h->stream);
     free_mlp(h);
+    if (prev >= 0) (void)hipSetDevice(prev);
 }
 
 int64_t sov_hip_mlp_num_params(const sov_hip_mlp* h) { return h ? h->nparams : -1; }

@@ -1186,6 +1186,7 @@ int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t count, 
     bt->m = h->m;
     bt->n = h->n;
     bt->device = ctx->device;
+    bt->stream = ctx->stream;
     bt->csr = h;
     const size_t m1 = std::max<int32_t>(h->m, 1), n1 = std::max<int32_t>(h->n, 1);
     hipError_t e = hipMalloc(&bt->row_inst, sizeof(int32_t) * m1);
@@ -1233,7 +1234,8 @@ int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t count, 
 void sov_hip_batch_destroy(sov_hip_batch* bt) {
     if (bt == nullptr) return;
     DeviceGuard3 g(bt->device);
-    (void)hipDeviceSynchronize();
+    // Only the creating context's stream can still be using these buffers.
+    if (bt->stream != nullptr) (void)hipStreamSynchronize(bt->stream);
     free_batch(bt);
 }
 

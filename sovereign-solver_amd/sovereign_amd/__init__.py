@@ -41,7 +41,7 @@ class PdhgOpts(C.Structure):
         ("restart_scheme", C.c_int32), ("autotune", C.c_int32), ("restart_beta_sufficient", C.c_double),
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
         ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p), ("tiles", C.c_int32),
-        ("autotune_after_seconds", C.c_double),
+        ("autotune_after_seconds", C.c_double), ("timing_mark_iteration", C.c_int64),
     ]
 
 
@@ -52,7 +52,8 @@ class PdhgRes(C.Structure):
         ("iterations", C.c_int64), ("restarts", C.c_int64), ("seconds", C.c_double), ("setup_seconds", C.c_double),
         ("loop_seconds", C.c_double), ("operator_norm_estimate", C.c_double), ("relative_primal", C.c_double),
         ("relative_dual", C.c_double), ("relative_gap", C.c_double), ("primal_weight", C.c_double),
-        ("fp64_polish_iterations", C.c_int64), ("trace_len", C.c_int64),
+        ("fp64_polish_iterations", C.c_int64), ("trace_len", C.c_int64), ("timed_seconds", C.c_double),
+        ("timed_iterations", C.c_int64),
     ]
 
 
@@ -274,6 +275,8 @@ class Result:
     trace_it: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
     farkas: np.ndarray = None     # row multipliers when status == "primal_infeasible"
     ray: np.ndarray = None        # primal ray when status == "dual_infeasible"
+    timed_seconds: float = 0.0    # from the timing mark to the end of the loop
+    timed_iterations: int = 0
 
 
 _ENGINES = {"auto": 0, "primitive": 1, "fused": 2}
@@ -284,12 +287,14 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
           restarts="reference", autotune=True, detect_infeasibility=False, primal_weight_smoothing=None,
-          warm_start_primal=None, tiles=None, autotune_after=None) -> Result:
+          warm_start_primal=None, tiles=None, autotune_after=None, timing_mark=None) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point.
     tiles: None = the autotuner decides, 0 = CSR step kernels only, k >= 1 =
     the LDS-staged tiled step kernels in k launches per product.
     autotune_after: seconds of iterations before the autotuner runs (None:
-    the engine's default; 0: before the first iteration)."""
+    the engine's default; 0: before the first iteration).
+    timing_mark: the iteration from which Result.timed_seconds measures the
+    loop (None: no timed part)."""
     L = engine()
     o = PdhgOpts()
     L.sov_pdhg_default_opts(C.byref(o))
@@ -311,6 +316,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
         o.tiles = int(tiles)
     if autotune_after is not None:
         o.autotune_after_seconds = float(autotune_after)
+    if timing_mark is not None:
+        o.timing_mark_iteration = int(timing_mark)
     o.detect_infeasibility = int(detect_infeasibility)
     if primal_weight_smoothing is not None:
         o.primal_weight_smoothing = primal_weight_smoothing
@@ -351,7 +358,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
                   r.operator_norm_estimate, (r.relative_primal, r.relative_dual, r.relative_gap), r.primal_weight,
                   int(r.fp64_polish_iterations), x[:n], y[:m], rc[:n], msg.value.decode(), tr[:nt], ti[:nt],
                   farkas[:m] if STATUS[r.status] == "primal_infeasible" else None,
-                  ray[:n] if STATUS[r.status] == "dual_infeasible" else None)
+                  ray[:n] if STATUS[r.status] == "dual_infeasible" else None,
+                  r.timed_seconds, int(r.timed_iterations))
 
 
 def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, restart_check_period=40,

@@ -191,6 +191,7 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->autotune = d.autotune ? 1 : 0;
     o->tiles = d.tiles;
     o->autotune_after_seconds = d.autotune_after_seconds;
+    o->timing_mark_iteration = d.timing_mark_iteration;
     o->restart_beta_sufficient = d.restart_beta_sufficient;
     o->restart_beta_necessary = d.restart_beta_necessary;
     o->restart_beta_artificial = d.restart_beta_artificial;
@@ -224,6 +225,7 @@ sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
     p.autotune = o->autotune != 0;
     p.tiles = o->tiles;
     p.autotune_after_seconds = o->autotune_after_seconds;
+    p.timing_mark_iteration = o->timing_mark_iteration;
     p.restart_scheme = o->restart_scheme == 1 ? sov::PdhgRestartScheme::Adaptive : sov::PdhgRestartScheme::Reference;
     p.restart_beta_sufficient = o->restart_beta_sufficient;
     p.restart_beta_necessary = o->restart_beta_necessary;
@@ -249,6 +251,8 @@ void fill_result(const sov::PdhgResult& r, sov_pdhg_result* res) {
     res->relative_gap = r.relative_gap;
     res->primal_weight = r.primal_weight;
     res->fp64_polish_iterations = r.fp64_polish_iterations;
+    res->timed_seconds = r.timed_seconds;
+    res->timed_iterations = r.timed_iterations;
 }
 
 }  // namespace

@@ -19,7 +19,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <system_error>
 #include <thread>
+
+#include <sched.h>
 #include <utility>
 #include <vector>
 
@@ -594,26 +597,54 @@ struct CsrHandle {
     ~CsrHandle() { sov_hip_csr_destroy(h); }
 };
 
-/// Host copy of the handle's device order (sov_hip_csr_order): every vector
-/// the fused kernels touch is stored with element p = natural index perm[p].
-/// f(begin, end) over [0, n) in contiguous chunks on up to 16 host threads
-/// (one below 2^18 elements).  For element-wise work: every element is
-/// computed as in a serial loop, so results do not depend on the split.
+/// Host threads this process may run on: the CPUs of its affinity mask (so a
+/// run pinned to one core gets one thread), at most 16.
+unsigned host_threads() {
+    static const unsigned count = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        int c = 0;
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) c = CPU_COUNT(&set);
+        if (c <= 0) c = static_cast<int>(std::thread::hardware_concurrency());
+        return static_cast<unsigned>(std::clamp(c, 1, 16));
+    }();
+    return count;
+}
+
+/// f(begin, end) over [0, n) in contiguous chunks on up to host_threads()
+/// threads (one below 2^18 elements).  For element-wise work: every element
+/// is computed as in a serial loop, so results do not depend on the split.
+/// A thread that cannot be started leaves its chunk to the calling thread;
+/// every started thread is joined before this returns (also when f throws).
 template <class F>
 void parallel_for(std::size_t n, F f) {
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const std::size_t parts = n < (std::size_t{1} << 18) ? 1 : hw;
+    const std::size_t parts = n < (std::size_t{1} << 18) ? 1 : host_threads();
     if (parts <= 1) {
         f(std::size_t{0}, n);
         return;
     }
-    std::vector<std::thread> pool;
-    pool.reserve(parts - 1);
-    for (std::size_t k = 1; k < parts; ++k) pool.emplace_back(f, n * k / parts, n * (k + 1) / parts);
+    struct Joiner {
+        std::vector<std::thread> pool;
+        ~Joiner() {
+            for (auto& t : pool)
+                if (t.joinable()) t.join();
+        }
+    } j;
+    j.pool.reserve(parts - 1);
+    std::size_t k = 1;
+    for (; k < parts; ++k) {
+        try {
+            j.pool.emplace_back(f, n * k / parts, n * (k + 1) / parts);
+        } catch (const std::system_error&) {
+            break;   // no more threads: the caller takes the rest
+        }
+    }
     f(std::size_t{0}, n / parts);
-    for (auto& t : pool) t.join();
+    for (; k < parts; ++k) f(n * k / parts, n * (k + 1) / parts);
 }
 
+/// Host copy of the handle's device order (sov_hip_csr_order): every vector
+/// the fused kernels touch is stored with element p = natural index perm[p].
 struct DeviceOrder {
     std::vector<int32_t> row, col;
     static std::vector<Real> to_dev(std::span<const Real> v, const std::vector<int32_t>& perm) {
@@ -968,6 +999,8 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         long long polish_start = -1;
         Real best_rel = kInfinity;
         int stall = 0;
+        bool timing = false;
+        auto t_timed = Clock::now();
 
         result.iterations = 0;
         while (result.iterations < o.max_iterations) {
@@ -979,7 +1012,24 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             const long long next_check = (result.iterations / period + 1) * period;
             const long long count = std::min(next_check, static_cast<long long>(o.max_iterations)) - result.iterations;
             const Real tau = eta / omega, sigma = eta * omega;
-            run_interval(count, since == 0, tau, sigma);
+            const long long mark = o.timing_mark_iteration;
+            if (mark >= 0 && mark == result.iterations && !timing) {
+                hip_ok(sov_hip_sync(ctx), "sync");
+                t_timed = Clock::now();
+                timing = true;
+            }
+            if (mark > result.iterations && mark < result.iterations + count) {
+                // the mark falls inside this interval: run it as two plain
+                // launch sequences with the clock started between them
+                const long long head = mark - result.iterations;
+                run_steps(head, since == 0, tau, sigma);
+                hip_ok(sov_hip_sync(ctx), "sync");
+                t_timed = Clock::now();
+                timing = true;
+                run_steps(count - head, false, tau, sigma);
+            } else {
+                run_interval(count, since == 0, tau, sigma);
+            }
             since += count;
             result.iterations += count;
             if (result.iterations != next_check) break;   // iteration limit inside the interval
@@ -1115,6 +1165,10 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         }
         hip_ok(sov_hip_sync(ctx), "sync");
         result.loop_seconds = seconds_since(t_loop);
+        if (timing) {
+            result.timed_seconds = seconds_since(t_timed);
+            result.timed_iterations = result.iterations - o.timing_mark_iteration;
+        }
         if (best_x.empty()) {
             best_x = S.download(ctx, S.x, n);
             best_y = S.download(ctx, S.y, m);

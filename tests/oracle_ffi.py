@@ -96,6 +96,8 @@ def ref() -> C.CDLL:
                                         C.c_int32, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_double, _f64p,
                                         _f64p, _f64p, _f64p, C.c_int64, _f64p, _i64p, _i64p]
         _ref.ref_scaling.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, C.c_int32, C.c_int32, _f64p, _f64p]
+        _ref.ref_pdhg_rate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                       C.c_int32, C.c_double, C.c_int64, _f64p, _i64p, _i64p, _f64p]
         _ref.ref_parse.argtypes = [C.c_char_p, C.c_int32, _i32p, _i32p, _i32p]
         _ref.ref_parsed_get.argtypes = [_i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _i32p, _f64p]
     return _ref
@@ -262,6 +264,24 @@ def ref_solve(M: Model, tolerance=1e-4, max_iterations=1000000, period=40, time_
     return dict(status=STATUS[st], objective=stats[0], primal_infeasibility=stats[1], dual_infeasibility=stats[2],
                 duality_gap=stats[3], iterations=int(stats[4]), restarts=int(stats[5]), seconds=stats[6],
                 operator_norm=stats[7], x=x[: M.n], y=y[: M.m], rc=rc[: M.n], trace=tr[:nt], trace_it=ti[:nt])
+
+
+def ref_rate(M: Model, checks=2):
+    """The reference's own PDHG loop on the Host backend, cancelled after
+    `checks` checkpoints: (checkpoint wall-clock stamps since the solve
+    started, their iterations, Problem::Builder seconds, status)."""
+    L = ref()
+    stamps, its = np.zeros(checks), np.zeros(checks, np.int64)
+    k, tb = C.c_int64(0), C.c_double(0.0)
+    st = L.ref_pdhg_rate(M.m, M.n, _p(M.row_ptr, C.c_int32), _p(M.col_idx, C.c_int32), _p(M.val, C.c_double),
+                         _p(M.c, C.c_double), _p(M.col_lo, C.c_double), _p(M.col_hi, C.c_double),
+                         _p(M.row_lo, C.c_double), _p(M.row_hi, C.c_double), int(M.maximize),
+                         C.c_double(M.obj_const), C.c_int64(checks), _p(stamps, C.c_double), _p(its, C.c_int64),
+                         C.byref(k), C.byref(tb))
+    if st < 0:
+        raise RuntimeError("reference solve threw")
+    n = int(k.value)
+    return stamps[:n], its[:n], tb.value, STATUS[st]
 
 
 def ref_scaling(M: Model, ruiz=True, pc=True):
