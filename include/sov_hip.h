@@ -312,6 +312,19 @@ typedef struct sov_hip_check_out {
 int sov_hip_pdhg_check(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_check_args* a,
                        sov_hip_check_out* out);
 
+/* The same check without any allocation.  No reference counterpart: it is
+ * for callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels, summation order and
+ * results as sov_hip_pdhg_check; its block partials live in a static device
+ * array of the library (8 MiB per GPU, kept until the process ends) instead
+ * of the handle's and context's buffers.  A check that needs more than 65536
+ * blocks of partials returns SOV_INVALID_ARGUMENT (use sov_hip_pdhg_check).
+ * All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (from two contexts of that device) race and must be serialised by the
+ * caller. */
+int sov_hip_pdhg_check_static(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_check_args* a,
+                              sov_hip_check_out* out);
+
 /* Restart to the average (pdhg.cpp:393-404): x = clamp(x_sum/k, col box),
  * y = y_sum/k, s = clamp(act_avg, row box).  If x_last/y_last are not NULL
  * (FP64, n and m), movement[0] = ||x_new - x_last||_2^2 and

@@ -208,6 +208,11 @@ int sov_hip_ctx_create(int device, sov_hip_ctx** out) {
         sov_hip_ctx_destroy(ctx);
         return rc;
     }
+    const int src = static_scratch_address(device, &ctx->check_scratch);
+    if (src != SOV_OK) {
+        sov_hip_ctx_destroy(ctx);
+        return src;
+    }
     *out = ctx;
     return SOV_OK;
 }

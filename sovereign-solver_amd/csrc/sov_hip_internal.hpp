@@ -29,6 +29,8 @@ constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;
 constexpr int kClasses = 8;          // row-length classes of a schedule
 constexpr int kCheckSlots = 16;      // FP64 partials per block in the check kernels
+// Blocks of check partials the allocation-free check's static scratch holds.
+constexpr int64_t kStaticCheckBlocks = 65536;
 
 // Row-length class boundaries: class c holds rows with length <= kClassMax[c].
 constexpr int kClassMax[kClasses] = {2, 4, 8, 16, 32, 64, 256, 0x7fffffff};
@@ -223,6 +225,9 @@ struct sov_hip_ctx {
     double* h_pinned = nullptr;   // 64 doubles
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool capturing = false;
+    // device address of the allocation-free check's static scratch on this
+    // context's device (sovhip::static_scratch_address)
+    double* check_scratch = nullptr;
     int ensure_red(size_t count);
 };
 
@@ -507,5 +512,8 @@ hipError_t check_launch();         // hipGetLastError wrapper
 int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32);
 int ensure_partials(sov_hip_csr* csr);
 int inclusive_scan(sov_hip_ctx* ctx, int32_t* a, int64_t len);   // in place, synchronous
+// Device address of the allocation-free check's static scratch on `device`
+// (which must be current); looked up once per device.
+int static_scratch_address(int device, double** addr);
 
 }  // namespace sovhip

@@ -7,6 +7,8 @@
 // the host restart with its H2D uploads (:393-404).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "sov_hip_internal.hpp"
@@ -552,8 +554,49 @@ static const T* vals_of(const Part& p) {
     return std::is_same<T, double>::value ? reinterpret_cast<const T*>(p.val64) : reinterpret_cast<const T*>(p.val32);
 }
 
+// Block partials the check writes: one set per block of the launch that
+// completes the rows of each orientation (its last part), plus one spare each.
+static int64_t check_blocks(const sov_hip_csr* h) {
+    return (int64_t)h->a.part(h->a.nparts() - 1).blocks() + (int64_t)h->t.part(h->t.nparts() - 1).blocks() + 2;
+}
+
+// The allocation-free check (sov_hip_pdhg_check_static) keeps its block
+// partials and its 22 reduced values in this file-scope device array, one
+// copy per GPU, held until the process ends: kStaticCheckBlocks blocks of
+// kCheckSlots doubles (8 MiB) and 64 doubles after them.  Host code must not
+// pass `scratch` itself to a kernel (on the host it names a stand-in, not
+// device memory): it uses the address hipGetSymbolAddress gives for the
+// current device, taken once per device by static_scratch_address() when a
+// context is created on it.
+static __device__ __align__(256) double scratch[kStaticCheckBlocks * kCheckSlots + 64];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxScratchDevices] = {};
+}  // namespace
+
+int static_scratch_address(int device, double** addr) {
+    *addr = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(SOV_UNSUPPORTED, "static scratch: device ordinal " + std::to_string(device) + " out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device] == nullptr) {
+        // the caller has made `device` current
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_addr[device] = static_cast<double*>(p);
+    }
+    *addr = g_scratch_addr[device];
+    return SOV_OK;
+}
+
+// partials: the block partials of both orientations, (rb + cb + 2) *
+// kCheckSlots doubles, checked by the caller; red: 22 doubles.
 template <typename T>
-static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a, sov_hip_check_out* out) {
+static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a, sov_hip_check_out* out,
+                      double* partials, double* red) {
     const sov_hip_pdhg_state* st = a->st;
     const double inv = 1.0 / static_cast<double>(a->since_restart > 0 ? a->since_restart : 1);
     const Orient& OA = h->a;
@@ -564,11 +607,8 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
     for (const Orient* o : {&OA, &OT})
         for (const Part& p : o->parts)
             if (p.nnz > 0 && vals_of<T>(p) == nullptr) return fail(SOV_INVALID_ARGUMENT, "check: value image missing");
-    double* rowpart = h->partials;
-    double* colpart = h->partials + (int64_t)(rb + 1) * kCheckSlots;
-    if ((int64_t)(rb + cb + 2) > h->partial_blocks) return fail(SOV_INVALID_ARGUMENT, "check: partials too small");
-    int rc = ctx->ensure_red(64);
-    if (rc != SOV_OK) return rc;
+    double* rowpart = partials;
+    double* colpart = partials + (int64_t)(rb + 1) * kCheckSlots;
     for (int k = 0; k < npa; ++k) {
         const Part& p = OA.part(k);
         const Sched s = p.sched();
@@ -601,9 +641,9 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
         op.mode = seg_mode(k, npt); op.s = s;
         k_check<CheckColOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
-    k_check_final<<<22, kBlock, 0, ctx->stream>>>(rowpart, rb, colpart, cb, ctx->d_red);
+    k_check_final<<<22, kBlock, 0, ctx->stream>>>(rowpart, rb, colpart, cb, red);
     SOV_HIP_TRY(check_launch());
-    SOV_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, ctx->d_red, 22 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SOV_HIP_TRY(hipMemcpyAsync(ctx->h_pinned, red, 22 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
     finish_check(ctx->h_pinned, a->obj_sign, a->obj_const, out);
     return SOV_OK;
@@ -1421,7 +1461,35 @@ int sov_hip_pdhg_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_che
         return fail(SOV_INVALID_ARGUMENT, "check: null original-space array");
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "check during capture");
     DeviceGuard3 g(ctx->device);
-    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out) : check_impl<float>(ctx, h, a, out);
+    if (check_blocks(h) > h->partial_blocks) return fail(SOV_INVALID_ARGUMENT, "check: partials too small");
+    rc = ctx->ensure_red(64);
+    if (rc != SOV_OK) return rc;
+    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out, h->partials, ctx->d_red)
+                                       : check_impl<float>(ctx, h, a, out, h->partials, ctx->d_red);
+}
+
+int sov_hip_pdhg_check_static(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a,
+                              sov_hip_check_out* out) {
+    if (ctx == nullptr || h == nullptr || a == nullptr || out == nullptr)
+        return fail(SOV_INVALID_ARGUMENT, "check_static args");
+    int rc = state_ok(h, a->st);
+    if (rc != SOV_OK) return rc;
+    if ((h->m > 0 && (!a->row_scale || !a->orig_row_lo || !a->orig_row_hi)) ||
+        (h->n > 0 && (!a->col_scale || !a->orig_col_lo || !a->orig_col_hi || !a->orig_c)))
+        return fail(SOV_INVALID_ARGUMENT, "check_static: null original-space array");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "check_static during capture");
+    if (ctx->check_scratch == nullptr)
+        return fail(SOV_BACKEND_ERROR, "check_static: the context has no static scratch address");
+    const int64_t need = check_blocks(h);
+    if (need > kStaticCheckBlocks)
+        return fail(SOV_INVALID_ARGUMENT, "check_static: the check needs " + std::to_string(need) +
+                                              " blocks of partials; the static scratch holds " +
+                                              std::to_string(kStaticCheckBlocks) + " (use sov_hip_pdhg_check)");
+    DeviceGuard3 g(ctx->device);
+    double* partials = ctx->check_scratch;
+    double* red = ctx->check_scratch + kStaticCheckBlocks * kCheckSlots;
+    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out, partials, red)
+                                       : check_impl<float>(ctx, h, a, out, partials, red);
 }
 
 int sov_hip_pdhg_restart(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, int64_t since, const double* act,

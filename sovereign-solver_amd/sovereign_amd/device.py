@@ -68,6 +68,7 @@ def lib():
     L.sov_hip_pdhg_primal_step.argtypes = [_v, _v, _v, d, C.c_int]
     L.sov_hip_pdhg_dual_step.argtypes = [_v, _v, _v, d, d, C.c_int]
     L.sov_hip_pdhg_check.argtypes = [_v, _v, _v, _v]
+    L.sov_hip_pdhg_check_static.argtypes = [_v, _v, _v, _v]
     L.sov_hip_pdhg_restart.argtypes = [_v, _v, C.c_int64, _v, _v, _v, C.POINTER(d)]
     L.sov_hip_pdhg_set_point.argtypes = [_v, _v, _v, _v, _v]
     L.sov_hip_graph_begin.argtypes = [_v]
