@@ -116,6 +116,23 @@ BackendKind default_backend_kind();
 std::unique_ptr<Backend> make_backend(BackendKind kind);
 std::string backend_report();
 
+namespace detail {
+/// One device allocation and the backend that owns it, released once.
+struct Block {
+    Backend* backend = nullptr;
+    void* ptr = nullptr;
+    std::size_t bytes = 0;
+    Block() = default;
+    Block(Backend& backend, std::size_t bytes, const char* what);
+    ~Block() { reset(); }
+    Block(const Block&) = delete;
+    Block& operator=(const Block&) = delete;
+    Block(Block&& other) noexcept;
+    Block& operator=(Block&& other) noexcept;
+    void reset() noexcept;
+};
+}  // namespace detail
+
 class DeviceBuffer {
 public:
     DeviceBuffer() = default;
@@ -130,16 +147,14 @@ public:
     Status download(std::span<Real> host_dst) const;
     std::vector<Real> to_host() const;
 
-    Real* data() noexcept { return ptr_; }
-    const Real* data() const noexcept { return ptr_; }
+    Real* data() noexcept { return static_cast<Real*>(mem_.ptr); }
+    const Real* data() const noexcept { return static_cast<const Real*>(mem_.ptr); }
     std::size_t size() const noexcept { return count_; }
     bool empty() const noexcept { return count_ == 0; }
-    Backend* backend() const noexcept { return backend_; }
+    Backend* backend() const noexcept { return mem_.backend; }
 
 private:
-    void release() noexcept;
-    Backend* backend_ = nullptr;
-    Real* ptr_ = nullptr;
+    detail::Block mem_;
     std::size_t count_ = 0;
 };
 
@@ -155,22 +170,20 @@ public:
     /// Validates, then exactly three host->device copies.
     static DeviceCsr upload(Backend& backend, const CsrMatrix& host_matrix);
 
-    CsrView view() const noexcept { return CsrView{rows_, cols_, nnz_, row_ptr_, col_idx_, values_}; }
+    CsrView view() const noexcept {
+        return CsrView{rows_, cols_, nnz_, static_cast<const Idx*>(ptr_.ptr), static_cast<const Idx*>(idx_.ptr),
+                       static_cast<const Real*>(val_.ptr)};
+    }
     Idx rows() const noexcept { return rows_; }
     Idx cols() const noexcept { return cols_; }
     Idx nnz() const noexcept { return nnz_; }
-    std::size_t device_bytes() const noexcept { return bytes_; }
-    bool empty() const noexcept { return values_ == nullptr; }
-    Backend* backend() const noexcept { return backend_; }
+    std::size_t device_bytes() const noexcept { return ptr_.bytes + idx_.bytes + val_.bytes; }
+    bool empty() const noexcept { return val_.ptr == nullptr; }
+    Backend* backend() const noexcept { return ptr_.backend; }
 
 private:
-    void release() noexcept;
-    Backend* backend_ = nullptr;
     Idx rows_ = 0, cols_ = 0, nnz_ = 0;
-    Idx* row_ptr_ = nullptr;
-    Idx* col_idx_ = nullptr;
-    Real* values_ = nullptr;
-    std::size_t bytes_ = 0;
+    detail::Block ptr_, idx_, val_;   // row pointers, column indices, values
 };
 
 }  // namespace sov

@@ -113,19 +113,26 @@ public:
     Real column_lower(Idx col) const { return col_lower_.at(static_cast<std::size_t>(col)); }
     Real column_upper(Idx col) const { return col_upper_.at(static_cast<std::size_t>(col)); }
     VarKind column_kind(Idx col) const { return col_kind_.at(static_cast<std::size_t>(col)); }
-    Idx num_rows() const noexcept { return static_cast<Idx>(row_names_.size()); }
-    Idx num_columns() const noexcept { return static_cast<Idx>(col_names_.size()); }
+    Idx num_rows() const noexcept { return rows_.size(); }
+    Idx num_columns() const noexcept { return cols_.size(); }
     Problem finish();
 
 private:
+    /// Names in insertion order plus their lookup; a repeated name throws.
+    struct NameTable {
+        std::vector<std::string> names;
+        std::unordered_map<std::string, Idx> index;
+        Idx size() const noexcept { return static_cast<Idx>(names.size()); }
+        Idx insert(std::string name, const char* what);
+        Idx find(const std::string& name) const;
+    };
     std::string name_, objective_name_;
     ObjSense sense_ = ObjSense::Minimize;
     Real objective_constant_ = 0.0;
-    std::vector<std::string> row_names_, col_names_;
+    NameTable rows_, cols_;
     std::vector<Real> row_lower_, row_upper_, col_lower_, col_upper_, objective_;
     std::vector<VarKind> col_kind_;
     std::vector<Triplet> entries_;
-    std::unordered_map<std::string, Idx> row_index_, col_index_;
 };
 
 }  // namespace sov
