@@ -76,6 +76,24 @@ int sov_hip_memcpy_d2d(sov_hip_ctx* ctx, void* dst, const void* src, size_t byte
 int sov_hip_fill_zero(sov_hip_ctx* ctx, void* dst, size_t bytes);
 int sov_hip_sync(sov_hip_ctx* ctx);
 
+/* ---- asynchronous device -> host copies ---------------------------------
+ * No reference counterpart (the reference's checkpoint hand-off copies
+ * synchronously, pdhg.cpp:336-345 / :373-381).  A copier owns a second
+ * stream of default priority: sov_hip_copier_d2h queues a copy into host
+ * memory that starts once everything queued so far on the context stream
+ * is done and returns at once, so the context stream goes on with later
+ * work (which must not write `src` until the copy is done: copy from a
+ * snapshot); sov_hip_copier_wait blocks until the copier's copies are done.
+ * dst should be pinned host memory (sov_hip_host_alloc) for the copy to run
+ * asynchronously.  A copier is used by one thread. */
+typedef struct sov_hip_copier sov_hip_copier;
+int sov_hip_copier_create(sov_hip_ctx* ctx, sov_hip_copier** out);
+void sov_hip_copier_destroy(sov_hip_copier* c);
+int sov_hip_copier_d2h(sov_hip_ctx* ctx, sov_hip_copier* c, void* dst, const void* src, size_t bytes);
+int sov_hip_copier_wait(sov_hip_copier* c);
+int sov_hip_host_alloc(sov_hip_ctx* ctx, size_t bytes, void** out);
+int sov_hip_host_free(sov_hip_ctx* ctx, void* ptr);
+
 /* ---- level 1, FP64 ----------------------------------------------------
  * Backend::axpy/scale/dot/norm2/max_abs (backend.hpp:111-115; reference
  * rocBLAS calls backend_hip.cpp:126-146, Unsupported max_abs :169-175).

@@ -142,6 +142,12 @@ struct PdhgOptions {
     /// PdhgResult::timed_seconds / timed_iterations then cover the rest of
     /// the loop, checks and restarts included, up to the device's completion.
     long long timing_mark_iteration = -1;
+    /// Fused engine: hand checkpoints over without stalling the device.  The
+    /// candidate is snapshot on the device at its check and copied to pinned
+    /// host memory while the next interval runs; the callback is called
+    /// after that interval has been launched (one interval late, in check
+    /// order, with the same arguments).  false = call it at the check.
+    bool checkpoint_async = true;
 };
 
 struct PdhgResult {

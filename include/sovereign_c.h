@@ -85,6 +85,13 @@ typedef struct sov_pdhg_opts {
     int32_t tiles;                   /* LDS-staged step kernels: -1 autotuner decides, 0 never, k >= 1 always (k launches) */
     double autotune_after_seconds;   /* the autotuner runs at the first check after this much loop time (0: before it) */
     int64_t timing_mark_iteration;   /* -1, or the iteration the timed part of the loop starts at (timed_seconds) */
+    /* checkpoint callback (PdhgOptions::checkpoint): NULL, or called at every
+     * check with the original-space candidate (x: n values, y: m values) and
+     * its original-space residual; the arrays are valid during the call */
+    void (*checkpoint_fn)(void* user, int64_t iteration, const double* x, int32_t n, const double* y, int32_t m,
+                          double residual);
+    void* checkpoint_user;
+    int32_t checkpoint_async;        /* PdhgOptions::checkpoint_async (default 1) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

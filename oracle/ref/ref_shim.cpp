@@ -130,6 +130,40 @@ REF_API int ref_pdhg_rate(int32_t m, int32_t n, const int32_t* rp, const int32_t
     }
 }
 
+// The reference's checkpoint hand-off (pdhg.hpp:121-133, called at
+// pdhg.cpp:373-381): the first `cap` checkpoints of a default-options solve,
+// each its iteration, original-space residual and unscaled candidate (x: n,
+// y: m values per checkpoint, row-major in xs / ys); *count = checkpoints
+// the solve made.  Returns the status.
+REF_API int ref_pdhg_checkpoints(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
+                                 const double* c, const double* clo, const double* chi, const double* rlo,
+                                 const double* rhi, int32_t maximize, double obj_const, double tol,
+                                 int64_t max_it, int64_t cap, int64_t* its, double* res, double* xs, double* ys,
+                                 int64_t* count) {
+    try {
+        const Problem p = build(m, n, rp, ci, v, c, clo, chi, rlo, rhi, maximize, obj_const);
+        PdhgOptions opt;
+        opt.tolerance = tol;
+        opt.max_iterations = max_it;
+        int64_t k = 0;
+        opt.checkpoint = [&](long long it, const std::vector<Real>& x, const std::vector<Real>& y, Real r) {
+            if (k < cap) {
+                its[k] = it;
+                res[k] = r;
+                std::memcpy(xs + k * n, x.data(), sizeof(double) * static_cast<std::size_t>(n));
+                std::memcpy(ys + k * m, y.data(), sizeof(double) * static_cast<std::size_t>(m));
+            }
+            ++k;
+        };
+        auto be = make_backend(BackendKind::Host);
+        const PdhgResult r = Pdhg(opt).solve(p, *be);
+        *count = k;
+        return static_cast<int>(r.status);
+    } catch (...) {
+        return -1;
+    }
+}
+
 REF_API int ref_scaling(int32_t m, int32_t n, const int32_t* rp, const int32_t* ci, const double* v,
                 int32_t apply_ruiz, int32_t apply_pc, double* rs, double* cs) {
     try {

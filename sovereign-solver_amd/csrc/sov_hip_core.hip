@@ -442,6 +442,78 @@ int sov_hip_timer_stop(sov_hip_ctx* ctx, double* ms) {
     return SOV_OK;
 }
 
+struct sov_hip_copier {
+    int device = 0;
+    hipStream_t stream = nullptr;   // default priority, non-blocking
+    hipEvent_t ready = nullptr;     // recorded on the context stream
+    hipEvent_t done = nullptr;      // recorded on the copier stream
+};
+
+int sov_hip_copier_create(sov_hip_ctx* ctx, sov_hip_copier** out) {
+    if (ctx == nullptr || out == nullptr) return fail(SOV_INVALID_ARGUMENT, "copier_create args");
+    *out = nullptr;
+    DeviceGuard g(ctx->device);
+    auto* c = new (std::nothrow) sov_hip_copier();
+    if (c == nullptr) return fail(SOV_OUT_OF_MEMORY, "copier");
+    c->device = ctx->device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        sov_hip_copier_destroy(c);
+        return hip_fail(e, "sov_hip_copier_create");
+    }
+    *out = c;
+    return SOV_OK;
+}
+
+void sov_hip_copier_destroy(sov_hip_copier* c) {
+    if (c == nullptr) return;
+    DeviceGuard g(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ready) (void)hipEventDestroy(c->ready);
+    if (c->done) (void)hipEventDestroy(c->done);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sov_hip_copier_d2h(sov_hip_ctx* ctx, sov_hip_copier* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return SOV_OK;
+    if (ctx == nullptr || c == nullptr || dst == nullptr || src == nullptr)
+        return fail(SOV_INVALID_ARGUMENT, "copier_d2h args");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "copier_d2h during capture");
+    DeviceGuard g(ctx->device);
+    // after everything already queued on the context stream
+    SOV_HIP_TRY(hipEventRecord(c->ready, ctx->stream));
+    SOV_HIP_TRY(hipStreamWaitEvent(c->stream, c->ready, 0));
+    SOV_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SOV_HIP_TRY(hipEventRecord(c->done, c->stream));
+    return SOV_OK;
+}
+
+int sov_hip_copier_wait(sov_hip_copier* c) {
+    if (c == nullptr) return fail(SOV_INVALID_ARGUMENT, "copier_wait args");
+    SOV_HIP_TRY(hipEventSynchronize(c->done));
+    return SOV_OK;
+}
+
+int sov_hip_host_alloc(sov_hip_ctx* ctx, size_t bytes, void** out) {
+    if (ctx == nullptr || out == nullptr) return fail(SOV_INVALID_ARGUMENT, "host_alloc args");
+    *out = nullptr;
+    if (bytes == 0) return SOV_OK;
+    DeviceGuard g(ctx->device);
+    SOV_HIP_TRY(hipHostMalloc(out, bytes));
+    return SOV_OK;
+}
+
+int sov_hip_host_free(sov_hip_ctx* ctx, void* p) {
+    if (p == nullptr) return SOV_OK;
+    if (ctx == nullptr) return fail(SOV_INVALID_ARGUMENT, "host_free without ctx");
+    DeviceGuard g(ctx->device);
+    SOV_HIP_TRY(hipHostFree(p));
+    return SOV_OK;
+}
+
 struct sov_hip_events {
     std::vector<hipEvent_t> ev;
 };

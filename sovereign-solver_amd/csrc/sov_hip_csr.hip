@@ -11,6 +11,8 @@
 #include <cstring>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "sov_hip_internal.hpp"
 #include "sov_hip_tiles.hpp"
 
@@ -167,33 +169,29 @@ __device__ __forceinline__ int32_t find_row(const int32_t* __restrict__ ptr, int
     return lo;
 }
 
-__global__ void k_fill_transpose(int32_t m, int32_t nnz, const int32_t* __restrict__ rp,
-                                 const int32_t* __restrict__ ci, int32_t* __restrict__ cursor,
-                                 int32_t* __restrict__ trow, int32_t* __restrict__ tsrc) {
+// The transpose as one sort: entry k of A gets the key (column << rbits) |
+// row, unique because a row's columns are distinct, so sorting the keys
+// lists A^T's rows in order with each one's entries by ascending A row --
+// the reference's scatter order -- whatever the column lengths (the sort is
+// linear in nnz).  The value carried along is k, the entry's position in A.
+__global__ void k_transpose_keys(int32_t m, int32_t nnz, const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ ci, int rbits, uint64_t* __restrict__ keys,
+                                 int32_t* __restrict__ pos) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int32_t r = find_row(rp, m, (int32_t)k);
-        const int32_t pos = atomicAdd(&cursor[ci[k]], 1);
-        trow[pos] = r;
-        tsrc[pos] = (int32_t)k;
+        keys[k] = ((uint64_t)(uint32_t)ci[k] << rbits) | (uint64_t)(uint32_t)r;
+        pos[k] = (int32_t)k;
     }
 }
 
-// Deterministic placement: the entries of one A^T row have distinct A-row
-// indices, so each entry's rank among them is its final slot.
-__global__ void k_rank_transpose(int32_t n, int32_t nnz, const int32_t* __restrict__ tptr,
-                                 const int32_t* __restrict__ trow_u, const int32_t* __restrict__ tsrc_u,
-                                 int32_t* __restrict__ tidx, int32_t* __restrict__ tsrc) {
+// Sorted keys -> A^T column indices (the A rows).
+__global__ void k_transpose_rows(int32_t nnz, int rbits, const uint64_t* __restrict__ keys,
+                                 int32_t* __restrict__ tidx) {
+    const uint64_t mask = ((uint64_t)1 << rbits) - 1;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t j = find_row(tptr, n, (int32_t)p);
-        const int32_t b = tptr[j], e = tptr[j + 1];
-        const int32_t me = trow_u[p];
-        int32_t rank = 0;
-        for (int32_t q = b; q < e; ++q) rank += (trow_u[q] < me) ? 1 : 0;
-        tidx[b + rank] = me;
-        tsrc[b + rank] = tsrc_u[p];
-    }
+         p += (int64_t)gridDim.x * blockDim.x)
+        tidx[p] = (int32_t)(keys[p] & mask);
 }
 
 __global__ void k_gather_vals(int32_t nnz, const int32_t* __restrict__ src, const double* __restrict__ v,
@@ -752,6 +750,45 @@ using namespace sovhip;
 
 extern "C" {
 
+// A^T's column indices (tidx, the A rows) and each entry's position in A
+// (tsrc), in A^T row order with ascending A rows inside a row: a stable
+// radix sort of the (column << rbits | row) keys over the bits they use.
+// Temporary device memory: 2 x 12 bytes per entry plus the sort's storage.
+static int sort_transpose(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, const int32_t* d_rp,
+                          const int32_t* d_ci, int32_t* tidx, int32_t* tsrc) {
+    int rbits = 1, cbits = 1;
+    while (rbits < 31 && (1LL << rbits) < m) ++rbits;
+    while (cbits < 31 && (1LL << cbits) < n) ++cbits;
+    uint64_t *keys_in = nullptr, *keys_out = nullptr;
+    int32_t* pos_in = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    hipError_t e = hipMalloc(&keys_in, sizeof(uint64_t) * nnz);
+    if (e == hipSuccess) e = hipMalloc(&keys_out, sizeof(uint64_t) * nnz);
+    if (e == hipSuccess) e = hipMalloc(&pos_in, sizeof(int32_t) * nnz);
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs(nullptr, temp_bytes, keys_in, keys_out, pos_in, tsrc, (size_t)nnz, 0,
+                                      (unsigned)(rbits + cbits), ctx->stream);
+    if (e == hipSuccess) e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 16));
+    if (e == hipSuccess) {
+        k_transpose_keys<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(m, nnz, d_rp, d_ci, rbits, keys_in, pos_in);
+        e = check_launch();
+    }
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, pos_in, tsrc, (size_t)nnz, 0,
+                                      (unsigned)(rbits + cbits), ctx->stream);
+    if (e == hipSuccess) {
+        k_transpose_rows<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, rbits, keys_out, tidx);
+        e = check_launch();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(keys_in);
+    (void)hipFree(keys_out);
+    (void)hipFree(pos_in);
+    (void)hipFree(temp);
+    return e == hipSuccess ? SOV_OK : hip_fail(e, "csr transpose sort");
+}
+
 int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, const int32_t* d_rp,
                        const int32_t* d_ci, const double* d_val, int flags, sov_hip_csr** out) {
     if (ctx == nullptr || out == nullptr) return fail(SOV_INVALID_ARGUMENT, "csr_create args");
@@ -787,35 +824,23 @@ int sov_hip_csr_create(sov_hip_ctx* ctx, int32_t m, int32_t n, int32_t nnz, cons
     } else {
         A.val64 = const_cast<double*>(d_val);
     }
-    // natural A^T, built on the device
-    int32_t *cursor = nullptr, *trow_u = nullptr, *tsrc_u = nullptr;
+    // natural A^T, built on the device: row pointers from a column count and
+    // a scan, entries from one radix sort of (column, row) keys
     hipError_t e = hipMalloc(&h->t_ptr, sizeof(int32_t) * ((int64_t)n + 1));
     if (e == hipSuccess) e = hipMalloc(&h->t_idx, sizeof(int32_t) * nz);
     if (e == hipSuccess) e = hipMalloc(&h->t_src, sizeof(int32_t) * nz);
     if (e == hipSuccess) e = hipMalloc(&h->t_val64, sizeof(double) * nz);
-    if (e == hipSuccess) e = hipMalloc(&cursor, sizeof(int32_t) * ((int64_t)n + 1));
-    if (e == hipSuccess) e = hipMalloc(&trow_u, sizeof(int32_t) * nz);
-    if (e == hipSuccess) e = hipMalloc(&tsrc_u, sizeof(int32_t) * nz);
-    if (e != hipSuccess) {
-        (void)hipFree(cursor); (void)hipFree(trow_u); (void)hipFree(tsrc_u);
-        return cleanup(hip_fail(e, "csr transpose buffers"));
-    }
+    if (e != hipSuccess) return cleanup(hip_fail(e, "csr transpose buffers"));
     k_zero_i32<<<launch_blocks((int64_t)n + 1), kBlock, 0, ctx->stream>>>((int64_t)n + 1, h->t_ptr);
     if (nnz > 0) k_count_cols<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, d_ci, h->t_ptr);
     int rc = inclusive_scan(ctx, h->t_ptr, (int64_t)n + 1);
+    if (rc == SOV_OK && nnz > 0) rc = sort_transpose(ctx, m, n, nnz, d_rp, d_ci, h->t_idx, h->t_src);
     if (rc == SOV_OK && nnz > 0) {
-        k_copy_i32<<<launch_blocks(n), kBlock, 0, ctx->stream>>>(n, h->t_ptr, cursor);
-        k_fill_transpose<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(m, nnz, d_rp, d_ci, cursor, trow_u, tsrc_u);
-        k_rank_transpose<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(n, nnz, h->t_ptr, trow_u, tsrc_u,
-                                                                          h->t_idx, h->t_src);
         k_gather_vals<<<launch_blocks(nnz), kBlock, 0, ctx->stream>>>(nnz, h->t_src, A.val64, h->t_val64);
         e = check_launch();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(e, "csr transpose");
     }
-    (void)hipFree(cursor);
-    (void)hipFree(trow_u);
-    (void)hipFree(tsrc_u);
     if (rc != SOV_OK) return cleanup(rc);
     Natural& T = h->t.nat;
     T.rows = n;

@@ -23,6 +23,8 @@ LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
 STATUS = ["optimal", "iteration_limit", "time_limit", "numerical_failure", "not_solved",
           "cancelled", "primal_infeasible", "dual_infeasible"]
 
+CHECKPOINT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_int32,
+                            C.POINTER(C.c_double), C.c_int32, C.c_double)
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _f64p = C.POINTER(C.c_double)
@@ -42,6 +44,7 @@ class PdhgOpts(C.Structure):
         ("restart_beta_necessary", C.c_double), ("restart_beta_artificial", C.c_double),
         ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p), ("tiles", C.c_int32),
         ("autotune_after_seconds", C.c_double), ("timing_mark_iteration", C.c_int64),
+        ("checkpoint_fn", CHECKPOINT_FN), ("checkpoint_user", C.c_void_p), ("checkpoint_async", C.c_int32),
     ]
 
 
@@ -287,14 +290,18 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           fp64_polish=True, primal_weight=False, initial_primal_weight=1.0, warm_start_dual=None,
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
           restarts="reference", autotune=True, detect_infeasibility=False, primal_weight_smoothing=None,
-          warm_start_primal=None, tiles=None, autotune_after=None, timing_mark=None) -> Result:
+          warm_start_primal=None, tiles=None, autotune_after=None, timing_mark=None, checkpoint=None,
+          checkpoint_async=True) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point.
     tiles: None = the autotuner decides, 0 = CSR step kernels only, k >= 1 =
     the LDS-staged tiled step kernels in k launches per product.
     autotune_after: seconds of iterations before the autotuner runs (None:
     the engine's default; 0: before the first iteration).
     timing_mark: the iteration from which Result.timed_seconds measures the
-    loop (None: no timed part)."""
+    loop (None: no timed part).
+    checkpoint: None, or f(iteration, x, y, residual) called at every check
+    with copies of the original-space candidate (PdhgOptions::checkpoint);
+    checkpoint_async: hand checkpoints over without stalling the device."""
     L = engine()
     o = PdhgOpts()
     L.sov_pdhg_default_opts(C.byref(o))
@@ -318,6 +325,14 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
         o.autotune_after_seconds = float(autotune_after)
     if timing_mark is not None:
         o.timing_mark_iteration = int(timing_mark)
+    o.checkpoint_async = int(bool(checkpoint_async))
+    callback = None   # kept alive for the whole native call
+    if checkpoint is not None:
+        def _fn(user, it, xp, nx, yp, ny, res):
+            checkpoint(int(it), np.ctypeslib.as_array(xp, (nx,)).copy() if nx else np.zeros(0),
+                       np.ctypeslib.as_array(yp, (ny,)).copy() if ny else np.zeros(0), float(res))
+        callback = CHECKPOINT_FN(_fn)
+        o.checkpoint_fn = callback
     o.detect_infeasibility = int(detect_infeasibility)
     if primal_weight_smoothing is not None:
         o.primal_weight_smoothing = primal_weight_smoothing

@@ -192,6 +192,7 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->tiles = d.tiles;
     o->autotune_after_seconds = d.autotune_after_seconds;
     o->timing_mark_iteration = d.timing_mark_iteration;
+    o->checkpoint_async = d.checkpoint_async ? 1 : 0;
     o->restart_beta_sufficient = d.restart_beta_sufficient;
     o->restart_beta_necessary = d.restart_beta_necessary;
     o->restart_beta_artificial = d.restart_beta_artificial;
@@ -226,6 +227,7 @@ sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
     p.tiles = o->tiles;
     p.autotune_after_seconds = o->autotune_after_seconds;
     p.timing_mark_iteration = o->timing_mark_iteration;
+    p.checkpoint_async = o->checkpoint_async != 0;
     p.restart_scheme = o->restart_scheme == 1 ? sov::PdhgRestartScheme::Adaptive : sov::PdhgRestartScheme::Reference;
     p.restart_beta_sufficient = o->restart_beta_sufficient;
     p.restart_beta_necessary = o->restart_beta_necessary;
@@ -269,13 +271,19 @@ int sov_pdhg_solve(const sov_lp* lp, int32_t backend, const sov_pdhg_opts* o, do
         if (o->warm_start_primal != nullptr)
             p.warm_start_primal.assign(o->warm_start_primal, o->warm_start_primal + lp->p.num_cols());
         int64_t nt = 0;
-        if (o->trace_res != nullptr && o->trace_cap > 0) {
-            p.checkpoint = [&](long long it, const std::vector<double>&, const std::vector<double>&, double r) {
-                if (nt < o->trace_cap) {
-                    o->trace_res[nt] = r;
-                    if (o->trace_it) o->trace_it[nt] = it;
+        const bool tracing = o->trace_res != nullptr && o->trace_cap > 0;
+        if (tracing || o->checkpoint_fn != nullptr) {
+            p.checkpoint = [&](long long it, const std::vector<double>& x, const std::vector<double>& y, double r) {
+                if (tracing) {
+                    if (nt < o->trace_cap) {
+                        o->trace_res[nt] = r;
+                        if (o->trace_it) o->trace_it[nt] = it;
+                    }
+                    ++nt;
                 }
-                ++nt;
+                if (o->checkpoint_fn != nullptr)
+                    o->checkpoint_fn(o->checkpoint_user, it, x.data(), static_cast<int32_t>(x.size()), y.data(),
+                                     static_cast<int32_t>(y.size()), r);
             };
         }
         const sov::BackendKind kind = backend == SOV_BACKEND_HIP ? sov::BackendKind::Hip : sov::BackendKind::Host;

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <system_error>
 #include <thread>
 
@@ -728,6 +729,85 @@ struct DeviceState {
     }
 };
 
+/// The fused engine's checkpoint hand-off (PdhgOptions::checkpoint, reference
+/// pdhg.hpp:121-133, called at every check with the unscaled candidate).
+/// post() snapshots the candidate on the device (ordered before every later
+/// kernel) and queues its copy to pinned host memory on a copier stream, so
+/// the next interval starts at once; drain(), called right after that
+/// interval has been launched, waits for the copy, converts and unscales on
+/// the host and calls the callback while the device computes.  Callbacks
+/// come in check order, one interval late, with the values the synchronous
+/// hand-off would give.
+class CheckpointPipe {
+public:
+    CheckpointPipe(sov_hip_ctx* ctx, std::size_t m, std::size_t n) : ctx_(ctx), m_(m), n_(n) {
+        hip_ok(sov_hip_copier_create(ctx, &copier_), "copier");
+        const std::size_t bytes = std::max<std::size_t>(m + n, 1) * 8;
+        staging_ = DevMem(ctx, bytes);
+        hip_ok(sov_hip_host_alloc(ctx, bytes, &host_), "pinned checkpoint buffer");
+    }
+    ~CheckpointPipe() {
+        sov_hip_copier_destroy(copier_);
+        (void)sov_hip_host_free(ctx_, host_);
+    }
+    CheckpointPipe(const CheckpointPipe&) = delete;
+    CheckpointPipe& operator=(const CheckpointPipe&) = delete;
+
+    /// The candidate of this check: the average (x_sum, y_sum) * inv when
+    /// avg, else (x, y), of the state S.
+    void post(const DeviceState& S, bool avg, Real inv, long long iteration, Real residual) {
+        const std::size_t es = S.es;
+        char* dst = staging_.as<char>();
+        if (n_) hip_ok(sov_hip_memcpy_d2d(ctx_, dst, (avg ? S.xsum : S.x).p, n_ * es), "checkpoint snapshot");
+        if (m_) hip_ok(sov_hip_memcpy_d2d(ctx_, dst + n_ * es, (avg ? S.ysum : S.y).p, m_ * es), "checkpoint snapshot");
+        hip_ok(sov_hip_copier_d2h(ctx_, copier_, host_, dst, (n_ + m_) * es), "checkpoint copy");
+        pending_ = true;
+        prec_ = S.prec;
+        scale_ = avg ? inv : Real{1};
+        iteration_ = iteration;
+        residual_ = residual;
+    }
+
+    /// Calls f(iteration, x, y, residual) for the pending checkpoint, with x
+    /// and y in device order and scaled space, as DeviceState::download
+    /// would give them.
+    template <class F>
+    void drain(F&& f) {
+        if (!pending_) return;
+        pending_ = false;
+        hip_ok(sov_hip_copier_wait(copier_), "checkpoint copy");
+        std::vector<Real> x(n_), y(m_);
+        auto widen = [&](std::vector<Real>& out, std::size_t offset) {
+            parallel_for(out.size(), [&](std::size_t b, std::size_t e) {
+                if (prec_ == SOV_PREC_F64) {
+                    const double* src = static_cast<const double*>(host_) + offset;
+                    for (std::size_t i = b; i < e; ++i) out[i] = src[i];
+                } else {
+                    const float* src = static_cast<const float*>(host_) + offset;
+                    for (std::size_t i = b; i < e; ++i) out[i] = static_cast<Real>(src[i]);
+                }
+                if (scale_ != 1.0)
+                    for (std::size_t i = b; i < e; ++i) out[i] *= scale_;
+            });
+        };
+        widen(x, 0);
+        widen(y, n_);
+        f(iteration_, x, y, residual_);
+    }
+
+private:
+    sov_hip_ctx* ctx_;
+    std::size_t m_, n_;
+    sov_hip_copier* copier_ = nullptr;
+    DevMem staging_;
+    void* host_ = nullptr;
+    bool pending_ = false;
+    int prec_ = SOV_PREC_F64;
+    Real scale_ = 1.0;
+    long long iteration_ = 0;
+    Real residual_ = 0.0;
+};
+
 }  // namespace
 
 PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const CancellationToken* cancel) const {
@@ -1001,6 +1081,14 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         int stall = 0;
         bool timing = false;
         auto t_timed = Clock::now();
+        // the checkpoint callback: unscale a device-order, scaled candidate
+        auto deliver = [&](long long it, const std::vector<Real>& cx, const std::vector<Real>& cy, Real res) {
+            auto [ux, uy] = unscale_point(sign, scaling, DeviceOrder::to_nat(cx, ord.col),
+                                          DeviceOrder::to_nat(cy, ord.row));
+            o.checkpoint(it, ux, uy, res);
+        };
+        std::unique_ptr<CheckpointPipe> pipe;
+        if (o.checkpoint && o.checkpoint_async) pipe = std::make_unique<CheckpointPipe>(ctx, m, n);
 
         result.iterations = 0;
         while (result.iterations < o.max_iterations) {
@@ -1030,6 +1118,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             } else {
                 run_interval(count, since == 0, tau, sigma);
             }
+            if (pipe) pipe->drain(deliver);   // the previous check's callback, while this interval runs
             since += count;
             result.iterations += count;
             if (result.iterations != next_check) break;   // iteration limit inside the interval
@@ -1075,11 +1164,11 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 p.second = avg_better ? S.download(ctx, S.ysum, m, inv) : S.download(ctx, S.y, m);
                 return p;
             };
-            if (o.checkpoint) {
+            if (pipe) {
+                pipe->post(S, avg_better, inv, result.iterations - 1, ores);
+            } else if (o.checkpoint) {
                 auto [cx, cy] = candidate();
-                auto [ux, uy] = unscale_point(sign, scaling, DeviceOrder::to_nat(cx, ord.col),
-                                              DeviceOrder::to_nat(cy, ord.row));
-                o.checkpoint(result.iterations - 1, ux, uy, ores);
+                deliver(result.iterations - 1, cx, cy, ores);
             }
             const bool stop = o.termination == PdhgTermination::Reference ? (ores < o.tolerance)
                                                                           : (rel_worst <= o.tolerance);
@@ -1163,6 +1252,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                                  static_cast<double>(omega));
             }
         }
+        if (pipe) pipe->drain(deliver);   // the last check's callback
         hip_ok(sov_hip_sync(ctx), "sync");
         result.loop_seconds = seconds_since(t_loop);
         if (timing) {

@@ -96,6 +96,9 @@ def ref() -> C.CDLL:
                                         C.c_int32, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_double, _f64p,
                                         _f64p, _f64p, _f64p, C.c_int64, _f64p, _i64p, _i64p]
         _ref.ref_scaling.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, C.c_int32, C.c_int32, _f64p, _f64p]
+        _ref.ref_pdhg_checkpoints.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                              _f64p, C.c_int32, C.c_double, C.c_double, C.c_int64, C.c_int64, _i64p,
+                                              _f64p, _f64p, _f64p, _i64p]
         _ref.ref_pdhg_rate.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                        C.c_int32, C.c_double, C.c_int64, _f64p, _i64p, _i64p, _f64p]
         _ref.ref_parse.argtypes = [C.c_char_p, C.c_int32, _i32p, _i32p, _i32p]
@@ -264,6 +267,25 @@ def ref_solve(M: Model, tolerance=1e-4, max_iterations=1000000, period=40, time_
     return dict(status=STATUS[st], objective=stats[0], primal_infeasibility=stats[1], dual_infeasibility=stats[2],
                 duality_gap=stats[3], iterations=int(stats[4]), restarts=int(stats[5]), seconds=stats[6],
                 operator_norm=stats[7], x=x[: M.n], y=y[: M.m], rc=rc[: M.n], trace=tr[:nt], trace_it=ti[:nt])
+
+
+def ref_checkpoints(M: Model, tolerance=1e-4, max_iterations=1000000, cap=16):
+    """The first `cap` checkpoints of the reference's own solve: (status,
+    total checkpoints, iterations, residuals, xs [k x n], ys [k x m])."""
+    its, res = np.zeros(cap, np.int64), np.zeros(cap)
+    xs, ys = np.zeros(cap * max(M.n, 1)), np.zeros(cap * max(M.m, 1))
+    k = C.c_int64(0)
+    st = ref().ref_pdhg_checkpoints(M.m, M.n, _p(M.row_ptr, C.c_int32), _p(M.col_idx, C.c_int32),
+                                    _p(M.val, C.c_double), _p(M.c, C.c_double), _p(M.col_lo, C.c_double),
+                                    _p(M.col_hi, C.c_double), _p(M.row_lo, C.c_double), _p(M.row_hi, C.c_double),
+                                    int(M.maximize), C.c_double(M.obj_const), C.c_double(tolerance),
+                                    C.c_int64(max_iterations), C.c_int64(cap), _p(its, C.c_int64),
+                                    _p(res, C.c_double), _p(xs, C.c_double), _p(ys, C.c_double), C.byref(k))
+    if st < 0:
+        raise RuntimeError("reference solve threw")
+    got = min(int(k.value), cap)
+    return (STATUS[st], int(k.value), its[:got], res[:got], xs[: got * M.n].reshape(got, M.n),
+            ys[: got * M.m].reshape(got, M.m))
 
 
 def ref_rate(M: Model, checks=2):

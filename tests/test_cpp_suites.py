@@ -19,12 +19,12 @@ def run(name, require_hip):
     assert out.returncode == 0, out.stdout[-4000:] + out.stderr[-2000:]
 
 
-@pytest.mark.parametrize("name", ["test_backend", "test_pdhg"])
+@pytest.mark.parametrize("name", ["test_backend", "test_pdhg", "test_race"])
 def test_cpp_suite_host(name):
     run(name, require_hip=False)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["test_backend", "test_pdhg"])
+@pytest.mark.parametrize("name", ["test_backend", "test_pdhg", "test_race"])
 def test_cpp_suite_hip(gpu, name):
     run(name, require_hip=True)

@@ -146,6 +146,42 @@ def test_device_transpose_is_the_csc_of_a(ctx):
     A.destroy()
 
 
+def test_device_transpose_with_a_dense_column(ctx):
+    """One column touching every row (m = 400,000) next to short ones: the
+    transpose is a sort, linear in nnz, so a dense column costs no more than
+    its entries; the result is still the CSC of A with ascending rows."""
+    import time
+    rng = np.random.default_rng(8)
+    m, n = 400_000, 64
+    extra = [np.sort(rng.choice(np.arange(1, n), int(k), replace=False)) for k in rng.integers(0, 3, m)]
+    rows_cols = [np.concatenate([[0], e]) for e in extra]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows_cols])]).astype(np.int32)
+    ci = np.concatenate(rows_cols).astype(np.int32)
+    v = rng.standard_normal(len(ci))
+    t0 = time.perf_counter()
+    A = Csr(ctx, m, n, rp, ci, v)
+    check(ctx.L.sov_hip_sync(ctx.h))
+    built = time.perf_counter() - t0
+    nnz = int(rp[-1])
+    tp, ti, tv = ctx.alloc(4 * (n + 1)), ctx.alloc(4 * nnz), ctx.alloc(8 * nnz)
+    check(ctx.L.sov_hip_csr_transpose_arrays(ctx.h, A.h, tp, ti, tv))
+    tp, ti, tv = ctx.download(tp, n + 1, np.int32), ctx.download(ti, nnz, np.int32), ctx.download(tv, nnz)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    order = np.lexsort((rows, ci))
+    print(f"dense column of {m} entries, nnz {nnz}: handle built in {built:.2f} s", flush=True)
+    np.testing.assert_array_equal(tp, np.concatenate([[0], np.cumsum(np.bincount(ci, minlength=n))]))
+    np.testing.assert_array_equal(ti, rows[order])
+    np.testing.assert_array_equal(tv, v[order])
+    x = rng.standard_normal(m)
+    zero = dict(c=np.zeros(n), col_lo=np.zeros(n), col_hi=np.zeros(n), row_lo=np.zeros(m), row_hi=np.zeros(m))
+    want = O.spmv_t(O.Model(dict(m=m, n=n, row_ptr=rp, col_idx=ci, val=v, **zero)), x)
+    mag = O.spmv_t(O.Model(dict(m=m, n=n, row_ptr=rp, col_idx=ci, val=np.abs(v), **zero)), np.abs(x))
+    bound = (np.bincount(ci, minlength=n) + 4) * 2.0 ** -53 * 2 * mag
+    assert np.all(np.abs(A.spmv(x, transpose=True) - want) <= bound)
+    assert built < 10.0
+    A.destroy()
+
+
 def test_empty_matrix_and_empty_rows(ctx):
     A = Csr(ctx, 3, 4, [0, 0, 0, 0], [], [])
     np.testing.assert_array_equal(A.spmv([1, 2, 3, 4], beta=0.0, y=[np.nan] * 3), [0, 0, 0])

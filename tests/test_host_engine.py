@@ -48,6 +48,33 @@ def test_host_primitive_engine_on_random_lp(golden):
     np.testing.assert_array_equal(r.x, g["x"])
 
 
+def checkpoint_fixture():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "reference_checkpoints.json")) as f:
+        return json.load(f)["cases"]
+
+
+def checkpoint_case_lp(name, case):
+    if "model" in case:
+        return lp_of(case["model"])
+    return S.LP.random(case["m"], case["n"], case["k"], case["seed"])
+
+
+def test_host_checkpoints_are_bitwise_the_reference():
+    """The checkpoint hand-off (reference pdhg.hpp:121-133): on the Host
+    backend every checkpoint's iteration, residual and unscaled candidate
+    equal the reference's own (tests/golden/reference_checkpoints.json)."""
+    for name, case in checkpoint_fixture().items():
+        seen = []
+        r = S.solve(checkpoint_case_lp(name, case), backend="host", tolerance=case["tolerance"],
+                    checkpoint=lambda it, x, y, res: seen.append((it, x, y, res)))
+        assert r.status == case["status"] and len(seen) == case["checkpoints"], name
+        for k, it in enumerate(case["iterations"]):
+            assert seen[k][0] == it and seen[k][3] == case["residuals"][k], (name, k)
+            np.testing.assert_array_equal(seen[k][1], case["x"][k])
+            np.testing.assert_array_equal(seen[k][2], case["y"][k])
+
+
 def test_reader_matches_the_reference_parser(golden):
     from golden.make_golden import TEST_PDHG_LPS
     for name, text in TEST_PDHG_LPS.items():
