@@ -24,6 +24,8 @@
 #pragma once
 
 #include <functional>
+#include <memory>
+#include <span>
 #include <string>
 #include <vector>
 
@@ -231,6 +233,53 @@ public:
 
 private:
     PdhgOptions options_;
+};
+
+/// Additive (no reference counterpart): a live batch for re-solving related
+/// LPs -- bounding the nodes of a branch-and-bound tree, parametric sweeps,
+/// rolling horizons.  The instances are stacked once (matrix, scaling,
+/// layouts and schedules, operator norms, iterate states all stay on the
+/// device); between solves the caller overwrites an instance's row bounds
+/// (right-hand sides), objective or column bounds in place, or starts an
+/// instance from a given point or from another instance's (a neighbour's)
+/// last solution, and solve() runs the batched engine again from the points
+/// the instances hold.  Scaling depends on A only, so no update rescales;
+/// the matrix itself cannot change.  By default an instance restarts from
+/// its own last candidate.  Updates are validated when they are made (an
+/// update that would make the instance invalid throws Error and changes
+/// nothing).  The backend must outlive the session; one thread uses it.
+class PdhgBatchSession {
+public:
+    /// Same arguments as PdhgBatch::solve; throws Error when the backend is
+    /// not a Hip backend or the device work fails.  Instances that fail
+    /// validation are not stacked: every solve reports them with the
+    /// validation status, and updating them throws.
+    PdhgBatchSession(PdhgOptions options, const std::vector<const Problem*>& problems, Backend& backend,
+                     const std::vector<std::vector<Real>>* warm_dual = nullptr,
+                     const std::vector<Real>* primal_weight = nullptr);
+    ~PdhgBatchSession();
+    PdhgBatchSession(const PdhgBatchSession&) = delete;
+    PdhgBatchSession& operator=(const PdhgBatchSession&) = delete;
+
+    std::size_t size() const noexcept;
+    /// The instance as the session holds it (with every update applied).
+    const Problem& problem(std::size_t b) const;
+    void set_row_bounds(std::size_t b, std::span<const Real> lower, std::span<const Real> upper);
+    void set_objective(std::size_t b, std::span<const Real> c);
+    void set_column_bounds(std::size_t b, std::span<const Real> lower, std::span<const Real> upper);
+    /// Start instance b's next solve from an original-space point (either
+    /// vector may be empty: keep that part); x is projected onto the column
+    /// bounds and the slacks follow from A x.
+    void set_start(std::size_t b, std::span<const Real> x, std::span<const Real> y);
+    /// Start instance b from instance `source`'s last solution (same
+    /// dimensions; source must have been solved).
+    void warm_start_from(std::size_t b, std::size_t source);
+    /// Run the batched engine on every instance; results in input order.
+    PdhgBatchResult solve();
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
 };
 
 }  // namespace sov

@@ -147,6 +147,28 @@ int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg
                          const double* const* warm_ys, const double* primal_weights, sov_pdhg_result* results,
                          double* const* xs, double* const* ys, double* timing);
 
+/* A live batch for re-solving related LPs (sov::PdhgBatchSession; no
+ * reference counterpart): created from count models like
+ * sov_pdhg_solve_batch, it keeps them on the device (on its own Hip
+ * backend); between solves an instance's row bounds, objective or column
+ * bounds are overwritten in place, or its next start is set from a point
+ * (original units; x or y may be NULL) or from another instance's last
+ * solution.  Each solve starts every instance from its last candidate
+ * unless told otherwise.  Functions return sov::Status codes; create
+ * returns NULL on failure (sov_c_last_error). */
+typedef struct sov_batch_session sov_batch_session;
+sov_batch_session* sov_batch_session_create(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* opts,
+                                            const double* const* warm_ys, const double* primal_weights);
+void sov_batch_session_destroy(sov_batch_session* s);
+int sov_batch_session_set_row_bounds(sov_batch_session* s, int32_t b, const double* lower, const double* upper);
+int sov_batch_session_set_objective(sov_batch_session* s, int32_t b, const double* c);
+int sov_batch_session_set_col_bounds(sov_batch_session* s, int32_t b, const double* lower, const double* upper);
+int sov_batch_session_set_start(sov_batch_session* s, int32_t b, const double* x, const double* y);
+int sov_batch_session_warm_start_from(sov_batch_session* s, int32_t b, int32_t source);
+/* results[count]; xs / ys / timing as sov_pdhg_solve_batch. */
+int sov_batch_session_solve(sov_batch_session* s, sov_pdhg_result* results, double* const* xs, double* const* ys,
+                            double* timing);
+
 /* Backend report line (available backends, default, index width). */
 int sov_backend_report(char* buf, int32_t len);
 

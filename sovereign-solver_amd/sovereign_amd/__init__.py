@@ -438,3 +438,132 @@ def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, res
                           int(r.fp64_polish_iterations), xs[b][:n], ys[b][:m], np.zeros(0)))
     return out, {"seconds": timing[0], "setup_seconds": timing[1], "loop_seconds": timing[2],
                  "iterations": int(timing[3])}
+
+
+def _batch_opts(tolerance, max_iterations, time_limit, restart_check_period, termination, precision, fp64_polish,
+                use_graphs, autotune, restarts, verbose):
+    L = engine()
+    o = PdhgOpts()
+    L.sov_pdhg_default_opts(C.byref(o))
+    o.tolerance = tolerance
+    o.max_iterations = max_iterations
+    o.time_limit_seconds = time_limit
+    o.restart_check_period = restart_check_period
+    o.termination = 1 if termination == "relative" else 0
+    o.precision = 1 if precision == "f32" else 0
+    o.fp64_polish = int(fp64_polish)
+    o.use_graphs = int(use_graphs)
+    o.autotune = int(autotune)
+    o.restart_scheme = 1 if restarts == "adaptive" else 0
+    o.verbose = int(verbose)
+    return o
+
+
+class BatchSession:
+    """A live batch for re-solving related LPs (sov::PdhgBatchSession through
+    sov_batch_session_* of sovereign_c.h): the models stay on the device;
+    between solve() calls an instance's row bounds, objective or column
+    bounds are overwritten in place, or its next start is set from a point
+    or from another instance's last solution.  Every solve starts each
+    instance from its last candidate unless told otherwise."""
+
+    def __init__(self, lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, restart_check_period=40,
+                 termination="relative", precision="f32", fp64_polish=True, use_graphs=True, autotune=True,
+                 restarts="adaptive", verbose=False, warm_duals=None, primal_weights=None):
+        L = engine()
+        L.sov_batch_session_create.restype = C.c_void_p
+        L.sov_batch_session_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(PdhgOpts),
+                                               C.POINTER(_f64p), _f64p]
+        L.sov_batch_session_destroy.argtypes = [C.c_void_p]
+        for f in ("sov_batch_session_set_row_bounds", "sov_batch_session_set_col_bounds",
+                  "sov_batch_session_set_start"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p]
+        L.sov_batch_session_set_objective.argtypes = [C.c_void_p, C.c_int32, _f64p]
+        L.sov_batch_session_warm_start_from.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.sov_batch_session_solve.argtypes = [C.c_void_p, C.POINTER(PdhgRes), C.POINTER(_f64p), C.POINTER(_f64p),
+                                              _f64p]
+        self.lps = list(lps)
+        self.dims = [lp.dims() for lp in self.lps]
+        o = _batch_opts(tolerance, max_iterations, time_limit, restart_check_period, termination, precision,
+                        fp64_polish, use_graphs, autotune, restarts, verbose)
+        count = len(self.lps)
+        handles = (C.c_void_p * count)(*[lp.handle for lp in self.lps])
+        keep, wyp, pwp = [], None, None
+        if warm_duals is not None:
+            arrs = [None if w is None else np.ascontiguousarray(w, np.float64) for w in warm_duals]
+            keep += [a for a in arrs if a is not None]
+            wyp = (_f64p * count)(*[ptr(a, C.c_double) if a is not None else _f64p() for a in arrs])
+        if primal_weights is not None:
+            pwa = np.ascontiguousarray(primal_weights, np.float64)
+            keep.append(pwa)
+            pwp = ptr(pwa, C.c_double)
+        h = L.sov_batch_session_create(handles, count, C.byref(o), wyp, pwp)
+        if not h:
+            raise RuntimeError("sov_batch_session_create failed: " + last_error())
+        self.h = C.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            engine().sov_batch_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        rc = getattr(engine(), name)(self.h, *args)
+        if rc != 0:
+            raise RuntimeError(f"{name} failed ({rc}): {last_error()}")
+
+    @staticmethod
+    def _p(a):
+        return None if a is None else ptr(np.ascontiguousarray(a, np.float64), C.c_double)
+
+    def set_row_bounds(self, b, lower, upper):
+        lo, hi = np.ascontiguousarray(lower, np.float64), np.ascontiguousarray(upper, np.float64)
+        assert lo.shape == hi.shape == (self.dims[b][0],)
+        self._call("sov_batch_session_set_row_bounds", b, ptr(lo, C.c_double), ptr(hi, C.c_double))
+
+    def set_col_bounds(self, b, lower, upper):
+        lo, hi = np.ascontiguousarray(lower, np.float64), np.ascontiguousarray(upper, np.float64)
+        assert lo.shape == hi.shape == (self.dims[b][1],)
+        self._call("sov_batch_session_set_col_bounds", b, ptr(lo, C.c_double), ptr(hi, C.c_double))
+
+    def set_objective(self, b, c):
+        cc = np.ascontiguousarray(c, np.float64)
+        assert cc.shape == (self.dims[b][1],)
+        self._call("sov_batch_session_set_objective", b, ptr(cc, C.c_double))
+
+    def set_start(self, b, x=None, y=None):
+        xa = None if x is None else np.ascontiguousarray(x, np.float64)
+        ya = None if y is None else np.ascontiguousarray(y, np.float64)
+        self._call("sov_batch_session_set_start", b, None if xa is None else ptr(xa, C.c_double),
+                   None if ya is None else ptr(ya, C.c_double))
+
+    def warm_start_from(self, b, source):
+        self._call("sov_batch_session_warm_start_from", b, source)
+
+    def solve(self):
+        """(list of Result, dict of batch timings), as solve_batch."""
+        count = len(self.lps)
+        res = (PdhgRes * count)()
+        xs = [np.zeros(max(d[1], 1)) for d in self.dims]
+        ys = [np.zeros(max(d[0], 1)) for d in self.dims]
+        xp = (_f64p * count)(*[ptr(x, C.c_double) for x in xs])
+        yp = (_f64p * count)(*[ptr(y, C.c_double) for y in ys])
+        timing = np.zeros(4)
+        self._call("sov_batch_session_solve", res, xp, yp, ptr(timing, C.c_double))
+        out = []
+        for b in range(count):
+            r = res[b]
+            m, n, _ = self.dims[b]
+            out.append(Result(STATUS[r.status], r.objective, r.primal_infeasibility, r.dual_infeasibility,
+                              r.duality_gap, int(r.iterations), int(r.restarts), r.seconds, r.setup_seconds,
+                              r.loop_seconds, r.operator_norm_estimate, (r.relative_primal, r.relative_dual,
+                                                                         r.relative_gap), r.primal_weight,
+                              int(r.fp64_polish_iterations), xs[b][:n], ys[b][:m], np.zeros(0)))
+        return out, {"seconds": timing[0], "setup_seconds": timing[1], "loop_seconds": timing[2],
+                     "iterations": int(timing[3])}

@@ -353,6 +353,137 @@ int sov_pdhg_solve_batch(const sov_lp* const* lps, int32_t count, const sov_pdhg
     }
 }
 
+}  // extern "C"
+
+struct sov_batch_session {
+    std::unique_ptr<sov::Backend> backend;   // declared first: outlives the session
+    std::unique_ptr<sov::PdhgBatchSession> session;
+};
+
+namespace {
+
+template <class F>
+int session_call(sov_batch_session* s, F&& f) {
+    if (s == nullptr) return set_err(3, "null session");
+    try {
+        f(*s->session);
+        return 0;
+    } catch (const sov::Error& e) {
+        return set_err(static_cast<int>(e.status()), e.what());
+    } catch (const std::exception& e) {
+        return set_err(6, e.what());
+    }
+}
+
+std::span<const double> span_of(const double* p, std::size_t n) {
+    return p == nullptr ? std::span<const double>{} : std::span<const double>(p, n);
+}
+
+}  // namespace
+
+extern "C" {
+
+sov_batch_session* sov_batch_session_create(const sov_lp* const* lps, int32_t count, const sov_pdhg_opts* o,
+                                            const double* const* warm_ys, const double* primal_weights) {
+    if (lps == nullptr || o == nullptr || count < 0) {
+        g_err = "null argument";
+        return nullptr;
+    }
+    try {
+        std::vector<const sov::Problem*> probs(static_cast<std::size_t>(count));
+        for (int32_t b = 0; b < count; ++b) {
+            if (lps[b] == nullptr) throw sov::Error(sov::Status::InvalidArgument, "null model in batch");
+            probs[static_cast<std::size_t>(b)] = &lps[b]->p;
+        }
+        std::vector<std::vector<double>> wy;
+        if (warm_ys != nullptr) {
+            wy.resize(static_cast<std::size_t>(count));
+            for (int32_t b = 0; b < count; ++b)
+                if (warm_ys[b] != nullptr)
+                    wy[static_cast<std::size_t>(b)].assign(warm_ys[b], warm_ys[b] + lps[b]->p.num_rows());
+        }
+        std::vector<double> pw;
+        if (primal_weights != nullptr) pw.assign(primal_weights, primal_weights + count);
+        auto* s = new sov_batch_session;
+        try {
+            s->backend = sov::make_backend(sov::BackendKind::Hip);
+            s->session = std::make_unique<sov::PdhgBatchSession>(to_options(o), probs, *s->backend,
+                                                                 warm_ys ? &wy : nullptr,
+                                                                 primal_weights ? &pw : nullptr);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        return s;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void sov_batch_session_destroy(sov_batch_session* s) {
+    if (s == nullptr) return;
+    s->session.reset();
+    delete s;
+}
+
+int sov_batch_session_set_row_bounds(sov_batch_session* s, int32_t b, const double* lo, const double* hi) {
+    return session_call(s, [&](sov::PdhgBatchSession& x) {
+        const auto m = static_cast<std::size_t>(x.problem(static_cast<std::size_t>(b)).num_rows());
+        x.set_row_bounds(static_cast<std::size_t>(b), span_of(lo, m), span_of(hi, m));
+    });
+}
+
+int sov_batch_session_set_objective(sov_batch_session* s, int32_t b, const double* c) {
+    return session_call(s, [&](sov::PdhgBatchSession& x) {
+        const auto n = static_cast<std::size_t>(x.problem(static_cast<std::size_t>(b)).num_cols());
+        x.set_objective(static_cast<std::size_t>(b), span_of(c, n));
+    });
+}
+
+int sov_batch_session_set_col_bounds(sov_batch_session* s, int32_t b, const double* lo, const double* hi) {
+    return session_call(s, [&](sov::PdhgBatchSession& x) {
+        const auto n = static_cast<std::size_t>(x.problem(static_cast<std::size_t>(b)).num_cols());
+        x.set_column_bounds(static_cast<std::size_t>(b), span_of(lo, n), span_of(hi, n));
+    });
+}
+
+int sov_batch_session_set_start(sov_batch_session* s, int32_t b, const double* xv, const double* yv) {
+    return session_call(s, [&](sov::PdhgBatchSession& x) {
+        const sov::Problem& p = x.problem(static_cast<std::size_t>(b));
+        x.set_start(static_cast<std::size_t>(b), span_of(xv, static_cast<std::size_t>(p.num_cols())),
+                    span_of(yv, static_cast<std::size_t>(p.num_rows())));
+    });
+}
+
+int sov_batch_session_warm_start_from(sov_batch_session* s, int32_t b, int32_t source) {
+    return session_call(s, [&](sov::PdhgBatchSession& x) {
+        if (b < 0 || source < 0) throw sov::Error(sov::Status::InvalidArgument, "negative instance");
+        x.warm_start_from(static_cast<std::size_t>(b), static_cast<std::size_t>(source));
+    });
+}
+
+int sov_batch_session_solve(sov_batch_session* s, sov_pdhg_result* results, double* const* xs, double* const* ys,
+                            double* timing) {
+    if (results == nullptr) return set_err(3, "null results");
+    return session_call(s, [&](sov::PdhgBatchSession& x) {
+        const sov::PdhgBatchResult br = x.solve();
+        for (std::size_t b = 0; b < br.results.size(); ++b) {
+            const sov::PdhgResult& r = br.results[b];
+            fill_result(r, &results[b]);
+            if (xs && xs[b] && !r.primal.empty()) std::memcpy(xs[b], r.primal.data(), r.primal.size() * sizeof(double));
+            if (ys && ys[b] && !r.dual.empty()) std::memcpy(ys[b], r.dual.data(), r.dual.size() * sizeof(double));
+        }
+        if (timing) {
+            timing[0] = br.seconds;
+            timing[1] = br.setup_seconds;
+            timing[2] = br.loop_seconds;
+            timing[3] = static_cast<double>(br.iterations);
+        }
+        if (!br.message.empty()) g_err = br.message;
+    });
+}
+
 int sov_backend_report(char* buf, int32_t len) {
     if (buf == nullptr || len <= 0) return set_err(3, "null buffer");
     const std::string s = sov::backend_report();

@@ -1279,7 +1279,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
 }
 
 // ===========================================================================
-// batched engine (additive)
+// batched engine and the live batch session (additive)
 // ===========================================================================
 
 namespace {
@@ -1289,248 +1289,362 @@ struct BatchHandle {
     ~BatchHandle() { sov_hip_batch_destroy(b); }
 };
 
+/// The same validation prevalidate() applies before a solve, for a session
+/// update: throws with the reason instead of reporting a status.
+void require_valid(const Problem& p, const char* what) {
+    PdhgResult r;
+    if (!prevalidate(p, r)) throw Error(Status::InvalidArgument, std::string(what) + ": " + r.message);
+}
+
 }  // namespace
 
-PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Backend& backend,
-                                  const std::vector<std::vector<Real>>* warm_dual,
-                                  const std::vector<Real>* primal_weight) const {
-    PdhgBatchResult out;
+struct PdhgBatchSession::Impl {
+    PdhgOptions o;
+    sov_hip_ctx* ctx = nullptr;
+    std::vector<Problem> probs;                 // the session's copies, in input order
+    std::vector<PdhgResult> rejected;           // prevalidate's verdict on instances not stacked
+    std::vector<std::size_t> idx;               // stacked instance -> input index
+    std::vector<std::ptrdiff_t> slot;           // input index -> stacked instance (-1: not stacked)
+    std::size_t B = 0, m = 0, n = 0, nnz = 0;
+    Real sign = 1.0;
+    std::vector<int32_t> row_off{0}, col_off{0};
+    DevMem a_rp, a_ci;                          // A's structure, borrowed by the handle (declared first: outlives it)
+    CsrHandle csr;
+    BatchHandle bh;
+    DeviceOrder ord;
+    std::vector<Real> rs, cs, oconst, est;
+    std::vector<double> tau, sigma, weight;
+    DevMem d_olo_r, d_ohi_r, d_olo_c, d_ohi_c, d_oc, d_act, d_act32, d_rs_dev, d_cs_dev;
+    DeviceState S32, S64;
+    bool f32 = false;
+    std::vector<Real> x_start, y_start;         // next solve's start: scaled space, natural order, FP64
+    std::vector<char> cold;                     // per stacked instance: start as the reference does (s = Proj_S(0))
+    std::vector<Real> clo_s, chi_s;             // scaled column bounds, natural order
+    std::vector<std::vector<Real>> last_x, last_y;   // last original-space solutions, input order
+    bool data_dirty = true, start_dirty = true;
+    double setup_seconds = 0.0;
+
+    Impl(PdhgOptions options, const std::vector<const Problem*>& problems, Backend& backend,
+         const std::vector<std::vector<Real>>* warm_dual, const std::vector<Real>* primal_weight);
+    void stacked_vectors(std::vector<Real>& c, std::vector<Real>& clo, std::vector<Real>& chi,
+                         std::vector<Real>& rlo, std::vector<Real>& rhi) const;
+    void upload_data();
+    void place_start();
+    std::size_t stacked(std::size_t b, const char* what) const;
+    void replace(std::size_t b, Problem p, const char* what);
+    PdhgBatchResult run();
+};
+
+PdhgBatchSession::Impl::Impl(PdhgOptions options, const std::vector<const Problem*>& problems, Backend& backend,
+                             const std::vector<std::vector<Real>>* warm_dual, const std::vector<Real>* primal_weight)
+    : o(std::move(options)) {
     const auto t0 = Clock::now();
-    const PdhgOptions& o = options_;
-    out.results.resize(problems.size());
-    for (PdhgResult& r : out.results) r.engine = "batched";
-    auto* ctx = static_cast<sov_hip_ctx*>(backend.native_context());
-    if (backend.kind() != BackendKind::Hip || ctx == nullptr) {
-        out.message = "the batched engine needs a Hip backend";
-        for (PdhgResult& r : out.results) {
-            r.status = PdhgStatus::NumericalFailure;
-            r.message = out.message;
+    ctx = static_cast<sov_hip_ctx*>(backend.native_context());
+    if (backend.kind() != BackendKind::Hip || ctx == nullptr)
+        throw Error(Status::Unsupported, "the batched engine needs a Hip backend");
+    probs.reserve(problems.size());
+    rejected.resize(problems.size());
+    slot.assign(problems.size(), -1);
+    last_x.resize(problems.size());
+    last_y.resize(problems.size());
+    for (std::size_t b = 0; b < problems.size(); ++b) {
+        probs.push_back(problems[b] != nullptr ? *problems[b] : Problem{});
+        rejected[b].engine = "batched";
+        if (problems[b] == nullptr) {
+            rejected[b].status = PdhgStatus::NumericalFailure;
+            rejected[b].message = "no problem";
+        } else if (prevalidate(probs[b], rejected[b])) {
+            slot[b] = static_cast<std::ptrdiff_t>(idx.size());
+            idx.push_back(b);
         }
-        return out;
     }
-    // instances that pass validation are stacked; the others keep prevalidate's status
-    std::vector<std::size_t> idx;
-    for (std::size_t b = 0; b < problems.size(); ++b)
-        if (problems[b] != nullptr && prevalidate(*problems[b], out.results[b])) idx.push_back(b);
-    const std::size_t B = idx.size();
-    if (B == 0) return out;
-    const ObjSense sense = problems[idx[0]]->sense();
+    B = idx.size();
+    if (B == 0) return;
+    const ObjSense sense = probs[idx[0]].sense();
     for (std::size_t k : idx)
-        if (problems[k]->sense() != sense) throw Error(Status::InvalidArgument, "PdhgBatch: instances differ in sense");
-    const Real sign = sense == ObjSense::Maximize ? -1.0 : 1.0;
-    try {
-        // -- stack the instances block-diagonally (natural order)
-        std::vector<int32_t> row_off{0}, col_off{0};
-        std::size_t nnz = 0;
-        for (std::size_t k : idx) {
-            const Problem& p = *problems[k];
-            row_off.push_back(row_off.back() + p.num_rows());
-            col_off.push_back(col_off.back() + p.num_cols());
-            nnz += static_cast<std::size_t>(p.matrix().nnz());
-        }
-        const auto m = static_cast<std::size_t>(row_off.back()), n = static_cast<std::size_t>(col_off.back());
-        std::vector<int32_t> rp(m + 1, 0), ci(std::max<std::size_t>(nnz, 1));
-        std::vector<double> vals(std::max<std::size_t>(nnz, 1));
-        std::vector<Real> c_nat(n), clo(n), chi(n), rlo(m), rhi(m), oconst(B);
-        {
-            std::size_t q = 0;
-            for (std::size_t b = 0; b < B; ++b) {
-                const Problem& p = *problems[idx[b]];
-                const CsrMatrix& A = p.matrix();
-                const auto r0 = static_cast<std::size_t>(row_off[b]), c0 = static_cast<std::size_t>(col_off[b]);
-                for (Idx r = 0; r < A.rows(); ++r) {
-                    for (Idx k = A.row_ptr()[r]; k < A.row_ptr()[r + 1]; ++k, ++q) {
-                        ci[q] = static_cast<int32_t>(c0) + A.col_idx()[k];
-                        vals[q] = A.values()[k];
-                    }
-                    rp[r0 + static_cast<std::size_t>(r) + 1] = static_cast<int32_t>(q);
+        if (probs[k].sense() != sense) throw Error(Status::InvalidArgument, "PdhgBatch: instances differ in sense");
+    sign = sense == ObjSense::Maximize ? -1.0 : 1.0;
+
+    // -- stack the instances block-diagonally (natural order)
+    for (std::size_t k : idx) {
+        const Problem& p = probs[k];
+        row_off.push_back(row_off.back() + p.num_rows());
+        col_off.push_back(col_off.back() + p.num_cols());
+        nnz += static_cast<std::size_t>(p.matrix().nnz());
+    }
+    m = static_cast<std::size_t>(row_off.back());
+    n = static_cast<std::size_t>(col_off.back());
+    std::vector<int32_t> rp(m + 1, 0), ci(std::max<std::size_t>(nnz, 1));
+    std::vector<double> vals(std::max<std::size_t>(nnz, 1));
+    oconst.resize(B);
+    {
+        std::size_t q = 0;
+        for (std::size_t b = 0; b < B; ++b) {
+            const Problem& p = probs[idx[b]];
+            const CsrMatrix& A = p.matrix();
+            const auto r0 = static_cast<std::size_t>(row_off[b]);
+            for (Idx r = 0; r < A.rows(); ++r) {
+                for (Idx k = A.row_ptr()[r]; k < A.row_ptr()[r + 1]; ++k, ++q) {
+                    ci[q] = col_off[b] + A.col_idx()[k];
+                    vals[q] = A.values()[k];
                 }
-                for (Idx j = 0; j < p.num_cols(); ++j) {
-                    c_nat[c0 + j] = p.objective()[j];
-                    clo[c0 + j] = p.col_lower()[j];
-                    chi[c0 + j] = p.col_upper()[j];
-                }
-                for (Idx i = 0; i < p.num_rows(); ++i) {
-                    rlo[r0 + i] = p.row_lower()[i];
-                    rhi[r0 + i] = p.row_upper()[i];
-                }
-                oconst[b] = p.objective_constant();
+                rp[r0 + static_cast<std::size_t>(r) + 1] = static_cast<int32_t>(q);
             }
+            oconst[b] = p.objective_constant();
         }
+    }
+    {
         DevMem d_rp(ctx, (m + 1) * sizeof(int32_t)), d_ci(ctx, ci.size() * sizeof(int32_t)),
             d_v(ctx, vals.size() * sizeof(double));
         hip_ok(sov_hip_memcpy_h2d(ctx, d_rp.p, rp.data(), (m + 1) * sizeof(int32_t)), "upload");
         hip_ok(sov_hip_memcpy_h2d(ctx, d_ci.p, ci.data(), ci.size() * sizeof(int32_t)), "upload");
         hip_ok(sov_hip_memcpy_h2d(ctx, d_v.p, vals.data(), vals.size() * sizeof(double)), "upload");
-        CsrHandle csr;
         hip_ok(sov_hip_csr_create(ctx, static_cast<int32_t>(m), static_cast<int32_t>(n), static_cast<int32_t>(nnz),
                                   d_rp.as<int32_t>(), d_ci.as<int32_t>(), d_v.as<double>(), SOV_CSR_OWN_VALUES, &csr.h),
                "csr_create");
-        d_v.reset();
+        // the handle copied the values and borrows the structure arrays
+        a_rp = std::move(d_rp);
+        a_ci = std::move(d_ci);
+    }
 
-        // -- Ruiz + Pock-Chambolle on the stack: row and column norms never
-        //    cross instances, so these are each instance's own factors (the
-        //    Ruiz early stop is taken over the whole batch)
+    // -- Ruiz + Pock-Chambolle on the stack: row and column norms never
+    //    cross instances, so these are each instance's own factors (the
+    //    Ruiz early stop is taken over the whole batch)
+    {
         DevMem d_rs(ctx, std::max<std::size_t>(m, 1) * 8), d_cs(ctx, std::max<std::size_t>(n, 1) * 8);
-        {
-            const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
-            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, d_rs.p, ones_m.data(), m * 8), "upload");
-            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, d_cs.p, ones_n.data(), n * 8), "upload");
-            const ScalingOptions so;
-            if (so.apply_ruiz)
-                for (int sweep = 0; sweep < so.ruiz_iterations; ++sweep) {
-                    double worst = 0.0;
-                    hip_ok(sov_hip_scale_ruiz_sweep(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>(), &worst), "ruiz");
-                    if (worst < so.ruiz_tolerance) break;
-                }
-            if (so.apply_pock_chambolle)
-                hip_ok(sov_hip_scale_pock_chambolle(ctx, csr.h, so.pock_chambolle_alpha, d_rs.as<double>(),
-                                                    d_cs.as<double>()),
-                       "pock_chambolle");
-        }
-        std::vector<Real> rs(m), cs(n);
+        const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
+        if (m) hip_ok(sov_hip_memcpy_h2d(ctx, d_rs.p, ones_m.data(), m * 8), "upload");
+        if (n) hip_ok(sov_hip_memcpy_h2d(ctx, d_cs.p, ones_n.data(), n * 8), "upload");
+        const ScalingOptions so;
+        if (so.apply_ruiz)
+            for (int sweep = 0; sweep < so.ruiz_iterations; ++sweep) {
+                double worst = 0.0;
+                hip_ok(sov_hip_scale_ruiz_sweep(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>(), &worst), "ruiz");
+                if (worst < so.ruiz_tolerance) break;
+            }
+        if (so.apply_pock_chambolle)
+            hip_ok(sov_hip_scale_pock_chambolle(ctx, csr.h, so.pock_chambolle_alpha, d_rs.as<double>(),
+                                                d_cs.as<double>()),
+                   "pock_chambolle");
+        rs.resize(m);
+        cs.resize(n);
         if (m) hip_ok(sov_hip_memcpy_d2h(ctx, rs.data(), d_rs.p, m * 8), "download scaling");
         if (n) hip_ok(sov_hip_memcpy_d2h(ctx, cs.data(), d_cs.p, n * 8), "download scaling");
         hip_ok(sov_hip_csr_scale(ctx, csr.h, d_rs.as<double>(), d_cs.as<double>()), "csr_scale");
-        DeviceOrder ord;
-        ord.row.resize(m);
-        ord.col.resize(n);
-        hip_ok(sov_hip_csr_order(ctx, csr.h, 0, ord.row.data()), "order");
-        hip_ok(sov_hip_csr_order(ctx, csr.h, 1, ord.col.data()), "order");
-        BatchHandle bh;
-        hip_ok(sov_hip_batch_create(ctx, csr.h, static_cast<int32_t>(B), row_off.data(), col_off.data(), &bh.b),
-               "batch_create");
+    }
+    ord.row.resize(m);
+    ord.col.resize(n);
+    hip_ok(sov_hip_csr_order(ctx, csr.h, 0, ord.row.data()), "order");
+    hip_ok(sov_hip_csr_order(ctx, csr.h, 1, ord.col.data()), "order");
+    hip_ok(sov_hip_batch_create(ctx, csr.h, static_cast<int32_t>(B), row_off.data(), col_off.data(), &bh.b),
+           "batch_create");
+    const std::size_t mb = std::max<std::size_t>(m, 1) * 8, nb = std::max<std::size_t>(n, 1) * 8;
+    d_olo_r = DevMem(ctx, mb);
+    d_ohi_r = DevMem(ctx, mb);
+    d_olo_c = DevMem(ctx, nb);
+    d_ohi_c = DevMem(ctx, nb);
+    d_oc = DevMem(ctx, nb);
+    d_act = DevMem(ctx, mb);
+    d_act32 = DevMem(ctx, mb);
+    d_rs_dev = DevMem(ctx, mb);
+    d_cs_dev = DevMem(ctx, nb);
+    {
+        const std::vector<Real> pr = DeviceOrder::to_dev(rs, ord.row), pc = DeviceOrder::to_dev(cs, ord.col);
+        if (m) hip_ok(sov_hip_memcpy_h2d(ctx, d_rs_dev.p, pr.data(), m * 8), "upload");
+        if (n) hip_ok(sov_hip_memcpy_h2d(ctx, d_cs_dev.p, pc.data(), n * 8), "upload");
+    }
 
-        // scaled vectors (device order)
-        std::vector<Real> c_s(n), clo_s(n), chi_s(n), rlo_s(m), rhi_s(m);
-        for (std::size_t j = 0; j < n; ++j) {
-            c_s[j] = (c_nat[j] * cs[j]) * sign;
-            clo_s[j] = is_finite_bound(clo[j]) ? clo[j] / cs[j] : clo[j];
-            chi_s[j] = is_finite_bound(chi[j]) ? chi[j] / cs[j] : chi[j];
-        }
-        for (std::size_t i = 0; i < m; ++i) {
-            rlo_s[i] = is_finite_bound(rlo[i]) ? rlo[i] * rs[i] : rlo[i];
-            rhi_s[i] = is_finite_bound(rhi[i]) ? rhi[i] * rs[i] : rhi[i];
-        }
-        c_s = DeviceOrder::to_dev(c_s, ord.col);
-        clo_s = DeviceOrder::to_dev(clo_s, ord.col);
-        chi_s = DeviceOrder::to_dev(chi_s, ord.col);
-        rlo_s = DeviceOrder::to_dev(rlo_s, ord.row);
-        rhi_s = DeviceOrder::to_dev(rhi_s, ord.row);
-        DevMem d_olo_r(ctx, std::max<std::size_t>(m, 1) * 8), d_ohi_r(ctx, std::max<std::size_t>(m, 1) * 8),
-            d_olo_c(ctx, std::max<std::size_t>(n, 1) * 8), d_ohi_c(ctx, std::max<std::size_t>(n, 1) * 8),
-            d_oc(ctx, std::max<std::size_t>(n, 1) * 8), d_act(ctx, std::max<std::size_t>(m, 1) * 8),
-            d_rs_dev(ctx, std::max<std::size_t>(m, 1) * 8), d_cs_dev(ctx, std::max<std::size_t>(n, 1) * 8);
-        auto upload_dev = [&](const DevMem& d, std::span<const Real> v, const std::vector<int32_t>& perm) {
-            if (v.empty()) return;
-            const std::vector<Real> pv = DeviceOrder::to_dev(v, perm);
-            hip_ok(sov_hip_memcpy_h2d(ctx, d.p, pv.data(), pv.size() * 8), "upload");
-        };
-        upload_dev(d_olo_r, rlo, ord.row);
-        upload_dev(d_ohi_r, rhi, ord.row);
-        upload_dev(d_rs_dev, rs, ord.row);
-        upload_dev(d_olo_c, clo, ord.col);
-        upload_dev(d_ohi_c, chi, ord.col);
-        upload_dev(d_oc, c_nat, ord.col);
-        upload_dev(d_cs_dev, cs, ord.col);
-
-        // -- per-instance power iteration (reference pdhg.cpp:48-79 on each block)
-        std::vector<Real> est(B, 0.0);
-        {
-            DevMem u(ctx, std::max<std::size_t>(n, 1) * 8), v(ctx, std::max<std::size_t>(m, 1) * 8),
-                w(ctx, std::max<std::size_t>(m, 1) * 8);
-            const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
-            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, u.p, ones_n.data(), n * 8), "upload");
-            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, v.p, ones_m.data(), m * 8), "upload");
-            std::vector<char> live(B, 1);
-            std::vector<double> sq(B), sq2(B), fac(B);
-            for (int it = 0; it < o.power_iterations; ++it) {
-                hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, -1.0, u.p, 0.0, w.p), "power spmv");
-                hip_ok(sov_hip_axpy_f64(ctx, static_cast<int64_t>(m), 1.0, v.as<double>(), w.as<double>()), "power axpy");
-                hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 0, w.as<double>(), sq.data()), "power norms");
-                for (std::size_t b = 0; b < B; ++b) {
-                    const double wn = std::sqrt(sq[b]);
-                    if (live[b] && wn < 1e-300) live[b] = 0;
-                    fac[b] = live[b] ? 1.0 / wn : 1.0;
-                }
-                hip_ok(sov_hip_batch_scale(ctx, bh.b, 0, fac.data(), w.as<double>()), "power scale");
-                hip_ok(sov_hip_spmv_device(ctx, csr.h, 1, SOV_PREC_F64, -1.0, w.p, 0.0, u.p), "power spmv_t");
-                if (m) hip_ok(sov_hip_memcpy_d2d(ctx, v.p, w.p, m * 8), "power copy");
-                hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 1, u.as<double>(), sq.data()), "power norms");
-                hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 0, v.as<double>(), sq2.data()), "power norms");
-                for (std::size_t b = 0; b < B; ++b) {
-                    if (live[b]) {
-                        est[b] = std::sqrt(sq[b] + sq2[b]);
-                        if (est[b] < 1e-300) live[b] = 0;
-                    }
-                    fac[b] = live[b] ? 1.0 / est[b] : 1.0;
-                }
-                hip_ok(sov_hip_batch_scale(ctx, bh.b, 1, fac.data(), u.as<double>()), "power scale");
-                hip_ok(sov_hip_batch_scale(ctx, bh.b, 0, fac.data(), v.as<double>()), "power scale");
-            }
-        }
-        std::vector<double> tau(B), sigma(B);
-        for (std::size_t b = 0; b < B; ++b) {
-            const double eta = est[b] > 1e-9 ? o.step_size_safety / est[b] : o.step_size_safety;
-            double w = 1.0;
-            if (primal_weight != nullptr && idx[b] < primal_weight->size()) {
-                w = (*primal_weight)[idx[b]];
-                if (!(w > 0.0) || !std::isfinite(w)) w = 1.0;
-            }
-            tau[b] = eta / w;
-            sigma[b] = eta * w;
-            out.results[idx[b]].operator_norm_estimate = est[b];
-            out.results[idx[b]].primal_weight = w;
-        }
-
-        // -- iterate states: FP64 always (the only one for an FP64 run, the
-        //    polish state of an FP32 run); FP32 for an FP32 run.  Each
-        //    instance lives in one of them (active = 1 or 2) and moves to
-        //    FP64 on its own once it is near tolerance or stalled.
-        const bool f32 = o.precision == Precision::F32;
-        if (f32) hip_ok(sov_hip_csr_make_f32(ctx, csr.h), "make_f32");
-        // optional per-instance dual warm starts: original units -> scaled, device order
-        std::vector<Real> y_start;
-        if (warm_dual != nullptr) {
-            y_start.assign(m, 0.0);
+    // -- per-instance power iteration (reference pdhg.cpp:48-79 on each block)
+    est.assign(B, 0.0);
+    {
+        DevMem u(ctx, nb), v(ctx, mb), w(ctx, mb);
+        const std::vector<Real> ones_m(m, 1.0), ones_n(n, 1.0);
+        if (n) hip_ok(sov_hip_memcpy_h2d(ctx, u.p, ones_n.data(), n * 8), "upload");
+        if (m) hip_ok(sov_hip_memcpy_h2d(ctx, v.p, ones_m.data(), m * 8), "upload");
+        std::vector<char> live(B, 1);
+        std::vector<double> sq(B), sq2(B), fac(B);
+        for (int it = 0; it < o.power_iterations; ++it) {
+            hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, -1.0, u.p, 0.0, w.p), "power spmv");
+            hip_ok(sov_hip_axpy_f64(ctx, static_cast<int64_t>(m), 1.0, v.as<double>(), w.as<double>()), "power axpy");
+            hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 0, w.as<double>(), sq.data()), "power norms");
             for (std::size_t b = 0; b < B; ++b) {
-                if (idx[b] >= warm_dual->size() || (*warm_dual)[idx[b]].empty()) continue;
-                const std::vector<Real>& w = (*warm_dual)[idx[b]];
-                const auto r0 = static_cast<std::size_t>(row_off[b]);
-                const std::size_t mb = static_cast<std::size_t>(row_off[b + 1] - row_off[b]);
-                if (w.size() != mb) throw Error(Status::DimensionMismatch, "PdhgBatch: warm_dual length");
-                for (std::size_t i = 0; i < mb; ++i) y_start[r0 + i] = sign * w[i] / rs[r0 + i];
+                const double wn = std::sqrt(sq[b]);
+                if (live[b] && wn < 1e-300) live[b] = 0;
+                fac[b] = live[b] ? 1.0 / wn : 1.0;
             }
-            y_start = DeviceOrder::to_dev(y_start, ord.row);
+            hip_ok(sov_hip_batch_scale(ctx, bh.b, 0, fac.data(), w.as<double>()), "power scale");
+            hip_ok(sov_hip_spmv_device(ctx, csr.h, 1, SOV_PREC_F64, -1.0, w.p, 0.0, u.p), "power spmv_t");
+            if (m) hip_ok(sov_hip_memcpy_d2d(ctx, v.p, w.p, m * 8), "power copy");
+            hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 1, u.as<double>(), sq.data()), "power norms");
+            hip_ok(sov_hip_batch_sumsq(ctx, bh.b, 0, v.as<double>(), sq2.data()), "power norms");
+            for (std::size_t b = 0; b < B; ++b) {
+                if (live[b]) {
+                    est[b] = std::sqrt(sq[b] + sq2[b]);
+                    if (est[b] < 1e-300) live[b] = 0;
+                }
+                fac[b] = live[b] ? 1.0 / est[b] : 1.0;
+            }
+            hip_ok(sov_hip_batch_scale(ctx, bh.b, 1, fac.data(), u.as<double>()), "power scale");
+            hip_ok(sov_hip_batch_scale(ctx, bh.b, 0, fac.data(), v.as<double>()), "power scale");
         }
-        DeviceState S32, S64;
-        auto init_state = [&](DeviceState& S, int p) {
-            S.allocate(ctx, p, m, n);
-            S.upload(ctx, S.c, c_s);
-            S.upload(ctx, S.clo, clo_s);
-            S.upload(ctx, S.chi, chi_s);
-            S.upload(ctx, S.rlo, rlo_s);
-            S.upload(ctx, S.rhi, rhi_s);
-            const std::vector<Real> zm(m, 0.0), zn(n, 0.0);
-            DevMem zx(ctx, std::max<std::size_t>(n, 1) * 8), zs(ctx, std::max<std::size_t>(m, 1) * 8);
-            if (n) hip_ok(sov_hip_memcpy_h2d(ctx, zx.p, zn.data(), n * 8), "upload");
-            if (m) hip_ok(sov_hip_memcpy_h2d(ctx, zs.p, zm.data(), m * 8), "upload");
-            DevMem wy;
-            const double* ysrc = nullptr;
-            if (!y_start.empty()) {
-                wy = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
-                if (m) hip_ok(sov_hip_memcpy_h2d(ctx, wy.p, y_start.data(), m * 8), "upload warm start");
-                ysrc = wy.as<double>();
-            }
-            hip_ok(sov_hip_pdhg_set_point(ctx, &S.st, zx.as<double>(), ysrc, zs.as<double>()), "set_point");
-            hip_ok(sov_hip_sync(ctx), "sync");
-        };
-        if (f32) init_state(S32, SOV_PREC_F32);
-        if (!f32 || o.fp64_polish) init_state(S64, SOV_PREC_F64);
-        DevMem d_act32(ctx, std::max<std::size_t>(m, 1) * 8);
-        if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, f32 ? SOV_PREC_F32 : SOV_PREC_F64), "tune");
-        // the batched step kernels run on the CSR parts only
-        for (int orient : {0, 1}) hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, 0), "tiles");
+    }
+    tau.resize(B);
+    sigma.resize(B);
+    weight.resize(B);
+    for (std::size_t b = 0; b < B; ++b) {
+        const double eta = est[b] > 1e-9 ? o.step_size_safety / est[b] : o.step_size_safety;
+        double w = 1.0;
+        if (primal_weight != nullptr && idx[b] < primal_weight->size()) {
+            w = (*primal_weight)[idx[b]];
+            if (!(w > 0.0) || !std::isfinite(w)) w = 1.0;
+        }
+        tau[b] = eta / w;
+        sigma[b] = eta * w;
+        weight[b] = w;
+    }
+
+    // -- iterate states: FP64 always (the only one for an FP64 run, the
+    //    polish state of an FP32 run); FP32 for an FP32 run.  Each instance
+    //    lives in one of them (active = 1 or 2) and moves to FP64 on its own
+    //    once it is near tolerance or stalled.
+    f32 = o.precision == Precision::F32;
+    if (f32) hip_ok(sov_hip_csr_make_f32(ctx, csr.h), "make_f32");
+    if (f32) S32.allocate(ctx, SOV_PREC_F32, m, n);
+    if (!f32 || o.fp64_polish) S64.allocate(ctx, SOV_PREC_F64, m, n);
+    // starting point: x = 0 and y = 0 or the optional dual warm starts
+    // (original units -> scaled)
+    x_start.assign(n, 0.0);
+    y_start.assign(m, 0.0);
+    cold.assign(B, 1);
+    if (warm_dual != nullptr)
+        for (std::size_t b = 0; b < B; ++b) {
+            if (idx[b] >= warm_dual->size() || (*warm_dual)[idx[b]].empty()) continue;
+            const std::vector<Real>& w = (*warm_dual)[idx[b]];
+            const auto r0 = static_cast<std::size_t>(row_off[b]);
+            const auto rows = static_cast<std::size_t>(row_off[b + 1] - row_off[b]);
+            if (w.size() != rows) throw Error(Status::DimensionMismatch, "PdhgBatch: warm_dual length");
+            for (std::size_t i = 0; i < rows; ++i) y_start[r0 + i] = sign * w[i] / rs[r0 + i];
+        }
+    if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, f32 ? SOV_PREC_F32 : SOV_PREC_F64), "tune");
+    // the batched step kernels run on the CSR parts only
+    for (int orient : {0, 1}) hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, 0), "tiles");
+    setup_seconds = seconds_since(t0);
+}
+
+/// The stacked natural-order objective and bounds of the instances as the
+/// session holds them now.
+void PdhgBatchSession::Impl::stacked_vectors(std::vector<Real>& c, std::vector<Real>& clo, std::vector<Real>& chi,
+                                             std::vector<Real>& rlo, std::vector<Real>& rhi) const {
+    c.resize(n);
+    clo.resize(n);
+    chi.resize(n);
+    rlo.resize(m);
+    rhi.resize(m);
+    for (std::size_t b = 0; b < B; ++b) {
+        const Problem& p = probs[idx[b]];
+        const auto r0 = static_cast<std::size_t>(row_off[b]), c0 = static_cast<std::size_t>(col_off[b]);
+        std::copy(p.objective().begin(), p.objective().end(), c.begin() + static_cast<std::ptrdiff_t>(c0));
+        std::copy(p.col_lower().begin(), p.col_lower().end(), clo.begin() + static_cast<std::ptrdiff_t>(c0));
+        std::copy(p.col_upper().begin(), p.col_upper().end(), chi.begin() + static_cast<std::ptrdiff_t>(c0));
+        std::copy(p.row_lower().begin(), p.row_lower().end(), rlo.begin() + static_cast<std::ptrdiff_t>(r0));
+        std::copy(p.row_upper().begin(), p.row_upper().end(), rhi.begin() + static_cast<std::ptrdiff_t>(r0));
+    }
+}
+
+/// Objective and bounds into the iterate states (scaled, internal sense,
+/// device order; reference scaling.cpp:172-192, pdhg.cpp:250-252) and the
+/// original-space arrays of the checks.
+void PdhgBatchSession::Impl::upload_data() {
+    std::vector<Real> c_nat, clo, chi, rlo, rhi;
+    stacked_vectors(c_nat, clo, chi, rlo, rhi);
+    std::vector<Real> c_s(n), rlo_s(m), rhi_s(m);
+    clo_s.resize(n);
+    chi_s.resize(n);
+    for (std::size_t j = 0; j < n; ++j) {
+        c_s[j] = (c_nat[j] * cs[j]) * sign;
+        clo_s[j] = is_finite_bound(clo[j]) ? clo[j] / cs[j] : clo[j];
+        chi_s[j] = is_finite_bound(chi[j]) ? chi[j] / cs[j] : chi[j];
+    }
+    for (std::size_t i = 0; i < m; ++i) {
+        rlo_s[i] = is_finite_bound(rlo[i]) ? rlo[i] * rs[i] : rlo[i];
+        rhi_s[i] = is_finite_bound(rhi[i]) ? rhi[i] * rs[i] : rhi[i];
+    }
+    const std::vector<Real> c_d = DeviceOrder::to_dev(c_s, ord.col), clo_d = DeviceOrder::to_dev(clo_s, ord.col),
+                            chi_d = DeviceOrder::to_dev(chi_s, ord.col), rlo_d = DeviceOrder::to_dev(rlo_s, ord.row),
+                            rhi_d = DeviceOrder::to_dev(rhi_s, ord.row);
+    for (DeviceState* S : {&S32, &S64}) {
+        if (S->x.p == nullptr) continue;   // state not allocated
+        S->upload(ctx, S->c, c_d);
+        S->upload(ctx, S->clo, clo_d);
+        S->upload(ctx, S->chi, chi_d);
+        S->upload(ctx, S->rlo, rlo_d);
+        S->upload(ctx, S->rhi, rhi_d);
+    }
+    auto upload_dev = [&](const DevMem& d, std::span<const Real> v, const std::vector<int32_t>& perm) {
+        if (v.empty()) return;
+        const std::vector<Real> pv = DeviceOrder::to_dev(v, perm);
+        hip_ok(sov_hip_memcpy_h2d(ctx, d.p, pv.data(), pv.size() * 8), "upload");
+    };
+    upload_dev(d_olo_r, rlo, ord.row);
+    upload_dev(d_ohi_r, rhi, ord.row);
+    upload_dev(d_olo_c, clo, ord.col);
+    upload_dev(d_ohi_c, chi, ord.col);
+    upload_dev(d_oc, c_nat, ord.col);
+}
+
+/// Every state's point from (x_start, y_start): x projected onto the column
+/// box and the slacks s = Proj_S(A x); a cold instance starts as the
+/// reference does, x = Proj_X(0), s = Proj_S(0) (pdhg.cpp:268-270), with y
+/// from the optional dual warm start.
+void PdhgBatchSession::Impl::place_start() {
+    const std::size_t mb = std::max<std::size_t>(m, 1) * 8, nb = std::max<std::size_t>(n, 1) * 8;
+    std::vector<Real> x_act(n, 0.0);   // the point whose activity sets s (0 for cold instances)
+    for (std::size_t b = 0; b < B; ++b) {
+        if (cold[b]) continue;
+        for (auto j = static_cast<std::size_t>(col_off[b]); j < static_cast<std::size_t>(col_off[b + 1]); ++j) {
+            Real v = x_start[j];
+            if (v < clo_s[j]) v = clo_s[j];
+            if (v > chi_s[j]) v = chi_s[j];
+            x_act[j] = v;
+        }
+    }
+    DevMem dx(ctx, nb), dy(ctx, mb), da(ctx, nb);
+    const std::vector<Real> xd = DeviceOrder::to_dev(x_start, ord.col), yd = DeviceOrder::to_dev(y_start, ord.row),
+                            ad = DeviceOrder::to_dev(x_act, ord.col);
+    if (n) hip_ok(sov_hip_memcpy_h2d(ctx, dx.p, xd.data(), n * 8), "upload start");
+    if (m) hip_ok(sov_hip_memcpy_h2d(ctx, dy.p, yd.data(), m * 8), "upload start");
+    if (n) hip_ok(sov_hip_memcpy_h2d(ctx, da.p, ad.data(), n * 8), "upload start");
+    hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, 1.0, da.p, 0.0, d_act.p), "start activity");
+    for (DeviceState* S : {&S32, &S64})
+        if (S->x.p != nullptr)
+            hip_ok(sov_hip_pdhg_set_point(ctx, &S->st, dx.as<double>(), dy.as<double>(), d_act.as<double>()),
+                   "set_point");
+    hip_ok(sov_hip_sync(ctx), "sync");
+}
+
+std::size_t PdhgBatchSession::Impl::stacked(std::size_t b, const char* what) const {
+    if (b >= probs.size()) throw Error(Status::InvalidArgument, std::string(what) + ": no instance " + std::to_string(b));
+    if (slot[b] < 0)
+        throw Error(Status::InvalidArgument, std::string(what) + ": instance " + std::to_string(b) +
+                                                 " failed validation and is not in the batch");
+    return static_cast<std::size_t>(slot[b]);
+}
+
+void PdhgBatchSession::Impl::replace(std::size_t b, Problem p, const char* what) {
+    require_valid(p, what);
+    probs[b] = std::move(p);
+    data_dirty = true;
+}
+
+PdhgBatchResult PdhgBatchSession::Impl::run() {
+    PdhgBatchResult out;
+    const auto t0 = Clock::now();
+    out.results = rejected;
+    if (B == 0) return out;
+    try {
+        if (data_dirty) upload_data();
+        place_start();
+        data_dirty = false;
 
         std::vector<int32_t> active(B, f32 ? 1 : 2), since(B, 0), flags32(B, 0), flags64(B, 0), since_stop(B, 0);
         std::vector<int32_t> stop_state(B, 0), promote(B, 0), use_avg(B, 0);
@@ -1702,8 +1816,7 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
                     prev_cand[b] = kInfinity;
                     promote[b] = 0;
                 }
-                if (o.verbose)
-                    std::fprintf(stderr, "  [pdhg batch] iter %lld: instances moved to FP64\n", it);
+                if (o.verbose) std::fprintf(stderr, "  [pdhg batch] iter %lld: instances moved to FP64\n", it);
             }
             for (std::size_t b = 0; b < B; ++b) {
                 flags32[b] = flags64[b] = 0;
@@ -1726,7 +1839,7 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
         out.iterations = it;
 
         // -- final points: each instance's candidate from the state it ended
-        //    in, natural order, unscaled
+        //    in, natural order, unscaled; the scaled one is the next start
         struct Host {
             std::vector<Real> x, y, xs, ys;
         };
@@ -1743,7 +1856,7 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
         if (!f32 || o.fp64_polish) h64 = fetch(S64);
         for (std::size_t b = 0; b < B; ++b) {
             PdhgResult& r = out.results[idx[b]];
-            const Problem& p = *problems[idx[b]];
+            const Problem& p = probs[idx[b]];
             const int where = (done[b] || failed[b]) ? stop_state[b] : active[b];
             const Host& hv = where == 1 ? h32 : h64;
             const auto r0 = static_cast<std::size_t>(row_off[b]), c0 = static_cast<std::size_t>(col_off[b]);
@@ -1753,11 +1866,19 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
             std::vector<Real> x(nb), y(mb);
             for (std::size_t j = 0; j < nb; ++j) x[j] = ua ? hv.xs[c0 + j] * inv : hv.x[c0 + j];
             for (std::size_t i = 0; i < mb; ++i) y[i] = ua ? hv.ys[r0 + i] * inv : hv.y[r0 + i];
+            // the next solve starts from this candidate (a failed instance from zero)
+            for (std::size_t j = 0; j < nb; ++j) x_start[c0 + j] = failed[b] ? 0.0 : x[j];
+            for (std::size_t i = 0; i < mb; ++i) y_start[r0 + i] = failed[b] ? 0.0 : y[i];
+            cold[b] = failed[b] ? 1 : 0;
             const Scaling sc = Scaling::from_factors(std::vector<Real>(rs.begin() + static_cast<std::ptrdiff_t>(r0),
                                                                        rs.begin() + static_cast<std::ptrdiff_t>(r0 + mb)),
                                                      std::vector<Real>(cs.begin() + static_cast<std::ptrdiff_t>(c0),
                                                                        cs.begin() + static_cast<std::ptrdiff_t>(c0 + nb)));
             auto [ux, uy] = unscale_point(sign, sc, x, y);
+            r = PdhgResult{};
+            r.engine = "batched";
+            r.operator_norm_estimate = est[b];
+            r.primal_weight = weight[b];
             r.iterations = done[b] || failed[b] ? its[b] : it;
             r.restarts = restarts[b];
             if (promoted_at[b] >= 0) r.fp64_polish_iterations = r.iterations - promoted_at[b];
@@ -1770,6 +1891,8 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
                 r.status = timed_out ? PdhgStatus::TimeLimit : PdhgStatus::IterationLimit;
             }
             finish_result(p, o, std::move(ux), std::move(uy), r);
+            last_x[idx[b]] = r.primal;
+            last_y[idx[b]] = r.dual;
             r.seconds = seconds_since(t0);
             r.setup_seconds = out.setup_seconds;
             r.loop_seconds = out.loop_seconds;
@@ -1783,6 +1906,133 @@ PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Ba
     }
     out.seconds = seconds_since(t0);
     return out;
+}
+
+// ----------------------------------------------------------------- sessions
+
+PdhgBatchSession::PdhgBatchSession(PdhgOptions options, const std::vector<const Problem*>& problems,
+                                   Backend& backend, const std::vector<std::vector<Real>>* warm_dual,
+                                   const std::vector<Real>* primal_weight)
+    : impl_(std::make_unique<Impl>(std::move(options), problems, backend, warm_dual, primal_weight)) {}
+
+PdhgBatchSession::~PdhgBatchSession() = default;
+
+std::size_t PdhgBatchSession::size() const noexcept { return impl_->probs.size(); }
+
+const Problem& PdhgBatchSession::problem(std::size_t b) const {
+    if (b >= impl_->probs.size()) throw Error(Status::InvalidArgument, "PdhgBatchSession::problem: no instance " + std::to_string(b));
+    return impl_->probs[b];
+}
+
+namespace {
+
+/// The instance rebuilt from its matrix with the given vectors (its
+/// sense and objective constant kept).
+Problem rebuilt(const Problem& p, std::vector<Real> c, std::vector<Real> clo, std::vector<Real> chi,
+                std::vector<Real> rlo, std::vector<Real> rhi) {
+    return Problem::from_csr(p.matrix(), std::move(c), std::move(clo), std::move(chi), std::move(rlo), std::move(rhi),
+                             p.sense(), p.objective_constant(), p.name());
+}
+
+std::vector<Real> vec(std::span<const Real> s) { return {s.begin(), s.end()}; }
+
+void require_length(std::size_t got, Idx want, const char* what) {
+    if (got != static_cast<std::size_t>(want))
+        throw Error(Status::DimensionMismatch, std::string(what) + ": " + std::to_string(got) + " values for " +
+                                                   std::to_string(want));
+}
+
+}  // namespace
+
+void PdhgBatchSession::set_row_bounds(std::size_t b, std::span<const Real> lower, std::span<const Real> upper) {
+    Impl& s = *impl_;
+    s.stacked(b, "set_row_bounds");
+    const Problem& p = s.probs[b];
+    require_length(lower.size(), p.num_rows(), "set_row_bounds");
+    require_length(upper.size(), p.num_rows(), "set_row_bounds");
+    s.replace(b, rebuilt(p, vec(p.objective()), vec(p.col_lower()), vec(p.col_upper()), vec(lower), vec(upper)),
+              "set_row_bounds");
+}
+
+void PdhgBatchSession::set_objective(std::size_t b, std::span<const Real> c) {
+    Impl& s = *impl_;
+    s.stacked(b, "set_objective");
+    const Problem& p = s.probs[b];
+    require_length(c.size(), p.num_cols(), "set_objective");
+    s.replace(b, rebuilt(p, vec(c), vec(p.col_lower()), vec(p.col_upper()), vec(p.row_lower()), vec(p.row_upper())),
+              "set_objective");
+}
+
+void PdhgBatchSession::set_column_bounds(std::size_t b, std::span<const Real> lower, std::span<const Real> upper) {
+    Impl& s = *impl_;
+    s.stacked(b, "set_column_bounds");
+    const Problem& p = s.probs[b];
+    require_length(lower.size(), p.num_cols(), "set_column_bounds");
+    require_length(upper.size(), p.num_cols(), "set_column_bounds");
+    s.replace(b, rebuilt(p, vec(p.objective()), vec(lower), vec(upper), vec(p.row_lower()), vec(p.row_upper())),
+              "set_column_bounds");
+}
+
+void PdhgBatchSession::set_start(std::size_t b, std::span<const Real> x, std::span<const Real> y) {
+    Impl& s = *impl_;
+    const std::size_t k = s.stacked(b, "set_start");
+    const Problem& p = s.probs[b];
+    const auto r0 = static_cast<std::size_t>(s.row_off[k]), c0 = static_cast<std::size_t>(s.col_off[k]);
+    if (!x.empty()) {
+        require_length(x.size(), p.num_cols(), "set_start (x)");
+        for (std::size_t j = 0; j < x.size(); ++j) s.x_start[c0 + j] = x[j] / s.cs[c0 + j];
+    }
+    if (!y.empty()) {
+        require_length(y.size(), p.num_rows(), "set_start (y)");
+        for (std::size_t i = 0; i < y.size(); ++i) s.y_start[r0 + i] = s.sign * y[i] / s.rs[r0 + i];
+    }
+    s.cold[k] = 0;
+}
+
+void PdhgBatchSession::warm_start_from(std::size_t b, std::size_t source) {
+    Impl& s = *impl_;
+    s.stacked(b, "warm_start_from");
+    s.stacked(source, "warm_start_from");
+    if (s.last_x[source].empty())
+        throw Error(Status::InvalidArgument, "warm_start_from: instance " + std::to_string(source) + " has no solution yet");
+    if (s.probs[b].num_rows() != s.probs[source].num_rows() || s.probs[b].num_cols() != s.probs[source].num_cols())
+        throw Error(Status::DimensionMismatch, "warm_start_from: instances " + std::to_string(b) + " and " +
+                                                   std::to_string(source) + " differ in size");
+    const std::vector<Real> x = s.last_x[source], y = s.last_y[source];
+    set_start(b, x, y);
+}
+
+PdhgBatchResult PdhgBatchSession::solve() {
+    const double first_setup = impl_->setup_seconds;
+    impl_->setup_seconds = 0.0;
+    PdhgBatchResult out = impl_->run();
+    // the first solve carries the session's construction
+    out.setup_seconds += first_setup;
+    out.seconds += first_setup;
+    for (PdhgResult& r : out.results) {
+        r.setup_seconds += first_setup;
+        r.seconds += first_setup;
+    }
+    return out;
+}
+
+PdhgBatchResult PdhgBatch::solve(const std::vector<const Problem*>& problems, Backend& backend,
+                                  const std::vector<std::vector<Real>>* warm_dual,
+                                  const std::vector<Real>* primal_weight) const {
+    try {
+        PdhgBatchSession session(options_, problems, backend, warm_dual, primal_weight);
+        return session.solve();
+    } catch (const Error& e) {
+        PdhgBatchResult out;
+        out.message = e.what();
+        out.results.resize(problems.size());
+        for (PdhgResult& r : out.results) {
+            r.engine = "batched";
+            r.status = PdhgStatus::NumericalFailure;
+            r.message = out.message;
+        }
+        return out;
+    }
 }
 
 }  // namespace sov
