@@ -37,4 +37,12 @@ ReadResult read_lp(const std::string& path);
 ReadResult read_lp_string(const std::string& text, const std::string& origin = "<string>");
 ReadResult read_model(const std::string& path);
 
+/// Additive: the model as free-format MPS text that read_mps reads back to
+/// the same arrays bit for bit (values in shortest round-trip decimal form).
+/// Rows and columns take the model's names (R<i> / C<j> for a model built
+/// without them); two-sided rows are written as G rows with a range, the
+/// objective constant as an RHS entry on the objective row.
+std::string write_mps_string(const Problem& problem);
+void write_mps(const Problem& problem, const std::string& path);
+
 }  // namespace sov

@@ -29,6 +29,8 @@ sov_lp* sov_lp_from_csr(int32_t m, int32_t n, int32_t nnz, const int32_t* row_pt
 sov_lp* sov_lp_from_text(const char* text, int32_t fmt);
 sov_lp* sov_lp_read_file(const char* path);
 void sov_lp_destroy(sov_lp* lp);
+/* Write the model as MPS (sov::write_mps); reading it back gives the same arrays. */
+int sov_lp_write_mps(const sov_lp* lp, const char* path);
 int sov_lp_dims(const sov_lp* lp, int32_t* m, int32_t* n, int32_t* nnz);
 /* Copy the model out; any pointer may be NULL. */
 int sov_lp_arrays(const sov_lp* lp, int32_t* row_ptr, int32_t* col_idx, double* val, double* c, double* col_lo,

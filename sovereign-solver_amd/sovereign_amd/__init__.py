@@ -185,6 +185,12 @@ class LP:
     def transportation(cls, sources, sinks, seed):
         return cls(engine().sov_gen_transportation(sources, sinks, seed))
 
+    def write_mps(self, path: str):
+        L = engine()
+        L.sov_lp_write_mps.argtypes = [C.c_void_p, C.c_char_p]
+        if L.sov_lp_write_mps(self._h, path.encode()) != 0:
+            raise RuntimeError("write_mps failed: " + last_error())
+
     def dims(self):
         m, n, z = C.c_int32(), C.c_int32(), C.c_int32()
         engine().sov_lp_dims(self._h, C.byref(m), C.byref(n), C.byref(z))

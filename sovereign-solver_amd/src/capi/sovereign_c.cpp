@@ -73,6 +73,16 @@ sov_lp* sov_lp_read_file(const char* path) {
 
 void sov_lp_destroy(sov_lp* lp) { delete lp; }
 
+int sov_lp_write_mps(const sov_lp* lp, const char* path) {
+    if (lp == nullptr || path == nullptr) return set_err(3, "null argument");
+    try {
+        sov::write_mps(lp->p, path);
+        return 0;
+    } catch (const std::exception& e) {
+        return set_err(6, e.what());
+    }
+}
+
 int sov_lp_dims(const sov_lp* lp, int32_t* m, int32_t* n, int32_t* nnz) {
     if (lp == nullptr) return set_err(3, "null model");
     if (m) *m = lp->p.num_rows();

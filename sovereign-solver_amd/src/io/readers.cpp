@@ -8,16 +8,18 @@
 // General/Integer; Binary; End.  Anything else is rejected with an Error.
 //
 // MPS: free-format tokens (names without blanks), sections NAME, OBJSENSE,
-// ROWS, COLUMNS (with MARKER INTORG/INTEND), RHS (an entry on the objective
-// row sets the constant to minus that value), RANGES, BOUNDS (UP LO FX FR MI
-// PL BV LI UI; UP < 0 on a column with lower bound 0 moves the lower bound to
-// -inf under the default rule), ENDATA.  Gzip input is not supported.
+// ROWS, COLUMNS (with MARKER INTORG/INTEND), RHS, RANGES, BOUNDS (UP LO FX FR
+// MI PL BV LI UI), ENDATA; rules below, at the MPS reader.  Gzip input is not
+// supported.
 #include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <cmath>
 #include <fstream>
 #include <map>
 #include <sstream>
+#include <string_view>
+#include <unordered_set>
 
 #include "sovereign/io.hpp"
 
@@ -160,6 +162,8 @@ public:
         ReadResult r;
         r.problem = b_.finish();
         r.format = "lp";
+        std::string why;
+        if (r.problem.validate(&why) != Status::Ok) throw Error(origin_ + ": invalid model: " + why);
         return r;
     }
 
@@ -370,188 +374,465 @@ ReadResult read_lp_string(const std::string& text, const std::string& origin) { 
 ReadResult read_lp(const std::string& path) { return read_lp_string(slurp(path), path); }
 
 // ----------------------------------------------------------------- MPS ---
+//
+// One pass over the text, tokens as views into it (no per-line streams), so
+// a million-column file reads at memory speed.  Semantics follow the
+// reference reader (src/io/mps_reader.cpp): numbers must parse completely
+// ("expected a number"); bound values accept INF / INFINITY / |v| >= 1e30 as
+// infinite; a valued bound card without a value is an error; the column of a
+// BOUNDS card is its last token (valueless types) or the one before the
+// value, and an unknown one is added with a warning; only UP moves a default
+// lower bound of 0 to -inf when its value is negative (MpsOptions); a second
+// N row is kept as a free row (keep_free_rows) or dropped together with every
+// coefficient, right-hand side and range naming it; an RHS entry on the
+// objective row sets the objective constant to minus its value; RANGES
+// widens L and G rows by |R| and E rows by the signed R.
+
+namespace {
+
+using Tokens = std::vector<std::string_view>;
+
+bool iequal(std::string_view a, std::string_view b) {
+    if (a.size() != b.size()) return false;
+    for (std::size_t i = 0; i < a.size(); ++i)
+        if (std::toupper(static_cast<unsigned char>(a[i])) != std::toupper(static_cast<unsigned char>(b[i])))
+            return false;
+    return true;
+}
+
+/// A complete decimal number, or nothing.
+bool to_number(std::string_view t, double* out) {
+    if (t.empty()) return false;
+    const char* b = t.data();
+    const char* e = b + t.size();
+    if (*b == '+') ++b;   // from_chars takes no leading '+'
+    const auto r = std::from_chars(b, e, *out);
+    return r.ec == std::errc() && r.ptr == e;
+}
+
+class MpsText {
+public:
+    MpsText(const std::string& text, const MpsOptions& opt, const std::string& origin)
+        : text_(text), opt_(opt), origin_(origin) {}
+
+    ReadResult read() {
+        std::size_t pos = 0;
+        Tokens tok;
+        while (pos < text_.size()) {
+            std::size_t eol = text_.find('\n', pos);
+            if (eol == std::string::npos) eol = text_.size();
+            const std::string_view line(text_.data() + pos, eol - pos);
+            pos = eol + 1;
+            ++line_;
+            if (line.empty() || line[0] == '*') continue;
+            split(line, tok);
+            if (tok.empty()) continue;
+            if (!std::isspace(static_cast<unsigned char>(line[0]))) {
+                if (!header(tok)) break;   // ENDATA
+                continue;
+            }
+            card(tok);
+        }
+        res_.problem = b_.finish();
+        res_.format = "mps";
+        // a model that cannot be solved is not handed on (the reference's
+        // readers validate too)
+        std::string why;
+        if (res_.problem.validate(&why) != Status::Ok) throw Error(origin_ + ": invalid model: " + why);
+        return std::move(res_);
+    }
+
+private:
+    enum class Sec { None, Name, ObjSense, Rows, Columns, Rhs, Ranges, Bounds };
+
+    [[noreturn]] void fail(const std::string& msg) const {
+        throw Error(origin_ + ":" + std::to_string(line_) + ": " + msg);
+    }
+    void warn(const char* section, std::string msg) { res_.warnings.push_back({line_, section, std::move(msg)}); }
+
+    static void split(std::string_view line, Tokens& tok) {
+        tok.clear();
+        std::size_t i = 0;
+        while (i < line.size()) {
+            while (i < line.size() && std::isspace(static_cast<unsigned char>(line[i]))) ++i;
+            const std::size_t s = i;
+            while (i < line.size() && !std::isspace(static_cast<unsigned char>(line[i]))) ++i;
+            if (i > s) tok.push_back(line.substr(s, i - s));
+        }
+    }
+
+    double number(std::string_view t) const {
+        double v = 0.0;
+        if (!to_number(t, &v)) fail("expected a number, got '" + std::string(t) + "'");
+        return v;
+    }
+
+    double bound_number(std::string_view t) const {
+        std::string_view u = t;
+        bool negative = false;
+        if (!u.empty() && (u[0] == '+' || u[0] == '-')) {
+            negative = u[0] == '-';
+            u.remove_prefix(1);
+        }
+        if (iequal(u, "INF") || iequal(u, "INFINITY")) return negative ? -kInfinity : kInfinity;
+        const double v = number(t);
+        if (std::abs(v) >= 1e30) return v > 0 ? kInfinity : -kInfinity;
+        return v;
+    }
+
+    void objsense(std::string_view t) {
+        if (iequal(t, "MAX") || iequal(t, "MAXIMIZE")) b_.set_sense(ObjSense::Maximize);
+        else if (iequal(t, "MIN") || iequal(t, "MINIMIZE")) b_.set_sense(ObjSense::Minimize);
+        else fail("unknown OBJSENSE '" + std::string(t) + "'");
+    }
+
+    /// A section header; false at ENDATA.
+    bool header(const Tokens& tok) {
+        const std::string_view h = tok[0];
+        if (iequal(h, "ENDATA")) return false;
+        if (iequal(h, "NAME")) {
+            sec_ = Sec::Name;
+            b_.set_name(tok.size() > 1 ? std::string(tok[1]) : std::string());
+        } else if (iequal(h, "OBJSENSE")) {
+            sec_ = Sec::ObjSense;
+            if (tok.size() > 1) objsense(tok[1]);
+        } else if (iequal(h, "ROWS")) {
+            sec_ = Sec::Rows;
+        } else if (iequal(h, "COLUMNS")) {
+            sec_ = Sec::Columns;
+        } else if (iequal(h, "RHS")) {
+            sec_ = Sec::Rhs;
+        } else if (iequal(h, "RANGES")) {
+            sec_ = Sec::Ranges;
+        } else if (iequal(h, "BOUNDS")) {
+            sec_ = Sec::Bounds;
+        } else {
+            fail("unknown section '" + std::string(h) + "'");
+        }
+        return true;
+    }
+
+    void card(const Tokens& tok) {
+        switch (sec_) {
+            case Sec::ObjSense: objsense(tok[0]); break;
+            case Sec::Rows: row(tok); break;
+            case Sec::Columns: column(tok); break;
+            case Sec::Rhs: rhs(tok); break;
+            case Sec::Ranges: range(tok); break;
+            case Sec::Bounds: bound(tok); break;
+            default: fail("data outside a known section");
+        }
+    }
+
+    void row(const Tokens& tok) {
+        if (tok.size() < 2) fail("ROWS card needs a type and a name");
+        const std::string name(tok[1]);
+        char sense = static_cast<char>(std::toupper(static_cast<unsigned char>(tok[0][0])));
+        if (tok[0].size() != 1 || std::string_view("NLGE").find(sense) == std::string_view::npos)
+            fail("unknown row type '" + std::string(tok[0]) + "'");
+        if (sense == 'N') {
+            if (obj_.empty()) {
+                obj_ = name;
+                b_.set_objective_name(name);
+                return;
+            }
+            if (!opt_.keep_free_rows) {
+                dropped_.insert(name);
+                return;
+            }
+        }
+        const double lo = sense == 'G' || sense == 'E' ? 0.0 : -kInfinity;
+        const double hi = sense == 'L' || sense == 'E' ? 0.0 : kInfinity;
+        b_.add_row(name, lo, hi);
+        sense_.push_back(sense);
+        rhs_.push_back(0.0);
+    }
+
+    /// The row a card names, or -1 for a dropped free row.
+    Idx row_of(std::string_view name, const char* section) {
+        const std::string key(name);
+        const Idx r = b_.find_row(key);
+        if (r >= 0) return r;
+        if (dropped_.count(key)) return -1;
+        fail("unknown row '" + key + "' in " + section);
+    }
+
+    void column(const Tokens& tok) {
+        bool marker = false, start = false, end = false;
+        for (std::string_view t : tok) {
+            std::string u;
+            for (char ch : t)
+                if (ch != '\'') u.push_back(static_cast<char>(std::toupper(static_cast<unsigned char>(ch))));
+            marker = marker || u == "MARKER";
+            start = start || u == "INTORG";
+            end = end || u == "INTEND";
+        }
+        if (marker && (start || end)) {
+            integer_ = start;
+            return;
+        }
+        if (tok.size() < 3) fail("COLUMNS card needs a column, a row and a value");
+        if (tok[0] != last_col_name_) {   // cards of one column come together
+            last_col_name_ = std::string(tok[0]);
+            last_col_ = b_.find_column(last_col_name_);
+            if (last_col_ < 0) last_col_ = b_.add_column(last_col_name_);
+        }
+        if (integer_) b_.set_column_kind(last_col_, VarKind::Integer);
+        for (std::size_t k = 1; k + 1 < tok.size(); k += 2) {
+            const double v = number(tok[k + 1]);
+            if (tok[k] == obj_) {
+                b_.set_objective_coefficient(last_col_, v);
+                continue;
+            }
+            const Idx r = row_of(tok[k], "COLUMNS");
+            if (r >= 0) b_.add_coefficient(r, last_col_, v);
+        }
+    }
+
+    void rhs(const Tokens& tok) {
+        if (tok.size() < 2) fail("RHS card needs a row and a value");
+        for (std::size_t k = tok.size() % 2; k + 1 < tok.size(); k += 2) {   // odd count: a set name first
+            const double v = number(tok[k + 1]);
+            if (tok[k] == obj_) {
+                b_.set_objective_constant(-v);
+                continue;
+            }
+            const Idx r = row_of(tok[k], "RHS");
+            if (r < 0) continue;
+            const auto i = static_cast<std::size_t>(r);
+            rhs_[i] = v;
+            switch (sense_[i]) {
+                case 'L': b_.set_row_bounds(r, -kInfinity, v); break;
+                case 'G': b_.set_row_bounds(r, v, kInfinity); break;
+                case 'E': b_.set_row_bounds(r, v, v); break;
+                default: warn("RHS", "right-hand side of free row '" + std::string(tok[k]) + "' ignored");
+            }
+        }
+    }
+
+    void range(const Tokens& tok) {
+        if (tok.size() < 2) fail("RANGES card needs a row and a value");
+        for (std::size_t k = tok.size() % 2; k + 1 < tok.size(); k += 2) {
+            const Idx r = row_of(tok[k], "RANGES");
+            const double R = number(tok[k + 1]);
+            if (r < 0) continue;
+            const auto i = static_cast<std::size_t>(r);
+            const double v = rhs_[i];
+            switch (sense_[i]) {
+                case 'L': b_.set_row_bounds(r, v - std::abs(R), v); break;
+                case 'G': b_.set_row_bounds(r, v, v + std::abs(R)); break;
+                case 'E':
+                    if (R >= 0) b_.set_row_bounds(r, v, v + R);
+                    else b_.set_row_bounds(r, v + R, v);
+                    break;
+                default: warn("RANGES", "range of free row '" + std::string(tok[k]) + "' ignored");
+            }
+        }
+    }
+
+    void bound(const Tokens& tok) {
+        if (tok.size() < 2) fail("BOUNDS card needs a type and a column");
+        std::string type;
+        for (char ch : tok[0]) type.push_back(static_cast<char>(std::toupper(static_cast<unsigned char>(ch))));
+        const bool valueless = type == "FR" || type == "MI" || type == "PL" || type == "BV";
+        if (!valueless && tok.size() < 3) fail("bound type " + type + " needs a value");
+        const std::string name(tok[valueless ? tok.size() - 1 : tok.size() - 2]);
+        Idx c = b_.find_column(name);
+        if (c < 0) {
+            warn("BOUNDS", "bound on column '" + name + "', which has no coefficients");
+            c = b_.add_column(name);
+        }
+        const double lo = b_.column_lower(c), hi = b_.column_upper(c);
+        const double v = valueless ? 0.0 : bound_number(tok.back());
+        if (type == "UP") {
+            if (v < 0.0 && lo == 0.0 && opt_.negative_upper_bound == NegativeUpperBoundRule::LowerBecomesMinusInfinity) {
+                b_.set_column_bounds(c, -kInfinity, v);
+                warn("BOUNDS", "negative UP bound on '" + name + "' with the default lower bound 0: lower bound "
+                               "set to -inf");
+            } else {
+                b_.set_column_bounds(c, lo, v);
+                if (v < 0.0 && lo == 0.0) warn("BOUNDS", "negative UP bound on '" + name + "' leaves crossed bounds");
+            }
+        } else if (type == "LO") {
+            b_.set_column_bounds(c, v, hi);
+        } else if (type == "FX") {
+            b_.set_column_bounds(c, v, v);
+        } else if (type == "FR") {
+            b_.set_column_bounds(c, -kInfinity, kInfinity);
+        } else if (type == "MI") {
+            b_.set_column_bounds(c, -kInfinity, hi);
+        } else if (type == "PL") {
+            b_.set_column_bounds(c, lo, kInfinity);
+        } else if (type == "BV") {
+            b_.set_column_bounds(c, 0.0, 1.0);
+            b_.set_column_kind(c, VarKind::Integer);
+        } else if (type == "LI") {
+            b_.set_column_bounds(c, v, hi);
+            b_.set_column_kind(c, VarKind::Integer);
+        } else if (type == "UI") {
+            b_.set_column_bounds(c, lo, v);
+            b_.set_column_kind(c, VarKind::Integer);
+        } else {
+            fail("unsupported bound type '" + std::string(tok[0]) + "'");
+        }
+    }
+
+    const std::string& text_;
+    const MpsOptions& opt_;
+    const std::string& origin_;
+    int line_ = 0;
+    Sec sec_ = Sec::None;
+    Problem::Builder b_;
+    ReadResult res_;
+    std::string obj_;
+    std::vector<char> sense_;
+    std::vector<double> rhs_;
+    std::unordered_set<std::string> dropped_;
+    bool integer_ = false;
+    std::string last_col_name_;
+    Idx last_col_ = -1;
+};
+
+}  // namespace
 
 ReadResult read_mps_string(const std::string& text, const MpsOptions& opt, const std::string& origin) {
-    Problem::Builder b;
-    ReadResult res;
-    std::string obj_row;
-    std::istringstream in(text);
-    std::string line, section;
-    int ln = 0;
-    bool integer_block = false;
-    std::map<Idx, double> rhs, ranges;
-    auto err = [&](const std::string& m) { return Error(origin + ":" + std::to_string(ln) + ": " + m); };
-    std::vector<std::string> tok;
-    while (std::getline(in, line)) {
-        ++ln;
-        if (line.empty() || line[0] == '*') continue;
-        tok.clear();
-        {
-            std::istringstream ls(line);
-            std::string w;
-            while (ls >> w) tok.push_back(w);
+    return MpsText(text, opt, origin).read();
+}
+
+// --------------------------------------------------------- MPS writer ---
+
+namespace {
+
+/// Appends `v` in its shortest round-trip decimal form.
+void put_number(std::string& out, double v) {
+    char buf[32];
+    const auto r = std::to_chars(buf, buf + sizeof buf, v);
+    out.append(buf, r.ptr);
+}
+
+}  // namespace
+
+std::string write_mps_string(const Problem& p) {
+    const std::span<const std::string> rn = p.row_names(), cn = p.col_names();
+    const Idx m = p.num_rows(), n = p.num_cols();
+    std::string out;
+    out.reserve(static_cast<std::size_t>(p.num_nonzeros()) * 40 + static_cast<std::size_t>(m + n) * 24 + 256);
+    std::string obj = "OBJ";   // the objective row: a name no row has
+    {
+        const std::unordered_set<std::string_view> used(rn.begin(), rn.end());
+        while (used.count(obj)) obj += '_';
+    }
+    out += "NAME ";
+    out += p.name().empty() ? std::string("model") : p.name();
+    out += "\n";
+    if (p.sense() == ObjSense::Maximize) out += "OBJSENSE\n    MAX\n";
+    out += "ROWS\n N  " + obj + "\n";
+    std::vector<char> sense(static_cast<std::size_t>(m));
+    for (Idx i = 0; i < m; ++i) {
+        const Real lo = p.row_lower()[i], hi = p.row_upper()[i];
+        char t = 'N';
+        if (is_finite_bound(lo) && is_finite_bound(hi)) t = lo == hi ? 'E' : 'G';
+        else if (is_finite_bound(lo)) t = 'G';
+        else if (is_finite_bound(hi)) t = 'L';
+        sense[static_cast<std::size_t>(i)] = t;
+        out += ' ';
+        out += t;
+        out += "  ";
+        out += rn[static_cast<std::size_t>(i)];
+        out += '\n';
+    }
+    out += "COLUMNS\n";
+    const CscMatrix A = p.matrix().to_csc();
+    bool in_integer = false;
+    int marker = 0;
+    for (Idx j = 0; j < n; ++j) {
+        const bool integer = p.col_kind()[static_cast<std::size_t>(j)] == VarKind::Integer;
+        if (integer != in_integer) {
+            out += "    M" + std::to_string(marker++) + "  'MARKER'  " + (integer ? "'INTORG'\n" : "'INTEND'\n");
+            in_integer = integer;
         }
-        if (tok.empty()) continue;
-        const bool header = !std::isspace(static_cast<unsigned char>(line[0]));
-        if (header) {
-            section = tok[0];
-            for (char& ch : section) ch = static_cast<char>(std::toupper(static_cast<unsigned char>(ch)));
-            if (section == "NAME") b.set_name(tok.size() > 1 ? tok[1] : "");
-            if (section == "OBJSENSE" && tok.size() > 1) {
-                const std::string s = lower(tok[1]);
-                b.set_sense(s.rfind("max", 0) == 0 ? ObjSense::Maximize : ObjSense::Minimize);
+        const std::string& name = cn[static_cast<std::size_t>(j)];
+        const Real c = p.objective()[j];
+        const Idx b = A.col_ptr()[j], e = A.col_ptr()[j + 1];
+        if (c != 0.0 || b == e) {   // every column appears at least once
+            out += "    " + name + "  " + obj + "  ";
+            put_number(out, c);
+            out += '\n';
+        }
+        for (Idx k = b; k < e; ++k) {
+            out += "    " + name + "  ";
+            out += rn[static_cast<std::size_t>(A.row_idx()[k])];
+            out += "  ";
+            put_number(out, A.values()[k]);
+            out += '\n';
+        }
+    }
+    if (in_integer) out += "    M" + std::to_string(marker) + "  'MARKER'  'INTEND'\n";
+    out += "RHS\n";
+    if (p.objective_constant() != 0.0) {
+        out += "    RHS  " + obj + "  ";
+        put_number(out, -p.objective_constant());
+        out += '\n';
+    }
+    std::string ranges;
+    for (Idx i = 0; i < m; ++i) {
+        const char t = sense[static_cast<std::size_t>(i)];
+        if (t == 'N') continue;
+        const Real lo = p.row_lower()[i], hi = p.row_upper()[i];
+        const Real v = t == 'L' ? hi : lo;
+        if (v != 0.0) {
+            out += "    RHS  " + rn[static_cast<std::size_t>(i)] + "  ";
+            put_number(out, v);
+            out += '\n';
+        }
+        if (t == 'G' && is_finite_bound(hi)) {
+            // G rows read a range R as [lo, lo + |R|]; hi - lo must give hi back
+            const Real R = hi - lo;
+            if (lo + R != hi) throw Error("write_mps: row " + rn[static_cast<std::size_t>(i)] +
+                                          " has a range that does not round-trip");
+            ranges += "    RNG  " + rn[static_cast<std::size_t>(i)] + "  ";
+            put_number(ranges, R);
+            ranges += '\n';
+        }
+    }
+    if (!ranges.empty()) out += "RANGES\n" + ranges;
+    out += "BOUNDS\n";
+    for (Idx j = 0; j < n; ++j) {
+        const Real lo = p.col_lower()[j], hi = p.col_upper()[j];
+        const std::string& name = cn[static_cast<std::size_t>(j)];
+        auto card = [&](const char* type, const Real* v) {
+            out += ' ';
+            out += type;
+            out += " BND  " + name;
+            if (v != nullptr) {
+                out += "  ";
+                put_number(out, *v);
             }
-            if (section == "ENDATA") break;
+            out += '\n';
+        };
+        const bool lf = is_finite_bound(lo), hf = is_finite_bound(hi);
+        if (lf && hf && lo == hi) {
+            card("FX", &lo);
             continue;
         }
-        if (section == "OBJSENSE") {
-            const std::string s = lower(tok[0]);
-            b.set_sense(s.rfind("max", 0) == 0 ? ObjSense::Maximize : ObjSense::Minimize);
-        } else if (section == "ROWS") {
-            if (tok.size() < 2) throw err("ROWS entry needs a type and a name");
-            const char ty = static_cast<char>(std::toupper(static_cast<unsigned char>(tok[0][0])));
-            if (ty == 'N') {
-                if (obj_row.empty()) {
-                    obj_row = tok[1];
-                    b.set_objective_name(tok[1]);
-                    continue;
-                }
-                if (!opt.keep_free_rows) continue;
-                b.add_row(tok[1], -kInfinity, kInfinity);
-            } else if (ty == 'L') {
-                b.add_row(tok[1], -kInfinity, 0.0);
-            } else if (ty == 'G') {
-                b.add_row(tok[1], 0.0, kInfinity);
-            } else if (ty == 'E') {
-                b.add_row(tok[1], 0.0, 0.0);
-            } else {
-                throw err("unknown row type " + tok[0]);
-            }
-        } else if (section == "COLUMNS") {
-            if (tok.size() >= 3 && lower(tok[1]) == "'marker'") {
-                const std::string mk = lower(tok[2]);
-                if (mk == "'intorg'") integer_block = true;
-                else if (mk == "'intend'") integer_block = false;
-                continue;
-            }
-            if (tok.size() < 3 || tok.size() % 2 == 0) throw err("COLUMNS entry needs name/value pairs");
-            Idx c = b.find_column(tok[0]);
-            if (c < 0) {
-                c = b.add_column(tok[0]);
-                if (integer_block) {
-                    b.set_column_kind(c, VarKind::Integer);
-                    b.set_column_bounds(c, 0.0, kInfinity);
-                }
-            }
-            for (std::size_t k = 1; k + 1 < tok.size(); k += 2) {
-                const double v = std::strtod(tok[k + 1].c_str(), nullptr);
-                if (tok[k] == obj_row) {
-                    b.set_objective_coefficient(c, v);
-                    continue;
-                }
-                const Idx r = b.find_row(tok[k]);
-                if (r < 0) throw err("unknown row " + tok[k]);
-                b.add_coefficient(r, c, v);
-            }
-        } else if (section == "RHS" || section == "RANGES") {
-            const std::size_t start = (tok.size() % 2 == 1) ? 1 : 0;   // optional set name
-            for (std::size_t k = start; k + 1 < tok.size(); k += 2) {
-                const double v = std::strtod(tok[k + 1].c_str(), nullptr);
-                if (section == "RHS" && tok[k] == obj_row) {
-                    b.set_objective_constant(-v);
-                    continue;
-                }
-                const Idx r = b.find_row(tok[k]);
-                if (r < 0) throw err("unknown row " + tok[k]);
-                (section == "RHS" ? rhs : ranges)[r] = v;
-            }
-        } else if (section == "BOUNDS") {
-            if (tok.size() < 3) throw err("BOUNDS entry too short");
-            const std::string ty = lower(tok[0]);
-            const bool no_value = ty == "fr" || ty == "mi" || ty == "pl" || ty == "bv";
-            const std::string& cname = tok[2];
-            (void)no_value;
-            Idx c = b.find_column(cname);
-            if (c < 0) throw err("unknown column " + cname);
-            const double v = tok.size() > 3 ? std::strtod(tok[3].c_str(), nullptr) : 0.0;
-            const double lo = b.column_lower(c), hi = b.column_upper(c);
-            if (ty == "up" || ty == "ui") {
-                double nlo = lo;
-                if (v < 0.0 && lo == 0.0 && opt.negative_upper_bound == NegativeUpperBoundRule::LowerBecomesMinusInfinity) {
-                    nlo = -kInfinity;
-                    res.warnings.push_back({ln, "BOUNDS", "negative UP bound moved the lower bound of " + cname + " to -inf"});
-                }
-                b.set_column_bounds(c, nlo, v);
-                if (ty == "ui") b.set_column_kind(c, VarKind::Integer);
-            } else if (ty == "lo" || ty == "li") {
-                b.set_column_bounds(c, v, hi);
-                if (ty == "li") b.set_column_kind(c, VarKind::Integer);
-            } else if (ty == "fx") {
-                b.set_column_bounds(c, v, v);
-            } else if (ty == "fr") {
-                b.set_column_bounds(c, -kInfinity, kInfinity);
-            } else if (ty == "mi") {
-                b.set_column_bounds(c, -kInfinity, hi);
-            } else if (ty == "pl") {
-                b.set_column_bounds(c, lo, kInfinity);
-            } else if (ty == "bv") {
-                b.set_column_bounds(c, 0.0, 1.0);
-                b.set_column_kind(c, VarKind::Integer);
-            } else {
-                throw err("unknown bound type " + tok[0]);
-            }
-        } else {
-            throw err("data outside a known section");
+        if (!lf && !hf) {
+            card("FR", nullptr);
+            continue;
         }
+        if (!lf) card("MI", nullptr);
+        else if (lo != 0.0) card("LO", &lo);
+        if (hf) card("UP", &hi);   // after MI / LO, so a negative UP keeps the lower bound written
     }
-    // apply RHS and RANGES to the row bounds
-    std::vector<char> types;
-    {
-        // recover the row order's types
-        std::istringstream in2(text);
-        std::string l2, sec2;
-        while (std::getline(in2, l2)) {
-            if (l2.empty() || l2[0] == '*') continue;
-            std::istringstream ls(l2);
-            std::vector<std::string> tk;
-            std::string w;
-            while (ls >> w) tk.push_back(w);
-            if (tk.empty()) continue;
-            if (!std::isspace(static_cast<unsigned char>(l2[0]))) {
-                sec2 = tk[0];
-                for (char& ch : sec2) ch = static_cast<char>(std::toupper(static_cast<unsigned char>(ch)));
-                continue;
-            }
-            if (sec2 == "ROWS" && tk.size() >= 2 && tk[1] != obj_row) {
-                const char ty = static_cast<char>(std::toupper(static_cast<unsigned char>(tk[0][0])));
-                if (ty == 'N' && !opt.keep_free_rows) continue;
-                types.push_back(ty);
-            }
-        }
-    }
-    for (Idx r = 0; r < b.num_rows(); ++r) {
-        const char ty = types[static_cast<std::size_t>(r)];
-        const double v = rhs.count(r) ? rhs[r] : 0.0;
-        double lo = -kInfinity, hi = kInfinity;
-        if (ty == 'L') hi = v;
-        else if (ty == 'G') lo = v;
-        else if (ty == 'E') lo = hi = v;
-        if (ranges.count(r) && ty != 'N') {
-            const double R = ranges[r];
-            if (ty == 'E') {
-                if (R > 0) hi = v + std::abs(R);
-                else lo = v - std::abs(R);
-            } else if (ty == 'L') {
-                lo = v - std::abs(R);
-            } else if (ty == 'G') {
-                hi = v + std::abs(R);
-            }
-        }
-        b.set_row_bounds(r, lo, hi);
-    }
-    res.problem = b.finish();
-    res.format = "mps";
-    return res;
+    out += "ENDATA\n";
+    return out;
+}
+
+void write_mps(const Problem& p, const std::string& path) {
+    const std::string text = write_mps_string(p);
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw Error("cannot write " + path);
+    f.write(text.data(), static_cast<std::streamsize>(text.size()));
+    if (!f) throw Error("write failed: " + path);
 }
 
 ReadResult read_mps(const std::string& path, const MpsOptions& opt) { return read_mps_string(slurp(path), opt, path); }

@@ -226,3 +226,60 @@ def test_default_options_through_the_c_api():
     S.engine().sov_pdhg_default_opts(C.byref(o))
     assert o.autotune == 1 and o.tiles == -1
     assert o.autotune_after_seconds == 0.5
+
+
+def test_mps_edge_cases_read_as_the_reference_reads_them():
+    """Bound card forms and infinite spellings, missing and malformed
+    numbers, negative UP vs UI bounds, bounds on new columns, ranges, the
+    objective constant, kept free rows, integer markers: each text is read
+    to the same arrays as the reference's reader, or rejected as it rejects
+    it (tests/golden/reader_cases.json)."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "reader_cases.json")) as f:
+        cases = json.load(f)["cases"]
+    for name, case in cases.items():
+        if case["rejected"]:
+            with pytest.raises(RuntimeError):
+                S.LP.from_text(case["text"], mps=True)
+            continue
+        a = S.LP.from_text(case["text"], mps=True).arrays()
+        g = fixture_model(case["model"])
+        for k in ("row_ptr", "col_idx", "val", "c", "col_lo", "col_hi", "row_lo", "row_hi"):
+            np.testing.assert_array_equal(a[k], g[k], err_msg=f"{name}:{k}")
+        assert a["maximize"] == g["maximize"] and a["obj_const"] == g["obj_const"], name
+
+
+@pytest.mark.parametrize("make", [lambda: S.LP.random(200, 500, 5, 1), lambda: S.LP.transportation(20, 30, 0),
+                                  lambda: S.LP.random_zipf(3000, 7000, 2),
+                                  lambda: S.LP.read(os.path.join(ROOT, "tests", "golden", "smoke", "tiny_lp.mps"))])
+def test_mps_writer_round_trips(tmp_path, make):
+    lp = make()
+    path = str(tmp_path / "m.mps")
+    lp.write_mps(path)
+    a, b = lp.arrays(), S.LP.read(path).arrays()
+    for k in ("row_ptr", "col_idx", "val", "c", "col_lo", "col_hi", "row_lo", "row_hi"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    assert a["maximize"] == b["maximize"] and a["obj_const"] == b["obj_const"]
+
+
+def test_mps_reader_at_scale(tmp_path):
+    """A 100,000 x 200,000 Zipf LP (about 4.3M nonzeros) through an MPS file:
+    the arrays come back exactly and the read runs at more than 1M nonzeros
+    per second on one core (the reference's map-based assembly is the
+    bottleneck this path replaces; reference src/l0/sparse.cpp:36)."""
+    import time
+    lp = S.LP.random_zipf(100_000, 200_000, 4)
+    path = str(tmp_path / "big.mps")
+    t0 = time.perf_counter()
+    lp.write_mps(path)
+    t_write = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    back = S.LP.read(path)
+    t_read = time.perf_counter() - t0
+    a, b = lp.arrays(), back.arrays()
+    size = os.path.getsize(path)
+    print(f"{a['nnz']} nonzeros, {size / 2**20:.0f} MiB: write {t_write:.2f} s, read {t_read:.2f} s "
+          f"({a['nnz'] / t_read / 1e6:.1f}M nonzeros/s)", flush=True)
+    for k in ("row_ptr", "col_idx", "val", "c", "col_lo", "col_hi", "row_lo", "row_hi"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    assert a["nnz"] / t_read > 1e6
