@@ -197,21 +197,26 @@ def time_to_tolerance(S):
     (Pdhg::solve, fused HIP engine, adaptive restarts, relative-KKT stop) on
     four workloads: the 2,000 x 5,000 LP (5 nonzeros per column, seed 0) in
     FP64; the 2,000 x 2,000 transportation LP (4M columns, seed 0) and the
-    1M-column Zipf LP (seed 0), each with FP32 iterates and the FP64 polish;
-    and the 1M-column Zipf LP again in FP64 (the reference's precision).
+    1M-column Zipf LP (seed 0), each with FP32 iterates and iterative
+    refinement (FP32 corrections of an FP64 anchor, no FP64 iterates); the
+    Zipf LP again with FP32 iterates and the FP64 polish instead; and the
+    Zipf LP in FP64 (the reference's precision).
     `seconds` is the whole solve (setup included: scaling, layouts,
     autotuning, power iteration); `loop_seconds` the iterations alone."""
     cases = [("lp_2000x5000_f64", lambda: S.LP.random(2000, 5000, 5, 0), "f64"),
              ("transport_2000x2000_f32", lambda: S.LP.transportation(2000, 2000, 0), "f32"),
              ("zipf_500000x1000000_f32", lambda: S.LP.random_zipf(500000, 1000000, 0), "f32"),
+             ("zipf_500000x1000000_f32_polish", lambda: S.LP.random_zipf(500000, 1000000, 0), "f32-polish"),
              ("zipf_500000x1000000_f64", lambda: S.LP.random_zipf(500000, 1000000, 0), "f64")]
     out = {}
     for name, make, prec in cases:
         lp = make()
         r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
-                    restarts="adaptive", precision=prec, max_iterations=2000000, time_limit=240)
+                    restarts="adaptive", precision=prec[:3], refinement=prec == "f32", max_iterations=2000000,
+                    time_limit=240)
         out[name] = {"status": r.status, "seconds": round(r.seconds, 3), "loop_seconds": round(r.loop_seconds, 3),
-                     "iterations": r.iterations, "fp64_polish_iterations": r.fp64_polish_iterations,
+                     "iterations": r.iterations, "refinement_iterations": r.refinement_iterations,
+                     "fp64_polish_iterations": r.fp64_polish_iterations,
                      "relative_kkt": max(r.relative_kkt), "objective": r.objective}
         log(f"time to 1e-6: {name}: {r.status} in {r.seconds:.2f} s ({r.iterations} iterations, "
             f"{r.fp64_polish_iterations} FP64 polish)")
@@ -414,11 +419,14 @@ def main():
     pname = "f32" if prec == 1 else "f64"
 
     # the headline: the engine's own loop (Pdhg::solve, fused HIP engine),
-    # exactly `warmup` untimed iterations then exactly `steps` timed ones
+    # exactly `warmup` untimed iterations then exactly `steps` timed ones, all
+    # of them plain iterations of the one precision (no switch to FP64
+    # iterates or to refinement corrections inside the timed part; the
+    # time-to-tolerance runs cover those)
     eng = None
     if not args.no_engine_loop:
         r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=0.0, termination="relative",
-                    restarts="adaptive", precision=pname, fp64_polish=False, autotune_after=0.0,
+                    restarts="adaptive", precision=pname, fp64_polish=False, refinement=False, autotune_after=0.0,
                     max_iterations=args.warmup + args.steps, timing_mark=args.warmup)
         assert r.timed_iterations == args.steps, (r.timed_iterations, args.steps, r.status, r.message)
         eng = {"iterations_per_s": args.steps / r.timed_seconds, "timed_seconds": r.timed_seconds,

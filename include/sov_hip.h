@@ -284,6 +284,11 @@ typedef struct sov_hip_pdhg_state {
     void* s;             /* m */
     void* y;             /* m */
     void* y_sum;         /* m */
+    /* optional (NULL = none), m: an objective on the slacks, so that the
+     * dual step's slack update is s+ = clamp(s - tau (y + s_grad)) -- the
+     * correction problems of iterative refinement need it; batch steps
+     * ignore it */
+    const void* s_grad;
 } sov_hip_pdhg_state;
 
 int sov_hip_pdhg_primal_step(sov_hip_ctx* ctx, const sov_hip_csr* csr,
@@ -363,6 +368,32 @@ int sov_hip_pdhg_movement(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, double
  * warm starts and for the FP32 -> FP64 switch. */
 int sov_hip_pdhg_set_point(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double* x_src,
                            const double* y_src, const double* act);
+
+/* Iterative refinement (no reference counterpart).  With an FP64 anchor
+ * (x0, y0, s0 = A x0) and a scale delta > 0, the substitution
+ *   x = x0 + dx / delta,  s = s0 + ds / delta,  y = y0 + dy / delta
+ * turns the scaled saddle problem c x + y (s - A x) into one of the same
+ * form in (dx, ds, dy): objective delta (c - A^T y0), column box
+ * delta (col - x0), row box delta (row - s0) and slack objective delta y0.
+ * PDHG on it, with the same step sizes, takes the same steps as PDHG on
+ * the original from the anchor, so FP32 storage of the corrections keeps
+ * FP64-level precision in the combined point.
+ *
+ * sov_hip_pdhg_anchor writes that problem into `correction` (c, col_lo,
+ * col_hi, row_lo, row_hi and the m values at s_grad, which must be set)
+ * from the FP64 scaled data of `base` (c, col_lo, col_hi, row_lo, row_hi)
+ * and sets its start: x = Proj(0), s = Proj(0), y = 0.  aty0 = A^T y0.
+ * All vectors in device order. */
+int sov_hip_pdhg_anchor(sov_hip_ctx* ctx, const sov_hip_pdhg_state* base, const sov_hip_pdhg_state* correction,
+                        const double* x0, const double* y0, const double* s0, const double* aty0, double delta);
+
+/* The point an anchor and a correction stand for, into the FP64 state dst:
+ *   x = x0 + inv_delta x_c,  x_sum = since x0 + inv_delta x_sum_c,
+ *   y = y0 + inv_delta y_c,  y_sum = since y0 + inv_delta y_sum_c
+ * (the correction of either precision), so sov_hip_pdhg_check on dst
+ * evaluates the combined current point and its average. */
+int sov_hip_pdhg_combine(sov_hip_ctx* ctx, const sov_hip_pdhg_state* correction, const sov_hip_pdhg_state* dst,
+                         const double* x0, const double* y0, int64_t since, double inv_delta);
 
 /* ---- batches of LPs ---------------------------------------------------
  * No reference counterpart (the north star's batched solve).  A batch is a

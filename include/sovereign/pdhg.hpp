@@ -102,6 +102,18 @@ struct PdhgOptions {
     bool fp64_polish = true;
     Real fp32_switch_factor = 20.0;
     int fp32_stall_checks = 250;
+    /// FP32 runs with iterative refinement (fused engine, single LP): at the
+    /// first restart whose candidate is below
+    /// max(fp32_switch_factor * tolerance, refinement_start) relative KKT (or
+    /// when FP32 iterates stall), the FP32 state turns into a correction
+    /// around an FP64 anchor, re-anchored at every restart to the average;
+    /// checks, restarts and the reported point are the combined FP64 point.
+    /// PDHG on a correction takes the steps PDHG on the LP would take from
+    /// the anchor, so no FP64 polish follows (fp64_polish applies to runs
+    /// without refinement and to batches).
+    bool fp32_refinement = true;
+    Real refinement_start = 1e-3;
+    Real refinement_max_scale = 1e8;   ///< the largest scale 1 / residual given to a correction
     /// PDLP primal weight omega: tau = eta/omega, sigma = eta*omega, updated at
     /// restarts from the restart-to-restart movement.  Off = the reference
     /// (omega = 1 throughout).
@@ -173,6 +185,8 @@ struct PdhgResult {
     Real relative_gap = 0.0;               ///< gap
     Real primal_weight = 1.0;              ///< final omega
     long long fp64_polish_iterations = 0;  ///< iterations run after an FP32->FP64 switch
+    long long refinement_iterations = 0;   ///< iterations run as FP32 corrections (fp32_refinement)
+    long long refinement_anchors = 0;      ///< anchors the corrections were taken around
     std::string engine;                    ///< "primitive" or "fused"
     FarkasCertificate certificate;         ///< set when PrimalInfeasible
     PrimalRay ray;                         ///< set when DualInfeasible

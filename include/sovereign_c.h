@@ -94,6 +94,9 @@ typedef struct sov_pdhg_opts {
                           double residual);
     void* checkpoint_user;
     int32_t checkpoint_async;        /* PdhgOptions::checkpoint_async (default 1) */
+    int32_t fp32_refinement;         /* PdhgOptions::fp32_refinement (default 0) */
+    double refinement_start;         /* PdhgOptions::refinement_start */
+    double refinement_max_scale;     /* PdhgOptions::refinement_max_scale */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {
@@ -117,6 +120,8 @@ typedef struct sov_pdhg_result {
     int64_t trace_len;
     double timed_seconds;            /* PdhgResult::timed_seconds */
     int64_t timed_iterations;        /* PdhgResult::timed_iterations */
+    int64_t refinement_iterations;   /* PdhgResult::refinement_iterations */
+    int64_t refinement_anchors;      /* PdhgResult::refinement_anchors */
 } sov_pdhg_result;
 
 void sov_pdhg_default_opts(sov_pdhg_opts* o);

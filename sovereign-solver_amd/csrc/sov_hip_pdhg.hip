@@ -127,6 +127,7 @@ struct DualOp {
     int mode;
     Sched s;
     BatchView bv;
+    const T* sg = nullptr;   // optional objective of the slacks (refinement corrections)
     __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     template <int G>
     __device__ void run(const ClassView& v) {
@@ -155,7 +156,8 @@ struct DualOp {
         if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
         if (mode == kLast) ax = acc[row] + ax;
         const T so = s_[row], yo = y[row];
-        T sn = so + (-tau_r) * yo;
+        const T gs = sg != nullptr ? yo + sg[row] : yo;   // the slacks' gradient
+        T sn = so + (-tau_r) * gs;
         const T l = lo[row], h = hi[row];
         if (sn < l) sn = l;
         if (sn > h) sn = h;
@@ -504,6 +506,47 @@ __global__ void k_set_clamped(int32_t n, const double* __restrict__ src, const T
             if (v > h) v = h;
         }
         dst[j] = T(v);
+    }
+}
+
+// dst = base + d * inv (current) and dst_sum = since * base + d_sum * inv
+// (running sum) for one vector, in FP64 from a correction of type T.
+template <typename T>
+__global__ void k_combine(int32_t len, const double* __restrict__ base, const T* __restrict__ d,
+                          const T* __restrict__ d_sum, double since, double inv, double* __restrict__ dst,
+                          double* __restrict__ dst_sum) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+        const double b = base[i];
+        dst[i] = b + double(d[i]) * inv;
+        dst_sum[i] = since * b + double(d_sum[i]) * inv;
+    }
+}
+
+// The state's data fields are read-only to the step kernels; a correction
+// problem is the one place that writes them.
+template <typename T>
+static T* writable(const void* p) { return static_cast<T*>(const_cast<void*>(p)); }
+
+// One side of a correction problem around the anchor point p0:
+//   box' = delta (box - p0), v = Proj_box'(0),
+//   grad' = delta (g - g_sub)  (columns: c - A^T y0; rows: y0 with g_sub = null)
+// and, for rows, the dual correction starts at zero (zero != null).
+template <typename T>
+__global__ void k_anchor(int32_t len, const double* __restrict__ lo, const double* __restrict__ hi,
+                         const double* __restrict__ p0, const double* __restrict__ g,
+                         const double* __restrict__ g_sub, double delta, T* __restrict__ lo_c, T* __restrict__ hi_c,
+                         T* __restrict__ v, T* __restrict__ grad, T* __restrict__ zero) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+        const double p = p0[i];
+        const T l = T(delta * (lo[i] - p)), h = T(delta * (hi[i] - p));
+        lo_c[i] = l;
+        hi_c[i] = h;
+        T z = T(0);
+        if (z < l) z = l;
+        if (z > h) z = h;
+        v[i] = z;
+        grad[i] = T(delta * (g_sub != nullptr ? g[i] - g_sub[i] : g[i]));
+        if (zero != nullptr) zero[i] = T(0);
     }
 }
 
@@ -1003,6 +1046,7 @@ int dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* 
         DualOp<T> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
                      static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
                      static_cast<T*>(st->y_sum), orient_acc<T>(o), T(tau), T(sigma), first, kOnly, Sched{}};
+        op.sg = static_cast<const T*>(st->s_grad);
         return launch_tile_step<T>(ctx, o, op, static_cast<const T*>(st->x_bar));
     }
     const int np = o.nparts();
@@ -1016,6 +1060,7 @@ int dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_pdhg_state* 
         DualOp<T> op{p.ptr, p.idx, val, static_cast<const T*>(st->x_bar), static_cast<const T*>(st->row_lo),
                      static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
                      static_cast<T*>(st->y_sum), orient_acc<T>(o), T(tau), T(sigma), first, seg_mode(k, np), s};
+        op.sg = static_cast<const T*>(st->s_grad);
         k_step<DualOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     SOV_HIP_TRY(check_launch());
@@ -1514,6 +1559,66 @@ int sov_hip_pdhg_movement(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, double
     DeviceGuard3 g(ctx->device);
     return st->prec == SOV_PREC_F64 ? movement_impl<double>(ctx, st, xl, yl, movement)
                                     : movement_impl<float>(ctx, st, xl, yl, movement);
+}
+
+int sov_hip_pdhg_combine(sov_hip_ctx* ctx, const sov_hip_pdhg_state* corr, const sov_hip_pdhg_state* dst,
+                         const double* x0, const double* y0, int64_t since, double inv_delta) {
+    if (ctx == nullptr || x0 == nullptr || y0 == nullptr) return fail(SOV_INVALID_ARGUMENT, "combine args");
+    int rc = state_ok(nullptr, corr);
+    if (rc == SOV_OK) rc = state_ok(nullptr, dst);
+    if (rc != SOV_OK) return rc;
+    if (dst->prec != SOV_PREC_F64) return fail(SOV_INVALID_ARGUMENT, "combine: the target state must be FP64");
+    if (corr->m != dst->m || corr->n != dst->n) return fail(SOV_DIMENSION_MISMATCH, "combine: state dims");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "combine during capture");
+    DeviceGuard3 g(ctx->device);
+    const double sn = static_cast<double>(since);
+    auto launch = [&](auto tag) {
+        using T = decltype(tag);
+        if (corr->n > 0)
+            k_combine<T><<<launch_blocks(corr->n), kBlock, 0, ctx->stream>>>(
+                corr->n, x0, static_cast<const T*>(corr->x), static_cast<const T*>(corr->x_sum), sn, inv_delta,
+                static_cast<double*>(dst->x), static_cast<double*>(dst->x_sum));
+        if (corr->m > 0)
+            k_combine<T><<<launch_blocks(corr->m), kBlock, 0, ctx->stream>>>(
+                corr->m, y0, static_cast<const T*>(corr->y), static_cast<const T*>(corr->y_sum), sn, inv_delta,
+                static_cast<double*>(dst->y), static_cast<double*>(dst->y_sum));
+    };
+    if (corr->prec == SOV_PREC_F64) launch(double{});
+    else launch(float{});
+    SOV_HIP_TRY(check_launch());
+    return SOV_OK;
+}
+
+int sov_hip_pdhg_anchor(sov_hip_ctx* ctx, const sov_hip_pdhg_state* base, const sov_hip_pdhg_state* corr,
+                        const double* x0, const double* y0, const double* s0, const double* aty0, double delta) {
+    if (ctx == nullptr || x0 == nullptr || y0 == nullptr || s0 == nullptr || aty0 == nullptr)
+        return fail(SOV_INVALID_ARGUMENT, "anchor args");
+    int rc = state_ok(nullptr, base);
+    if (rc == SOV_OK) rc = state_ok(nullptr, corr);
+    if (rc != SOV_OK) return rc;
+    if (base->prec != SOV_PREC_F64) return fail(SOV_INVALID_ARGUMENT, "anchor: the base state must be FP64");
+    if (corr->m != base->m || corr->n != base->n) return fail(SOV_DIMENSION_MISMATCH, "anchor: state dims");
+    if (corr->m > 0 && corr->s_grad == nullptr) return fail(SOV_INVALID_ARGUMENT, "anchor: s_grad is not set");
+    if (!(delta > 0.0) || !(delta < INFINITY)) return fail(SOV_INVALID_ARGUMENT, "anchor: delta must be positive");
+    if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "anchor during capture");
+    DeviceGuard3 g(ctx->device);
+    auto launch = [&](auto tag) {
+        using T = decltype(tag);
+        if (corr->n > 0)
+            k_anchor<T><<<launch_blocks(corr->n), kBlock, 0, ctx->stream>>>(
+                corr->n, static_cast<const double*>(base->col_lo), static_cast<const double*>(base->col_hi), x0,
+                static_cast<const double*>(base->c), aty0, delta, writable<T>(corr->col_lo),
+                writable<T>(corr->col_hi), static_cast<T*>(corr->x), writable<T>(corr->c), nullptr);
+        if (corr->m > 0)
+            k_anchor<T><<<launch_blocks(corr->m), kBlock, 0, ctx->stream>>>(
+                corr->m, static_cast<const double*>(base->row_lo), static_cast<const double*>(base->row_hi), s0,
+                y0, nullptr, delta, writable<T>(corr->row_lo), writable<T>(corr->row_hi),
+                static_cast<T*>(corr->s), writable<T>(corr->s_grad), static_cast<T*>(corr->y));
+    };
+    if (corr->prec == SOV_PREC_F64) launch(double{});
+    else launch(float{});
+    SOV_HIP_TRY(check_launch());
+    return SOV_OK;
 }
 
 int sov_hip_pdhg_set_point(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double* xs, const double* ys,

@@ -45,6 +45,7 @@ class PdhgOpts(C.Structure):
         ("farkas_out", _f64p), ("ray_out", _f64p), ("warm_start_primal", _f64p), ("tiles", C.c_int32),
         ("autotune_after_seconds", C.c_double), ("timing_mark_iteration", C.c_int64),
         ("checkpoint_fn", CHECKPOINT_FN), ("checkpoint_user", C.c_void_p), ("checkpoint_async", C.c_int32),
+        ("fp32_refinement", C.c_int32), ("refinement_start", C.c_double), ("refinement_max_scale", C.c_double),
     ]
 
 
@@ -56,7 +57,7 @@ class PdhgRes(C.Structure):
         ("loop_seconds", C.c_double), ("operator_norm_estimate", C.c_double), ("relative_primal", C.c_double),
         ("relative_dual", C.c_double), ("relative_gap", C.c_double), ("primal_weight", C.c_double),
         ("fp64_polish_iterations", C.c_int64), ("trace_len", C.c_int64), ("timed_seconds", C.c_double),
-        ("timed_iterations", C.c_int64),
+        ("timed_iterations", C.c_int64), ("refinement_iterations", C.c_int64), ("refinement_anchors", C.c_int64),
     ]
 
 
@@ -286,6 +287,8 @@ class Result:
     ray: np.ndarray = None        # primal ray when status == "dual_infeasible"
     timed_seconds: float = 0.0    # from the timing mark to the end of the loop
     timed_iterations: int = 0
+    refinement_iterations: int = 0   # iterations run as FP32 corrections around an FP64 anchor
+    refinement_anchors: int = 0
 
 
 _ENGINES = {"auto": 0, "primitive": 1, "fused": 2}
@@ -297,7 +300,7 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
           use_graphs=True, trace_cap=0, verbose=False, fp32_switch_factor=None, fp32_stall_checks=None,
           restarts="reference", autotune=True, detect_infeasibility=False, primal_weight_smoothing=None,
           warm_start_primal=None, tiles=None, autotune_after=None, timing_mark=None, checkpoint=None,
-          checkpoint_async=True) -> Result:
+          checkpoint_async=True, refinement=None, refinement_start=None) -> Result:
     """Pdhg(options).solve(problem, backend) through the C entry point.
     tiles: None = the autotuner decides, 0 = CSR step kernels only, k >= 1 =
     the LDS-staged tiled step kernels in k launches per product.
@@ -307,7 +310,10 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
     loop (None: no timed part).
     checkpoint: None, or f(iteration, x, y, residual) called at every check
     with copies of the original-space candidate (PdhgOptions::checkpoint);
-    checkpoint_async: hand checkpoints over without stalling the device."""
+    checkpoint_async: hand checkpoints over without stalling the device.
+    refinement: FP32 iterates become corrections around an FP64 anchor
+    (PdhgOptions::fp32_refinement; None: the engine's default, on);
+    refinement_start: the relative KKT at which they do."""
     L = engine()
     o = PdhgOpts()
     L.sov_pdhg_default_opts(C.byref(o))
@@ -346,6 +352,10 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
         o.fp32_switch_factor = fp32_switch_factor
     if fp32_stall_checks is not None:
         o.fp32_stall_checks = fp32_stall_checks
+    if refinement is not None:
+        o.fp32_refinement = int(bool(refinement))
+    if refinement_start is not None:
+        o.refinement_start = float(refinement_start)
     m, n, _ = lp.dims()
     keep = []
     if warm_start_dual is not None:
@@ -380,7 +390,8 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
                   int(r.fp64_polish_iterations), x[:n], y[:m], rc[:n], msg.value.decode(), tr[:nt], ti[:nt],
                   farkas[:m] if STATUS[r.status] == "primal_infeasible" else None,
                   ray[:n] if STATUS[r.status] == "dual_infeasible" else None,
-                  r.timed_seconds, int(r.timed_iterations))
+                  r.timed_seconds, int(r.timed_iterations), int(r.refinement_iterations),
+                  int(r.refinement_anchors))
 
 
 def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, restart_check_period=40,

@@ -71,6 +71,8 @@ def lib():
     L.sov_hip_pdhg_check_static.argtypes = [_v, _v, _v, _v]
     L.sov_hip_pdhg_restart.argtypes = [_v, _v, C.c_int64, _v, _v, _v, C.POINTER(d)]
     L.sov_hip_pdhg_set_point.argtypes = [_v, _v, _v, _v, _v]
+    L.sov_hip_pdhg_anchor.argtypes = [_v, _v, _v, _v, _v, _v, _v, d]
+    L.sov_hip_pdhg_combine.argtypes = [_v, _v, _v, _v, _v, C.c_int64, d]
     L.sov_hip_graph_begin.argtypes = [_v]
     L.sov_hip_graph_end.argtypes = [_v, C.POINTER(_v)]
     L.sov_hip_graph_launch.argtypes = [_v, _v]
@@ -173,7 +175,7 @@ class PdhgState(C.Structure):
     """sov_hip_pdhg_state (include/sov_hip.h)."""
     _fields_ = [("prec", C.c_int32), ("m", C.c_int32), ("n", C.c_int32), ("c", _v), ("col_lo", _v),
                 ("col_hi", _v), ("row_lo", _v), ("row_hi", _v), ("x", _v), ("x_bar", _v), ("x_sum", _v), ("s", _v),
-                ("y", _v), ("y_sum", _v)]
+                ("y", _v), ("y_sum", _v), ("s_grad", _v)]
 
 
 class CheckArgs(C.Structure):

@@ -192,6 +192,9 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->fp64_polish = d.fp64_polish ? 1 : 0;
     o->fp32_switch_factor = d.fp32_switch_factor;
     o->fp32_stall_checks = d.fp32_stall_checks;
+    o->fp32_refinement = d.fp32_refinement ? 1 : 0;
+    o->refinement_start = d.refinement_start;
+    o->refinement_max_scale = d.refinement_max_scale;
     o->use_primal_weight = 0;
     o->initial_primal_weight = d.initial_primal_weight;
     o->primal_weight_smoothing = d.primal_weight_smoothing;
@@ -228,6 +231,9 @@ sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
     p.fp64_polish = o->fp64_polish != 0;
     p.fp32_switch_factor = o->fp32_switch_factor;
     p.fp32_stall_checks = o->fp32_stall_checks;
+    p.fp32_refinement = o->fp32_refinement != 0;
+    p.refinement_start = o->refinement_start;
+    p.refinement_max_scale = o->refinement_max_scale;
     p.use_primal_weight = o->use_primal_weight != 0;
     p.initial_primal_weight = o->initial_primal_weight;
     p.primal_weight_smoothing = o->primal_weight_smoothing;
@@ -265,6 +271,8 @@ void fill_result(const sov::PdhgResult& r, sov_pdhg_result* res) {
     res->fp64_polish_iterations = r.fp64_polish_iterations;
     res->timed_seconds = r.timed_seconds;
     res->timed_iterations = r.timed_iterations;
+    res->refinement_iterations = r.refinement_iterations;
+    res->refinement_anchors = r.refinement_anchors;
 }
 
 }  // namespace

@@ -1090,6 +1090,59 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
         std::unique_ptr<CheckpointPipe> pipe;
         if (o.checkpoint && o.checkpoint_async) pipe = std::make_unique<CheckpointPipe>(ctx, m, n);
 
+        // iterative refinement: once on, S holds FP32 corrections (dx, ds, dy)
+        // of the FP64 anchor (x0, A x0, y0) at scale delta, and R holds the
+        // point they stand for (FP64, the original scaled data) for checks,
+        // restarts, candidates and checkpoints
+        const bool refine = o.fp32_refinement && prec == SOV_PREC_F32;
+        bool refining = false;
+        long long refine_from = -1;
+        DeviceState R;
+        DevMem d_x0, d_y0, d_s0, d_aty0, d_sgrad;
+        Real delta = 1.0;
+        auto combine = [&]() {
+            hip_ok(sov_hip_pdhg_combine(ctx, &S.st, &R.st, d_x0.as<double>(), d_y0.as<double>(), since, 1.0 / delta),
+                   "combine");
+        };
+        // R.x, R.y become the anchor; the correction restarts at zero
+        auto anchor = [&](Real rel) {
+            if (n) hip_ok(sov_hip_memcpy_d2d(ctx, d_x0.p, R.x.p, n * 8), "anchor");
+            if (m) hip_ok(sov_hip_memcpy_d2d(ctx, d_y0.p, R.y.p, m * 8), "anchor");
+            hip_ok(sov_hip_spmv_device(ctx, csr.h, 0, SOV_PREC_F64, 1.0, d_x0.p, 0.0, d_s0.p), "anchor activity");
+            hip_ok(sov_hip_spmv_device(ctx, csr.h, 1, SOV_PREC_F64, 1.0, d_y0.p, 0.0, d_aty0.p), "anchor A^T y");
+            delta = std::clamp(1.0 / std::max(rel, Real{1e-300}), Real{1}, std::max(o.refinement_max_scale, Real{1}));
+            hip_ok(sov_hip_pdhg_anchor(ctx, &R.st, &S.st, d_x0.as<double>(), d_y0.as<double>(), d_s0.as<double>(),
+                                       d_aty0.as<double>(), delta),
+                   "anchor");
+            ++result.refinement_anchors;
+        };
+        // the switch: R takes S's point in FP64 (a zero anchor at scale 1)
+        auto begin_refinement = [&]() {
+            R.allocate(ctx, SOV_PREC_F64, m, n);
+            R.upload(ctx, R.c, c_s);
+            R.upload(ctx, R.clo, clo_s);
+            R.upload(ctx, R.chi, chi_s);
+            R.upload(ctx, R.rlo, rlo_s);
+            R.upload(ctx, R.rhi, rhi_s);
+            d_x0 = DevMem(ctx, std::max<std::size_t>(n, 1) * 8);
+            d_y0 = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
+            d_s0 = DevMem(ctx, std::max<std::size_t>(m, 1) * 8);
+            d_aty0 = DevMem(ctx, std::max<std::size_t>(n, 1) * 8);
+            d_sgrad = DevMem(ctx, std::max<std::size_t>(m, 1) * S.es);
+            hip_ok(sov_hip_fill_zero(ctx, d_x0.p, std::max<std::size_t>(n, 1) * 8), "fill");
+            hip_ok(sov_hip_fill_zero(ctx, d_y0.p, std::max<std::size_t>(m, 1) * 8), "fill");
+            delta = 1.0;
+            combine();
+            S.st.s_grad = d_sgrad.p;   // a new kernel argument: captured intervals are captured again
+            g_first.reset();
+            g_cont.reset();
+            graph_prec = -1;
+            refining = true;
+            refine_from = result.iterations;
+            if (o.verbose)
+                std::fprintf(stderr, "  [pdhg] fp32 refinement from iter %lld\n", result.iterations);
+        };
+
         result.iterations = 0;
         while (result.iterations < o.max_iterations) {
             if (cancel != nullptr && cancel->is_cancelled()) { result.status = PdhgStatus::Cancelled; break; }
@@ -1133,7 +1186,9 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 graph_prec = -1;
             }
 
-            ca.st = &S.st;
+            if (refining) combine();
+            DeviceState& V = refining ? R : S;   // the point checked and reported
+            ca.st = &V.st;
             ca.since_restart = since;
             sov_hip_check_out co{};
             hip_ok(sov_hip_pdhg_check(ctx, csr.h, &ca, &co), "check");
@@ -1160,12 +1215,12 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             const Real inv = 1.0 / static_cast<Real>(since);
             auto candidate = [&]() {
                 std::pair<std::vector<Real>, std::vector<Real>> p;
-                p.first = avg_better ? S.download(ctx, S.xsum, n, inv) : S.download(ctx, S.x, n);
-                p.second = avg_better ? S.download(ctx, S.ysum, m, inv) : S.download(ctx, S.y, m);
+                p.first = avg_better ? V.download(ctx, V.xsum, n, inv) : V.download(ctx, V.x, n);
+                p.second = avg_better ? V.download(ctx, V.ysum, m, inv) : V.download(ctx, V.y, m);
                 return p;
             };
             if (pipe) {
-                pipe->post(S, avg_better, inv, result.iterations - 1, ores);
+                pipe->post(V, avg_better, inv, result.iterations - 1, ores);
             } else if (o.checkpoint) {
                 auto [cx, cy] = candidate();
                 deliver(result.iterations - 1, cx, cy, ores);
@@ -1180,7 +1235,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 break;
             }
             if (o.detect_infeasibility) {
-                std::vector<Real> xd = S.download(ctx, S.x, n), yd = S.download(ctx, S.y, m);
+                std::vector<Real> xd = V.download(ctx, V.x, n), yd = V.download(ctx, V.y, m);
                 auto [dxo, dyo] = unscale_point(sign, scaling, DeviceOrder::to_nat(xd, ord.col),
                                                 DeviceOrder::to_nat(yd, ord.row));
                 if (drift.test(problem, std::move(dxo), std::move(dyo), result)) {
@@ -1189,16 +1244,18 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                     break;
                 }
             }
-            // FP32 -> FP64 switch (additive): near tolerance or stalled
-            if (S.prec == SOV_PREC_F32 && o.fp64_polish) {
+            // FP32 -> FP64 switch (additive): near tolerance or stalled.  With
+            // refinement there is none: the corrections carry FP64 precision.
+            const Real gauge = o.termination == PdhgTermination::Reference ? ores : rel_worst;
+            if (S.prec == SOV_PREC_F32 && !refining && (o.fp64_polish || refine)) {
                 if (rel_worst < best_rel * 0.99) {
                     best_rel = rel_worst;
                     stall = 0;
                 } else {
                     ++stall;
                 }
-                const Real gauge = o.termination == PdhgTermination::Reference ? ores : rel_worst;
-                if (gauge <= o.fp32_switch_factor * o.tolerance || stall >= o.fp32_stall_checks) {
+                if (!refine && o.fp64_polish &&
+                    (gauge <= o.fp32_switch_factor * o.tolerance || stall >= o.fp32_stall_checks)) {
                     auto [cx, cy] = candidate();
                     init_state(SOV_PREC_F64);
                     DevMem hx(ctx, std::max<std::size_t>(n, 1) * 8), hy(ctx, std::max<std::size_t>(m, 1) * 8);
@@ -1231,15 +1288,30 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                              static_cast<Real>(since) >= o.restart_beta_artificial * static_cast<Real>(result.iterations);
                 prev_cand = rel_worst;
             }
+            // refinement begins at a restart near tolerance, or at once when
+            // the FP32 iterates have stalled
+            bool enter = false;
+            if (refine && !refining && S.prec == SOV_PREC_F32) {
+                if (stall >= o.fp32_stall_checks) {
+                    do_restart = true;
+                    enter = true;
+                } else if (do_restart && gauge <= std::max(o.fp32_switch_factor * o.tolerance, o.refinement_start)) {
+                    enter = true;
+                }
+            }
             if (do_restart) {
                 drift.reset();
+                if (enter) begin_refinement();
+                DeviceState& Q = refining ? R : S;   // the state a restart moves
                 double mv[2] = {0.0, 0.0};
                 double* xl = o.use_primal_weight ? d_xlast.as<double>() : nullptr;
                 double* yl = o.use_primal_weight ? d_ylast.as<double>() : nullptr;
                 if (!adaptive || avg_better) {
-                    hip_ok(sov_hip_pdhg_restart(ctx, &S.st, since, d_act.as<double>(), xl, yl, mv), "restart");
-                } else if (o.use_primal_weight) {
-                    hip_ok(sov_hip_pdhg_movement(ctx, &S.st, xl, yl, mv), "movement");
+                    hip_ok(sov_hip_pdhg_restart(ctx, &Q.st, since, d_act.as<double>(), xl, yl, mv), "restart");
+                    if (refining) anchor(adaptive ? rel_a : std::max({co.rel_avg[0], co.rel_avg[1], co.rel_avg[2]}));
+                } else {
+                    if (o.use_primal_weight) hip_ok(sov_hip_pdhg_movement(ctx, &Q.st, xl, yl, mv), "movement");
+                    if (enter) anchor(rel_c);
                 }
                 if (o.use_primal_weight) omega = next_primal_weight(omega, mv[0], mv[1], o.primal_weight_smoothing);
                 res_at_restart = adaptive ? rel_worst : avg;
@@ -1259,9 +1331,12 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             result.timed_seconds = seconds_since(t_timed);
             result.timed_iterations = result.iterations - o.timing_mark_iteration;
         }
+        if (refining) result.refinement_iterations = result.iterations - refine_from;
         if (best_x.empty()) {
-            best_x = S.download(ctx, S.x, n);
-            best_y = S.download(ctx, S.y, m);
+            if (refining && since > 0) combine();
+            DeviceState& V = refining ? R : S;
+            best_x = V.download(ctx, V.x, n);
+            best_y = V.download(ctx, V.y, m);
         }
         if (converged) result.status = PdhgStatus::Optimal;
         else if (result.status == PdhgStatus::NotSolved) result.status = PdhgStatus::IterationLimit;

@@ -3,15 +3,15 @@
   * the 2,000 x 5,000 LP (5 nonzeros per column, seed 0) in FP64 on the fused
     engine against the reference's own host PDHG run on it (the committed
     golden: 20,000 iterations, status, restarts, every check's residual), and
-    to 1e-6 relative KKT in FP64 and FP32 + FP64 polish against its exact
-    optimum;
+    to 1e-6 relative KKT in FP64, FP32 with iterative refinement and FP32 +
+    FP64 polish against its exact optimum;
   * the 2,000 x 2,000 transportation LP (4M columns, 8M nonzeros) with FP32
-    iterates + FP64 polish to 1e-6 relative KKT, on the default schedule and
+    iterates + iterative refinement to 1e-6 relative KKT, on the default schedule and
     on the LDS-staged tiles, against its exact optimum;
   * the 1M-column Zipf LP (m = 500,000, about 21.5M nonzeros): one FP32 and
     one FP64 primal step and dual step on the tiles against the oracle's
-    products under the dot-product rounding bound, and the FP32 + FP64
-    polish solve to 1e-6 relative KKT.
+    products under the dot-product rounding bound, and the FP32 solve with
+    iterative refinement to 1e-6 relative KKT.
 
 Exact optima come from tests/golden/exact_optima.json (HiGHS, the solver the
 reference's own PDHG-vs-oracle check uses, via scipy; written by
@@ -93,14 +93,21 @@ def test_lp_2000x5000_fp64_follows_the_reference_golden(gpu, golden):
     assert rel(r.objective, g["objective"]) < 1e-10
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32-polish"])
 def test_lp_2000x5000_to_1e6_reaches_the_exact_optimum(gpu, exact, precision):
     e = exact["lp_2000x5000_s0"]
     lp = S.LP.random(2000, 5000, 5, 0)
     assert arrays_digest(lp.arrays()) == e["digest"]
+    # "f32": FP32 iterates with iterative refinement (the default);
+    # "f32-polish": FP32 iterates, then FP64 iterates (refinement off)
     r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
-                restarts="adaptive", precision=precision, max_iterations=2000000)
+                restarts="adaptive", precision=precision[:3], refinement=precision == "f32",
+                max_iterations=2000000)
     assert_solved(lp, r, 1e-6, e["objective"], f"lp_2000x5000 {precision}")
+    if precision == "f32":
+        assert r.refinement_iterations > 0 and r.fp64_polish_iterations == 0
+    if precision == "f32-polish":
+        assert r.refinement_iterations == 0 and r.fp64_polish_iterations > 0
 
 
 @pytest.fixture(scope="module")
@@ -184,12 +191,13 @@ def test_zipf_1m_tiled_steps_match_oracle(gpu, zipf_scaled, prec):
         ctx.close()
 
 
-def test_zipf_1m_fp32_polish_to_1e6(gpu, zipf_scaled):
-    """The headline solve: the 1M-column Zipf LP with FP32 iterates and the
-    FP64 polish to 1e-6 relative KKT.  No exact optimum is available at this
-    size; the host re-check of the relative KKT (gap term <= 1e-6) bounds the
-    objective error."""
+def test_zipf_1m_fp32_refined_to_1e6(gpu, zipf_scaled):
+    """The headline solve: the 1M-column Zipf LP with FP32 iterates and
+    iterative refinement to 1e-6 relative KKT, every iteration in FP32.  No
+    exact optimum is available at this size; the host re-check of the
+    relative KKT (gap term <= 1e-6) bounds the objective error."""
     lp, _ = zipf_scaled
     r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
                 restarts="adaptive", precision="f32", max_iterations=2000000, time_limit=110)
     assert_solved(lp, r, 1e-6, None, "zipf_500000x1000000 f32")
+    assert r.refinement_iterations > 0 and r.fp64_polish_iterations == 0
