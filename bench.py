@@ -19,10 +19,11 @@ each launch on the engine stream (FP32 and FP64 step kernels both reported
 in config); cpu_baseline times the reference's own host PDHG (oracle/_ref,
 compiled from the reference sources) pinned to one core, in a child process
 that runs beside the GPU work.  config also carries the seconds to 1e-6
-relative KKT of four solves (the 2,000 x 5,000 LP in FP64, the 2,000 x 2,000
-transportation LP, the 1M-column Zipf LP with FP32 iterates + FP64 polish and
-in FP64), the batched workload's LPs/s, the learned warm start's cold vs
-warm time to 1e-6 and the host baselines on the 2,000 x 5,000 LP.
+relative KKT of five solves (the 2,000 x 5,000 LP in FP64, the 2,000 x 2,000
+transportation LP with FP32 iterates and iterative refinement, the 1M-column
+Zipf LP with FP32 iterates and iterative refinement, with FP32 iterates + FP64
+polish, and in FP64), the batched workload's LPs/s, the learned warm start's
+cold vs warm time to 1e-6 and the host baselines on the 2,000 x 5,000 LP.
 """
 import argparse
 import ctypes as C
@@ -231,16 +232,27 @@ def learned_warm_start(ctx, count=200, time_subset=24):
     transportation family with the committed network (trained on the GPU by
     `python -m sovereign_amd.learned train`): median iterations and wall time
     to 1e-6 relative KKT, cold start vs the network's dual + primal weight
-    (and a constant primal weight, no network, for reference)."""
+    (and a constant primal weight, no network, for reference), and each
+    start's largest objective error on those instances against their exact
+    optima (HiGHS through scipy.optimize.linprog, on the host)."""
     if not os.path.exists(MODEL):
         return None
     import json as _json
     import sovereign_amd as S
     from sovereign_amd import learned as W
+    from scipy.optimize import linprog
+    from scipy.sparse import csr_matrix
+    fam = S.Family(100, 200, 16, 0)
+    exact = []
+    for i in range(count):
+        a = fam.instance(2, i)[0].arrays()
+        A = csr_matrix((a["val"][: a["nnz"]], a["col_idx"][: a["nnz"]], a["row_ptr"]), shape=(a["m"], a["n"]))
+        r = linprog(a["c"], A_eq=A, b_eq=a["row_lo"], bounds=(0, None), method="highs")
+        exact.append(r.fun if r.status == 0 else np.nan)
     net, xn, yn = W.load(ctx, MODEL)
     meta = _json.loads(str(np.load(MODEL, allow_pickle=False)["meta"]))
-    rep = W.evaluate(ctx, S.Family(100, 200, 16, 0), net, xn, yn, count, seed=2,
-                     const_logw=meta.get("mean_log_weight"), chunk=count, time_subset=time_subset)
+    rep = W.evaluate(ctx, fam, net, xn, yn, count, seed=2, const_logw=meta.get("mean_log_weight"), chunk=count,
+                     time_subset=time_subset, exact=exact)
     rep["trained_on"] = meta.get("train")
     net.close()
     return rep

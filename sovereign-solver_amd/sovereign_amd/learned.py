@@ -106,11 +106,15 @@ def init_params(widths, rng):
     return np.concatenate(parts).astype(np.float32)
 
 
-def dataset(family, seed, count, chunk=500, tolerance=1e-6, precision="f32", verbose=False):
+def dataset(family, seed, count, chunk=500, tolerance=1e-6, precision="f32", verbose=False, basis=False):
     """Solve `count` instances (indices 0..count-1 of draw stream `seed`)
     cold with the batched engine.  Returns features (count x F), optimal
-    duals (count x m), log primal-weight targets (count), cold iterations."""
-    feats, duals, logw, its = [], [], [], []
+    duals (count x m), log primal-weight targets (count), cold iterations;
+    with basis=True also the optimal basis of each (count x n, uint8): the
+    maximum spanning tree under the solve's flows, which is the optimal basis
+    of a non-degenerate transportation LP once the flows are accurate enough
+    to rank every basic arc above every other."""
+    feats, duals, logw, its, bases = [], [], [], [], []
     for start in range(0, count, chunk):
         idx = range(start, min(count, start + chunk))
         lps, fs = zip(*(family.instance(seed, i) for i in idx))
@@ -123,17 +127,122 @@ def dataset(family, seed, count, chunk=500, tolerance=1e-6, precision="f32", ver
             duals.append(r.y)
             logw.append(np.log(np.linalg.norm(r.y / rs) / max(np.linalg.norm(r.x / cs), 1e-300)))
             its.append(r.iterations)
+            if basis:
+                src, snk = _spanning_tree(family.S, family.D, -r.x)
+                lab = np.zeros(len(r.x), np.uint8)
+                lab[src * family.D + (snk - family.S)] = 1
+                bases.append(lab)
         if verbose:
             print(f"  dataset: {min(count, start + chunk)}/{count} solved, batch {tim['seconds']:.1f}s", flush=True)
-    return np.array(feats), np.array(duals), np.array(logw), np.array(its)
+    out = np.array(feats), np.array(duals), np.array(logw), np.array(its)
+    return out + (np.array(bases),) if basis else out
+
+
+def _spanning_tree(sources, sinks, weight):
+    """The minimum spanning tree of the complete source/sink graph under
+    `weight` (one value per arc i * sinks + j): (source row, sink row) of its
+    sources + sinks - 1 arcs."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+    S, Dn = int(sources), int(sinks)
+    w = np.asarray(weight, np.float64)
+    w = w - w.min() + 1e-9    # positive weights (a stored zero would be a missing arc)
+    G = coo_matrix((w, (np.repeat(np.arange(S), Dn), S + np.tile(np.arange(Dn), S))), shape=(S + Dn, S + Dn))
+    T = minimum_spanning_tree(G.tocsr()).tocoo()
+    return np.minimum(T.row, T.col), np.maximum(T.row, T.col)
+
+
+def _tree_flows(a, sources, sinks, src, snk):
+    """The flows of tree arcs (src, snk) that meet every supply and demand of
+    the equality rows, by peeling leaves (a leaf's arc carries its whole
+    supply or demand); negative flows, which a tree that is not an optimal
+    basis can need, are clipped to the bound 0."""
+    S, Dn = int(sources), int(sinks)
+    rem = np.array(a["row_lo"][: S + Dn], np.float64)
+    arcs = [[] for _ in range(S + Dn)]
+    for k, (i, j) in enumerate(zip(src, snk)):
+        arcs[i].append(k)
+        arcs[j].append(k)
+    deg = np.array([len(l) for l in arcs])
+    used = np.zeros(len(src), bool)
+    x = np.zeros(S * Dn)
+    leaves = [v for v in range(S + Dn) if deg[v] == 1]
+    while leaves:
+        v = leaves.pop()
+        if deg[v] != 1:
+            continue
+        k = next(e for e in arcs[v] if not used[e])
+        i, j = int(src[k]), int(snk[k])
+        q = rem[v]
+        x[i * Dn + (j - S)] = q
+        rem[i] -= q
+        rem[j] -= q
+        used[k] = True
+        deg[i] -= 1
+        deg[j] -= 1
+        other = j if v == i else i
+        if deg[other] == 1:
+            leaves.append(other)
+    return np.maximum(x, 0.0)
+
+
+def _tree_duals(a, sources, sinks, src, snk):
+    """Row duals with zero reduced cost on every tree arc,
+    y_i + y_(sources + j) = c_ij, from y_0 = 0 outward."""
+    S, Dn = int(sources), int(sinks)
+    c = np.asarray(a["c"][: S * Dn], np.float64)
+    nbr = [[] for _ in range(S + Dn)]
+    for i, j in zip(src, snk):
+        nbr[int(i)].append(int(j))
+        nbr[int(j)].append(int(i))
+    y = np.zeros(S + Dn)
+    seen = np.zeros(S + Dn, bool)
+    seen[0] = True
+    stack = [0]
+    while stack:
+        v = stack.pop()
+        for w in nbr[v]:
+            if not seen[w]:
+                i, j = (v, w) if v < S else (w, v)
+                y[w] = c[i * Dn + (j - S)] - y[v]
+                seen[w] = True
+                stack.append(w)
+    return y
+
+
+def transport_primal(a, y, sources, sinks):
+    """A primal start recovered from a dual y by complementary slackness on
+    a balanced transportation LP (column i * sinks + j joins source row i and
+    sink row sources + j; equality rows).  An optimal basis is a spanning
+    tree of the source/sink graph whose arcs have zero reduced cost
+    c - A^T y, and its flows follow from the supplies and demands alone: the
+    minimum spanning tree under reduced cost (the optimal basis when y is
+    optimal and the LP is not degenerate), then its flows."""
+    S, Dn = int(sources), int(sinks)
+    yy = np.asarray(y, np.float64)
+    d = np.asarray(a["c"][: S * Dn], np.float64) - (yy[:S][:, None] + yy[S:S + Dn][None, :]).ravel()
+    src, snk = _spanning_tree(S, Dn, d)
+    return _tree_flows(a, S, Dn, src, snk)
+
+
+def transport_point(a, score, sources, sinks):
+    """A primal-dual start from per-arc basis scores (higher: more likely in
+    the optimal basis): the maximum spanning tree under the scores, its
+    flows, and the duals that price its arcs at zero reduced cost.  When the
+    tree is the optimal basis of a non-degenerate LP, this is the optimal
+    vertex and an optimal dual."""
+    S, Dn = int(sources), int(sinks)
+    src, snk = _spanning_tree(S, Dn, -np.asarray(score, np.float64))
+    return _tree_flows(a, S, Dn, src, snk), _tree_duals(a, S, Dn, src, snk)
 
 
 class Normalizer:
-    """Per-column affine maps of the network's inputs and outputs."""
+    """Per-column affine maps of the network's inputs and outputs
+    (identity=True: no map, for targets already in a fixed range)."""
 
-    def __init__(self, a):
-        self.mu = a.mean(axis=0)
-        self.sd = a.std(axis=0) + 1e-8
+    def __init__(self, a, identity=False):
+        self.mu = np.zeros(a.shape[1]) if identity else a.mean(axis=0)
+        self.sd = np.ones(a.shape[1]) if identity else a.std(axis=0) + 1e-8
 
     def fwd(self, a):
         return (a - self.mu) / self.sd
@@ -142,11 +251,13 @@ class Normalizer:
         return a * self.sd + self.mu
 
 
-def train(ctx, X, Y, hidden=(512, 512, 512), epochs=60, batch=256, lr=1e-3, seed=0, verbose=False):
+def train(ctx, X, Y, hidden=(512, 512, 512), epochs=60, batch=256, lr=1e-3, seed=0, verbose=False,
+          normalize_outputs=True):
     """Fit the network to (X, Y) on the GPU; returns (mlp, x_norm, y_norm, losses)."""
     rng = np.random.default_rng(seed)
-    xn, yn = Normalizer(X), Normalizer(Y)
-    Xs, Ys = xn.fwd(X).astype(np.float32), yn.fwd(Y).astype(np.float32)
+    xn, yn = Normalizer(X), Normalizer(Y, identity=not normalize_outputs)
+    Xs = xn.fwd(X).astype(np.float32)
+    Ys = Y.astype(np.float32) if not normalize_outputs else yn.fwd(Y).astype(np.float32)
     widths = [X.shape[1], *hidden, Y.shape[1]]
     net = Mlp(ctx, widths, max(batch, 2048))
     net.set_params(init_params(widths, rng))
@@ -189,7 +300,8 @@ def load(ctx, path, max_batch=2048):
     return net, xn, yn
 
 
-def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500, time_subset=0, verbose=False):
+def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500, time_subset=0, verbose=False,
+             exact=None):
     """Warm vs cold on `count` held-out instances (draw stream `seed`),
     each solved to 1e-6 relative KKT by the batched GPU engine:
       cold     y0 = 0, omega = 1 (the engine's default cold start)
@@ -197,7 +309,8 @@ def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500
       learned  y0 = predicted dual, omega = exp(predicted log weight)
     Returns per-variant iterations, statuses, objectives and batch times; with
     time_subset > 0 also single-instance wall times (fused engine) cold vs
-    learned on that many instances."""
+    learned on that many instances; with `exact` (the instances' optimal
+    objectives) each variant's largest |objective - exact| / (1 + |exact|)."""
     import time
     from . import solve
     m = family.S + family.D
@@ -236,6 +349,10 @@ def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500
         its = np.array(o["iterations"])
         summary[name] = {"median_iterations": float(np.median(its)), "mean_iterations": float(its.mean()),
                          "optimal": int(o["optimal"]), "batch_seconds": round(o["batch_seconds"], 2)}
+        if exact is not None:
+            ex = np.asarray(exact, np.float64)
+            summary[name]["max_objective_error"] = float(np.max(np.abs(np.array(o["objective"]) - ex) /
+                                                                (1.0 + np.abs(ex))))
     c, l = np.array(out["cold"]["iterations"]), np.array(out["learned"]["iterations"])
     summary["median_iteration_ratio_cold_over_learned"] = float(np.median(c / np.maximum(l, 1)))
     obj_c, obj_l = np.array(out["cold"]["objective"]), np.array(out["learned"]["objective"])
@@ -261,14 +378,84 @@ def evaluate(ctx, family, net, xn, yn, count, seed=2, const_logw=None, chunk=500
     return summary
 
 
+def study(ctx, family, ntrain=3000, ntest=300, epochs=60, chunk=500, verbose=False):
+    """What a warm start has to supply to beat a constant primal weight on a
+    transportation family.  Trains two networks on `ntrain` cold solves
+    (draw stream 1) -- the shipped one (dual + log primal weight) and a
+    basis scorer (per-arc probability of being basic) -- and solves `ntest`
+    held-out instances (stream 2) with the batched engine to 1e-6 relative
+    KKT from each start:
+      cold          x = 0, y = 0, omega = 1
+      constant      x = 0, y = 0, omega = the training mean
+      learned       y = predicted dual, omega = predicted
+      exact_dual    y = the cold solve's dual, omega = constant
+      learned_dual_tree  y = predicted dual, x = its spanning-tree primal
+                    (transport_primal), omega = predicted
+      learned_tree  (x, y) = the spanning tree of the predicted basis scores
+                    (transport_point), omega = constant
+      exact_vertex  (x, y) = the maximum spanning tree under the cold solve's
+                    flows: the optimal vertex and dual, omega = constant
+    Returns median/mean iterations and batch seconds per start, the share of
+    test instances whose predicted tree is the exact basis, and the median
+    number of basic arcs it misses."""
+    from . import BatchSession
+    S_, Dn = family.S, family.D
+    m = S_ + Dn
+    X, Y, LW, its, B = dataset(family, 1, ntrain, chunk=chunk, verbose=verbose, basis=True)
+    net_y, xn_y, yn_y, _ = train(ctx, X, np.column_stack([Y, LW]), epochs=epochs, verbose=verbose)
+    net_b, xn_b, yn_b, _ = train(ctx, X, B, epochs=epochs, verbose=verbose, normalize_outputs=False)
+    inst = [family.instance(2, i) for i in range(ntest)]
+    lps, F = [t[0] for t in inst], np.array([t[1] for t in inst])
+    arrs = [lp.arrays() for lp in lps]
+    cold, _ = solve_batch(lps, tolerance=1e-6, precision="f32")
+    pr = predict(net_y, xn_y, yn_y, F)
+    scores = predict(net_b, xn_b, yn_b, F)
+    const = float(np.exp(LW.mean()))
+    trees, vertices, missing = [], [], []
+    for a, sc, r in zip(arrs, scores, cold):
+        trees.append(transport_point(a, sc, S_, Dn))
+        vertices.append(transport_point(a, r.x, S_, Dn))
+        ps, pk = _spanning_tree(S_, Dn, -np.asarray(sc, np.float64))
+        ts, tk = _spanning_tree(S_, Dn, -r.x)
+        missing.append(len(set(zip(ts.tolist(), tk.tolist())) - set(zip(ps.tolist(), pk.tolist()))))
+    starts = {
+        "cold": dict(),
+        "constant": dict(primal_weights=np.full(ntest, const)),
+        "learned": dict(warm_duals=list(pr[:, :m]), primal_weights=np.exp(pr[:, m])),
+        "exact_dual": dict(warm_duals=[r.y for r in cold], primal_weights=np.full(ntest, const)),
+        "learned_dual_tree": dict(primal_weights=np.exp(pr[:, m]),
+                                  points=[(transport_primal(a, y, S_, Dn), y) for a, y in zip(arrs, pr[:, :m])]),
+        "learned_tree": dict(primal_weights=np.full(ntest, const), points=trees),
+        "exact_vertex": dict(primal_weights=np.full(ntest, const), points=vertices),
+    }
+    out = {"train": int(len(X)), "test": ntest, "exact_basis_share": float(np.mean(np.array(missing) == 0)),
+           "median_missing_basic_arcs": float(np.median(missing)), "basis_size": float(B.sum(axis=1).mean())}
+    for name, kw in starts.items():
+        points = kw.pop("points", None)
+        sess = BatchSession(lps, tolerance=1e-6, precision="f32", **kw)
+        if points is not None:
+            for b, (x, y) in enumerate(points):
+                sess.set_start(b, x, y)
+        res, tim = sess.solve()
+        sess.close()
+        it = np.array([r.iterations for r in res])
+        out[name] = {"median_iterations": float(np.median(it)), "mean_iterations": float(it.mean()),
+                     "optimal": int(sum(r.status == "optimal" for r in res)), "batch_seconds": round(tim["seconds"], 2)}
+        if verbose:
+            print(f"  study: {name}: median {np.median(it):.0f} iterations", flush=True)
+    net_y.close()
+    net_b.close()
+    return out
+
+
 def main(argv=None):
     import argparse
     import json
     import time
     from . import Family
     ap = argparse.ArgumentParser(description="learned PDHG warm start on the 100 x 200 transportation family")
-    ap.add_argument("command", choices=["train", "evaluate"])
-    ap.add_argument("--model", required=True, help="model file (.npz) to write (train) or read (evaluate)")
+    ap.add_argument("command", choices=["train", "evaluate", "study"])
+    ap.add_argument("--model", help="model file (.npz) to write (train) or read (evaluate)")
     ap.add_argument("--train", type=int, default=20000)
     ap.add_argument("--test", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=60)
@@ -278,7 +465,9 @@ def main(argv=None):
     ctx = D.Ctx(0)
     fam = Family(100, 200, 16, 0)
     rep = {}
-    if args.command == "train":
+    if args.command == "study":
+        rep = study(ctx, fam, args.train, args.test, epochs=args.epochs, chunk=args.chunk, verbose=True)
+    elif args.command == "train":
         t0 = time.perf_counter()
         X, Y, LW, its = dataset(fam, 1, args.train, chunk=args.chunk, verbose=True)
         rep["dataset"] = {"instances": int(len(X)), "seconds": round(time.perf_counter() - t0, 1),

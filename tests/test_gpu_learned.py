@@ -3,9 +3,11 @@
 the network trains on the GPU with this project's kernels, and on held-out
 instances its warm start (dual + primal weight) must cut the median
 iterations to 1e-6 relative KKT against the cold start, reach the same
-objective (FP64 oracle solve, 1e-6 relative), and keep every instance
-optimal.  Also reported: the gain of a constant primal weight (the training
-set's mean, no network), the baseline the network has to match."""
+objective (1e-6 relative of the exact optimum from HiGHS), and keep every
+instance optimal.  Also reported: the gain of a constant primal weight (the
+training set's mean, no network), the baseline the network has to match.
+A start at the optimal vertex (the spanning tree of the optimal basis,
+learned.transport_point) must stop at the first check."""
 import numpy as np
 import pytest
 
@@ -13,6 +15,7 @@ import oracle_ffi as O
 import sovereign_amd as S
 from sovereign_amd.device import Ctx
 from sovereign_amd import learned as W
+from test_learned_tree import highs
 
 pytestmark = pytest.mark.gpu
 
@@ -38,14 +41,25 @@ def test_trained_network_cuts_iterations(ctx):
     assert rep["median_iteration_ratio_cold_over_learned"] > 1.3
     assert rep["learned"]["median_iterations"] < 0.8 * rep["cold"]["median_iterations"]
     assert rep["max_objective_rel_diff"] < 1e-4
-    # warm-started answers against the FP64 oracle on a few instances
+    # warm-started answers against the exact optimum (HiGHS) on a few
+    # instances, and the same instances started at their optimal vertex (the
+    # spanning tree of the optimal basis): a start at an optimum stops at the
+    # first check
     for i in range(3):
         lp, f = fam.instance(2, i)
+        a = lp.arrays()
+        _, xs, _, exact = highs(a)
         pr = W.predict(net, xn, yn, f[None, :])[0]
-        r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative",
-                    restarts="adaptive", warm_start_dual=pr[:60], primal_weight=True,
-                    initial_primal_weight=float(np.exp(pr[60])), primal_weight_smoothing=0.0)
-        o = O.pdhg(O.Model(lp.arrays()), tolerance=1e-6, termination=1, restart_scheme=1, max_iterations=2000000)
-        assert r.status == "optimal" and o["status"] == "optimal"
-        assert abs(r.objective - o["objective"]) <= 1e-5 * max(1.0, abs(o["objective"]))
+        kw = dict(backend="hip", engine_kind="fused", tolerance=1e-6, termination="relative", restarts="adaptive")
+        r = S.solve(lp, warm_start_dual=pr[:60], primal_weight=True, initial_primal_weight=float(np.exp(pr[60])),
+                    primal_weight_smoothing=0.0, **kw)
+        host = O.relative_kkt(O.Model(a), r.x, r.y)[:3]
+        print(f"instance {i}: warm {r.status} it={r.iterations} obj {r.objective:.12g} exact {exact:.12g} "
+              f"host rel {host}", flush=True)
+        assert r.status == "optimal" and max(host) <= 1e-6 * 1.001
+        assert abs(r.objective - exact) <= 1e-6 * (1.0 + abs(exact))
+        xv, yv = W.transport_point(a, xs, 20, 40)
+        v = S.solve(lp, warm_start_primal=xv, warm_start_dual=yv, **kw)
+        assert v.status == "optimal" and v.iterations == 40, (v.status, v.iterations)
+        assert abs(v.objective - exact) <= 1e-9 * (1.0 + abs(exact))
     net.close()
