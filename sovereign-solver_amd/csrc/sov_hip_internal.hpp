@@ -132,7 +132,13 @@ constexpr int kTileRowCap = 4032;                     // local rows per group (1
 constexpr int kTileColBits = 20;                      // key bits of the column within a unit
 constexpr uint32_t kTileColMask = (1u << kTileColBits) - 1u;
 constexpr int kTileGatherSlices = 63;                 // slices per gather unit (columns < 2^20 - 1)
-constexpr int kTileStageMin = 384;                    // entries per group that make a slice staged
+// Entries per group that make a slice staged.  Measured on the 1M-column
+// Zipf LP (FP32 dual step, where slices of 400-1100 entries per group lie
+// between the short columns and the dense ones): 384 stages them, 69.6 us;
+// 800 gathers the sparser ones from memory in one unit, 65.6 us (fewer
+// barrier-ended units); 1200, 65.7 us; the FP64 steps and the FP32 primal
+// step do not move.
+constexpr int kTileStageMin = 800;
 // Whether the tiled kernels of element size tsize add passes in pairs (and
 // the pass list evens every (wave, unit) count for them).  Measured on the
 // 1M-column Zipf LP: pairs cut the FP32 primal step from 66 to 60 us; the

@@ -2,7 +2,7 @@
 device path uploads), checked on the CPU against the CSR it is built from:
 every entry stored exactly once with its row, column and source position
 decodable from (group, unit, key); units that cover the slices in order
-(a staged unit is one slice with at least 384 entries per group, a gather
+(a staged unit is one slice with at least 800 entries per group, a gather
 unit a run of up to 63 sparser slices); each (group, unit) tile split into 8
 row-aligned sub-ranges (one per entry wave), each a multiple of 4 entries with padding only
 at its end; rows dealt to groups in 64-row chunks; the pass list the
@@ -96,7 +96,7 @@ def test_layout_stores_the_csr(name, tsize):
     assert W == 65536 // tsize and S == -(-cols // W)       # a slice is 64 KiB in either precision
     assert rmax <= 4032 and nb * rmax >= rows and rmax % 64 == 0
     assert nb == max(min(cus, nchunks), -(-nchunks // 63))
-    # units: contiguous slice ranges; staged = one slice with >= 384 entries per group
+    # units: contiguous slice ranges; staged = one slice with >= 800 entries per group
     first = units[0:4 * U:4]
     staged = units[1:4 * U:4]
     last = np.append(first[1:], S) if U else first
@@ -104,7 +104,7 @@ def test_layout_stores_the_csr(name, tsize):
     per_slice = np.bincount(idx // W, minlength=S) if S else np.zeros(0, np.int64)
     for u in range(U):
         width = last[u] - first[u]
-        dense = per_slice[first[u]:last[u]] >= 384 * nb
+        dense = per_slice[first[u]:last[u]] >= 800 * nb
         if staged[u]:
             assert width == 1 and dense.all()
         else:
@@ -210,7 +210,7 @@ def ragged_csr():
     so the last columns are used."""
     rng = np.random.default_rng(11)
     m, n = 301, 40003
-    lens = rng.integers(300, 500, m)
+    lens = rng.integers(600, 900, m)
     lens[7] = n
     ptr, idx = random_csr(m, n, lens, rng)
     return m, n, ptr, idx
