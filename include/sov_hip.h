@@ -323,6 +323,22 @@ typedef struct sov_hip_check_args {
     const double* orig_col_hi;  /* n */
     const double* orig_c;       /* n, problem sense */
     double* act_avg;            /* m, optional */
+    /* Iterative refinement (no reference counterpart; all NULL / ignored
+     * otherwise): with base set, st is a correction (sov_hip_pdhg_anchor)
+     * and the check evaluates the point anchor + inv_delta * correction,
+     * the current one and the average, against the LP whose FP64 scaled
+     * data (c, col_lo, col_hi, row_lo, row_hi) base holds.  base_x = x0,
+     * base_y = y0, base_act = A x0, base_aty = A^T y0 (FP64, device
+     * order).  The products run on the correction, in its precision; the
+     * anchor's terms are added in FP64, so an FP32 correction is checked to
+     * FP32 rounding of the distance from the anchor, not of the point.
+     * act_avg then receives the combined average's activity. */
+    const sov_hip_pdhg_state* base;
+    const double* base_x;       /* n */
+    const double* base_y;       /* m */
+    const double* base_act;     /* m */
+    const double* base_aty;     /* n */
+    double inv_delta;
 } sov_hip_check_args;
 
 typedef struct sov_hip_check_out {

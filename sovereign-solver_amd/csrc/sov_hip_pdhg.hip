@@ -289,6 +289,14 @@ struct CheckRowOp {
     int mode;
     Sched s;
     BatchView bv;
+    // a refinement correction (b_act != null): the point checked is the
+    // anchor plus idelta times the state's vectors, with A x0 = b_act,
+    // y0 = b_y and the LP's scaled row box b_lo, b_hi (FP64)
+    const double* b_act = nullptr;
+    const double* b_y = nullptr;
+    const double* b_lo = nullptr;
+    const double* b_hi = nullptr;
+    double idelta = 1.0;
     __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     // sums: pv2_c, pv2_a, bn2, yres2_c, yres2_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
@@ -300,11 +308,24 @@ struct CheckRowOp {
         row_dot2<G>(ptr, idx, val, x, xsum, inv_r, i, lane, ac, aa);
         if (i >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)i, ac, aa)) return;
+            double yc = double(y[i]), ya = double(ysum[i]) * inv_r;
+            double ls, hs;
+            if (b_act != nullptr) {
+                const double a0 = b_act[i], y0 = b_y[i];
+                ac = a0 + ac * idelta;
+                aa = a0 + aa * idelta;
+                yc = y0 + yc * idelta;
+                ya = y0 + ya * idelta;
+                ls = b_lo[i];
+                hs = b_hi[i];
+            } else {
+                ls = double(lo_s[i]);
+                hs = double(hi_s[i]);
+            }
             if (act_avg != nullptr) act_avg[i] = aa;
             double t[11];
-            const double yc = double(y[i]), ya = double(ysum[i]) * inv_r;
-            t[7] = row_res(ac, double(lo_s[i]), double(hi_s[i]), yc, tol_s);
-            t[8] = row_res(aa, double(lo_s[i]), double(hi_s[i]), ya, tol_s);
+            t[7] = row_res(ac, ls, hs, yc, tol_s);
+            t[8] = row_res(aa, ls, hs, ya, tol_s);
             const double r = rs[i], lo = lo_o[i], hi = hi_o[i];
             const double aoc = ac / r, aoa = aa / r;
             const double yic = yc * r, yia = ya * r;
@@ -354,6 +375,15 @@ struct CheckColOp {
     int mode;
     Sched s;
     BatchView bv;
+    // a refinement correction (b_aty != null): the point checked is the
+    // anchor plus idelta times the state's vectors, with A^T y0 = b_aty,
+    // x0 = b_x and the LP's scaled objective b_c and column box b_lo, b_hi
+    const double* b_aty = nullptr;
+    const double* b_x = nullptr;
+    const double* b_c = nullptr;
+    const double* b_lo = nullptr;
+    const double* b_hi = nullptr;
+    double idelta = 1.0;
     __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     // sums: dres2_c, dres2_a, cn2, pobj_c, pobj_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
@@ -366,10 +396,24 @@ struct CheckColOp {
         if (j >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)j, tc, ta)) return;
             double t[11];
-            const double xc = double(x[j]), xa = double(xsum[j]) * inv_r;
-            const double cj = double(c_s[j]);
-            t[7] = dual_class(cj - tc, double(lo_s[j]), double(hi_s[j]), xc, tol_s);
-            t[8] = dual_class(cj - ta, double(lo_s[j]), double(hi_s[j]), xa, tol_s);
+            double xc = double(x[j]), xa = double(xsum[j]) * inv_r;
+            double cj, ls, hs;
+            if (b_aty != nullptr) {
+                const double t0 = b_aty[j], x0 = b_x[j];
+                tc = t0 + tc * idelta;
+                ta = t0 + ta * idelta;
+                xc = x0 + xc * idelta;
+                xa = x0 + xa * idelta;
+                cj = b_c[j];
+                ls = b_lo[j];
+                hs = b_hi[j];
+            } else {
+                cj = double(c_s[j]);
+                ls = double(lo_s[j]);
+                hs = double(hi_s[j]);
+            }
+            t[7] = dual_class(cj - tc, ls, hs, xc, tol_s);
+            t[8] = dual_class(cj - ta, ls, hs, xa, tol_s);
             const double sc = cs[j], lo = lo_o[j], hi = hi_o[j];
             const double ci = sign * c_o[j];
             const double xoc = xc * sc, xoa = xa * sc;
@@ -665,6 +709,10 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
         op.lo_o = a->orig_row_lo; op.hi_o = a->orig_row_hi; op.rs = a->row_scale;
         op.act_avg = a->act_avg; op.partial = rowpart; op.acc = OA.chk_acc;
         op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.mode = seg_mode(k, npa); op.s = s;
+        if (a->base != nullptr) {
+            op.b_act = a->base_act; op.b_y = a->base_y; op.idelta = a->inv_delta;
+            op.b_lo = static_cast<const double*>(a->base->row_lo); op.b_hi = static_cast<const double*>(a->base->row_hi);
+        }
         k_check<CheckRowOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     for (int k = 0; k < npt; ++k) {
@@ -682,6 +730,11 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
         op.partial = colpart; op.acc = OT.chk_acc;
         op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign;
         op.mode = seg_mode(k, npt); op.s = s;
+        if (a->base != nullptr) {
+            op.b_aty = a->base_aty; op.b_x = a->base_x; op.idelta = a->inv_delta;
+            op.b_c = static_cast<const double*>(a->base->c);
+            op.b_lo = static_cast<const double*>(a->base->col_lo); op.b_hi = static_cast<const double*>(a->base->col_hi);
+        }
         k_check<CheckColOp<T>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     k_check_final<<<22, kBlock, 0, ctx->stream>>>(rowpart, rb, colpart, cb, red);
@@ -1497,6 +1550,20 @@ int sov_hip_pdhg_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip
                                     : dual_impl<float>(ctx, h, st, tau, sigma, first);
 }
 
+// The refinement fields of a check: all set (with an FP64 base of the
+// state's dimensions) or base null.
+static int check_base_ok(const sov_hip_csr* h, const sov_hip_check_args* a) {
+    if (a->base == nullptr) return SOV_OK;
+    const int rc = state_ok(h, a->base);
+    if (rc != SOV_OK) return rc;
+    if (a->base->prec != SOV_PREC_F64) return fail(SOV_INVALID_ARGUMENT, "check: the base state must be FP64");
+    if ((h->n > 0 && (!a->base_x || !a->base_aty)) || (h->m > 0 && (!a->base_y || !a->base_act)))
+        return fail(SOV_INVALID_ARGUMENT, "check: null anchor array");
+    if (!(a->inv_delta > 0.0) || !(a->inv_delta < INFINITY))
+        return fail(SOV_INVALID_ARGUMENT, "check: inv_delta must be positive");
+    return SOV_OK;
+}
+
 int sov_hip_pdhg_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a, sov_hip_check_out* out) {
     if (ctx == nullptr || h == nullptr || a == nullptr || out == nullptr) return fail(SOV_INVALID_ARGUMENT, "check args");
     int rc = state_ok(h, a->st);
@@ -1504,6 +1571,8 @@ int sov_hip_pdhg_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_che
     if ((h->m > 0 && (!a->row_scale || !a->orig_row_lo || !a->orig_row_hi)) ||
         (h->n > 0 && (!a->col_scale || !a->orig_col_lo || !a->orig_col_hi || !a->orig_c)))
         return fail(SOV_INVALID_ARGUMENT, "check: null original-space array");
+    rc = check_base_ok(h, a);
+    if (rc != SOV_OK) return rc;
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "check during capture");
     DeviceGuard3 g(ctx->device);
     if (check_blocks(h) > h->partial_blocks) return fail(SOV_INVALID_ARGUMENT, "check: partials too small");
@@ -1522,6 +1591,8 @@ int sov_hip_pdhg_check_static(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_
     if ((h->m > 0 && (!a->row_scale || !a->orig_row_lo || !a->orig_row_hi)) ||
         (h->n > 0 && (!a->col_scale || !a->orig_col_lo || !a->orig_col_hi || !a->orig_c)))
         return fail(SOV_INVALID_ARGUMENT, "check_static: null original-space array");
+    rc = check_base_ok(h, a);
+    if (rc != SOV_OK) return rc;
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "check_static during capture");
     if (ctx->check_scratch == nullptr)
         return fail(SOV_BACKEND_ERROR, "check_static: the context has no static scratch address");

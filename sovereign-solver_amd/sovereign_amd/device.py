@@ -182,7 +182,8 @@ class CheckArgs(C.Structure):
     _fields_ = [("st", C.POINTER(PdhgState)), ("since_restart", C.c_int64), ("tol_scaled", C.c_double),
                 ("tol_orig", C.c_double), ("obj_sign", C.c_double), ("obj_const", C.c_double), ("row_scale", _v),
                 ("col_scale", _v), ("orig_row_lo", _v), ("orig_row_hi", _v), ("orig_col_lo", _v),
-                ("orig_col_hi", _v), ("orig_c", _v), ("act_avg", _v)]
+                ("orig_col_hi", _v), ("orig_c", _v), ("act_avg", _v), ("base", C.POINTER(PdhgState)),
+                ("base_x", _v), ("base_y", _v), ("base_act", _v), ("base_aty", _v), ("inv_delta", C.c_double)]
 
 
 class CheckOut(C.Structure):

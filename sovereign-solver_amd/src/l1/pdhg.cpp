@@ -1186,9 +1186,25 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 graph_prec = -1;
             }
 
-            if (refining) combine();
-            DeviceState& V = refining ? R : S;   // the point checked and reported
-            ca.st = &V.st;
+            // refining: the check reads the corrections and adds the anchor's
+            // terms; R (the combined point) is formed only where it is read
+            ca.st = &S.st;
+            ca.base = refining ? &R.st : nullptr;
+            if (refining) {
+                ca.base_x = d_x0.as<double>();
+                ca.base_y = d_y0.as<double>();
+                ca.base_act = d_s0.as<double>();
+                ca.base_aty = d_aty0.as<double>();
+                ca.inv_delta = 1.0 / delta;
+            }
+            bool r_current = false;
+            auto point = [&]() -> DeviceState& {   // the point checked, for reading
+                if (refining && !r_current) {
+                    combine();
+                    r_current = true;
+                }
+                return refining ? R : S;
+            };
             ca.since_restart = since;
             sov_hip_check_out co{};
             hip_ok(sov_hip_pdhg_check(ctx, csr.h, &ca, &co), "check");
@@ -1214,13 +1230,14 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             }
             const Real inv = 1.0 / static_cast<Real>(since);
             auto candidate = [&]() {
+                DeviceState& V = point();
                 std::pair<std::vector<Real>, std::vector<Real>> p;
                 p.first = avg_better ? V.download(ctx, V.xsum, n, inv) : V.download(ctx, V.x, n);
                 p.second = avg_better ? V.download(ctx, V.ysum, m, inv) : V.download(ctx, V.y, m);
                 return p;
             };
             if (pipe) {
-                pipe->post(V, avg_better, inv, result.iterations - 1, ores);
+                pipe->post(point(), avg_better, inv, result.iterations - 1, ores);
             } else if (o.checkpoint) {
                 auto [cx, cy] = candidate();
                 deliver(result.iterations - 1, cx, cy, ores);
@@ -1235,6 +1252,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
                 break;
             }
             if (o.detect_infeasibility) {
+                DeviceState& V = point();
                 std::vector<Real> xd = V.download(ctx, V.x, n), yd = V.download(ctx, V.y, m);
                 auto [dxo, dyo] = unscale_point(sign, scaling, DeviceOrder::to_nat(xd, ord.col),
                                                 DeviceOrder::to_nat(yd, ord.row));
@@ -1302,7 +1320,7 @@ PdhgResult Pdhg::solve_fused(const Problem& problem, Backend& backend, const Can
             if (do_restart) {
                 drift.reset();
                 if (enter) begin_refinement();
-                DeviceState& Q = refining ? R : S;   // the state a restart moves
+                DeviceState& Q = point();   // the state a restart moves
                 double mv[2] = {0.0, 0.0};
                 double* xl = o.use_primal_weight ? d_xlast.as<double>() : nullptr;
                 double* yl = o.use_primal_weight ? d_ylast.as<double>() : nullptr;

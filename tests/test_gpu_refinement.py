@@ -16,6 +16,9 @@ src/l1/pdhg.cpp:218-502); what fixes the method is the algebra itself:
     corrections: to FP32 rounding of the distance moved), on the CSR and on
     the tiled step kernels.  A flipped sign in the slack objective, a box
     taken around the wrong point or a dropped 1/delta fails these;
+  * the check of a correction with its anchor (the products on the FP32
+    correction, the anchor's terms in FP64) against the oracle's residuals
+    of the combined point;
   * solve level: FP32 + refinement to 1e-6 relative KKT on LPs with exact
     optima (HiGHS through scipy, tests/golden/exact_optima.json), with the
     objective within 1e-6, the host oracle's relative KKT of the returned
@@ -31,7 +34,7 @@ import pytest
 
 import oracle_ffi as O
 import sovereign_amd as S
-from sovereign_amd.device import Ctx, PdhgState, check
+from sovereign_amd.device import CheckArgs, CheckOut, Ctx, PdhgState, check
 from test_gpu_steps import Device, scaled_problem
 from test_gpu_tiles import set_tiles
 
@@ -135,6 +138,74 @@ def test_correction_steps_are_the_original_steps(ctx, layout, prec):
     s_moved = np.abs(s_dir - s0)
     assert np.all(np.abs(s_cor - s_dir) <= 64 * u * (s_moved + s_moved.max()) + 1e-12 * (1 + np.abs(s_dir)))
     for D in (base, corr):
+        D.A.destroy()
+
+
+def test_anchored_check_is_the_check_of_the_combined_point(ctx):
+    """sov_hip_pdhg_check on an FP32 correction with its anchor (the
+    refinement fields of sov_hip_check_args) against the oracle's residuals
+    of the combined point x0 + x_c / delta, y0 + y_c / delta and of its
+    average, with the activity of the average written to act_avg."""
+    rng = np.random.default_rng(23)
+    M, Ms, rs, cs = scaled_problem(11)
+    m, n = Ms.m, Ms.n
+    base = Device(ctx, Ms, 1)
+    corr = Device(ctx, Ms, 2, 1)
+    pr, pc = base.pr, base.pc
+    lo, hi = Ms.col_lo, Ms.col_hi
+    x0 = lo + rng.uniform(0, 1, n) * np.where(np.isfinite(hi - lo), hi - lo, 1.0)
+    y0 = rng.standard_normal(m)
+    s0, aty0 = O.spmv(Ms, x0), O.spmv_t(Ms, y0)
+    delta, since = 4096.0, 5
+    # a correction of FP32 values (what the device holds), and its sums
+    xc = (rng.standard_normal(n) * 50).astype(np.float32).astype(np.float64)
+    yc = (rng.standard_normal(m) * 50).astype(np.float32).astype(np.float64)
+    xs = ((xc + rng.standard_normal(n)) * since).astype(np.float32).astype(np.float64)
+    ys = ((yc + rng.standard_normal(m)) * since).astype(np.float32).astype(np.float64)
+    corr.put(x=xc, y=yc, x_sum=xs, y_sum=ys)
+    a = CheckArgs()
+    a.st = C.pointer(corr.st)
+    a.since_restart = since
+    a.tol_scaled, a.tol_orig = 1e-4, 1e-4
+    a.obj_sign, a.obj_const = 1.0, M.obj_const
+    a.row_scale, a.col_scale = ctx.upload(rs[pr]), ctx.upload(cs[pc])
+    a.orig_row_lo, a.orig_row_hi = ctx.upload(M.row_lo[pr]), ctx.upload(M.row_hi[pr])
+    a.orig_col_lo, a.orig_col_hi, a.orig_c = ctx.upload(M.col_lo[pc]), ctx.upload(M.col_hi[pc]), ctx.upload(M.c[pc])
+    act = ctx.upload(np.zeros(m))
+    a.act_avg = act
+    a.base = C.pointer(base.st)
+    a.base_x, a.base_y = dev_vec(ctx, base, x0, "x"), dev_vec(ctx, base, y0, "y")
+    a.base_act, a.base_aty = dev_vec(ctx, base, s0, "s"), dev_vec(ctx, base, aty0, "x")
+    a.inv_delta = 1.0 / delta
+    out = CheckOut()
+    check(ctx.L.sov_hip_pdhg_check(ctx.h, corr.A.h, C.byref(a), C.byref(out)))
+    # the combined points on the host
+    x, y = x0 + xc / delta, y0 + yc / delta
+    xa, ya = x0 + xs / since / delta, y0 + ys / since / delta
+    # The products ran on the FP32 image of A_s times the correction; the
+    # anchor's products are exact FP64.  So the residuals match the oracle's
+    # of the combined point up to FP32 rounding of |A_s| |correction| / delta
+    # (the bound below) plus summation order.
+    Ma = O.Model(dict(m=m, n=n, row_ptr=Ms.row_ptr, col_idx=Ms.col_idx[: Ms.nnz], val=np.abs(Ms.val[: Ms.nnz]),
+                      c=Ms.c, col_lo=Ms.col_lo, col_hi=Ms.col_hi, row_lo=Ms.row_lo, row_hi=Ms.row_hi))
+    for pt, cr, kkt, rel_, p, d in (((x, y), (xc, yc), out.kkt_cur, out.rel_cur, out.pobj_cur, out.dobj_cur),
+                                     ((xa, ya), (xs / since, ys / since), out.kkt_avg, out.rel_avg, out.pobj_avg,
+                                      out.dobj_avg)):
+        want = O.kkt_residual(Ms, Ms.c, pt[0], pt[1], 1e-4)
+        err = 4 * 2.0 ** -24 / delta * max(np.max(O.spmv(Ma, np.abs(cr[0]))), np.max(O.spmv_t(Ma, np.abs(cr[1]))))
+        assert abs(kkt - want) <= 1e-12 * max(1.0, abs(want)) + err, (kkt, want, err)
+        w = O.relative_kkt(M, pt[0] * cs, pt[1] * rs)
+        np.testing.assert_allclose(list(rel_), w[:3], rtol=1e-6, atol=1e-12)
+        np.testing.assert_allclose([p, d], w[3:5], rtol=1e-7)
+    act_nat = np.empty(m)
+    act_nat[pr] = ctx.download(act, m)
+    bound = 4 * 2.0 ** -24 / delta * O.spmv(Ma, np.abs(xs / since)) + 1e-12 * (1 + np.abs(O.spmv(Ms, xa)))
+    assert np.all(np.abs(act_nat - O.spmv(Ms, xa)) <= bound)
+    # a base state in FP32 is refused
+    bad = Device(ctx, Ms, 1, 1)
+    a.base = C.pointer(bad.st)
+    assert ctx.L.sov_hip_pdhg_check(ctx.h, corr.A.h, C.byref(a), C.byref(out)) != 0
+    for D in (base, corr, bad):
         D.A.destroy()
 
 
