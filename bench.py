@@ -197,6 +197,24 @@ def batched(S, count=256, time_limit=150.0, use_graphs=True):
             "sum_m": int(sum(sizes)), "sum_nnz": int(sum(lp.dims()[2] for lp in lps))}
 
 
+def checkpoint_cost(S, lp, steps=2000, warmup=400):
+    """The engine loop's rate on the bench workload without a checkpoint
+    callback, with the asynchronous hand-off (copier stream, pinned buffers,
+    delivered one interval late) and with the synchronous one (the loop
+    waits for each copy): iterations/s over `steps` timed iterations, one
+    checkpoint per 40-iteration check."""
+    out = {}
+    for name, kw in (("none", {}), ("async", {"checkpoint_async": True}), ("sync", {"checkpoint_async": False})):
+        got = []
+        if kw:
+            kw = dict(kw, checkpoint=lambda it, x, y, res: got.append(it))
+        r = S.solve(lp, backend="hip", engine_kind="fused", tolerance=0.0, termination="relative",
+                    restarts="adaptive", precision="f32", fp64_polish=False, refinement=False, autotune_after=0.0,
+                    max_iterations=warmup + steps, timing_mark=warmup, **kw)
+        out[name] = {"iterations_per_s": round(r.timed_iterations / r.timed_seconds, 1), "checkpoints": len(got)}
+    return out
+
+
 def time_to_tolerance(S):
     """Seconds to 1e-6 relative KKT through the engine's own entry point
     (Pdhg::solve, fused HIP engine, adaptive restarts, relative-KKT stop) on
@@ -464,9 +482,13 @@ def main():
     its = eng["iterations_per_s"] if eng else 1e6 / (k["primal_us"] + k["dual_us"])
     value = (bp + bd) * its / 1e9
 
-    ttt = None
+    ttt = ckpt = None
     if not args.no_solve:
         ttt = time_to_tolerance(S)
+    if not args.no_engine_loop and prec == 1:
+        ckpt = checkpoint_cost(S, lp)
+        log(f"checkpoints: {ckpt['none']['iterations_per_s']} it/s without, {ckpt['async']['iterations_per_s']} "
+            f"asynchronous, {ckpt['sync']['iterations_per_s']} synchronous")
 
     bat = None
     if not args.no_batched:
@@ -523,6 +545,7 @@ def main():
                                                               HBM_PEAK_GBS, 4)}
                                        for p, v in kt.items()},
                    "time_to_1e-6_relative_kkt": ttt,
+                   "checkpoint_hand_off": ckpt,
                    "batched": bat,
                    "learned_warm_start": lws,
                    "host_baselines": host},
