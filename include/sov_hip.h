@@ -196,15 +196,15 @@ int sov_hip_csr_tiles_usable(const sov_hip_ctx* ctx, const sov_hip_csr* csr, int
  * [0, cols), and a source position per entry), as sov_hip_csr_set_tiles
  * builds it for a device with `cus` compute units and the products of
  * element size tsize (4: FP32, slices of 16384 columns; 8: FP64, slices of
- * 8192).  shape = {groups, slices, slice width, local rows per group,
- * units, sub-ranges per tile}; *entries = stored entries including
- * padding.  keys / src_out (entries each), wstart (groups * units *
+ * 4096).  shape = {groups, slices, slice width, local rows per group,
+ * units, sub-ranges per tile, workgroups per group, LDS slice buffers};
+ * *entries = stored entries including padding.  keys / src_out (entries each), wstart (groups * units *
  * sub-ranges + 1) and units (4 * units + 1: first slice, staged, buffer,
  * next staged unit; then the slice count) may be null (first call for the
  * sizes).  Host only: no device is touched. */
 int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
                         const int32_t* col_idx, const int32_t* src, int32_t cus, int32_t tsize,
-                        int32_t shape[6], int64_t* entries, uint32_t* keys, int32_t* src_out,
+                        int32_t shape[8], int64_t* entries, uint32_t* keys, int32_t* src_out,
                         int32_t* wstart, int32_t* units);
 /* The pass list the tiled kernels read for the same layout: *passes = pass
  * count; pidx (groups * sub-ranges per tile * (units + 1)) = index of the

@@ -93,27 +93,33 @@ struct Natural {
 // Rows (device order) are cut into 64-row chunks; chunk c belongs to group
 // c % nb, where it holds local rows (c / nb) * 64 .. + 63, so every group
 // mixes all length classes and the per-row work is coalesced per chunk.
-// One workgroup runs one group and keeps its row sums in LDS.
+// A group is run by Q workgroups ("halves"), one per CU: each takes a
+// contiguous range of the units (below) and keeps the group's row sums over
+// its units in LDS; the last half to finish adds the halves' sums in half
+// order and runs the per-row epilogue.  So a CU stages only its half of the
+// gathered vector, and each of its units holds the entries of twice as many
+// rows as one workgroup per group would give it.
 //
-// The gathered dimension is cut into slices of W = kTileSliceBytes / (element
-// size) columns, and consecutive slices form units: a layout serves the
-// products of one precision (an orientation holds one per precision), so a
-// slice fills one 64 KB LDS buffer in both and every launch double-buffers.  A staged unit is one slice whose entries are dense
-// enough per group (on average at least kTileStageMin per group) to repay
-// copying the vector slice into LDS with 16-byte loads; its entries then
-// gather from LDS.  A gather unit is a run of up to kTileGatherSlices
-// sparser slices whose entries gather straight from the vector in memory
-// (staging them would move more bytes than it saves).
+// The gathered dimension is cut into slices of W = tile_slice_bytes(element
+// size) / (element size) columns, and consecutive slices form units.  A
+// staged unit is one slice whose entries are dense enough per group (on
+// average at least kTileStageMin per group) to repay copying the vector
+// slice into one of the tile_buffers() LDS buffers; its entries then gather
+// from LDS.  A gather unit is a run of up to tile_gather_slices(W) sparser
+// slices whose entries gather straight from the vector in memory (staging
+// them would move more bytes than it saves).
 //
 // Entries are stored group-major, then unit, then wave: each (group, unit)
 // tile is split at row boundaries into kTileWaves contiguous sub-ranges of
 // about equal size, one per entry wave, each padded to a multiple of kTileLane
 // entries with key kTileSentinel (value 0); kTileLane more sentinels follow
-// the last entry (what a lane with no entries in a pass loads).  key = (local row << 20) | (column -
-// first column of the unit).  wstart[(b U + u) kTileWaves + w] is the first
-// entry of wave w of tile (b, u); the array ends with the total count E.
-// units[4 u .. 4 u + 3] = {first slice, staged, LDS buffer of a staged
-// unit, next staged unit (U if none)}, units[4 U] = S.
+// the last entry (what a lane with no entries in a pass loads).  key = (local
+// row << kTileColBits) | (column - first column of the unit).
+// wstart[(b U + u) kTileWaves + w] is the first entry of wave w of tile
+// (b, u); the array ends with the total count E.  units[4 u .. 4 u + 3] =
+// {first slice, staged, LDS buffer of a staged unit (its ordinal among the
+// staged units modulo tile_buffers), next staged unit (U if none)},
+// units[4 U] = S.
 //
 // What the kernels read instead of wstart: the pass list.  A pass is up to
 // kWave * kTileLane consecutive entries of one sub-range; only a sub-range's
@@ -123,15 +129,31 @@ struct Natural {
 // w) (U + 1) + u] is the index of its first pass in unit u (entry U: one
 // past its last), and pdesc[2 i], pdesc[2 i + 1] = first entry and entry
 // count of pass i.
-constexpr int kTileThreads = 1024;                    // workgroup of a group
-constexpr int kTileStagers = 8;                       // waves that copy vector slices into LDS
-constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 8 waves that add the entries
+constexpr int kTileThreads = 1024;                    // workgroup of a half
+constexpr int kTileStagers = 4;                       // waves that copy vector slices into LDS (one per SIMD)
+constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 12 waves that add the entries
 constexpr int kTileLane = 4;                          // entries per lane per pass
-constexpr int kTileSliceBytes = 65536;                // bytes of a slice: 16384 FP32 or 8192 FP64 columns
-constexpr int kTileRowCap = 4032;                     // local rows per group (12 key bits; FP64 LDS)
-constexpr int kTileColBits = 20;                      // key bits of the column within a unit
+constexpr int kTileHalves = 2;                        // workgroups (CUs) per row group
+constexpr int kTileLdsBytes = 163840;                 // LDS of one gfx950 workgroup
+constexpr int kTileColBits = 19;                      // key bits of the column within a unit
 constexpr uint32_t kTileColMask = (1u << kTileColBits) - 1u;
-constexpr int kTileGatherSlices = 63;                 // slices per gather unit (columns < 2^20 - 1)
+constexpr uint32_t kTileSentinel = 0xFFF80000u;       // row 8191, column 0: never a real key (rows < tile_row_cap)
+// Slice bytes and LDS buffers per element size.  FP32: two 64 KiB buffers
+// (one is consumed while the next slice lands in the other).  FP64: three
+// 32 KiB buffers, because its row sums take twice the LDS; the third
+// buffer gives each slice's loads two units' time to land instead of one.
+__host__ __device__ constexpr int tile_slice_bytes(int tsize) { return tsize == 4 ? 65536 : 32768; }
+__host__ __device__ constexpr int tile_buffers(int tsize) { return tsize == 4 ? 2 : 3; }
+// local rows per group: what the LDS holds beside the buffers (and one
+// spare slot per lane), 8128 in both precisions (13 key bits)
+__host__ __device__ constexpr int tile_row_cap(int tsize) {
+    return ((kTileLdsBytes - tile_buffers(tsize) * tile_slice_bytes(tsize)) / tsize - kWave) / 64 * 64;
+}
+// slices per gather unit: its columns must fit the key's column bits
+__host__ __device__ constexpr int tile_gather_slices(int W) {
+    return ((1 << kTileColBits) / W - 1) < 63 ? ((1 << kTileColBits) / W - 1) : 63;
+}
+static_assert(tile_row_cap(4) < 8191 && tile_row_cap(8) < 8191, "sentinel row");
 // Entries per group that make a slice staged.  Measured on the 1M-column
 // Zipf LP (FP32 dual step, where slices of 400-1100 entries per group lie
 // between the short columns and the dense ones): 384 stages them, 69.6 us;
@@ -145,10 +167,10 @@ constexpr int kTileStageMin = 800;
 // FP64 tiles hold about one pass per wave and unit, where the padding only
 // halves the load queue's lookahead.
 constexpr bool tile_pairs(int tsize) { return tsize == 4; }
-constexpr uint32_t kTileSentinel = 0xFFF00000u;       // row 4095, column 0: never a real key (rows < kTileRowCap)
 
 struct Tiles {
     int32_t nb = 0, S = 0, W = 0, rmax = 0;           // groups, slices, slice width, local rows per group
+    int32_t Q = 1;                                    // halves (workgroups) per group
     int32_t U = 0;                                    // units
     int32_t cols = 0;                                 // length of the gathered vector
     int64_t E = 0;                                    // stored entries (with padding)
@@ -159,6 +181,9 @@ struct Tiles {
     int64_t passes = 0;
     int32_t* units = nullptr;                         // 4 U + 1
     std::vector<int32_t> h_units;                     // host copy of units
+    std::vector<double> h_cost;                       // U: each unit's estimated time (splits a launch into halves)
+    void* part = nullptr;                             // Q * nb * rmax elements: the halves' row sums
+    int32_t* cnt = nullptr;                           // nb: halves finished (0 between launches)
     float* val32 = nullptr;                           // E + kTileLane: the FP32 layout's values
     double* val64 = nullptr;                          // E + kTileLane: the FP64 layout's values
     int parts = 0;                                    // FP32 layout: launches per product (0 = on the CSR parts)

@@ -1057,7 +1057,7 @@ int launch_tile_step(sov_hip_ctx* ctx, const Orient& o, Op op, const T* x) {
     for (int k = 0; k < np; ++k) {
         op.mode = seg_mode(k, np);
         const TileArgs ta = tile_args(t, o.nat.rows, o.nat.cols, sizeof(T) == 8, k, np);
-        k_tile_step<T, Op><<<t.nb, kTileThreads, lds, ctx->stream>>>(op, ta, x);
+        k_tile_step<T, Op><<<t.nb * t.Q, kTileThreads, lds, ctx->stream>>>(op, ta, x);
     }
     SOV_HIP_TRY(check_launch());
     return SOV_OK;

@@ -41,6 +41,12 @@ void Tiles::free() {
     (void)hipFree(units);
     (void)hipFree(val32);
     (void)hipFree(val64);
+    (void)hipFree(part);
+    (void)hipFree(cnt);
+    part = nullptr;
+    cnt = nullptr;
+    h_cost.clear();
+    Q = 1;
     keys = nullptr;
     src = nullptr;
     pidx = nullptr;
@@ -57,7 +63,9 @@ void Tiles::free() {
 }
 
 // Launch k of `parts` covers the units whose first slice lies in
-// [k S / parts, (k + 1) S / parts).
+// [k S / parts, (k + 1) S / parts), split into t.Q halves of contiguous
+// units with about equal estimated time (h_cost): the split that makes the
+// slower half fastest.
 TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, int parts) {
     TileArgs a{};
     a.keys = t.keys;
@@ -66,21 +74,37 @@ TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, 
     a.pdesc = t.pdesc;
     a.sent = static_cast<int32_t>(t.E);
     a.units = t.units;
+    a.part = t.part;
+    a.cnt = t.cnt;
     a.nb = t.nb;
     a.U = t.U;
     a.W = t.W;
     a.rmax = t.rmax;
     a.rows = rows;
     a.cols = cols;
+    a.Q = t.Q;
     const int64_t lo = (int64_t)k * t.S / parts, hi = (int64_t)(k + 1) * t.S / parts;
     int u0 = 0;
     while (u0 < t.U && t.h_units[4 * static_cast<size_t>(u0)] < lo) ++u0;
     int u1 = u0;
     while (u1 < t.U && t.h_units[4 * static_cast<size_t>(u1)] < hi) ++u1;
-    a.u0 = u0;
-    a.u1 = u1;
-    a.staged = 0;
-    for (int u = u0; u < u1; ++u) a.staged += t.h_units[4 * static_cast<size_t>(u) + 1] != 0;
+    a.hu[0] = u0;
+    a.hu[t.Q] = u1;
+    if (t.Q == 2) {
+        double total = 0.0;
+        for (int u = u0; u < u1; ++u) total += t.h_cost[static_cast<size_t>(u)];
+        double left = 0.0, best = total;
+        int cut = u0;
+        for (int c = u0; c <= u1; ++c) {
+            const double worse = std::max(left, total - left);
+            if (worse < best) {
+                best = worse;
+                cut = c;
+            }
+            if (c < u1) left += t.h_cost[static_cast<size_t>(c)];
+        }
+        a.hu[1] = cut;
+    }
     return a;
 }
 
@@ -95,8 +119,8 @@ bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* 
 }
 
 hipError_t tile_lds_attr(const sov_hip_ctx* ctx, const void* kernel, int tsize) {
-    const int64_t want =
-        std::min<int64_t>(ctx->lds_max, (int64_t)tile_lds_bytes(kTileSliceBytes / tsize, kTileRowCap, tsize));
+    const int64_t want = std::min<int64_t>(
+        ctx->lds_max, (int64_t)tile_lds_bytes(tile_slice_bytes(tsize) / tsize, tile_row_cap(tsize), tsize));
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(want));
 }
 
@@ -231,20 +255,33 @@ static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, bool 
     return SOV_OK;
 }
 
+// Estimated time of a unit per group, in entry-equivalents: entries
+// gathered from memory cost about four times those gathered from LDS, and
+// every unit pays its barrier and pipeline fill.  Measured on the
+// 1M-column Zipf LP's FP32 dual step (two gather units of about 8,000
+// entries per group, then staged units of 1,500 to 21,000): gather weight
+// 1 / 2 / 4 / 8 with unit cost 1,000 gave 70.6 / 65.3 / 65.1 / 63.2 us,
+// weight 4 with unit cost 3,000 61.9 us; the primal step (31 equal staged
+// units) does not move.
+constexpr double kGatherEntryCost = 4.0;
+constexpr double kUnitCost = 3000.0;
+
 // The tiled layout of a device-order CSR (rows x cols, nnz entries with
-// source positions src), for a device with `cus` compute units: fills the
-// shape fields and unit table of t and the host arrays.  Host only.
+// source positions src) for products of element size tsize, for a device
+// with `cus` compute units: fills the shape fields, unit table and unit
+// costs of t and the host arrays.  Host only.
 static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& ptr, const std::vector<int32_t>& idx,
-                       const std::vector<int32_t>& srcv, int cus, int W, Tiles& t, std::vector<uint32_t>& keys,
+                       const std::vector<int32_t>& srcv, int cus, int tsize, Tiles& t, std::vector<uint32_t>& keys,
                        std::vector<int32_t>& src, std::vector<int32_t>& wstart) {
     if (rows <= 0) return fail(SOV_INVALID_ARGUMENT, "tiles: no rows");
     const int64_t nchunks = (rows + 63) / 64;
-    const int64_t cap = kTileRowCap / 64;
-    int64_t nb = std::min<int64_t>(std::max(cus, 1), nchunks);
+    const int64_t cap = tile_row_cap(tsize) / 64;
+    t.Q = kTileHalves;
+    int64_t nb = std::min<int64_t>(std::max(cus / t.Q, 1), nchunks);
     nb = std::max<int64_t>(nb, (nchunks + cap - 1) / cap);
     t.nb = static_cast<int32_t>(nb);
     t.rmax = static_cast<int32_t>((nchunks + nb - 1) / nb * 64);
-    t.W = W;
+    t.W = tile_slice_bytes(tsize) / tsize;
     t.cols = cols;
     t.S = static_cast<int32_t>((static_cast<int64_t>(cols) + t.W - 1) / t.W);
     // units: a slice with at least kTileStageMin entries per group on
@@ -253,11 +290,12 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     for (int32_t q = 0; q < ptr[rows]; ++q) ++per_slice[static_cast<size_t>(idx[q] / t.W)];
     std::vector<int32_t> unit_of_slice(static_cast<size_t>(t.S));
     t.h_units.clear();
+    const int gmax = tile_gather_slices(t.W);
     for (int s = 0; s < t.S;) {
         const bool staged = per_slice[static_cast<size_t>(s)] >= static_cast<int64_t>(kTileStageMin) * nb;
         int e = s + 1;
         if (!staged)
-            while (e < t.S && e - s < kTileGatherSlices &&
+            while (e < t.S && e - s < gmax &&
                    per_slice[static_cast<size_t>(e)] < static_cast<int64_t>(kTileStageMin) * nb)
                 ++e;
         const int u = static_cast<int>(t.h_units.size() / 4);
@@ -268,7 +306,8 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     t.U = static_cast<int32_t>(t.h_units.size() / 4);
     int ordinal = 0, next = t.U;
     for (int u = 0; u < t.U; ++u)
-        if (t.h_units[4 * static_cast<size_t>(u) + 1]) t.h_units[4 * static_cast<size_t>(u) + 2] = (ordinal++) & 1;
+        if (t.h_units[4 * static_cast<size_t>(u) + 1])
+            t.h_units[4 * static_cast<size_t>(u) + 2] = (ordinal++) % tile_buffers(tsize);
     for (int u = t.U - 1; u >= 0; --u) {
         t.h_units[4 * static_cast<size_t>(u) + 3] = next;
         if (t.h_units[4 * static_cast<size_t>(u) + 1]) next = u;
@@ -303,6 +342,17 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     }
     wstart.back() = static_cast<int32_t>(E);
     t.E = E;
+    t.h_cost.assign(static_cast<size_t>(t.U), 0.0);
+    for (int u = 0; u < t.U; ++u) {
+        int64_t eu = 0;
+        for (int64_t b = 0; b < nb; ++b) {
+            const size_t at = (static_cast<size_t>(b) * t.U + u) * kTileWaves;
+            eu += wstart[at + kTileWaves] - wstart[at];
+        }
+        const double per_group = static_cast<double>(eu) / static_cast<double>(nb);
+        t.h_cost[static_cast<size_t>(u)] =
+            per_group * (t.h_units[4 * static_cast<size_t>(u) + 1] ? 1.0 : kGatherEntryCost) + kUnitCost;
+    }
     return SOV_OK;
 }
 
@@ -323,8 +373,7 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
     SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
     std::vector<uint32_t> keys;
     std::vector<int32_t> src, wstart;
-    int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, kTileSliceBytes / (f64 ? 8 : 4), t, keys, src,
-                         wstart);
+    int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, f64 ? 8 : 4, t, keys, src, wstart);
     std::vector<int32_t> pidx, pdesc;
     if (rc == SOV_OK) rc = pass_tables(t, wstart, tile_pairs(f64 ? 8 : 4), pidx, pdesc);
     if (rc != SOV_OK) {
@@ -342,6 +391,9 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
     if (e == hipSuccess) e = hipMalloc(&t.units, sizeof(int32_t) * t.h_units.size());
     if (e == hipSuccess)
         e = hipMemcpy(t.units, t.h_units.data(), sizeof(int32_t) * t.h_units.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && t.Q > 1) e = hipMalloc(&t.part, (size_t)(f64 ? 8 : 4) * t.Q * t.nb * t.rmax);
+    if (e == hipSuccess && t.Q > 1) e = hipMalloc(&t.cnt, sizeof(int32_t) * t.nb);
+    if (e == hipSuccess && t.Q > 1) e = hipMemset(t.cnt, 0, sizeof(int32_t) * t.nb);
     if (e == hipSuccess && f64) e = hipMalloc(&t.val64, sizeof(double) * e1);
     if (e == hipSuccess && !f64) e = hipMalloc(&t.val32, sizeof(float) * e1);
     if (e == hipSuccess) e = hipMemcpy(t.keys, keys.data(), sizeof(uint32_t) * e1, hipMemcpyHostToDevice);
@@ -407,7 +459,7 @@ static int spmv_tiles_t(sov_hip_ctx* ctx, const Orient& o, double alpha, const v
         TileSpmvOp<T> op{static_cast<T*>(y),
                          std::is_same<T, double>::value ? reinterpret_cast<T*>(o.acc64) : reinterpret_cast<T*>(o.acc32),
                          T(alpha), T(beta), seg_mode(k, np)};
-        k_tile_spmv<T><<<t.nb, kTileThreads, lds, ctx->stream>>>(op, ta, static_cast<const T*>(x));
+        k_tile_spmv<T><<<t.nb * t.Q, kTileThreads, lds, ctx->stream>>>(op, ta, static_cast<const T*>(x));
     }
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
@@ -451,7 +503,7 @@ int sov_hip_csr_set_tiles(sov_hip_ctx* ctx, sov_hip_csr* h, int orient, int part
 }
 
 int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* ptr, const int32_t* idx,
-                        const int32_t* src, int32_t cus, int32_t tsize, int32_t shape[6], int64_t* entries,
+                        const int32_t* src, int32_t cus, int32_t tsize, int32_t shape[8], int64_t* entries,
                         uint32_t* keys, int32_t* src_out, int32_t* wstart, int32_t* units) {
     if (rows <= 0 || cols < 0 || nnz < 0 || ptr == nullptr || shape == nullptr || entries == nullptr ||
         (nnz > 0 && (idx == nullptr || src == nullptr)) || (tsize != 4 && tsize != 8))
@@ -466,7 +518,7 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     Tiles t;
     std::vector<uint32_t> k;
     std::vector<int32_t> so, ws;
-    const int rc = layout_host(rows, cols, p, ix, sv, cus, kTileSliceBytes / tsize, t, k, so, ws);
+    const int rc = layout_host(rows, cols, p, ix, sv, cus, tsize, t, k, so, ws);
     if (rc != SOV_OK) return rc;
     shape[0] = t.nb;
     shape[1] = t.S;
@@ -474,6 +526,8 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     shape[3] = t.rmax;
     shape[4] = t.U;
     shape[5] = kTileWaves;
+    shape[6] = t.Q;
+    shape[7] = tile_buffers(tsize);
     *entries = t.E;
     if (keys) std::copy(k.begin(), k.begin() + t.E, keys);
     if (src_out) std::copy(so.begin(), so.begin() + t.E, src_out);
@@ -498,7 +552,7 @@ int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     Tiles t;
     std::vector<uint32_t> k;
     std::vector<int32_t> so, ws, pi, pd;
-    int rc = layout_host(rows, cols, p, ix, sv, cus, kTileSliceBytes / tsize, t, k, so, ws);
+    int rc = layout_host(rows, cols, p, ix, sv, cus, tsize, t, k, so, ws);
     if (rc == SOV_OK) rc = pass_tables(t, ws, tile_pairs(tsize), pi, pd);
     if (rc != SOV_OK) return rc;
     *passes = static_cast<int64_t>(pd.size() / 2);

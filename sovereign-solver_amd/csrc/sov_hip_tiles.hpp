@@ -31,28 +31,30 @@ __device__ __forceinline__ const cint32* const_table(const int32_t* p) {
     return (const cint32*)(p);
 }
 
-// What one tiled launch reads: units [u0, u1) of every group's tiles.
+// What one tiled launch reads: units [hu[0], hu[Q]) of every group's
+// tiles, half h of a group taking units [hu[h], hu[h + 1]).
 struct TileArgs {
     const uint32_t* keys;
     const void* vals;         // T
     const int32_t* pidx;      // nb * kTileWaves * (U + 1): first pass of (group, wave) in unit u
     const int32_t* pdesc;     // 2 per pass: first entry, entries in the pass
     const int32_t* units;     // 4 U + 1
+    void* part;               // Q * nb * rmax elements of T: the halves' row sums
+    int32_t* cnt;             // nb: halves of the group finished in this launch
     int32_t nb, U, W, rmax, rows, cols;
-    int32_t u0, u1;
-    int32_t staged;           // staged units in [u0, u1)
+    int32_t Q;
+    int32_t hu[kTileHalves + 1];
     int32_t sent;             // first entry of the sentinel block
 };
 
 // Device row of local row l of group b.
 __host__ __device__ __forceinline__ int tile_row(int b, int nb, int l) { return ((l >> 6) * nb + b) * 64 + (l & 63); }
 
-// LDS of a tiled launch: two slice buffers (one is consumed while the
-// other is filled), the row sums and one slot per lane that takes the
-// writes a lane makes no real write with.  A slice is kTileSliceBytes
-// whatever the element type (W = kTileSliceBytes / tsize columns).
+// LDS of a tiled launch: tile_buffers() slice buffers (one is consumed
+// while the next slices land in the others), the row sums and one slot per
+// lane that takes the writes a lane makes no real write with.
 __host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize) {
-    return (size_t)2 * W * tsize + (size_t)(rmax + kWave) * tsize;
+    return (size_t)tile_buffers(tsize) * W * tsize + (size_t)(rmax + kWave) * tsize;
 }
 
 // The four values of one lane's entries (T = float: one 16-byte load;
@@ -346,37 +348,6 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One staging thread's share of a whole slice (kTileSliceBytes), held in
-// registers between its loads and its store to LDS, so the loads of the
-// next slice are in flight while the entry waves consume the current one.
-template <typename T>
-struct SliceRegs {
-    static constexpr int NV = 16 / sizeof(T);
-    typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
-    static constexpr int kStageThreads = kTileStagers * kWave;
-    static constexpr int kPer = kTileSliceBytes / (kStageThreads * 16);   // 16-byte loads per thread
-    vec r[kPer];
-
-    // Buffer loads: the vector is one buffer resource (its length in bytes
-    // bounds every access, and the hardware returns zeros past it, which is
-    // what a short last slice needs), each thread's offset inside a chunk is
-    // one register, and the chunk's offset a scalar.  So the loads need no
-    // address registers, every register set takes exactly kPer loads on
-    // every path, and the compiler counts the loads in flight exactly.
-    // The resource ends at a whole number of 16-byte vectors, so a load is
-    // either inside it or wholly past it (zeros).
-    __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t rs, const TileArgs& ta, int s, int st) {
-        const int base = s * ta.W * static_cast<int>(sizeof(T));
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-            r[k] = __builtin_bit_cast(vec, __builtin_amdgcn_raw_buffer_load_b128(rs, st * 16, base + k * kStageThreads * 16, 0));
-    }
-    __device__ __forceinline__ void store(T* __restrict__ buf, int st) const {
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) *reinterpret_cast<vec*>(buf + (k * kStageThreads + st) * NV) = r[k];
-    }
-};
-
 // The gathered vector as a buffer resource of cols elements (raw: no
 // stride, no format conversion; 32-bit data format as the raw loads need).
 template <typename T>
@@ -385,9 +356,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
                                              0x00020000);
 }
 
-// The group's row sums over units [u0, u1) into acc (LDS, rmax entries),
-// then op.finish(device row, sum) for each row of the group.  x: the
-// gathered vector (device order, 16-byte aligned).  All threads call it.
+// The group's row sums over this half's units into acc (LDS, rmax
+// entries), then op.finish(device row, sum) for each row of the group,
+// with the halves' sums added in half order.  x: the gathered vector
+// (device order, 16-byte aligned).  All threads call it.
 //
 // Waves 0 .. kTileWaves-1 add the entries.  Each walks its stream of
 // passes (its sub-ranges of the units, in order) with the loads of the
@@ -395,128 +367,127 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
 // consumed in place and refilled with the pass kRing further on, so passes
 // are used in the order their loads were issued (a wave's vector memory
 // counter completes in that order) and no slot is copied while its load is
-// pending.  The last kTileStagers waves copy vector slices into LDS: while
-// the entry waves consume staged unit u from one buffer they fill the other
-// with the next staged slice, whose loads they issued two staged units
-// earlier.  Staging runs in waves of its own so its loads never sit in an
-// entry wave's queue.  Every unit ends in a
-// barrier of all waves: the next unit's sub-ranges give rows to other waves.
+// pending.  The last kTileStagers waves (one per SIMD) copy vector slices
+// into LDS with direct buffer-to-LDS loads (16 bytes per lane, no
+// registers), tile_buffers() - 1 staged units ahead of the entry waves.
+// Staging runs in waves of its own so its loads never sit in an entry
+// wave's queue.  Every unit ends in a barrier of all waves: the next
+// unit's sub-ranges give rows to other waves.
 constexpr int kRing = 4;
+
+// s_waitcnt immediate that waits until at most n vector memory operations
+// of the wave are in flight (expcnt and lgkmcnt left alone).
+__host__ __device__ constexpr int vmcnt_imm(int n) { return 0x0F70 | ((n >> 4) << 14) | (n & 15); }
 
 template <typename T, class Op>
 __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restrict__ x, Op& op) {
+    constexpr int NB = tile_buffers(sizeof(T));
+    constexpr int SB = tile_slice_bytes(sizeof(T));
     extern __shared__ __align__(16) unsigned char tile_lds[];
     T* xs = reinterpret_cast<T*>(tile_lds);
-    T* acc = xs + (size_t)2 * ta.W;
-    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1);
+    T* acc = xs + (size_t)NB * ta.W;
+    // group g, half h: consecutive workgroups are the halves of one group
+    // (workgroups are dealt to the XCDs round-robin, so with two halves the
+    // even XCDs stage one half of the vector and the odd ones the other)
+    const int g = blockIdx.x / ta.Q, h = blockIdx.x - g * ta.Q;
+    const int u0 = ta.hu[h], u1 = ta.hu[h + 1];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
     // wave-uniform, so the unit table and sub-range bounds are scalar loads
     // (their own counter: they never wait for the vector loads in flight)
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     const cint32* un = const_table(ta.units);
     for (int l = tid; l < ta.rmax; l += kTileThreads) acc[l] = T(0);
     if (wave >= kTileWaves) {
-        const int st = tid - kTileWaves * kWave;
-        // The compiled code joins the two branches' control flow at this
-        // point, so the compiler's count of loads in flight here includes
-        // the entry waves' loads, which it would then take as newer than the
-        // register sets' own.  Nothing is in flight on this path: an
-        // explicit vmcnt(0) clears that count at no cost.
-        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
-        // ---- staging waves: two register sets; with s_0, s_1, ... the
-        // staged units of the range, R0 holds the slices s_j of even j and
-        // R1 those of odd j.  During s_i they store s_{i+1} (loaded two
-        // staged units earlier) into its buffer, freed by the barrier that
-        // ended s_{i-1}, and load s_{i+3} into the set just stored: every
-        // slice's loads have two units' time to arrive.  The loop takes two
-        // staged units per turn, so each set stays in its own registers.
-        SliceRegs<T> R0, R1;
-        // A gathered length that is not a whole number of 16-byte vectors:
-        // the resource ends at the last whole vector (the partial one loads
-        // as zeros), and the thread that stores the partial vector then
-        // writes its rem elements, read with scalar loads (x does not
-        // change during the launch).
-        constexpr int NV = SliceRegs<T>::NV;
+        // ---- staging waves.  With s_0, s_1, ... the staged units of the
+        // range: s_0 .. s_{NB-1} are issued before the first barrier (every
+        // buffer is free), and s_{j+NB-1} (j >= 1) right after the barrier
+        // that starts s_j, into the buffer s_{j-1} used (the unit table
+        // gives staged unit j buffer j mod NB).  Before the barrier that
+        // starts s_j the wave waits for its own part of s_j, leaving the
+        // younger slices' loads in flight.  A slice is kStageChunks chunks
+        // of kWave x 16 bytes per staging wave, chunk k of wave w at
+        // (k kTileStagers + w) kWave x 16 bytes into the slice.
+        const int sw = wave - kTileWaves;
+        constexpr int NV = 16 / static_cast<int>(sizeof(T));
+        constexpr int kChunkBytes = kWave * 16;
+        constexpr int kStageChunks = SB / (kChunkBytes * kTileStagers);
+        static_assert(kStageChunks * kChunkBytes * kTileStagers == SB, "slice split");
+        static_assert((NB - 1) * kStageChunks < 64, "vmcnt range");
+        // The resource ends at the last whole 16-byte vector, so a lane's
+        // load is either inside it or wholly past it (the hardware returns
+        // zeros there).  A gathered length that is not a whole number of
+        // 16-byte vectors leaves rem elements: the lane whose load covers
+        // them writes them after its load has landed, read with scalar
+        // loads (x does not change during the launch).
         const int rem = ta.cols % NV;
         const __amdgpu_buffer_rsrc_t rs = vector_rsrc(x, ta.cols - rem);
-        const int last = (ta.cols - 1) / ta.W;              // the slice holding it
-        const int lo = ta.cols - rem - last * ta.W;         // its first element there
-        const bool owner = rem != 0 && st == (lo / NV) % SliceRegs<T>::kStageThreads;
-        auto next_staged = [&](int v) { return v < ta.u1 ? un[4 * v + 3] : ta.u1; };
-        auto buffer = [&](int v) { return xs + (size_t)(un[4 * v + 2] & 1) * ta.W; };
-        int q0 = ta.u1;   // s_i; then s_{i+1}, s_{i+2}, s_{i+3}
-        if (ta.u0 < ta.u1) q0 = un[4 * ta.u0 + 1] ? ta.u0 : un[4 * ta.u0 + 3];
-        int q1 = next_staged(q0), q2 = next_staged(q1), q3 = next_staged(q2);
-        // The order of the loads matters: at the loop's head R1's loads are
-        // the older set, as after every turn (scheduling barriers keep the
-        // compiler from reordering the prologue).  Every step issues its
-        // set's loads, from slice s_i when no slice is left to load, so the
-        // compiler counts the loads in flight exactly on every path.
-        auto slice_of = [&](int v, int fallback) { return un[4 * (v < ta.u1 ? v : fallback)]; };
-        // where s_{i+1} goes; past the last staged unit, the buffer s_i does
-        // not use (free since s_{i-1} ended), so the store is unconditional
-        auto store_target = [&](int v, int cur) {
-            return v < ta.u1 ? buffer(v) : xs + (size_t)(1 - (un[4 * cur + 2] & 1)) * ta.W;
+        const int last = (ta.cols - 1) / ta.W;                        // the slice holding them
+        const int lo = ta.cols - rem - last * ta.W;                   // their first element there
+        const int lo_chunk = lo * static_cast<int>(sizeof(T)) / kChunkBytes;
+        const bool owner = rem != 0 && sw == lo_chunk % kTileStagers &&
+                           lane == (lo * static_cast<int>(sizeof(T)) % kChunkBytes) / 16;
+        auto next_staged = [&](int v) { return v < u1 ? un[4 * v + 3] : u1; };
+        auto buffer = [&](int v) { return xs + (size_t)un[4 * v + 2] * ta.W; };
+        int issued = 0;   // slices of the range issued so far
+        int upcoming = u0 < u1 ? (un[4 * u0 + 1] ? u0 : un[4 * u0 + 3]) : u1;   // the next to issue
+        auto issue_next = [&]() {
+            const int base = un[4 * upcoming] * SB;
+            unsigned char* buf = reinterpret_cast<unsigned char*>(buffer(upcoming));
+#pragma unroll
+            for (int k = 0; k < kStageChunks; ++k) {
+                const int c = (k * kTileStagers + sw) * kChunkBytes;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void*)(buf + c), 16, lane * 16, base + c, 0, 0);
+            }
+            ++issued;
+            upcoming = next_staged(upcoming);
         };
-        // store a set holding the slice of unit v into buf
-        auto store_set = [&](const SliceRegs<T>& R, T* buf, int v) {
-            R.store(buf, st);
-            if (owner && v < ta.u1 && un[4 * v] == last) {
+        int j = 0;   // ordinal of the next staged unit to start
+        // before the barrier that starts staged unit v (ordinal j): the
+        // wave's part of v has landed, and the rem tail is written over it
+        auto land = [&](int v) {
+            const int younger = issued - j - 1;
+            if (NB > 2 && younger >= 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * kStageChunks));
+            else if (younger >= 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(kStageChunks));
+            else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            if (owner && un[4 * v] == last) {
                 typedef const T __attribute__((address_space(4))) ctail;
                 const ctail* xt = (const ctail*)(x + (ta.cols - rem));
+                T* buf = buffer(v);
                 for (int i = 0; i < NV - 1; ++i)
                     if (i < rem) buf[lo + i] = xt[i];
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the tail is in LDS
             }
         };
-        if (q0 < ta.u1) {
-            R0.load(rs, ta, un[4 * q0], st);
-            R1.load(rs, ta, slice_of(q1, q0), st);
-            __builtin_amdgcn_sched_barrier(0);
-            store_set(R0, buffer(q0), q0);
-            R0.load(rs, ta, slice_of(q2, q0), st);
-            __builtin_amdgcn_sched_barrier(0);
+        // The staging waves' barrier.  The entry waves' lds_barrier is a
+        // release of the LDS address space, which the compiler implements
+        // here by waiting for every direct-to-LDS load in flight
+        // (vmcnt(0)), the younger slices' included.  What must be in LDS
+        // before this barrier, land() has already waited for, so it is a
+        // bare s_barrier (the memory clobber keeps the compiler from moving
+        // LDS accesses across it).
+        auto stage_barrier = []() { asm volatile("s_barrier" ::: "memory"); };
+        for (int k = 0; k < NB && upcoming < u1; ++k) issue_next();
+        if (u0 < u1 && un[4 * u0 + 1]) land(u0);
+        stage_barrier();
+        for (int u = u0; u < u1; ++u) {
+            if (un[4 * u + 1]) {   // staged unit j has started
+                if (j >= 1 && upcoming < u1) issue_next();
+                ++j;
+            }
+            if (u + 1 < u1 && un[4 * (u + 1) + 1]) land(u + 1);
+            stage_barrier();
         }
-        lds_barrier();
-        int u = ta.u0;
-        auto shift = [&]() {
-            q0 = q1;
-            q1 = q2;
-            q2 = q3;
-            q3 = next_staged(q3);
-        };
-        // The turns are counted (ta.staged staged units), so the only way
-        // back to the loop's head is the end of an odd step: at the head
-        // R1's loads are always the older set and the even step's stores
-        // wait for them alone.  An odd count ends with one even step.
-        auto even_step = [&]() {
-            for (; u < q0; ++u) lds_barrier();   // the gather units before s_i
-            store_set(R1, store_target(q1, q0), q1);
-            R1.load(rs, ta, slice_of(q3, q0), st);
-            lds_barrier();
-            u = q0 + 1;
-            shift();
-        };
-        int n = ta.staged;
-        for (; n >= 2; n -= 2) {
-            even_step();   // i even: s_{i+1} and s_{i+3} are odd
-            for (; u < q0; ++u) lds_barrier();
-            // i odd: s_{i+1} and s_{i+3} are even
-            store_set(R0, store_target(q1, q0), q1);
-            R0.load(rs, ta, slice_of(q3, q0), st);
-            lds_barrier();
-            u = q0 + 1;
-            shift();
-        }
-        if (n) even_step();
-        for (; u < ta.u1; ++u) lds_barrier();
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));   // nothing is left in flight
     } else {
         // ---- entry waves
         // The wave's passes of units [u0, u1) are [i, iend) of the pass
         // list.  Lane l holds the descriptor of pass dbase + l (reloaded
         // every kWave passes); a refill reads its pass's descriptor with
         // readlane, so it waits for no memory.
-        const cint32* pi = const_table(ta.pidx) + (int64_t)(b * kTileWaves + wave) * (ta.U + 1);
-        int i = pi[ta.u0];
-        const int iend = pi[ta.u1];
+        const cint32* pi = const_table(ta.pidx) + (int64_t)(g * kTileWaves + wave) * (ta.U + 1);
+        int i = pi[u0];
+        const int iend = pi[u1];
         int ld = i, dbase = i;
         auto load_desc = [&]() {
             const int at = dbase + lane;
@@ -540,7 +511,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         PassData<T> q0 = fetch(), q1 = fetch(), q2 = fetch(), q3 = fetch();
         lds_barrier();
         // consumer state: unit cu, its end in the pass list
-        int cu = ta.u0, nxt = i, staged = 0, carry_row = -1;
+        int cu = u0, nxt = i, staged = 0, carry_row = -1;
         T carry_val = T(0);
         const T* xl = xs;
         const T* xg = x;
@@ -602,10 +573,10 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // close the units this wave has finished (and those it has no
         // passes in) before pass i
         auto close_done = [&]() {
-            while (cu < ta.u1 && i == nxt) {
+            while (cu < u1 && i == nxt) {
                 close_unit(cu);
                 ++cu;
-                if (cu < ta.u1) open_unit(cu);
+                if (cu < u1) open_unit(cu);
             }
         };
         // two queue slots, passes i and i + 1: when both lie in the current
@@ -618,7 +589,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // number of loads, and the loop has its only exit at the top).
         auto step2 = [&](PassData<T>& sa, PassData<T>& sb) {
             close_done();
-            if (cu < ta.u1) {
+            if (cu < u1) {
                 if (i + 1 < nxt && sb.n == 0) {
                     consume_pass(sa);
                     i += 2;
@@ -631,7 +602,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
                     consume_pass(sa);
                     ++i;
                     close_done();
-                    if (cu < ta.u1) {
+                    if (cu < u1) {
                         consume_pass(sb);
                         ++i;
                     }
@@ -644,20 +615,20 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // empty passes)
         auto step = [&](PassData<T>& slot) {
             close_done();
-            if (cu < ta.u1) {
+            if (cu < u1) {
                 consume_pass(slot);
                 ++i;
             }
             slot = fetch();
         };
-        if (cu < ta.u1) open_unit(cu);
+        if (cu < u1) open_unit(cu);
         if constexpr (tile_pairs(sizeof(T))) {
-            while (cu < ta.u1) {
+            while (cu < u1) {
                 step2(q0, q1);
                 step2(q2, q3);
             }
         } else {
-            while (cu < ta.u1) {
+            while (cu < u1) {
                 step(q0);
                 step(q1);
                 step(q2);
@@ -666,9 +637,43 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         }
     }
     lds_barrier();
+    if (ta.Q == 1) {
+        for (int l = tid; l < ta.rmax; l += kTileThreads) {
+            const int p = tile_row(g, ta.nb, l);
+            if (p < ta.rows) op.finish(p, acc[l]);
+        }
+        return;
+    }
+    // Halves: each publishes its row sums with device-scope stores (sc1:
+    // written through to where every XCD reads them, and counted as done
+    // only once they are), then counts itself in at the group's counter
+    // (a device-scope atomic).  The last half to arrive resets the counter
+    // for the next launch and finishes the rows, reading the other halves'
+    // sums with device-scope loads (past any stale cache line).
+    T* part = static_cast<T*>(ta.part);
+    T* mine = part + ((size_t)h * ta.nb + g) * ta.rmax;
+    for (int l = tid; l < ta.rmax; l += kTileThreads)
+        __hip_atomic_store(mine + l, acc[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    // the arrival order goes through lane 0's spare slot (no pass writes
+    // it any more), so the kernel needs no LDS beyond tile_lds_bytes
+    volatile int* arrived = reinterpret_cast<volatile int*>(acc + ta.rmax);
+    __syncthreads();
+    if (tid == 0) *arrived = __hip_atomic_fetch_add(ta.cnt + g, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (*arrived != ta.Q - 1) return;
+    if (tid == 0) __hip_atomic_store(ta.cnt + g, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int l = tid; l < ta.rmax; l += kTileThreads) {
-        const int p = tile_row(b, ta.nb, l);
-        if (p < ta.rows) op.finish(p, acc[l]);
+        const int p = tile_row(g, ta.nb, l);
+        if (p >= ta.rows) continue;
+        T sum = T(0);
+        for (int k = 0; k < ta.Q; ++k) {
+            const T v = k == h ? acc[l]
+                               : __hip_atomic_load(part + ((size_t)k * ta.nb + g) * ta.rmax + l, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            sum = k == 0 ? v : sum + v;
+        }
+        op.finish(p, sum);
     }
 }
 

@@ -98,6 +98,35 @@ def test_tiled_spmv_matches_oracle(ctx, name, prec, parts):
     A.destroy()
 
 
+@pytest.mark.parametrize("name", ["zipf", "tall"])
+@pytest.mark.parametrize("prec,parts", [(0, 1), (1, 1), (1, 2)])
+def test_tiled_products_in_sequence(ctx, name, prec, parts):
+    """Two workgroups share each row group, and the last to finish adds the
+    other's published row sums: products of different vectors back to back
+    (both orientations interleaved) must each be the product of their own
+    vector, which a stale or missing half would break far beyond rounding."""
+    M = model(name)
+    A = Csr(ctx, M.m, M.n, M.row_ptr, M.col_idx[: M.nnz], M.val[: M.nnz])
+    set_tiles(ctx, A, parts, prec)
+    dt = np.float64 if prec == 0 else np.float32
+    vals = M.val[: M.nnz].astype(dt).astype(np.float64)
+    Mr = O.Model(dict(m=M.m, n=M.n, row_ptr=M.row_ptr, col_idx=M.col_idx[: M.nnz], val=vals, c=M.c,
+                      col_lo=M.col_lo, col_hi=M.col_hi, row_lo=M.row_lo, row_hi=M.row_hi))
+    Ma = O.Model(dict(m=M.m, n=M.n, row_ptr=M.row_ptr, col_idx=M.col_idx[: M.nnz], val=np.abs(vals), c=M.c,
+                      col_lo=M.col_lo, col_hi=M.col_hi, row_lo=M.row_lo, row_hi=M.row_hi))
+    rng = np.random.default_rng(40 + parts + 10 * prec)
+    for k in range(6):
+        transpose = k % 2 == 1
+        x = rng.standard_normal(M.m if transpose else M.n).astype(dt).astype(np.float64) * (k + 1)
+        got = A.spmv(x, transpose=transpose, prec=prec).astype(np.float64)
+        want = O.spmv_t(Mr, x) if transpose else O.spmv(Mr, x)
+        bound = (lens(M, transpose) + 4) * U[prec] * 2 * (O.spmv_t(Ma, np.abs(x)) if transpose
+                                                         else O.spmv(Ma, np.abs(x)))
+        bad = np.abs(got - want) > bound
+        assert not bad.any(), (k, np.flatnonzero(bad)[:5], got[bad][:5], want[bad][:5])
+    A.destroy()
+
+
 @pytest.mark.parametrize("prec,parts", [(0, 1), (0, 2), (1, 1), (1, 2)])
 def test_tiled_spmv_with_ragged_lengths(ctx, prec, parts):
     """Gathered lengths past the last whole 16-byte vector (n = 40003,

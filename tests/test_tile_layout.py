@@ -3,14 +3,17 @@ device path uploads), checked on the CPU against the CSR it is built from:
 every entry stored exactly once with its row, column and source position
 decodable from (group, unit, key); units that cover the slices in order
 (a staged unit is one slice with at least 800 entries per group, a gather
-unit a run of up to 63 sparser slices); each (group, unit) tile split into 8
-row-aligned sub-ranges (one per entry wave), each a multiple of 4 entries with padding only
-at its end; rows dealt to groups in 64-row chunks; the pass list the
+unit a run of sparser slices whose columns fit the key's 19 column bits);
+staged units cycling through the LDS buffers; each (group, unit) tile split
+into 12 row-aligned sub-ranges (one per entry wave), each a multiple of 4
+entries with padding only at its end; rows dealt to groups in 64-row
+chunks, two workgroups per group; the pass list the
 kernels read (every sub-range cut into passes of at most 256 entries, padded to an even count for FP32,
 listed per group and wave, unit by unit); and the product the layout
 encodes equal to the CSR product.  Cases cover empty rows, a row
 denser than one wave pass, several slices (ragged last slice), and more
-row groups than compute units when a group would exceed 4032 rows."""
+row groups than half the compute units when a group would exceed 8128
+rows."""
 import ctypes as C
 
 import numpy as np
@@ -18,18 +21,20 @@ import pytest
 
 from sovereign_amd.device import lib
 
-SENT = 0xFFF00000          # row 4095, column 0
+SENT = 0xFFF80000          # row 8191, column 0
+COLBITS = 19
 
 
 def layout(ptr, idx, src, rows, cols, cus, tsize):
     L = lib()
-    shape = np.zeros(6, np.int32)
+    shape = np.zeros(8, np.int32)
     E = np.zeros(1, np.int64)
     nnz = len(idx)
     args = (rows, cols, nnz, ptr.ctypes.data, idx.ctypes.data if nnz else None, src.ctypes.data if nnz else None,
             cus, tsize, shape.ctypes.data, E.ctypes.data)
     assert L.sov_hip_tile_layout(*args, None, None, None, None) == 0
-    nb, S, W, rmax, U, waves = (int(v) for v in shape)
+    nb, S, W, rmax, U, waves, Q, NBUF = (int(v) for v in shape)
+    assert Q == 2 and NBUF == (2 if tsize == 4 else 3)
     keys = np.zeros(max(int(E[0]), 1), np.uint32)
     so = np.zeros(max(int(E[0]), 1), np.int32)
     ws = np.zeros(nb * U * waves + 1, np.int32)
@@ -67,7 +72,7 @@ CASES = {
     "two_slices": (300, 20000, "0-40", 2),
     "dense_row": (130, 33000, [33000] + [0] * 128 + [5], 3),
     "long_rows": (200, 40000, [0] * 50 + [1500] + [7] * 149, 1),
-    "many_groups": (4096 * 3 + 5, 100, "1-4", 1),
+    "many_groups": (8192 * 3 + 5, 100, "1-4", 2),
     "no_entries": (64, 10, [0] * 64, 8),
 }
 
@@ -91,11 +96,11 @@ def test_layout_stores_the_csr(name, tsize):
     nnz = int(ptr[-1])
     src = rng.permutation(nnz).astype(np.int32)      # arbitrary source positions
     nb, S, W, rmax, U, WAVES, keys, so, ws, units = layout(ptr, idx, src, rows, cols, cus, tsize)
-    assert WAVES == 8
+    assert WAVES == 12
     nchunks = -(-rows // 64)
-    assert W == 65536 // tsize and S == -(-cols // W)       # a slice is 64 KiB in either precision
-    assert rmax <= 4032 and nb * rmax >= rows and rmax % 64 == 0
-    assert nb == max(min(cus, nchunks), -(-nchunks // 63))
+    assert W == (16384 if tsize == 4 else 4096) and S == -(-cols // W)   # 64 KiB FP32 / 32 KiB FP64 slices
+    assert rmax <= 8128 and nb * rmax >= rows and rmax % 64 == 0
+    assert nb == max(min(max(cus // 2, 1), nchunks), -(-nchunks // 127))
     # units: contiguous slice ranges; staged = one slice with >= 800 entries per group
     first = units[0:4 * U:4]
     staged = units[1:4 * U:4]
@@ -108,9 +113,10 @@ def test_layout_stores_the_csr(name, tsize):
         if staged[u]:
             assert width == 1 and dense.all()
         else:
-            assert 1 <= width <= 63 and not dense.any()
+            assert 1 <= width <= min(63, 2 ** COLBITS // W - 1) and not dense.any()
     sidx = np.flatnonzero(staged)
-    assert np.all(units[2:4 * U:4][sidx] == np.arange(len(sidx)) % 2)          # alternating LDS buffers
+    nbuf = 2 if tsize == 4 else 3
+    assert np.all(units[2:4 * U:4][sidx] == np.arange(len(sidx)) % nbuf)       # LDS buffers in turn
     for u in range(U):
         later = sidx[sidx > u]
         assert units[4 * u + 3] == (later[0] if len(later) else U)
@@ -132,7 +138,7 @@ def test_layout_stores_the_csr(name, tsize):
             nreal = int(real.sum())
             assert np.all(real[:nreal]) and e1 - e0 - nreal < 4      # padding only at the end
             k, sv = k[:nreal].astype(np.int64), so[e0:e0 + nreal]
-            l, q = k >> 20, k & 0xFFFFF
+            l, q = k >> COLBITS, k & (2 ** COLBITS - 1)
             assert np.all(np.diff(l) >= 0) and np.all(l < rmax)
             p = ((l >> 6) * nb + b) * 64 + (l & 63)
             assert np.all(p < rows)
@@ -176,16 +182,16 @@ def test_layout_stores_the_csr(name, tsize):
     t_of = np.searchsorted(ws, np.arange(len(keys)), side="right") - 1
     g, u = np.divmod(t_of // WAVES, max(U, 1))
     k = keys.astype(np.int64)
-    l = k >> 20
+    l = k >> COLBITS
     p = ((l >> 6) * nb + g) * 64 + (l & 63)
-    np.add.at(y, p[real], val[pos_of_src[so[real]]] * x[first[u[real]] * W + (k[real] & 0xFFFFF)])
+    np.add.at(y, p[real], val[pos_of_src[so[real]]] * x[first[u[real]] * W + (k[real] & (2 ** COLBITS - 1))])
     want = np.array([val[ptr[r]:ptr[r + 1]] @ x[idx[ptr[r]:ptr[r + 1]]] for r in range(rows)])
     np.testing.assert_allclose(y, want, rtol=1e-12, atol=1e-12)
 
 
 def test_layout_rejects_bad_input():
     L = lib()
-    shape = np.zeros(6, np.int32)
+    shape = np.zeros(8, np.int32)
     E = np.zeros(1, np.int64)
     ptr = np.array([0, 2], np.int32)
     idx = np.array([0, 7], np.int32)      # column 7 out of range for cols = 5
