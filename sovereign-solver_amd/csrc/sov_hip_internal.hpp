@@ -111,9 +111,9 @@ struct Natural {
 //
 // Entries are stored group-major, then unit, then wave: each (group, unit)
 // tile is split at row boundaries into kTileWaves contiguous sub-ranges of
-// about equal size, one per entry wave, each padded to a multiple of kTileLane
-// entries with key kTileSentinel (value 0); kTileLane more sentinels follow
-// the last entry (what a lane with no entries in a pass loads).  key = (local
+// about equal size, one per entry wave, each padded to a multiple of
+// tile_lane() entries with key kTileSentinel (value 0); kTileLaneMax more
+// sentinels follow the last entry (what a lane with no entries in a pass loads).  key = (local
 // row << kTileColBits) | (column - first column of the unit).
 // wstart[(b U + u) kTileWaves + w] is the first entry of wave w of tile
 // (b, u); the array ends with the total count E.  units[4 u .. 4 u + 3] =
@@ -122,7 +122,7 @@ struct Natural {
 // units[4 U] = S.
 //
 // What the kernels read instead of wstart: the pass list.  A pass is up to
-// kWave * kTileLane consecutive entries of one sub-range; only a sub-range's
+// kWave * tile_lane() consecutive entries of one sub-range; only a sub-range's
 // last pass is short, and for kernels that add passes in pairs
 // (tile_pairs) an empty pass (count 0) evens each (wave, unit) count.  The
 // passes of wave w of group b are listed unit by unit: pidx[(b kTileWaves +
@@ -132,7 +132,14 @@ struct Natural {
 constexpr int kTileThreads = 1024;                    // workgroup of a half
 constexpr int kTileStagers = 4;                       // waves that copy vector slices into LDS (one per SIMD)
 constexpr int kTileWaves = kTileThreads / kWave - kTileStagers;   // 12 waves that add the entries
-constexpr int kTileLane = 4;                          // entries per lane per pass
+// Entries per lane per pass.  Measured on the 1M-column Zipf LP: 8 FP32
+// entries per lane (one segmented scan and one row-sum batch per 512
+// entries) made the FP32 steps slower, 64.5 / 68.4 us against 59.1 /
+// 61.8 us with 4 and pairs of passes (with pairs of 8-entry passes: 69.0 /
+// 70.4 us, 109 VGPRs): the pass's latency chain, not its instruction
+// count, sets the time.
+__host__ __device__ constexpr int tile_lane(int) { return 4; }
+constexpr int kTileLaneMax = 8;                       // sentinel entries after the last one
 constexpr int kTileHalves = 2;                        // workgroups (CUs) per row group
 constexpr int kTileLdsBytes = 163840;                 // LDS of one gfx950 workgroup
 constexpr int kTileColBits = 19;                      // key bits of the column within a unit
@@ -184,8 +191,8 @@ struct Tiles {
     std::vector<double> h_cost;                       // U: each unit's estimated time (splits a launch into halves)
     void* part = nullptr;                             // Q * nb * rmax elements: the halves' row sums
     int32_t* cnt = nullptr;                           // nb: halves finished (0 between launches)
-    float* val32 = nullptr;                           // E + kTileLane: the FP32 layout's values
-    double* val64 = nullptr;                          // E + kTileLane: the FP64 layout's values
+    float* val32 = nullptr;                           // E + kTileLaneMax: the FP32 layout's values
+    double* val64 = nullptr;                          // E + kTileLaneMax: the FP64 layout's values
     int parts = 0;                                    // FP32 layout: launches per product (0 = on the CSR parts)
     int parts64 = 0;                                  // FP64 layout: the same
     bool built() const { return pidx != nullptr; }

@@ -105,6 +105,11 @@ TileArgs tile_args(const Tiles& t, int32_t rows, int32_t cols, bool f64, int k, 
         }
         a.hu[1] = cut;
     }
+    for (int h = 0; h < t.Q; ++h) {
+        int v = a.hu[h];
+        while (v < a.hu[h + 1] && !t.h_units[4 * static_cast<size_t>(v) + 1]) ++v;
+        a.hs[h] = v;
+    }
     return a;
 }
 
@@ -133,7 +138,7 @@ struct GroupOut {
     std::vector<int32_t> wrel;   // U * kTileWaves starts, relative to the group
 };
 
-void build_group(int b, const Tiles& t, int32_t rows, const std::vector<int32_t>& ptr,
+void build_group(int b, const Tiles& t, int lanew, int32_t rows, const std::vector<int32_t>& ptr,
                  const std::vector<int32_t>& idx, const std::vector<int32_t>& srcv,
                  const std::vector<int32_t>& unit_of_slice, GroupOut& g) {
     const int U = t.U, W = t.W;
@@ -164,11 +169,11 @@ void build_group(int b, const Tiles& t, int32_t rows, const std::vector<int32_t>
         }
     }
     // split each tile at row boundaries into kTileWaves sub-ranges, pad each
-    // to a multiple of kTileLane
+    // to a multiple of lanew (the entries per lane per pass)
     g.wrel.assign(static_cast<size_t>(U) * kTileWaves, 0);
     g.keys.clear();
     g.src.clear();
-    g.keys.reserve(bk.size() + static_cast<size_t>(U) * kTileWaves * (kTileLane - 1));
+    g.keys.reserve(bk.size() + static_cast<size_t>(U) * kTileWaves * (lanew - 1));
     g.src.reserve(g.keys.capacity());
     for (int u = 0; u < U; ++u) {
         const int64_t b0 = cnt[static_cast<size_t>(u)], n = cnt[static_cast<size_t>(u) + 1] - b0;
@@ -189,7 +194,7 @@ void build_group(int b, const Tiles& t, int32_t rows, const std::vector<int32_t>
                 g.keys.push_back(bk[static_cast<size_t>(b0 + i)]);
                 g.src.push_back(bs[static_cast<size_t>(b0 + i)]);
             }
-            while (g.keys.size() % kTileLane != 0) {
+            while (g.keys.size() % lanew != 0) {
                 g.keys.push_back(kTileSentinel);
                 g.src.push_back(-1);
             }
@@ -208,7 +213,7 @@ __global__ void k_tile_vals(int64_t E, const int32_t* __restrict__ src, const do
 
 template <typename T>
 int gather_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, const Tiles& t, T* out) {
-    const int64_t n = t.E + kTileLane;   // the arrays' full length (padding and the sentinel block read 0)
+    const int64_t n = t.E + kTileLaneMax;   // the arrays' full length (padding and the sentinel block read 0)
     k_tile_vals<T><<<launch_blocks(n), kBlock, 0, ctx->stream>>>(n, t.src, h->a.nat.val64, out);
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
@@ -218,11 +223,11 @@ int gather_tile_values(sov_hip_ctx* ctx, const sov_hip_csr* h, const Tiles& t, T
 
 // The pass list of a layout (what the kernels read instead of wstart): for
 // each (group, wave), unit by unit, the passes of its sub-range, each up to
-// kWave * kTileLane entries.  Host only.
-static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, bool pairs, std::vector<int32_t>& pidx,
-                       std::vector<int32_t>& pdesc) {
+// kWave * lanew entries.  Host only.
+static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, int lanew, bool pairs,
+                       std::vector<int32_t>& pidx, std::vector<int32_t>& pdesc) {
     const int U = t.U;
-    constexpr int kPass = kWave * kTileLane;
+    const int kPass = kWave * lanew;
     pidx.assign(static_cast<size_t>(t.nb) * kTileWaves * (U + 1), 0);
     pdesc.clear();
     for (int b = 0; b < t.nb; ++b)
@@ -256,13 +261,14 @@ static int pass_tables(const Tiles& t, const std::vector<int32_t>& wstart, bool 
 }
 
 // Estimated time of a unit per group, in entry-equivalents: entries
-// gathered from memory cost about four times those gathered from LDS, and
-// every unit pays its barrier and pipeline fill.  Measured on the
-// 1M-column Zipf LP's FP32 dual step (two gather units of about 8,000
-// entries per group, then staged units of 1,500 to 21,000): gather weight
-// 1 / 2 / 4 / 8 with unit cost 1,000 gave 70.6 / 65.3 / 65.1 / 63.2 us,
-// weight 4 with unit cost 3,000 61.9 us; the primal step (31 equal staged
-// units) does not move.
+// gathered from memory cost more than those gathered from LDS, and every
+// unit pays its barrier, table reads and the fill and drain of the entry
+// waves' passes.  Measured on the 1M-column Zipf LP's FP32 dual step (two
+// gather units of about 8,000 entries per group, then staged units of
+// 1,500 to 21,000): gather weight 1 / 2 / 4 / 8 with unit cost 1,000 gave
+// 70.6 / 65.3 / 65.1 / 63.2 us; 4 with 3,000, 60.6-61.9 us; 2 with 3,500
+// (a fit to per-unit timestamps of an instrumented copy), 64.9-65.2 us;
+// the primal step (31 equal staged units) does not move.
 constexpr double kGatherEntryCost = 4.0;
 constexpr double kUnitCost = 3000.0;
 
@@ -320,16 +326,16 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
         for (int th = 0; th < nth; ++th)
             pool.emplace_back([&, th] {
                 for (int64_t b = th; b < nb; b += nth)
-                    build_group(static_cast<int>(b), t, rows, ptr, idx, srcv, unit_of_slice,
+                    build_group(static_cast<int>(b), t, tile_lane(tsize), rows, ptr, idx, srcv, unit_of_slice,
                                 groups[static_cast<size_t>(b)]);
             });
         for (auto& th : pool) th.join();
     }
     int64_t E = 0;
     for (const GroupOut& g : groups) E += static_cast<int64_t>(g.keys.size());
-    if (E + kTileLane >= INT32_MAX) return fail(SOV_INVALID_ARGUMENT, "tiles: too many entries for 32-bit offsets");
+    if (E + kTileLaneMax >= INT32_MAX) return fail(SOV_INVALID_ARGUMENT, "tiles: too many entries for 32-bit offsets");
     wstart.assign(static_cast<size_t>(nb) * t.U * kTileWaves + 1, 0);
-    keys.assign(static_cast<size_t>(E + kTileLane), kTileSentinel);   // entries, then the sentinel block
+    keys.assign(static_cast<size_t>(E + kTileLaneMax), kTileSentinel);   // entries, then the sentinel block
     src.assign(keys.size(), -1);
     int64_t off = 0;
     for (int64_t b = 0; b < nb; ++b) {
@@ -375,7 +381,7 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
     std::vector<int32_t> src, wstart;
     int rc = layout_host(rows, o.nat.cols, ptr, idx, srcv, ctx->cus, f64 ? 8 : 4, t, keys, src, wstart);
     std::vector<int32_t> pidx, pdesc;
-    if (rc == SOV_OK) rc = pass_tables(t, wstart, tile_pairs(f64 ? 8 : 4), pidx, pdesc);
+    if (rc == SOV_OK) rc = pass_tables(t, wstart, tile_lane(f64 ? 8 : 4), tile_pairs(f64 ? 8 : 4), pidx, pdesc);
     if (rc != SOV_OK) {
         t.free();
         return rc;
@@ -383,7 +389,7 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
     t.passes = static_cast<int64_t>(pdesc.size() / 2);
     if (pdesc.empty()) pdesc.assign(2, 0);
     const int64_t E = t.E;
-    const int64_t e1 = E + kTileLane;   // entries and the sentinel block
+    const int64_t e1 = E + kTileLaneMax;   // entries and the sentinel block
     hipError_t e = hipMalloc(&t.keys, sizeof(uint32_t) * e1);
     if (e == hipSuccess) e = hipMalloc(&t.src, sizeof(int32_t) * e1);
     if (e == hipSuccess) e = hipMalloc(&t.pidx, sizeof(int32_t) * pidx.size());
@@ -553,7 +559,7 @@ int sov_hip_tile_passes(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     std::vector<uint32_t> k;
     std::vector<int32_t> so, ws, pi, pd;
     int rc = layout_host(rows, cols, p, ix, sv, cus, tsize, t, k, so, ws);
-    if (rc == SOV_OK) rc = pass_tables(t, ws, tile_pairs(tsize), pi, pd);
+    if (rc == SOV_OK) rc = pass_tables(t, ws, tile_lane(tsize), tile_pairs(tsize), pi, pd);
     if (rc != SOV_OK) return rc;
     *passes = static_cast<int64_t>(pd.size() / 2);
     if (pidx) std::copy(pi.begin(), pi.end(), pidx);

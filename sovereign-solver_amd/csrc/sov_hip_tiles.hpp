@@ -44,6 +44,7 @@ struct TileArgs {
     int32_t nb, U, W, rmax, rows, cols;
     int32_t Q;
     int32_t hu[kTileHalves + 1];
+    int32_t hs[kTileHalves];  // first staged unit of each half (its end if none)
     int32_t sent;             // first entry of the sentinel block
 };
 
@@ -57,51 +58,53 @@ __host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize) 
     return (size_t)tile_buffers(tsize) * W * tsize + (size_t)(rmax + kWave) * tsize;
 }
 
-// The four values of one lane's entries (T = float: one 16-byte load;
-// double: two).
-template <typename T>
-struct Quad {
-    T v[4];
-};
-
-template <typename T>
-__device__ __forceinline__ Quad<T> load_quad(const T* __restrict__ vals, uint32_t e);
-
-template <>
-__device__ __forceinline__ Quad<float> load_quad<float>(const float* __restrict__ vals, uint32_t e) {
-    const f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(vals + e));
-    return Quad<float>{{a.x, a.y, a.z, a.w}};
-}
-
-template <>
-__device__ __forceinline__ Quad<double> load_quad<double>(const double* __restrict__ vals, uint32_t e) {
-    const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals + e));
-    const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals + e + 2));
-    return Quad<double>{{a.x, a.y, b.x, b.y}};
-}
-
-// One lane's four entries of a pass as loaded (sentinels where it has
-// none).
-template <typename T>
+// One lane's L entries of a pass as loaded (sentinels where it has none):
+// keys and values, each read with 16-byte loads.
+template <typename T, int L>
 struct PassData {
-    u32x4 k;
-    Quad<T> v;
+    uint32_t k[L];
+    T v[L];
     int32_t n;   // entries in the pass (wave-uniform; 0: an empty pass)
 };
 
-// The lane's four entries of the pass whose first entry is `start` and
-// which holds `cnt` entries; a lane at or past cnt loads the sentinel block
-// (keys kTileSentinel, values 0), so what a lane holds needs no mask.  The
-// loads are issued on every path and nothing reads the loaded registers
-// here, so every refill issues the same loads, the compiler can count
-// exactly how many loads follow each queue slot, and the refill never waits
-// for its own loads.
-template <typename T>
-__device__ __forceinline__ PassData<T> load_pass(const TileArgs& ta, int32_t start, int32_t cnt, int lane) {
-    PassData<T> d;
-    const uint32_t at = static_cast<uint32_t>(lane * kTileLane < cnt ? start + lane * kTileLane : ta.sent);
-    d.k = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ta.keys + at));
-    d.v = load_quad<T>(static_cast<const T*>(ta.vals), at);
+// The lane's L entries of the pass whose first entry is `start` and which
+// holds `cnt` entries; a lane at or past cnt loads the sentinel block (keys
+// kTileSentinel, values 0), so what a lane holds needs no mask.  The loads
+// are issued on every path and nothing reads the loaded registers here, so
+// every refill issues the same loads, the compiler can count exactly how
+// many loads follow each queue slot, and the refill never waits for its own
+// loads.
+template <typename T, int L>
+__device__ __forceinline__ PassData<T, L> load_pass(const TileArgs& ta, int32_t start, int32_t cnt, int lane) {
+    static_assert(L % 4 == 0, "16-byte key loads");
+    PassData<T, L> d;
+    const uint32_t at = static_cast<uint32_t>(lane * L < cnt ? start + lane * L : ta.sent);
+#pragma unroll
+    for (int j = 0; j < L / 4; ++j) {
+        const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ta.keys + at) + j);
+        d.k[4 * j] = q.x;
+        d.k[4 * j + 1] = q.y;
+        d.k[4 * j + 2] = q.z;
+        d.k[4 * j + 3] = q.w;
+    }
+    const T* vals = static_cast<const T*>(ta.vals) + at;
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int j = 0; j < L / 4; ++j) {
+            const f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(vals) + j);
+            d.v[4 * j] = q.x;
+            d.v[4 * j + 1] = q.y;
+            d.v[4 * j + 2] = q.z;
+            d.v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < L / 2; ++j) {
+            const f64x2 q = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(vals) + j);
+            d.v[2 * j] = q.x;
+            d.v[2 * j + 1] = q.y;
+        }
+    }
     d.n = cnt;
     return d;
 }
@@ -163,37 +166,38 @@ __device__ __forceinline__ void seg_scan(T& S, int& F) {
 
 // What a pass computes from its own entries alone: rows, validity, the
 // lanes' running sums and the row exchanges with the neighbouring lanes.
-template <typename T>
+template <typename T, int L>
 struct PassLocal {
-    int r[4];
-    T s[4];
-    bool ok[4];
-    bool endin[3];   // a run ends inside the lane at u < 3 (entry u + 1 has another row)
-    bool single;     // the lane's entries are one run
+    int r[L];
+    T s[L];
+    bool ok[L];
+    bool endin[L - 1];   // a run ends inside the lane at u < L - 1 (entry u + 1 has another row)
+    bool single;         // the lane's entries are one run
     // rows are encoded +1 in the lane exchanges so that 0 (what a lane with
     // no source reads, or a lane whose entry is a sentinel) means "no row"
     int fr1, tl1, prev_tl1, next_fr1, lane0_fr1;
 };
 
-template <typename T>
-__device__ __forceinline__ PassLocal<T> pass_local(const u32x4& key, const Quad<T>& vals, const T (&xv)[4]) {
-    PassLocal<T> p;
-    const uint32_t kk[4] = {key.x, key.y, key.z, key.w};
+template <typename T, int L>
+__device__ __forceinline__ PassLocal<T, L> pass_local(const PassData<T, L>& d, const T (&xv)[L]) {
+    PassLocal<T, L> p;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        p.ok[u] = kk[u] < kTileSentinel;
-        p.r[u] = static_cast<int>(kk[u] >> kTileColBits);
+    for (int u = 0; u < L; ++u) {
+        p.ok[u] = d.k[u] < kTileSentinel;
+        p.r[u] = static_cast<int>(d.k[u] >> kTileColBits);
     }
-    p.s[0] = vals.v[0] * xv[0];
+    p.s[0] = d.v[0] * xv[0];
+    bool any = false;
 #pragma unroll
-    for (int u = 1; u < 4; ++u) {
-        const T q = vals.v[u] * xv[u];
+    for (int u = 1; u < L; ++u) {
+        const T q = d.v[u] * xv[u];
         p.endin[u - 1] = p.r[u] != p.r[u - 1];
+        any = any | p.endin[u - 1];
         p.s[u] = p.endin[u - 1] ? q : p.s[u - 1] + q;
     }
-    p.single = !(p.endin[0] | p.endin[1] | p.endin[2]);
+    p.single = !any;
     p.fr1 = p.ok[0] ? p.r[0] + 1 : 0;
-    p.tl1 = p.ok[3] ? p.r[3] + 1 : 0;
+    p.tl1 = p.ok[L - 1] ? p.r[L - 1] + 1 : 0;
     p.prev_tl1 = dpp<kDppWaveShr1, 0xF>(p.tl1);   // lane - 1's tail row (+1), 0 for lane 0
     p.next_fr1 = dpp<kDppWaveShl1, 0xF>(p.fr1);   // lane + 1's first row (+1), 0 for lane 63
     p.lane0_fr1 = __builtin_amdgcn_readfirstlane(p.fr1);
@@ -202,37 +206,37 @@ __device__ __forceinline__ PassLocal<T> pass_local(const u32x4& key, const Quad<
 
 // The segmented inclusive scan of a pass's tails, before the carry is
 // added.  carried: lane 0's first run continues the incoming carry row.
-template <typename T>
-__device__ __forceinline__ void pass_scan(const PassLocal<T>& p, bool carried, int lane, T& S, int& F, bool& cont) {
+template <typename T, int L>
+__device__ __forceinline__ void pass_scan(const PassLocal<T, L>& p, bool carried, int lane, T& S, int& F, bool& cont) {
     cont = (p.fr1 != 0) & (lane > 0 ? p.prev_tl1 == p.fr1 : carried);
-    S = p.ok[3] ? p.s[3] : T(0);
+    S = p.ok[L - 1] ? p.s[L - 1] : T(0);
     F = (p.single & cont) ? 0 : 1;
     seg_scan(S, F);
 }
 
 // The pass's writes once its scan has the carry in (cin: what the lane's
-// first run takes from earlier lanes): runs ending at u = 0..2, and the tail
-// when its row ends in this lane (lane 63's goes to the carry instead);
-// wm / wv: 4 entries each.
-template <typename T>
-__device__ __forceinline__ void pass_writes(const PassLocal<T>& p, T S, bool cont, bool carried, T carry_val, int lane,
-                                            bool* wm, T* wv) {
+// first run takes from earlier lanes): runs ending at u = 0 .. L - 2, and
+// the tail when its row ends in this lane (lane 63's goes to the carry
+// instead); wm / wv: L entries each.
+template <typename T, int L>
+__device__ __forceinline__ void pass_writes(const PassLocal<T, L>& p, T S, bool cont, bool carried, T carry_val,
+                                            int lane, bool* wm, T* wv) {
     T Sprev = dpp<kDppWaveShr1, 0xF>(S);
     if (lane == 0) Sprev = carried ? carry_val : T(0);
     const T cin = cont ? Sprev : T(0);
 #pragma unroll
-    for (int u = 0; u < 3; ++u) {
+    for (int u = 0; u < L - 1; ++u) {
         wm[u] = p.endin[u] & p.ok[u];
         wv[u] = p.s[u] + (p.r[u] == p.r[0] ? cin : T(0));
     }
-    wm[3] = (p.tl1 != 0) & (lane < kWave - 1) & (p.next_fr1 != p.tl1);
-    wv[3] = S;
+    wm[L - 1] = (p.tl1 != 0) & (lane < kWave - 1) & (p.next_fr1 != p.tl1);
+    wv[L - 1] = S;
 }
 
-// Add one pass (64 lanes x kTileLane entries, rows ascending, sentinels
-// last; xv = the gathered vector values of the lane's entries) into acc.
-// carry_row / carry_val: the last row of the previous pass of this
-// sub-range, not yet written (-1: none); wave-uniform.
+// Add one pass (64 lanes x L entries, rows ascending, sentinels last; xv =
+// the gathered vector values of the lane's entries) into acc.  carry_row /
+// carry_val: the last row of the previous pass of this sub-range, not yet
+// written (-1: none); wave-uniform.
 //
 // Each lane sums its runs of equal rows in storage order.  A run that ends
 // inside the lane (the next entry has another row, a sentinel included) is
@@ -240,7 +244,7 @@ __device__ __forceinline__ void pass_writes(const PassLocal<T>& p, T S, bool con
 // lanes, whose part arrives as cin.  The lane's tail run (the one holding
 // its last entry, if real) may continue into later lanes: the tails are
 // joined by a segmented scan over the wave and written by the lane where
-// the row ends (lane 63's goes to the carry).  Sentinels have row 4095 and
+// the row ends (lane 63's goes to the carry).  Sentinels have row 8191 and
 // value 0, so they never join a real run.
 //
 // The writes are one batch: all reads of acc first, then all writes.  The
@@ -248,28 +252,28 @@ __device__ __forceinline__ void pass_writes(const PassLocal<T>& p, T S, bool con
 // a run crossing lanes is written once), so the batch reads no value it
 // writes; a write a lane does not make goes to its own slot acc[dummy]
 // (dummy = rmax + lane), so the batch needs no branches.
-template <typename T>
-__device__ __forceinline__ void tile_pass(const u32x4& key, const Quad<T>& vals, const T (&xv)[4],
-                                          T* __restrict__ acc, int dummy, int lane, int& carry_row, T& carry_val) {
-    const PassLocal<T> p = pass_local<T>(key, vals, xv);
+template <typename T, int L>
+__device__ __forceinline__ void tile_pass(const PassData<T, L>& d, const T (&xv)[L], T* __restrict__ acc, int dummy,
+                                          int lane, int& carry_row, T& carry_val) {
+    const PassLocal<T, L> p = pass_local<T, L>(d, xv);
     const bool carried = (carry_row >= 0) & (p.lane0_fr1 == carry_row + 1);
     T S;
     int F;
     bool cont;
     pass_scan(p, carried, lane, S, F, cont);
     if (carried & !F) S = carry_val + S;
-    bool wm[4];
-    T wv[4];
+    bool wm[L];
+    T wv[L];
     pass_writes(p, S, cont, carried, carry_val, lane, wm, wv);
     // the carried row, when this pass does not continue it (wave-uniform)
     if ((carry_row >= 0) & !carried) {
         if (lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
     }
-    T old[4];
+    T old[L];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) old[u] = acc[wm[u] ? p.r[u] : 0];
+    for (int u = 0; u < L; ++u) old[u] = acc[wm[u] ? p.r[u] : 0];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[wm[u] ? p.r[u] : dummy] = old[u] + wv[u];
+    for (int u = 0; u < L; ++u) acc[wm[u] ? p.r[u] : dummy] = old[u] + wv[u];
     const int tl63 = __builtin_amdgcn_readlane(p.tl1, kWave - 1);
     const T s63 = readlane(S, kWave - 1);
     carry_row = tl63 - 1;
@@ -283,12 +287,12 @@ __device__ __forceinline__ void tile_pass(const u32x4& key, const Quad<T>& vals,
 // lane-63 sum) joins them afterwards.  The two batches are one: A's rows,
 // the flushed carries and B's rows are all distinct (rows ascend through
 // the sub-range, and A's tail row is written by B or flushed, never both).
-template <typename T>
-__device__ __forceinline__ void tile_pass2(const u32x4& keyA, const Quad<T>& valsA, const T (&xa)[4],
-                                           const u32x4& keyB, const Quad<T>& valsB, const T (&xb)[4],
-                                           T* __restrict__ acc, int dummy, int lane, int& carry_row, T& carry_val) {
-    const PassLocal<T> a = pass_local<T>(keyA, valsA, xa);
-    const PassLocal<T> b = pass_local<T>(keyB, valsB, xb);
+template <typename T, int L>
+__device__ __forceinline__ void tile_pass2(const PassData<T, L>& da, const T (&xa)[L], const PassData<T, L>& db,
+                                           const T (&xb)[L], T* __restrict__ acc, int dummy, int lane, int& carry_row,
+                                           T& carry_val) {
+    const PassLocal<T, L> a = pass_local<T, L>(da, xa);
+    const PassLocal<T, L> b = pass_local<T, L>(db, xb);
     const bool carriedA = (carry_row >= 0) & (a.lane0_fr1 == carry_row + 1);
     const int tlA63 = __builtin_amdgcn_readlane(a.tl1, kWave - 1);
     const int rowA = tlA63 - 1;   // A's outgoing carry row
@@ -302,22 +306,27 @@ __device__ __forceinline__ void tile_pass2(const u32x4& keyA, const Quad<T>& val
     const T sA63 = readlane(SA, kWave - 1);
     const T valA = tlA63 ? sA63 : T(0);   // A's outgoing carry value
     if (carriedB & !FB) SB = valA + SB;
-    bool wm[8];
-    T wv[8];
+    bool wm[2 * L];
+    T wv[2 * L];
     pass_writes(a, SA, contA, carriedA, carry_val, lane, wm, wv);
-    pass_writes(b, SB, contB, carriedB, valA, lane, wm + 4, wv + 4);
+    pass_writes(b, SB, contB, carriedB, valA, lane, wm + L, wv + L);
     if ((carry_row >= 0) & !carriedA) {
         if (lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
     }
     if ((rowA >= 0) & !carriedB) {
         if (lane == 0) acc[rowA] = acc[rowA] + valA;
     }
-    const int rw[8] = {a.r[0], a.r[1], a.r[2], a.r[3], b.r[0], b.r[1], b.r[2], b.r[3]};
-    T old[8];
+    int rw[2 * L];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) old[u] = acc[wm[u] ? rw[u] : 0];
+    for (int u = 0; u < L; ++u) {
+        rw[u] = a.r[u];
+        rw[L + u] = b.r[u];
+    }
+    T old[2 * L];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc[wm[u] ? rw[u] : dummy] = old[u] + wv[u];
+    for (int u = 0; u < 2 * L; ++u) old[u] = acc[wm[u] ? rw[u] : 0];
+#pragma unroll
+    for (int u = 0; u < 2 * L; ++u) acc[wm[u] ? rw[u] : dummy] = old[u] + wv[u];
     const int tl63 = __builtin_amdgcn_readlane(b.tl1, kWave - 1);
     const T s63 = readlane(SB, kWave - 1);
     carry_row = tl63 - 1;
@@ -361,13 +370,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
 // with the halves' sums added in half order.  x: the gathered vector
 // (device order, 16-byte aligned).  All threads call it.
 //
-// Waves 0 .. kTileWaves-1 add the entries.  Each walks its stream of
+// Waves kTileStagers .. 15 add the entries.  Each walks its stream of
 // passes (its sub-ranges of the units, in order) with the loads of the
 // next kRing passes in flight: a queue of kRing register slots, each
 // consumed in place and refilled with the pass kRing further on, so passes
 // are used in the order their loads were issued (a wave's vector memory
 // counter completes in that order) and no slot is copied while its load is
-// pending.  The last kTileStagers waves (one per SIMD) copy vector slices
+// pending.  The first kTileStagers waves (one per SIMD, the first to start:
+// a workgroup's waves are launched one after another) copy vector slices
 // into LDS with direct buffer-to-LDS loads (16 bytes per lane, no
 // registers), tile_buffers() - 1 staged units ahead of the entry waves.
 // Staging runs in waves of its own so its loads never sit in an entry
@@ -383,6 +393,7 @@ template <typename T, class Op>
 __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restrict__ x, Op& op) {
     constexpr int NB = tile_buffers(sizeof(T));
     constexpr int SB = tile_slice_bytes(sizeof(T));
+    constexpr int L = tile_lane(sizeof(T));
     extern __shared__ __align__(16) unsigned char tile_lds[];
     T* xs = reinterpret_cast<T*>(tile_lds);
     T* acc = xs + (size_t)NB * ta.W;
@@ -397,7 +408,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     const cint32* un = const_table(ta.units);
     for (int l = tid; l < ta.rmax; l += kTileThreads) acc[l] = T(0);
-    if (wave >= kTileWaves) {
+    if (wave < kTileStagers) {
         // ---- staging waves.  With s_0, s_1, ... the staged units of the
         // range: s_0 .. s_{NB-1} are issued before the first barrier (every
         // buffer is free), and s_{j+NB-1} (j >= 1) right after the barrier
@@ -407,7 +418,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // younger slices' loads in flight.  A slice is kStageChunks chunks
         // of kWave x 16 bytes per staging wave, chunk k of wave w at
         // (k kTileStagers + w) kWave x 16 bytes into the slice.
-        const int sw = wave - kTileWaves;
+        const int sw = wave;
         constexpr int NV = 16 / static_cast<int>(sizeof(T));
         constexpr int kChunkBytes = kWave * 16;
         constexpr int kStageChunks = SB / (kChunkBytes * kTileStagers);
@@ -429,7 +440,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         auto next_staged = [&](int v) { return v < u1 ? un[4 * v + 3] : u1; };
         auto buffer = [&](int v) { return xs + (size_t)un[4 * v + 2] * ta.W; };
         int issued = 0;   // slices of the range issued so far
-        int upcoming = u0 < u1 ? (un[4 * u0 + 1] ? u0 : un[4 * u0 + 3]) : u1;   // the next to issue
+        int upcoming = ta.hs[h];   // the next to issue (the half's first staged unit: no table load)
         auto issue_next = [&]() {
             const int base = un[4 * upcoming] * SB;
             unsigned char* buf = reinterpret_cast<unsigned char*>(buffer(upcoming));
@@ -485,7 +496,8 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // list.  Lane l holds the descriptor of pass dbase + l (reloaded
         // every kWave passes); a refill reads its pass's descriptor with
         // readlane, so it waits for no memory.
-        const cint32* pi = const_table(ta.pidx) + (int64_t)(g * kTileWaves + wave) * (ta.U + 1);
+        const int ew = wave - kTileStagers;   // entry wave
+        const cint32* pi = const_table(ta.pidx) + (int64_t)(g * kTileWaves + ew) * (ta.U + 1);
         int i = pi[u0];
         const int iend = pi[u1];
         int ld = i, dbase = i;
@@ -494,7 +506,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             return at < iend ? *reinterpret_cast<const int2*>(ta.pdesc + 2 * (int64_t)at) : make_int2(0, 0);
         };
         int2 dl = load_desc();
-        auto fetch = [&]() -> PassData<T> {
+        auto fetch = [&]() -> PassData<T, L> {
             if (ld - dbase == kWave) {
                 dbase = ld;
                 dl = load_desc();
@@ -506,9 +518,9 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             const int32_t s0 = __builtin_amdgcn_readlane(dl.x, j);
             const int32_t c0 = __builtin_amdgcn_readlane(dl.y, j);
             ++ld;
-            return load_pass<T>(ta, s0, c0, lane);
+            return load_pass<T, L>(ta, s0, c0, lane);
         };
-        PassData<T> q0 = fetch(), q1 = fetch(), q2 = fetch(), q3 = fetch();
+        PassData<T, L> q0 = fetch(), q1 = fetch(), q2 = fetch(), q3 = fetch();
         lds_barrier();
         // consumer state: unit cu, its end in the pass list
         int cu = u0, nxt = i, staged = 0, carry_row = -1;
@@ -528,47 +540,50 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             if (carry_row >= 0 && lane == 0) acc[carry_row] = acc[carry_row] + carry_val;
             lds_barrier();
         };
-        auto gather = [&](const PassData<T>& slot, T (&xv)[4]) {
-            const uint32_t q[4] = {slot.k.x & kTileColMask, slot.k.y & kTileColMask, slot.k.z & kTileColMask,
-                                   slot.k.w & kTileColMask};
+        auto gather = [&](const PassData<T, L>& slot, T (&xv)[L]) {
+            uint32_t q[L];
+#pragma unroll
+            for (int k = 0; k < L; ++k) q[k] = slot.k[k] & kTileColMask;
             if (staged) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) xv[k] = xl[q[k]];
+                for (int k = 0; k < L; ++k) xv[k] = xl[q[k]];
             } else {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) xv[k] = lane_identity(xg[q[k]]);
+                for (int k = 0; k < L; ++k) xv[k] = lane_identity(xg[q[k]]);
             }
         };
         // the gathers of two passes, all issued before any is waited for
-        auto gather2 = [&](const PassData<T>& sa, const PassData<T>& sb, T (&xa)[4], T (&xb)[4]) {
-            const uint32_t qa[4] = {sa.k.x & kTileColMask, sa.k.y & kTileColMask, sa.k.z & kTileColMask,
-                                    sa.k.w & kTileColMask};
-            const uint32_t qb[4] = {sb.k.x & kTileColMask, sb.k.y & kTileColMask, sb.k.z & kTileColMask,
-                                    sb.k.w & kTileColMask};
+        auto gather2 = [&](const PassData<T, L>& sa, const PassData<T, L>& sb, T (&xa)[L], T (&xb)[L]) {
+            uint32_t qa[L], qb[L];
+#pragma unroll
+            for (int k = 0; k < L; ++k) {
+                qa[k] = sa.k[k] & kTileColMask;
+                qb[k] = sb.k[k] & kTileColMask;
+            }
             if (staged) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < L; ++k) {
                     xa[k] = xl[qa[k]];
                     xb[k] = xl[qb[k]];
                 }
             } else {
-                T ga[4], gb[4];
+                T ga[L], gb[L];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < L; ++k) {
                     ga[k] = xg[qa[k]];
                     gb[k] = xg[qb[k]];
                 }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < L; ++k) {
                     xa[k] = lane_identity(ga[k]);
                     xb[k] = lane_identity(gb[k]);
                 }
             }
         };
-        auto consume_pass = [&](const PassData<T>& slot) {
-            T xv[4];
+        auto consume_pass = [&](const PassData<T, L>& slot) {
+            T xv[L];
             gather(slot, xv);
-            tile_pass<T>(slot.k, slot.v, xv, acc, dummy, lane, carry_row, carry_val);
+            tile_pass<T, L>(slot, xv, acc, dummy, lane, carry_row, carry_val);
         };
         // close the units this wave has finished (and those it has no
         // passes in) before pass i
@@ -587,16 +602,16 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // change between them.  Then both slots are refilled (loads are
         // issued on every path, so each slot is always followed by the same
         // number of loads, and the loop has its only exit at the top).
-        auto step2 = [&](PassData<T>& sa, PassData<T>& sb) {
+        auto step2 = [&](PassData<T, L>& sa, PassData<T, L>& sb) {
             close_done();
             if (cu < u1) {
                 if (i + 1 < nxt && sb.n == 0) {
                     consume_pass(sa);
                     i += 2;
                 } else if (i + 1 < nxt) {
-                    T xa[4], xb[4];
+                    T xa[L], xb[L];
                     gather2(sa, sb, xa, xb);
-                    tile_pass2<T>(sa.k, sa.v, xa, sb.k, sb.v, xb, acc, dummy, lane, carry_row, carry_val);
+                    tile_pass2<T, L>(sa, xa, sb, xb, acc, dummy, lane, carry_row, carry_val);
                     i += 2;
                 } else {
                     consume_pass(sa);
@@ -613,7 +628,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         };
         // one queue slot, pass i (the pass list of unpaired kernels has no
         // empty passes)
-        auto step = [&](PassData<T>& slot) {
+        auto step = [&](PassData<T, L>& slot) {
             close_done();
             if (cu < u1) {
                 consume_pass(slot);

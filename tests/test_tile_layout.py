@@ -5,10 +5,10 @@ decodable from (group, unit, key); units that cover the slices in order
 (a staged unit is one slice with at least 800 entries per group, a gather
 unit a run of sparser slices whose columns fit the key's 19 column bits);
 staged units cycling through the LDS buffers; each (group, unit) tile split
-into 12 row-aligned sub-ranges (one per entry wave), each a multiple of 4
-entries with padding only at its end; rows dealt to groups in 64-row
+into 12 row-aligned sub-ranges (one per entry wave), each a multiple of the
+entries per lane (4) with padding only at its end; rows dealt to groups in 64-row
 chunks, two workgroups per group; the pass list the
-kernels read (every sub-range cut into passes of at most 256 entries, padded to an even count for FP32,
+kernels read (every sub-range cut into passes of at most 64 lanes' entries, padded to an even count for FP32,
 listed per group and wave, unit by unit); and the product the layout
 encodes equal to the CSR product.  Cases cover empty rows, a row
 denser than one wave pass, several slices (ragged last slice), and more
@@ -123,7 +123,8 @@ def test_layout_stores_the_csr(name, tsize):
     if name == "staged_and_gather" and tsize == 4:
         assert list(staged) == [1, 1, 0] and list(first) == [0, 1, 2]
     assert len(ws) == nb * U * WAVES + 1 and ws[-1] == len(keys)
-    assert np.all(np.diff(ws) >= 0) and np.all(ws % 4 == 0)
+    lane = 4                                                             # entries per lane per pass
+    assert np.all(np.diff(ws) >= 0) and np.all(ws % lane == 0)
     row_of_src = np.repeat(np.arange(rows), np.diff(ptr))
     pos_of_src = np.empty(nnz, np.int64)
     pos_of_src[src] = np.arange(nnz)                 # src value -> CSR position
@@ -136,7 +137,7 @@ def test_layout_stores_the_csr(name, tsize):
             k = keys[e0:e1]
             real = k != SENT
             nreal = int(real.sum())
-            assert np.all(real[:nreal]) and e1 - e0 - nreal < 4      # padding only at the end
+            assert np.all(real[:nreal]) and e1 - e0 - nreal < lane   # padding only at the end
             k, sv = k[:nreal].astype(np.int64), so[e0:e0 + nreal]
             l, q = k >> COLBITS, k & (2 ** COLBITS - 1)
             assert np.all(np.diff(l) >= 0) and np.all(l < rmax)
@@ -155,7 +156,7 @@ def test_layout_stores_the_csr(name, tsize):
                 assert not (tile_rows[w] & tile_rows[v])
     assert seen.all()
     # the pass list: per (group, wave), unit by unit, each sub-range cut into
-    # passes of 256 entries (the last one short), plus, for the FP32 tiles,
+    # passes of 64 lanes' entries (the last one short), plus, for the FP32 tiles,
     # an empty pass when that count is odd (their kernels add passes in
     # pairs, and a pass with a successor must fill all 64 lanes)
     pidx, pdesc = passes(ptr, idx, src, rows, cols, cus, tsize, nb, U, WAVES)
@@ -167,7 +168,7 @@ def test_layout_stores_the_csr(name, tsize):
                 assert pidx[base + u] == len(want)
                 t = b * U + u
                 e0, e1 = ws[t * WAVES + w], ws[t * WAVES + w + 1]
-                cut = [(e, min(256, e1 - e)) for e in range(e0, e1, 256)]
+                cut = [(e, min(64 * lane, e1 - e)) for e in range(e0, e1, 64 * lane)]
                 want += cut + ([(e1, 0)] if len(cut) % 2 and tsize == 4 else [])
             assert pidx[base + U] == len(want)
             if tsize == 4:
