@@ -227,6 +227,8 @@ __global__ void k_fill_value(int64_t n, T v, T* __restrict__ d) {
 }
 
 int ensure_partials(sov_hip_csr* csr) {
+    if (csr->pairs == nullptr)
+        SOV_HIP_TRY(hipMalloc(&csr->pairs, sizeof(double) * 2 * ((size_t)csr->m + (size_t)csr->n + 1)));
     const int64_t need = (int64_t)csr->a.max_blocks() + (int64_t)csr->t.max_blocks() + 64;
     if (need <= csr->partial_blocks) return SOV_OK;
     if (csr->partials) (void)hipFree(csr->partials);
@@ -886,6 +888,7 @@ void sov_hip_csr_destroy(sov_hip_csr* h) {
     (void)hipFree(h->t_src);
     (void)hipFree(h->t_val64);
     (void)hipFree(h->partials);
+    (void)hipFree(h->pairs);
     delete h;
 }
 

@@ -280,6 +280,7 @@ struct sov_hip_csr {
     bool owns_a_val64 = false;
     bool f32 = false;           // parts carry FP32 value images (sov_hip_csr_make_f32)
     double* partials = nullptr; // owned: kCheckSlots * (largest part blocks of both orientations + 64) FP64
+    void* pairs = nullptr;      // owned: 2 (n + m) FP64, a check's (current, sum) pairs of x then y
     void* vx = nullptr;         // owned: max(m, n) FP64, device-order staging of natural-order SpMV operands
     void* vy = nullptr;
     int64_t partial_blocks = 0;
@@ -360,10 +361,11 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Block-wide reduction of NS sums and NM maxima; thread 0 writes them to
-// out[0..NS) then out[NS..NS+NM).  All threads of the block must call it.
+// out[0..NS) then out[NS..NS+NM), element i at out[i * stride].  All
+// threads of the block must call it.
 template <int NS, int NM>
 __device__ __forceinline__ void block_reduce_write(const double* s, const double* mx,
-                                                   double* out) {
+                                                   double* out, int64_t stride = 1) {
     constexpr int W = kBlock / kWave;
     __shared__ double lds[(NS + NM) * W];
     const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
@@ -383,13 +385,13 @@ __device__ __forceinline__ void block_reduce_write(const double* s, const double
         for (int i = 0; i < NS; ++i) {
             double acc = 0.0;
             for (int w = 0; w < W; ++w) acc += lds[i * W + w];
-            out[i] = acc;
+            out[i * stride] = acc;
         }
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
             double acc = 0.0;
             for (int w = 0; w < W; ++w) acc = nmax(acc, lds[(NS + i) * W + w]);
-            out[NS + i] = acc;
+            out[(NS + i) * stride] = acc;
         }
     }
 }
@@ -485,12 +487,16 @@ __device__ __forceinline__ Acc row_dot(const int32_t* __restrict__ ptr,
 }
 
 // Two dots of the same row (current and second vector scaled by s2), FP64.
-template <int G, typename V, typename X>
+// P: x1 holds both vectors interleaved (x1[2 j], x1[2 j + 1] = the first's
+// and the second's element j) and x2 is unused, so each entry makes one
+// gather of a pair instead of two; the sums are the same.
+template <int G, bool P = false, typename V, typename X>
 __device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
                                          const int32_t* __restrict__ idx,
                                          const V* __restrict__ val, const X* __restrict__ x1,
                                          const X* __restrict__ x2, double s2, int r, int lane,
                                          double& d1, double& d2) {
+    typedef X X2 __attribute__((ext_vector_type(2)));
     double a1 = 0.0, a2 = 0.0;
     if (r >= 0) {
         const int b = ptr[r], e = ptr[r + 1];
@@ -513,8 +519,14 @@ __device__ __forceinline__ void row_dot2(const int32_t* __restrict__ ptr,
                 v1[u] = X(0);
                 v2[u] = X(0);
                 if (j[u] >= 0) {
-                    v1[u] = x1[j[u]];
-                    v2[u] = x2[j[u]];
+                    if constexpr (P) {
+                        const X2 q = reinterpret_cast<const X2*>(x1)[j[u]];
+                        v1[u] = q.x;
+                        v2[u] = q.y;
+                    } else {
+                        v1[u] = x1[j[u]];
+                        v2[u] = x2[j[u]];
+                    }
                 }
             }
 #pragma unroll

@@ -297,6 +297,9 @@ struct CheckRowOp {
     const double* b_lo = nullptr;
     const double* b_hi = nullptr;
     double idelta = 1.0;
+    // (x, x_sum) interleaved (k_pack_pairs), or null: gather them separately
+    const T* pair = nullptr;
+    int64_t pstride = 1;   // slot stride of the block partials (slot-major)
     __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     // sums: pv2_c, pv2_a, bn2, yres2_c, yres2_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
@@ -305,7 +308,8 @@ struct CheckRowOp {
         const int i = row_of<G>(v, lane);
         const double inv_r = row_inv<B>(bv, inv, i);
         double ac, aa;
-        row_dot2<G>(ptr, idx, val, x, xsum, inv_r, i, lane, ac, aa);
+        if (pair != nullptr) row_dot2<G, true>(ptr, idx, val, pair, pair, inv_r, i, lane, ac, aa);
+        else row_dot2<G>(ptr, idx, val, x, xsum, inv_r, i, lane, ac, aa);
         if (i >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)i, ac, aa)) return;
             double yc = double(y[i]), ya = double(ysum[i]) * inv_r;
@@ -384,6 +388,9 @@ struct CheckColOp {
     const double* b_lo = nullptr;
     const double* b_hi = nullptr;
     double idelta = 1.0;
+    // (y, y_sum) interleaved (k_pack_pairs), or null: gather them separately
+    const T* pair = nullptr;
+    int64_t pstride = 1;   // slot stride of the block partials (slot-major)
     __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
     // sums: dres2_c, dres2_a, cn2, pobj_c, pobj_a, dobj_c, dobj_a; maxes: kkt_c, kkt_a, orig_c, orig_a
     template <int G>
@@ -392,7 +399,8 @@ struct CheckColOp {
         const int j = row_of<G>(v, lane);
         const double inv_r = row_inv<B>(bv, inv, j);
         double tc, ta;
-        row_dot2<G>(ptr, idx, val, y, ysum, inv_r, j, lane, tc, ta);
+        if (pair != nullptr) row_dot2<G, true>(ptr, idx, val, pair, pair, inv_r, j, lane, tc, ta);
+        else row_dot2<G>(ptr, idx, val, y, ysum, inv_r, j, lane, tc, ta);
         if (j >= 0 && lane == 0) {
             if (!combine_parts(mode, acc + 2 * (int64_t)j, tc, ta)) return;
             double t[11];
@@ -457,20 +465,32 @@ __global__ void __launch_bounds__(kBlock) k_check(Op op) {
     dispatch_G(v, r);
     // mode is uniform over the launch: only a launch that completes rows reduces
     if ((op.mode == kOnly || op.mode == kLast) && op.rowout == nullptr)
-        block_reduce_write<7, 4>(sm, mx, op.partial + (int64_t)blockIdx.x * kCheckSlots);
+        block_reduce_write<7, 4>(sm, mx, op.partial + blockIdx.x, op.pstride);
 }
 
-// Block b reduces slot (b % 11) of region (b / 11): slots 0..6 sums, 7..10 maxima.
+// Block b reduces slot (b % 11) of region (b / 11): slots 0..6 sums, 7..10
+// maxima.  A region holds slot k of block partial j at k * (blocks + 1) + j
+// (slot-major, so a slot's partials are contiguous).
 __global__ void k_check_final(const double* __restrict__ rowpart, int64_t rblocks,
                               const double* __restrict__ colpart, int64_t cblocks, double* __restrict__ out) {
     const int region = blockIdx.x / 11, slot = blockIdx.x % 11;
-    const double* p = region == 0 ? rowpart : colpart;
     const int64_t nb = region == 0 ? rblocks : cblocks;
+    const double* p = (region == 0 ? rowpart : colpart) + slot * (nb + 1);
     const bool is_max = slot >= 7;
     double acc = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
-        const double v = p[b * kCheckSlots + slot];
-        acc = is_max ? nmax(acc, v) : acc + v;
+    // thread t folds partials t, t + blockDim, t + 2 blockDim, ... in that
+    // order, with the loads of kAhead of them in flight at a time
+    constexpr int kAhead = 8;
+    for (int64_t b0 = threadIdx.x; b0 < nb; b0 += (int64_t)kAhead * blockDim.x) {
+        double v[kAhead];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const int64_t b = b0 + (int64_t)u * blockDim.x;
+            v[u] = b < nb ? p[b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u)
+            if (b0 + (int64_t)u * blockDim.x < nb) acc = is_max ? nmax(acc, v[u]) : acc + v[u];
     }
     double sm[1] = {acc}, mx[1] = {acc};
     if (is_max) block_reduce_write<0, 1>(sm, mx, out + blockIdx.x);
@@ -679,11 +699,26 @@ int static_scratch_address(int device, double** addr) {
     return SOV_OK;
 }
 
+// out[2 j] = a[j], out[2 j + 1] = b[j]: a vector and its running sum as
+// pairs, so a check's gather of both is one load.
+template <typename T>
+__global__ void k_pack_pairs(int32_t n, const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        T2 q;
+        q.x = a[j];
+        q.y = b[j];
+        reinterpret_cast<T2*>(out)[j] = q;
+    }
+}
+
 // partials: the block partials of both orientations, (rb + cb + 2) *
-// kCheckSlots doubles, checked by the caller; red: 22 doubles.
+// kCheckSlots doubles, checked by the caller; red: 22 doubles; pairs:
+// 2 (n + m) elements of T for the interleaved vectors (null: the check
+// gathers each vector separately, with the same sums).
 template <typename T>
 static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a, sov_hip_check_out* out,
-                      double* partials, double* red) {
+                      double* partials, double* red, void* pairs) {
     const sov_hip_pdhg_state* st = a->st;
     const double inv = 1.0 / static_cast<double>(a->since_restart > 0 ? a->since_restart : 1);
     const Orient& OA = h->a;
@@ -696,6 +731,18 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
             if (p.nnz > 0 && vals_of<T>(p) == nullptr) return fail(SOV_INVALID_ARGUMENT, "check: value image missing");
     double* rowpart = partials;
     double* colpart = partials + (int64_t)(rb + 1) * kCheckSlots;
+    T* xpair = nullptr;
+    T* ypair = nullptr;
+    if (pairs != nullptr) {
+        xpair = static_cast<T*>(pairs);
+        ypair = xpair + 2 * (int64_t)h->n;
+        if (h->n > 0)
+            k_pack_pairs<T><<<launch_blocks(h->n), kBlock, 0, ctx->stream>>>(
+                h->n, static_cast<const T*>(st->x), static_cast<const T*>(st->x_sum), xpair);
+        if (h->m > 0)
+            k_pack_pairs<T><<<launch_blocks(h->m), kBlock, 0, ctx->stream>>>(
+                h->m, static_cast<const T*>(st->y), static_cast<const T*>(st->y_sum), ypair);
+    }
     for (int k = 0; k < npa; ++k) {
         const Part& p = OA.part(k);
         const Sched s = p.sched();
@@ -709,6 +756,8 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
         op.lo_o = a->orig_row_lo; op.hi_o = a->orig_row_hi; op.rs = a->row_scale;
         op.act_avg = a->act_avg; op.partial = rowpart; op.acc = OA.chk_acc;
         op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.mode = seg_mode(k, npa); op.s = s;
+        op.pair = xpair;
+        op.pstride = rb + 1;
         if (a->base != nullptr) {
             op.b_act = a->base_act; op.b_y = a->base_y; op.idelta = a->inv_delta;
             op.b_lo = static_cast<const double*>(a->base->row_lo); op.b_hi = static_cast<const double*>(a->base->row_hi);
@@ -730,6 +779,8 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
         op.partial = colpart; op.acc = OT.chk_acc;
         op.inv = inv; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign;
         op.mode = seg_mode(k, npt); op.s = s;
+        op.pair = ypair;
+        op.pstride = cb + 1;
         if (a->base != nullptr) {
             op.b_aty = a->base_aty; op.b_x = a->base_x; op.idelta = a->inv_delta;
             op.b_c = static_cast<const double*>(a->base->c);
@@ -1578,8 +1629,8 @@ int sov_hip_pdhg_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_che
     if (check_blocks(h) > h->partial_blocks) return fail(SOV_INVALID_ARGUMENT, "check: partials too small");
     rc = ctx->ensure_red(64);
     if (rc != SOV_OK) return rc;
-    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out, h->partials, ctx->d_red)
-                                       : check_impl<float>(ctx, h, a, out, h->partials, ctx->d_red);
+    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out, h->partials, ctx->d_red, h->pairs)
+                                       : check_impl<float>(ctx, h, a, out, h->partials, ctx->d_red, h->pairs);
 }
 
 int sov_hip_pdhg_check_static(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_check_args* a,
@@ -1604,8 +1655,8 @@ int sov_hip_pdhg_check_static(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_
     DeviceGuard3 g(ctx->device);
     double* partials = ctx->check_scratch;
     double* red = ctx->check_scratch + kStaticCheckBlocks * kCheckSlots;
-    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out, partials, red)
-                                       : check_impl<float>(ctx, h, a, out, partials, red);
+    return a->st->prec == SOV_PREC_F64 ? check_impl<double>(ctx, h, a, out, partials, red, nullptr)
+                                       : check_impl<float>(ctx, h, a, out, partials, red, nullptr);
 }
 
 int sov_hip_pdhg_restart(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, int64_t since, const double* act,
