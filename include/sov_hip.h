@@ -196,7 +196,7 @@ int sov_hip_csr_tiles_usable(const sov_hip_ctx* ctx, const sov_hip_csr* csr, int
  * [0, cols), and a source position per entry), as sov_hip_csr_set_tiles
  * builds it for a device with `cus` compute units and the products of
  * element size tsize (4: FP32, slices of 16384 columns; 8: FP64, slices of
- * 4096).  shape = {groups, slices, slice width, local rows per group,
+ * 6144; either fewer when a group's row sums leave less LDS).  shape = {groups, slices, slice width, local rows per group,
  * units, sub-ranges per tile, workgroups per group, LDS slice buffers};
  * *entries = stored entries including padding.  keys / src_out (entries each), wstart (groups * units *
  * sub-ranges + 1) and units (4 * units + 1: first slice, staged, buffer,

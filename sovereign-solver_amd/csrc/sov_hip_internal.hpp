@@ -100,11 +100,11 @@ struct Natural {
 // gathered vector, and each of its units holds the entries of twice as many
 // rows as one workgroup per group would give it.
 //
-// The gathered dimension is cut into slices of W = tile_slice_bytes(element
-// size) / (element size) columns, and consecutive slices form units.  A
+// The gathered dimension is cut into slices of W = tile_slice_bytes() /
+// (element size) columns, and consecutive slices form units.  A
 // staged unit is one slice whose entries are dense enough per group (on
 // average at least kTileStageMin per group) to repay copying the vector
-// slice into one of the tile_buffers() LDS buffers; its entries then gather
+// slice into one of the kTileBuffers LDS buffers; its entries then gather
 // from LDS.  A gather unit is a run of up to tile_gather_slices(W) sparser
 // slices whose entries gather straight from the vector in memory (staging
 // them would move more bytes than it saves).
@@ -118,7 +118,7 @@ struct Natural {
 // wstart[(b U + u) kTileWaves + w] is the first entry of wave w of tile
 // (b, u); the array ends with the total count E.  units[4 u .. 4 u + 3] =
 // {first slice, staged, LDS buffer of a staged unit (its ordinal among the
-// staged units modulo tile_buffers), next staged unit (U if none)},
+// staged units modulo kTileBuffers), next staged unit (U if none)},
 // units[4 U] = S.
 //
 // What the kernels read instead of wstart: the pass list.  A pass is up to
@@ -144,23 +144,30 @@ constexpr int kTileHalves = 2;                        // workgroups (CUs) per ro
 constexpr int kTileLdsBytes = 163840;                 // LDS of one gfx950 workgroup
 constexpr int kTileColBits = 19;                      // key bits of the column within a unit
 constexpr uint32_t kTileColMask = (1u << kTileColBits) - 1u;
-constexpr uint32_t kTileSentinel = 0xFFF80000u;       // row 8191, column 0: never a real key (rows < tile_row_cap)
-// Slice bytes and LDS buffers per element size.  FP32: two 64 KiB buffers
-// (one is consumed while the next slice lands in the other).  FP64: three
-// 32 KiB buffers, because its row sums take twice the LDS; the third
-// buffer gives each slice's loads two units' time to land instead of one.
-__host__ __device__ constexpr int tile_slice_bytes(int tsize) { return tsize == 4 ? 65536 : 32768; }
-__host__ __device__ constexpr int tile_buffers(int tsize) { return tsize == 4 ? 2 : 3; }
-// local rows per group: what the LDS holds beside the buffers (and one
-// spare slot per lane), 8128 in both precisions (13 key bits)
-__host__ __device__ constexpr int tile_row_cap(int tsize) {
-    return ((kTileLdsBytes - tile_buffers(tsize) * tile_slice_bytes(tsize)) / tsize - kWave) / 64 * 64;
-}
+constexpr uint32_t kTileSentinel = 0xFFF80000u;       // row 8191, column 0: never a real key (rows < kTileRowCap)
+// Two LDS slice buffers: one is consumed while the next slice lands in the
+// other.  A layout's slice is 64 KiB for FP32 and 48 KiB for FP64, or the
+// largest smaller one (48, 32 KiB) whose two buffers fit beside the row
+// sums (tile_slice_bytes).  Measured on the 1M-column Zipf LP (7,872 /
+// 3,968 rows per group in the primal / dual step): FP64 with three 32 KiB
+// buffers 108.7 / 93.4 us, two 48 KiB 92.5-94.4 / 84.2-85.3 us, two 64 KiB
+// (dual only) 88.5 us; FP32 with two 64 KiB 58.1 / 60.1 us, two 48 KiB
+// 62.4 / 65.4 us.
+constexpr int kTileBuffers = 2;
+// local rows per group: 13 key bits, and the LDS of two 32 KiB buffers
+constexpr int kTileRowCap = 8128;
+static_assert(kTileRowCap < 8191, "sentinel row");
+__host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize);
 // slices per gather unit: its columns must fit the key's column bits
 __host__ __device__ constexpr int tile_gather_slices(int W) {
     return ((1 << kTileColBits) / W - 1) < 63 ? ((1 << kTileColBits) / W - 1) : 63;
 }
-static_assert(tile_row_cap(4) < 8191 && tile_row_cap(8) < 8191, "sentinel row");
+// The slice bytes of a layout with rmax local rows per group (0: none fits).
+__host__ __device__ constexpr int tile_slice_bytes(int rmax, int tsize) {
+    for (int sb = tsize == 4 ? 65536 : 49152; sb >= 32768; sb -= 16384)
+        if ((size_t)kTileBuffers * sb + (size_t)(rmax + kWave) * tsize <= (size_t)kTileLdsBytes) return sb;
+    return 0;
+}
 // Entries per group that make a slice staged.  Measured on the 1M-column
 // Zipf LP (FP32 dual step, where slices of 400-1100 entries per group lie
 // between the short columns and the dense ones): 384 stages them, 69.6 us;

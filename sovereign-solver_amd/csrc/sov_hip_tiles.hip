@@ -124,8 +124,8 @@ bool tiles_usable(const sov_hip_ctx* ctx, const Tiles& t, bool f64, const void* 
 }
 
 hipError_t tile_lds_attr(const sov_hip_ctx* ctx, const void* kernel, int tsize) {
-    const int64_t want = std::min<int64_t>(
-        ctx->lds_max, (int64_t)tile_lds_bytes(tile_slice_bytes(tsize) / tsize, tile_row_cap(tsize), tsize));
+    (void)tsize;   // every layout fits kTileLdsBytes
+    const int64_t want = std::min<int64_t>(ctx->lds_max, (int64_t)kTileLdsBytes);
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(want));
 }
 
@@ -281,13 +281,13 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
                        std::vector<int32_t>& src, std::vector<int32_t>& wstart) {
     if (rows <= 0) return fail(SOV_INVALID_ARGUMENT, "tiles: no rows");
     const int64_t nchunks = (rows + 63) / 64;
-    const int64_t cap = tile_row_cap(tsize) / 64;
+    const int64_t cap = kTileRowCap / 64;
     t.Q = kTileHalves;
     int64_t nb = std::min<int64_t>(std::max(cus / t.Q, 1), nchunks);
     nb = std::max<int64_t>(nb, (nchunks + cap - 1) / cap);
     t.nb = static_cast<int32_t>(nb);
     t.rmax = static_cast<int32_t>((nchunks + nb - 1) / nb * 64);
-    t.W = tile_slice_bytes(tsize) / tsize;
+    t.W = tile_slice_bytes(t.rmax, tsize) / tsize;
     t.cols = cols;
     t.S = static_cast<int32_t>((static_cast<int64_t>(cols) + t.W - 1) / t.W);
     // units: a slice with at least kTileStageMin entries per group on
@@ -313,7 +313,7 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     int ordinal = 0, next = t.U;
     for (int u = 0; u < t.U; ++u)
         if (t.h_units[4 * static_cast<size_t>(u) + 1])
-            t.h_units[4 * static_cast<size_t>(u) + 2] = (ordinal++) % tile_buffers(tsize);
+            t.h_units[4 * static_cast<size_t>(u) + 2] = (ordinal++) % kTileBuffers;
     for (int u = t.U - 1; u >= 0; --u) {
         t.h_units[4 * static_cast<size_t>(u) + 3] = next;
         if (t.h_units[4 * static_cast<size_t>(u) + 1]) next = u;
@@ -533,7 +533,7 @@ int sov_hip_tile_layout(int32_t rows, int32_t cols, int32_t nnz, const int32_t* 
     shape[4] = t.U;
     shape[5] = kTileWaves;
     shape[6] = t.Q;
-    shape[7] = tile_buffers(tsize);
+    shape[7] = kTileBuffers;
     *entries = t.E;
     if (keys) std::copy(k.begin(), k.begin() + t.E, keys);
     if (src_out) std::copy(so.begin(), so.begin() + t.E, src_out);

@@ -51,11 +51,11 @@ struct TileArgs {
 // Device row of local row l of group b.
 __host__ __device__ __forceinline__ int tile_row(int b, int nb, int l) { return ((l >> 6) * nb + b) * 64 + (l & 63); }
 
-// LDS of a tiled launch: tile_buffers() slice buffers (one is consumed
-// while the next slices land in the others), the row sums and one slot per
-// lane that takes the writes a lane makes no real write with.
+// LDS of a tiled launch: kTileBuffers slice buffers (one is consumed while
+// the next slice lands in the other), the row sums and one slot per lane
+// that takes the writes a lane makes no real write with.
 __host__ __device__ constexpr size_t tile_lds_bytes(int W, int rmax, int tsize) {
-    return (size_t)tile_buffers(tsize) * W * tsize + (size_t)(rmax + kWave) * tsize;
+    return (size_t)kTileBuffers * W * tsize + (size_t)(rmax + kWave) * tsize;
 }
 
 // One lane's L entries of a pass as loaded (sentinels where it has none):
@@ -379,7 +379,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
 // pending.  The first kTileStagers waves (one per SIMD, the first to start:
 // a workgroup's waves are launched one after another) copy vector slices
 // into LDS with direct buffer-to-LDS loads (16 bytes per lane, no
-// registers), tile_buffers() - 1 staged units ahead of the entry waves.
+// registers), one staged unit ahead of the entry waves.
 // Staging runs in waves of its own so its loads never sit in an entry
 // wave's queue.  Every unit ends in a barrier of all waves: the next
 // unit's sub-ranges give rows to other waves.
@@ -391,8 +391,8 @@ __host__ __device__ constexpr int vmcnt_imm(int n) { return 0x0F70 | ((n >> 4) <
 
 template <typename T, class Op>
 __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restrict__ x, Op& op) {
-    constexpr int NB = tile_buffers(sizeof(T));
-    constexpr int SB = tile_slice_bytes(sizeof(T));
+    constexpr int NB = kTileBuffers;
+    const int SB = ta.W * static_cast<int>(sizeof(T));   // slice bytes: 64, 48 or 32 KiB
     constexpr int L = tile_lane(sizeof(T));
     extern __shared__ __align__(16) unsigned char tile_lds[];
     T* xs = reinterpret_cast<T*>(tile_lds);
@@ -416,14 +416,13 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // gives staged unit j buffer j mod NB).  Before the barrier that
         // starts s_j the wave waits for its own part of s_j, leaving the
         // younger slices' loads in flight.  A slice is kStageChunks chunks
-        // of kWave x 16 bytes per staging wave, chunk k of wave w at
-        // (k kTileStagers + w) kWave x 16 bytes into the slice.
+        // of kWave x 16 bytes per staging wave (16, 12 or 8), chunk k of
+        // wave w at (k kTileStagers + w) kWave x 16 bytes into the slice.
         const int sw = wave;
         constexpr int NV = 16 / static_cast<int>(sizeof(T));
         constexpr int kChunkBytes = kWave * 16;
-        constexpr int kStageChunks = SB / (kChunkBytes * kTileStagers);
-        static_assert(kStageChunks * kChunkBytes * kTileStagers == SB, "slice split");
-        static_assert((NB - 1) * kStageChunks < 64, "vmcnt range");
+        const int kStageChunks = SB / (kChunkBytes * kTileStagers);
+        static_assert(NB == 2, "one younger slice at most");
         // The resource ends at the last whole 16-byte vector, so a lane's
         // load is either inside it or wholly past it (the hardware returns
         // zeros there).  A gathered length that is not a whole number of
@@ -444,7 +443,7 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         auto issue_next = [&]() {
             const int base = un[4 * upcoming] * SB;
             unsigned char* buf = reinterpret_cast<unsigned char*>(buffer(upcoming));
-#pragma unroll
+#pragma unroll 4
             for (int k = 0; k < kStageChunks; ++k) {
                 const int c = (k * kTileStagers + sw) * kChunkBytes;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -458,9 +457,10 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
         // wave's part of v has landed, and the rem tail is written over it
         auto land = [&](int v) {
             const int younger = issued - j - 1;
-            if (NB > 2 && younger >= 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * kStageChunks));
-            else if (younger >= 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(kStageChunks));
-            else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            if (younger < 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            else if (kStageChunks == 16) __builtin_amdgcn_s_waitcnt(vmcnt_imm(16));
+            else if (kStageChunks == 12) __builtin_amdgcn_s_waitcnt(vmcnt_imm(12));
+            else __builtin_amdgcn_s_waitcnt(vmcnt_imm(8));
             if (owner && un[4 * v] == last) {
                 typedef const T __attribute__((address_space(4))) ctail;
                 const ctail* xt = (const ctail*)(x + (ta.cols - rem));

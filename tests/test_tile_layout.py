@@ -4,7 +4,8 @@ every entry stored exactly once with its row, column and source position
 decodable from (group, unit, key); units that cover the slices in order
 (a staged unit is one slice with at least 800 entries per group, a gather
 unit a run of sparser slices whose columns fit the key's 19 column bits);
-staged units cycling through the LDS buffers; each (group, unit) tile split
+staged units alternating between the two LDS buffers, slices of 64 KiB
+(FP32) or 48 KiB (FP64) unless the row sums need the room; each (group, unit) tile split
 into 12 row-aligned sub-ranges (one per entry wave), each a multiple of the
 entries per lane (4) with padding only at its end; rows dealt to groups in 64-row
 chunks, two workgroups per group; the pass list the
@@ -34,7 +35,7 @@ def layout(ptr, idx, src, rows, cols, cus, tsize):
             cus, tsize, shape.ctypes.data, E.ctypes.data)
     assert L.sov_hip_tile_layout(*args, None, None, None, None) == 0
     nb, S, W, rmax, U, waves, Q, NBUF = (int(v) for v in shape)
-    assert Q == 2 and NBUF == (2 if tsize == 4 else 3)
+    assert Q == 2 and NBUF == 2
     keys = np.zeros(max(int(E[0]), 1), np.uint32)
     so = np.zeros(max(int(E[0]), 1), np.int32)
     ws = np.zeros(nb * U * waves + 1, np.int32)
@@ -98,8 +99,12 @@ def test_layout_stores_the_csr(name, tsize):
     nb, S, W, rmax, U, WAVES, keys, so, ws, units = layout(ptr, idx, src, rows, cols, cus, tsize)
     assert WAVES == 12
     nchunks = -(-rows // 64)
-    assert W == (16384 if tsize == 4 else 4096) and S == -(-cols // W)   # 64 KiB FP32 / 32 KiB FP64 slices
     assert rmax <= 8128 and nb * rmax >= rows and rmax % 64 == 0
+    # slices of 64 KiB (FP32) or 48 KiB (FP64), or the largest smaller one whose two buffers fit beside the
+    # row sums in 160 KiB of LDS
+    sb = next(b for b in ((65536, 49152, 32768) if tsize == 4 else (49152, 32768))
+              if 2 * b + (rmax + 64) * tsize <= 163840)
+    assert W == sb // tsize and S == -(-cols // W)
     assert nb == max(min(max(cus // 2, 1), nchunks), -(-nchunks // 127))
     # units: contiguous slice ranges; staged = one slice with >= 800 entries per group
     first = units[0:4 * U:4]
@@ -115,8 +120,7 @@ def test_layout_stores_the_csr(name, tsize):
         else:
             assert 1 <= width <= min(63, 2 ** COLBITS // W - 1) and not dense.any()
     sidx = np.flatnonzero(staged)
-    nbuf = 2 if tsize == 4 else 3
-    assert np.all(units[2:4 * U:4][sidx] == np.arange(len(sidx)) % nbuf)       # LDS buffers in turn
+    assert np.all(units[2:4 * U:4][sidx] == np.arange(len(sidx)) % 2)          # the two LDS buffers in turn
     for u in range(U):
         later = sidx[sidx > u]
         assert units[4 * u + 3] == (later[0] if len(later) else U)
