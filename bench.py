@@ -423,8 +423,9 @@ def main():
         cpu_child(args.cpu_child)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is described "
-              "in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this engine has no multi-GPU path (a single LP would split by row "
+              "blocks with an all-gather of x and a reduce-scatter of A^T y per iteration); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)

@@ -181,6 +181,11 @@ constexpr int kTileStageMin = 800;
 // FP64 tiles hold about one pass per wave and unit, where the padding only
 // halves the load queue's lookahead.
 constexpr bool tile_pairs(int tsize) { return tsize == 4; }
+// Passes in an entry wave's load queue (register slots).  Measured on the
+// 1M-column Zipf LP with 12 entry waves: 8 FP32 / 6 FP64 slots (95 / 92
+// VGPRs) made the FP32 primal step slower (58.5 -> 61.3 us) and left the
+// others within noise, so the entry data is not what the waves wait for.
+constexpr int tile_ring(int) { return 4; }
 
 struct Tiles {
     int32_t nb = 0, S = 0, W = 0, rmax = 0;           // groups, slices, slice width, local rows per group

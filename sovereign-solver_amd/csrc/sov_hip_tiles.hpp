@@ -372,8 +372,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
 //
 // Waves kTileStagers .. 15 add the entries.  Each walks its stream of
 // passes (its sub-ranges of the units, in order) with the loads of the
-// next kRing passes in flight: a queue of kRing register slots, each
-// consumed in place and refilled with the pass kRing further on, so passes
+// next tile_ring() passes in flight: a queue of that many register slots,
+// each consumed in place and refilled with the pass that far on, so passes
 // are used in the order their loads were issued (a wave's vector memory
 // counter completes in that order) and no slot is copied while its load is
 // pending.  The first kTileStagers waves (one per SIMD, the first to start:
@@ -383,7 +383,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const T* x, int co
 // Staging runs in waves of its own so its loads never sit in an entry
 // wave's queue.  Every unit ends in a barrier of all waves: the next
 // unit's sub-ranges give rows to other waves.
-constexpr int kRing = 4;
 
 // s_waitcnt immediate that waits until at most n vector memory operations
 // of the wave are in flight (expcnt and lgkmcnt left alone).
@@ -520,7 +519,18 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             ++ld;
             return load_pass<T, L>(ta, s0, c0, lane);
         };
+        constexpr int R = tile_ring(sizeof(T));
+        static_assert(R == 4 || R == 6 || R == 8, "ring");
         PassData<T, L> q0 = fetch(), q1 = fetch(), q2 = fetch(), q3 = fetch();
+        PassData<T, L> q4, q5, q6, q7;
+        if constexpr (R >= 6) {
+            q4 = fetch();
+            q5 = fetch();
+        }
+        if constexpr (R >= 8) {
+            q6 = fetch();
+            q7 = fetch();
+        }
         lds_barrier();
         // consumer state: unit cu, its end in the pass list
         int cu = u0, nxt = i, staged = 0, carry_row = -1;
@@ -641,6 +651,8 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
             while (cu < u1) {
                 step2(q0, q1);
                 step2(q2, q3);
+                if constexpr (R >= 6) step2(q4, q5);
+                if constexpr (R >= 8) step2(q6, q7);
             }
         } else {
             while (cu < u1) {
@@ -648,6 +660,14 @@ __device__ __forceinline__ void tile_rows(const TileArgs& ta, const T* __restric
                 step(q1);
                 step(q2);
                 step(q3);
+                if constexpr (R >= 6) {
+                    step(q4);
+                    step(q5);
+                }
+                if constexpr (R >= 8) {
+                    step(q6);
+                    step(q7);
+                }
             }
         }
     }
