@@ -430,6 +430,42 @@ typedef struct sov_hip_batch sov_hip_batch;
 int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* csr, int32_t count, const int32_t* row_off,
                          const int32_t* col_off, sov_hip_batch** out);
 void sov_hip_batch_destroy(sov_hip_batch* batch);
+/* Resident instances: an instance whose iterates, objective and bounds
+ * ((6 n_b + 5 m_b) values of the state's precision) fit `bytes` of LDS, and
+ * whose rows fit the resident kernel's registers (with the default lanes per
+ * length class: at most 80 wave-sized chunks of A^T's rows with 2 entries
+ * per lane, 48 of A's rows with 4), runs whole intervals in one workgroup
+ * (sov_hip_batch_resident_steps) while every instance iterating in its
+ * precision fits (then that precision makes no step launch; otherwise all
+ * of its instances stay on the step launches), and the batched step
+ * launches skip it.  Default: the device's LDS per workgroup (160 KiB);
+ * 0 = none resident.
+ * Takes effect at the next sov_hip_batch_set with `active`.  No reference
+ * counterpart. */
+int sov_hip_batch_set_stage_bytes(sov_hip_batch* batch, int64_t bytes);
+/* Tiled batch steps: while the instances iterating in a precision hold at
+ * least `share` of the batch's entries, that precision's step launches run
+ * on the csr handle's tiled layouts of the whole stacked matrix (built by
+ * sov_hip_csr_set_tiles; rows of other instances are summed and dropped),
+ * else on the CSR parts' block lists.  Negative = never (the default).
+ * Takes effect at the next sov_hip_batch_set with `active`.  No reference
+ * counterpart. */
+int sov_hip_batch_set_tile_share(sov_hip_batch* batch, double share);
+/* `count` iterations (k = 0 .. count-1 of an interval) of the resident
+ * instances iterating in st's precision, in one launch: each instance's
+ * workgroup keeps its iterates in LDS and its entries in registers for the
+ * whole interval.  With the csr handle's default lanes (no autotuning) they
+ * are the iterations `count` pairs of sov_hip_batch_primal_step / _dual_step
+ * calls would make, bit for bit; with tuned lanes they differ in rounding.
+ * The launch runs on the batch's own stream, after everything queued so far
+ * on the context stream and beside what is queued there next (the step
+ * launches, which skip the resident instances); sov_hip_batch_resident_join
+ * makes the context stream wait for it, and must come before anything that
+ * reads the iterates (a check, a restart, a copy).  No reference counterpart
+ * (a persistent batched launch). */
+int sov_hip_batch_resident_steps(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_batch* batch,
+                                 const sov_hip_pdhg_state* st, int count);
+int sov_hip_batch_resident_join(sov_hip_ctx* ctx, const sov_hip_batch* batch);
 /* Upload per-instance parameters from host arrays of `count` (NULL leaves
  * that parameter as it is).  Defaults: tau = sigma = 1, active = 2, since = 0. */
 int sov_hip_batch_set(sov_hip_ctx* ctx, sov_hip_batch* batch, const double* tau, const double* sigma,

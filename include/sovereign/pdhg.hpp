@@ -162,6 +162,18 @@ struct PdhgOptions {
     /// after that interval has been launched (one interval late, in check
     /// order, with the same arguments).  false = call it at the check.
     bool checkpoint_async = true;
+    /// Batches (PdhgBatch, PdhgBatchSession): instances whose iterates fit
+    /// this many bytes of LDS run every check interval in one workgroup
+    /// launch, iterates resident in LDS (the same iterations, bit for bit);
+    /// -1 = the device's LDS per workgroup, 0 = none.
+    long long batch_stage_bytes = -1;
+    /// Batches: a precision's steps run on the tiled layouts of the whole
+    /// stacked matrix while the instances iterating in it hold at least this
+    /// share of the entries, else on the CSR block lists; negative = never
+    /// (the default: the stacked device order spreads an instance's rows over
+    /// the length classes, so a row group's column slices hold few entries
+    /// each and the tiles run slower than the block lists).
+    double batch_tile_share = -1.0;
 };
 
 struct PdhgResult {

@@ -97,6 +97,8 @@ typedef struct sov_pdhg_opts {
     int32_t fp32_refinement;         /* PdhgOptions::fp32_refinement (default 0) */
     double refinement_start;         /* PdhgOptions::refinement_start */
     double refinement_max_scale;     /* PdhgOptions::refinement_max_scale */
+    int64_t batch_stage_bytes;       /* PdhgOptions::batch_stage_bytes (default -1) */
+    double batch_tile_share;         /* PdhgOptions::batch_tile_share (default -1) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

@@ -366,6 +366,7 @@ __global__ void k_gather_f64_to_f32(int32_t nnz, const int32_t* __restrict__ src
 // device-order CSR and A's current FP64 values.
 int build_parts(sov_hip_ctx* ctx, sov_hip_csr* h, Orient& o, int K, bool want_f32) {
     o.free_parts();
+    ++o.parts_version;
     const int32_t rows = o.nat.rows, cols = o.nat.cols, nnz = o.nat.nnz;
     if (K < 1 || cols < 2 * K || nnz == 0) K = 1;
     const int32_t W = (K == 1) ? std::max<int32_t>(cols, 1) : (cols + K - 1) / K;

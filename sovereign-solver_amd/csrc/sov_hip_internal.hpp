@@ -27,6 +27,28 @@ int hip_fail(hipError_t e, const char* what);
 
 constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;
+constexpr int kXcds = 8;             // gfx950: workgroup p of a launch runs on XCD p % 8
+
+// XCD-aware order of a launch's n work items: workgroup p (on XCD p % kXcds)
+// takes item xcd_slot(p, n), so XCD k runs the k-th contiguous share of the
+// items (q + 1 items for k < n % kXcds, else q = n / kXcds) and its L2 holds
+// the operands of that share only.  A bijection of [0, n) (checked below).
+__host__ __device__ constexpr int xcd_slot(int p, int n) {
+    const int q = n / kXcds, r = n % kXcds, k = p % kXcds;
+    return k * q + (k < r ? k : r) + p / kXcds;
+}
+constexpr bool xcd_slot_bijective(int nmax) {
+    for (int n = 0; n <= nmax; ++n) {
+        bool seen[200] = {};
+        for (int p = 0; p < n; ++p) {
+            const int s = xcd_slot(p, n);
+            if (s < 0 || s >= n || seen[s]) return false;
+            seen[s] = true;
+        }
+    }
+    return true;
+}
+static_assert(xcd_slot_bijective(200), "xcd_slot must permute [0, n)");
 constexpr int kClasses = 8;          // row-length classes of a schedule
 constexpr int kCheckSlots = 16;      // FP64 partials per block in the check kernels
 // Blocks of check partials the allocation-free check's static scratch holds.
@@ -227,6 +249,7 @@ struct Orient {
     int32_t class_count[kClasses] = {};
     int32_t class_offset[kClasses] = {};
     std::vector<Part> parts;
+    int64_t parts_version = 0;      // bumped whenever the parts are rebuilt
     int32_t seg_width = 0;          // 0 when unsegmented
     // owned storage of the parts
     int32_t* p_ptr = nullptr;       // nparts * (rows + 1)
@@ -323,9 +346,11 @@ struct sov_hip_batch {
     double* fac = nullptr;          // count
     double* terms = nullptr;        // 11 * (m + n): per-row check terms
     // per step-kernel block of each part of A (blk[0]) and A^T (blk[1]):
-    // bit 0 = holds a row of an instance iterating in FP32, bit 1 = in FP64;
-    // and per part the compacted lists of the blocks with bit 0 (list[.][2q])
-    // and bit 1 (list[.][2q+1]) set, which the batched step launches run.
+    // bit 0 / 1 = holds a row the CSR step launch of the FP32 / FP64 state
+    // runs (an instance iterating in it and not run by the instance-staged
+    // launch), bit 2 / 3 = holds a row of an instance iterating in FP32 /
+    // FP64; and per part the compacted list of the blocks with bit j set
+    // (list[.][4q + j]): bits 0-1 for the step launches, 2-3 for the checks.
     // Recomputed whenever `active` is set.
     std::vector<int32_t*> blk[2];
     std::vector<int32_t*> list[2];
@@ -337,6 +362,45 @@ struct sov_hip_batch {
     const sov_hip_csr* csr = nullptr;
     double* red = nullptr;          // count * 22
     double* h_red = nullptr;        // pinned host copy, count * 22
+    // Resident instances: small instances whose iterates, objective and
+    // bounds fit stage_bytes of LDS and whose rows fit the resident kernel's
+    // registers run whole intervals in one workgroup each
+    // (sov_hip_batch_resident_steps) while every instance iterating in
+    // their precision fits; the CSR step launches then skip them (mask).
+    // Built from single-part A and A^T (none when either has several
+    // parts), again whenever a part is rebuilt.
+    struct Resident {
+        int64_t built_from[2] = {-1, -1};          // parts_version of A and A^T the indices follow
+        uint16_t* lidx[2] = {nullptr, nullptr};    // per orientation: per entry, its column's place in its instance
+        int32_t* mask[2] = {nullptr, nullptr};     // per precision (0 FP32, 1 FP64), per instance: 1 = resident now
+        std::vector<int32_t> h_mask[2];            // per precision, per instance: 1 = fits
+        std::vector<int32_t> h_list[2];            // host copy of list (the source of its upload)
+        std::vector<int32_t> h_inst[2];            // per precision: the resident instances
+        int32_t* list[2] = {nullptr, nullptr};     // per precision: the resident instances iterating in it
+        int32_t nlist[2] = {0, 0};
+        int64_t lds[2] = {0, 0};                   // per precision: LDS bytes of the largest resident instance
+        int on[2] = {0, 0};                        // per precision: mask holds h_mask (1) or zeros (0)
+    } res;
+    // The resident launches run on the side stream, beside the step launches
+    // of the context stream: sov_hip_batch_resident_steps forks it from the
+    // context stream (ev_fork) and sov_hip_batch_resident_join makes the
+    // context stream wait for it (ev_join).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    mutable bool forked = false;    // a resident launch is queued since the last join
+    int64_t stage_bytes = 0;        // LDS a resident instance may take (0: none resident)
+    int64_t lds_max = 0;            // the device's LDS per workgroup
+    // Tiled steps: a precision's step launches run on the tiled layouts of
+    // the whole stacked matrix (the rows of instances not iterating in it
+    // are summed and dropped) while its iterating instances hold at least
+    // tile_frac of the entries, else on the CSR block lists.  Decided
+    // whenever `active` is set; negative tile_frac: never.
+    std::vector<int64_t> h_nnz;     // entries of each instance
+    int64_t nnz_total = 0;
+    double tile_frac = -1.0;
+    bool tiles_on[2] = {false, false};
+    std::vector<int32_t> h_seg;     // host copy of seg
+    std::vector<int32_t> h_active;  // host copy of active
 };
 
 struct sov_hip_graph {

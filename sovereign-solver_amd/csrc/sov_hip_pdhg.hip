@@ -47,7 +47,37 @@ struct BatchView {
     int k = 0;
     int want = 0;   // the active value this launch's state serves
     const int32_t* list = nullptr;  // the blocks this launch runs (compacted), or null for all
+    const int32_t* skip = nullptr;  // per instance: 1 = its rows run in the instance-staged launch instead
 };
+
+// The per-row updates of the two steps, shared by every kernel that makes
+// them (so that they make them bit for bit alike).  x, x_bar, x_sum (s, y,
+// y_sum) in and out; x_sum's (y_sum's) input is unused when first.
+template <typename T>
+__device__ __forceinline__ void primal_update(T c, T aty, T l, T h, T tau, int first, T& x, T& xbar, T& xsum) {
+    const T g = c - aty;
+    const T xo = x;
+    T xn = xo + (-tau) * g;
+    if (xn < l) xn = l;
+    if (xn > h) xn = h;
+    xbar = xn * T(2) - xo;
+    x = xn;
+    xsum = first ? xn : xsum + xn;
+}
+template <typename T>
+__device__ __forceinline__ void dual_update(T ax, T l, T h, T sgv, bool has_sg, T tau, T sigma, int first, T& s,
+                                            T& y, T& ysum) {
+    const T so = s, yo = y;
+    const T gs = has_sg ? yo + sgv : yo;   // the slacks' gradient
+    T sn = so + (-tau) * gs;
+    if (sn < l) sn = l;
+    if (sn > h) sn = h;
+    const T sb = sn * T(2) - so;
+    const T yn = yo + sigma * ((-ax) + sb);
+    s = sn;
+    y = yn;
+    ysum = first ? yn : ysum + yn;
+}
 
 // Per column j (a row of A^T): g = c_j - (A^T y)_j; x+ = clamp(x - tau g);
 // x_bar = 2 x+ - x; x = x+; x_sum = first ? x+ : x_sum + x+.
@@ -69,43 +99,78 @@ struct PrimalOp {
     int mode;
     Sched s;
     BatchView bv;
-    __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
+    // a batch launch runs the listed blocks in XCD-aware order: an XCD's
+    // share of the list is a run of consecutive blocks (fewer instances)
+    __device__ int block() const {
+        return (B && bv.list != nullptr) ? bv.list[xcd_slot((int)blockIdx.x, (int)gridDim.x)] : (int)blockIdx.x;
+    }
+    // The row's own operand loads (bounds, iterates, a segment's partial sum
+    // and, in a batch, its instance) are issued before the product's, so that
+    // the row waits for one chain of memory round trips, not for one after
+    // the other; a batch row whose instance does not iterate in this launch
+    // computes its product and drops it.  Same arithmetic as apply().
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        int row = row_of<G>(v, lane);
+        const int row = row_of<G>(v, lane);
+        const bool mine = row >= 0 && lane == 0;
+        T cr = T(0), lr = T(0), hr = T(0), xv = T(0), xs = T(0), ar = T(0);
+        int b = 0;
+        if (mine) {
+            if (mode == kMiddle || mode == kLast) ar = acc[row];
+            if (mode == kOnly || mode == kLast) {
+                cr = c[row];
+                lr = lo[row];
+                hr = hi[row];
+                xv = x[row];
+                xs = xsum[row];
+            }
+            if constexpr (B) b = bv.inst[row];
+        }
+        T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
+        if (!mine) return;
         T tau_r = tau;
         int first_r = first;
         if constexpr (B) {
-            if (row >= 0) {
-                const int b = bv.inst[row];
-                if (bv.active[b] != bv.want) {
-                    row = -1;
-                } else {
-                    tau_r = T(bv.tau[b]);
-                    first_r = (bv.k == 0 && bv.since[b] == 0) ? 1 : 0;
-                }
-            }
+            if (bv.active[b] != bv.want || (bv.skip != nullptr && bv.skip[b])) return;
+            tau_r = T(bv.tau[b]);
+            first_r = (bv.k == 0 && bv.since[b] == 0) ? 1 : 0;
         }
-        const T aty = row_dot<G, T>(ptr, idx, val, y, row, lane);
-        if (row >= 0 && lane == 0) apply(row, aty, tau_r, first_r);
+        if (mode == kFirst) { acc[row] = aty; return; }
+        if (mode == kMiddle) { acc[row] = ar + aty; return; }
+        if (mode == kLast) aty = ar + aty;
+        T xb;
+        if (first_r) xs = T(0);
+        primal_update(cr, aty, lr, hr, tau_r, first_r, xv, xb, xs);
+        xbar[row] = xb;
+        x[row] = xv;
+        xsum[row] = xs;
+    }
+    // a row of batch instance b, with b's step size and Cesaro state
+    __device__ __forceinline__ void apply_batch(int row, T aty, int b) {
+        apply(row, aty, T(bv.tau[b]), (bv.k == 0 && bv.since[b] == 0) ? 1 : 0);
     }
     // the row's full product (A^T y)_row, or a segment's part of it
     __device__ __forceinline__ void apply(int row, T aty, T tau_r, int first_r) {
         if (mode == kFirst) { acc[row] = aty; return; }
         if (mode == kMiddle) { acc[row] = acc[row] + aty; return; }
         if (mode == kLast) aty = acc[row] + aty;
-        const T g = c[row] - aty;
-        const T xo = x[row];
-        T xn = xo + (-tau_r) * g;
-        const T l = lo[row], h = hi[row];
-        if (xn < l) xn = l;
-        if (xn > h) xn = h;
-        xbar[row] = xn * T(2) - xo;
-        x[row] = xn;
-        xsum[row] = first_r ? xn : xsum[row] + xn;
+        T xv = x[row], xb, xs = first_r ? T(0) : xsum[row];
+        primal_update(c[row], aty, lo[row], hi[row], tau_r, first_r, xv, xb, xs);
+        xbar[row] = xb;
+        x[row] = xv;
+        xsum[row] = xs;
     }
-    __device__ __forceinline__ void finish(int row, T aty) { apply(row, aty, tau, first); }
+    // a row's sum from the tiled kernels; in a batch, only the rows of
+    // instances iterating in this launch's precision (and not resident)
+    __device__ __forceinline__ void finish(int row, T aty) {
+        if constexpr (B) {
+            const int b = bv.inst[row];
+            if (bv.active[b] == bv.want && (bv.skip == nullptr || !bv.skip[b])) apply_batch(row, aty, b);
+        } else {
+            apply(row, aty, tau, first);
+        }
+    }
 };
 
 // Per row i: s+ = clamp(s - tau y); s_bar = 2 s+ - s;
@@ -128,46 +193,73 @@ struct DualOp {
     Sched s;
     BatchView bv;
     const T* sg = nullptr;   // optional objective of the slacks (refinement corrections)
-    __device__ int block() const { return (B && bv.list != nullptr) ? bv.list[blockIdx.x] : (int)blockIdx.x; }
+    __device__ int block() const {
+        return (B && bv.list != nullptr) ? bv.list[xcd_slot((int)blockIdx.x, (int)gridDim.x)] : (int)blockIdx.x;
+    }
+    // as PrimalOp::run
     template <int G>
     __device__ void run(const ClassView& v) {
         int lane;
-        int row = row_of<G>(v, lane);
+        const int row = row_of<G>(v, lane);
+        const bool mine = row >= 0 && lane == 0;
+        T lr = T(0), hr = T(0), sv = T(0), yv = T(0), ys = T(0), sgv = T(0), ar = T(0);
+        int b = 0;
+        if (mine) {
+            if (mode == kMiddle || mode == kLast) ar = acc[row];
+            if (mode == kOnly || mode == kLast) {
+                lr = lo[row];
+                hr = hi[row];
+                sv = s_[row];
+                yv = y[row];
+                ys = ysum[row];
+                if (sg != nullptr) sgv = sg[row];
+            }
+            if constexpr (B) b = bv.inst[row];
+        }
+        T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
+        if (!mine) return;
         T tau_r = tau, sigma_r = sigma;
         int first_r = first;
         if constexpr (B) {
-            if (row >= 0) {
-                const int b = bv.inst[row];
-                if (bv.active[b] != bv.want) {
-                    row = -1;
-                } else {
-                    tau_r = T(bv.tau[b]);
-                    sigma_r = T(bv.sigma[b]);
-                    first_r = (bv.k == 0 && bv.since[b] == 0) ? 1 : 0;
-                }
-            }
+            if (bv.active[b] != bv.want || (bv.skip != nullptr && bv.skip[b])) return;
+            tau_r = T(bv.tau[b]);
+            sigma_r = T(bv.sigma[b]);
+            first_r = (bv.k == 0 && bv.since[b] == 0) ? 1 : 0;
         }
-        const T ax = row_dot<G, T>(ptr, idx, val, xbar, row, lane);
-        if (row >= 0 && lane == 0) apply(row, ax, tau_r, sigma_r, first_r);
+        if (mode == kFirst) { acc[row] = ax; return; }
+        if (mode == kMiddle) { acc[row] = ar + ax; return; }
+        if (mode == kLast) ax = ar + ax;
+        if (first_r) ys = T(0);
+        dual_update(ax, lr, hr, sgv, sg != nullptr, tau_r, sigma_r, first_r, sv, yv, ys);
+        s_[row] = sv;
+        y[row] = yv;
+        ysum[row] = ys;
+    }
+    // a row of batch instance b, with b's step sizes and Cesaro state
+    __device__ __forceinline__ void apply_batch(int row, T ax, int b) {
+        apply(row, ax, T(bv.tau[b]), T(bv.sigma[b]), (bv.k == 0 && bv.since[b] == 0) ? 1 : 0);
     }
     // the row's full product (A x_bar)_row, or a segment's part of it
     __device__ __forceinline__ void apply(int row, T ax, T tau_r, T sigma_r, int first_r) {
         if (mode == kFirst) { acc[row] = ax; return; }
         if (mode == kMiddle) { acc[row] = acc[row] + ax; return; }
         if (mode == kLast) ax = acc[row] + ax;
-        const T so = s_[row], yo = y[row];
-        const T gs = sg != nullptr ? yo + sg[row] : yo;   // the slacks' gradient
-        T sn = so + (-tau_r) * gs;
-        const T l = lo[row], h = hi[row];
-        if (sn < l) sn = l;
-        if (sn > h) sn = h;
-        const T sb = sn * T(2) - so;
-        const T yn = yo + sigma_r * ((-ax) + sb);
-        s_[row] = sn;
-        y[row] = yn;
-        ysum[row] = first_r ? yn : ysum[row] + yn;
+        T sv = s_[row], yv = y[row], ys = first_r ? T(0) : ysum[row];
+        dual_update(ax, lo[row], hi[row], sg != nullptr ? sg[row] : T(0), sg != nullptr, tau_r, sigma_r, first_r,
+                    sv, yv, ys);
+        s_[row] = sv;
+        y[row] = yv;
+        ysum[row] = ys;
     }
-    __device__ __forceinline__ void finish(int row, T ax) { apply(row, ax, tau, sigma, first); }
+    // as PrimalOp::finish
+    __device__ __forceinline__ void finish(int row, T ax) {
+        if constexpr (B) {
+            const int b = bv.inst[row];
+            if (bv.active[b] == bv.want && (bv.skip == nullptr || !bv.skip[b])) apply_batch(row, ax, b);
+        } else {
+            apply(row, ax, tau, sigma, first);
+        }
+    }
 };
 
 template <class Op>
@@ -181,6 +273,247 @@ __global__ void __launch_bounds__(kBlock) k_step(Op op) {
 template <typename T, class Op>
 __global__ void __launch_bounds__(kTileThreads) k_tile_step(Op op, TileArgs ta, const T* x) {
     tile_rows<T>(ta, x, op);
+}
+
+// (start, length) of instance b's rows of class c in dimension dim.
+__device__ __forceinline__ int2 batch_range(const int32_t* seg, int b, int dim, int c) {
+    const int32_t* p = seg + (((int64_t)b * 2 + dim) * kClasses + c) * 2;
+    return make_int2(p[0], p[1]);
+}
+
+// Resident batch intervals.  A small instance of a block-diagonal batch runs
+// a whole interval of iterations in one workgroup: its iterates (x, x_bar,
+// x_sum over its columns, s, y, y_sum over its rows) and its rows' bounds
+// and objective are copied into LDS, its entries of both orientations into
+// registers, and each iteration makes the primal step over its rows of A^T,
+// a barrier, the dual step over its rows of A, a barrier; the iterates are
+// written back at the end.  An instance's columns (rows) in device order are
+// the runs seg(b, 1, c) (seg(b, 0, c)) of the 8 length classes; LDS holds
+// them concatenated in class order, and an entry's lidx is its column's place
+// in that concatenation.
+//
+// The work of a pass: the rows of class c take G_c lanes each (the default
+// lanes of class c, kDefaultLanes), so the class is ceil(len_c G_c / 64)
+// wave-sized chunks;
+// the chunks of all classes are numbered in class order and wave w holds
+// chunks w, w + 16, ... (at most kResidentSlotsP / kResidentSlotsD of them,
+// with at most kResidentEntriesP / kResidentEntriesD entries per lane: the
+// host admits only instances that fit).  Each lane adds its entries in
+// ascending position, the G lanes of a row combine by group_sum's xor
+// butterfly, and every row is updated by primal_update / dual_update, so
+// where the CSR schedule has its default lanes (no autotuning) an interval
+// ends where `count` CSR step launches would, bit for bit; with tuned lanes
+// the sums differ from theirs in rounding only.  Which instances are
+// resident depends on each instance alone, never on the tuning.
+template <typename T>
+struct ResidentArgs {
+    const int32_t* a_ptr;     // A (rows: instance rows), one part, device order
+    const uint16_t* a_lidx;
+    const T* a_val;
+    const int32_t* t_ptr;     // A^T (rows: instance columns)
+    const uint16_t* t_lidx;
+    const T* t_val;
+    const int32_t* seg;
+    const int32_t* list;      // the instances this launch runs
+    T* x;
+    T* xbar;
+    T* xsum;
+    const T* c;
+    const T* clo;
+    const T* chi;
+    T* y;
+    T* s;
+    T* ysum;
+    const T* rlo;
+    const T* rhi;
+    BatchView bv;
+    int count;                // iterations
+    int ga[kClasses], gt[kClasses];   // lanes per class of A's and A^T's rows (kDefaultLanes)
+};
+
+constexpr int kResidentThreads = 1024;
+constexpr int kResidentWaves = kResidentThreads / kWave;
+constexpr int kResidentSlotsP = 5;     // chunks per wave of the primal pass
+constexpr int kResidentEntriesP = 2;   // entries per lane of the primal pass
+constexpr int kResidentSlotsD = 3;     // chunks per wave of the dual pass
+constexpr int kResidentEntriesD = 4;   // entries per lane of the dual pass
+
+// One pass's rows of one instance: per class, the first device row, the
+// count, the first local row, the lanes per row, and the first chunk
+// (pre[kClasses] = the pass's chunk count).
+struct ResidentPass {
+    int r0[kClasses];
+    int len[kClasses];
+    int lb[kClasses];
+    int g[kClasses];
+    int pre[kClasses + 1];
+};
+
+// group_sum with a runtime (power-of-two) G: the same additions in the same order
+template <typename T>
+__device__ __forceinline__ T group_sum_rt(T v, int G) {
+    for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// A wave's chunks of one pass, in registers for the whole interval: per
+// chunk, the lanes per row, the local row this lane updates (-1: none),
+// and the lane's entries (value, local index of the gathered element).
+template <typename T, int S, int E>
+struct ResidentChunks {
+    int G[S];
+    int upd[S];
+    int cnt[S];
+    T w[S][E];
+    int li[S][E];
+};
+
+template <typename T, int S, int E>
+__device__ __forceinline__ void resident_load(const ResidentPass& P, const int32_t* __restrict__ ptr,
+                                              const T* __restrict__ val, const uint16_t* __restrict__ lidx,
+                                              ResidentChunks<T, S, E>& ch) {
+    // the chunk numbers and lanes per row are wave-uniform: scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    const int lane = static_cast<int>(threadIdx.x) % kWave;
+    const int C = P.pre[kClasses];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        const int j = wave + u * kResidentWaves;   // wave-uniform
+        int c = 0;
+#pragma unroll
+        for (int cc = 1; cc < kClasses; ++cc)
+            if (j >= P.pre[cc]) c = cc;
+        const int G = P.g[c];
+        const int rr = ((j - P.pre[c]) * kWave + lane) / G;
+        const int q = lane & (G - 1);
+        const bool ok = j < C && rr < P.len[c];
+        ch.G[u] = j < C ? G : 1;
+        ch.upd[u] = (ok && q == 0) ? P.lb[c] + rr : -1;
+        int e0 = 0, e1 = 0;
+        if (ok) {
+            e0 = ptr[P.r0[c] + rr] + q;
+            e1 = ptr[P.r0[c] + rr + 1];
+        }
+        ch.cnt[u] = e1 > e0 ? min((e1 - e0 + G - 1) / G, E) : 0;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            ch.w[u][k] = T(0);
+            ch.li[u][k] = 0;
+            if (k < ch.cnt[u]) {
+                ch.w[u][k] = val[e0 + k * G];
+                ch.li[u][k] = lidx[e0 + k * G];
+            }
+        }
+    }
+}
+
+// One pass (primal: Pr, over A^T's rows, gathering y; dual: over A's rows,
+// gathering x_bar) of one iteration, in LDS and registers.  k0/k1/k2: the
+// rows' objective (primal only) and bounds.
+template <bool Pr, typename T, int S, int E>
+__device__ __forceinline__ void resident_pass(const ResidentChunks<T, S, E>& ch, const T* __restrict__ vin,
+                                              const T* __restrict__ k0, const T* __restrict__ k1,
+                                              const T* __restrict__ k2, T* __restrict__ u0, T* __restrict__ u1,
+                                              T* __restrict__ u2, T tau, T sigma, int first) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (k < ch.cnt[u]) acc += ch.w[u][k] * vin[ch.li[u][k]];
+        acc = group_sum_rt(acc, ch.G[u]);
+        const int l = ch.upd[u];
+        if (l >= 0) {
+            if constexpr (Pr) primal_update(k0[l], acc, k1[l], k2[l], tau, first, u0[l], u1[l], u2[l]);
+            else dual_update(acc, k1[l], k2[l], T(0), false, tau, sigma, first, u0[l], u1[l], u2[l]);
+        }
+    }
+}
+
+// copy an instance's runs of a dimension between a device vector and its
+// concatenation in LDS
+template <bool In, typename D, typename T>
+__device__ __forceinline__ void resident_copy(const ResidentPass& P, D* dev, T* lds) {
+    for (int c = 0; c < kClasses; ++c)
+        for (int i = threadIdx.x; i < P.len[c]; i += kResidentThreads) {
+            if constexpr (In) lds[P.lb[c] + i] = dev[P.r0[c] + i];
+            else dev[P.r0[c] + i] = lds[P.lb[c] + i];
+        }
+}
+
+// LDS bytes of a resident instance with n columns and m rows: x, x_bar,
+// x_sum, c, col_lo, col_hi, then s, y, y_sum, row_lo, row_hi.
+__host__ __device__ constexpr int64_t resident_lds_bytes(int64_t n, int64_t m, int64_t tsize) {
+    return (6 * n + 5 * m) * tsize;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kResidentThreads) k_batch_resident(ResidentArgs<T> a) {
+    __shared__ ResidentPass P[2];   // 0: the primal pass (A^T), 1: the dual pass (A)
+    const int b = a.list[blockIdx.x];
+    if (a.bv.active[b] != a.bv.want) return;   // uniform: the whole instance
+    if (threadIdx.x < 2) {
+        const int p = static_cast<int>(threadIdx.x);
+        int lb = 0, pre = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            const int2 rr = batch_range(a.seg, b, p == 0 ? 1 : 0, c);
+            const int g = p == 0 ? a.gt[c] : a.ga[c];
+            P[p].r0[c] = rr.x;
+            P[p].len[c] = rr.y;
+            P[p].lb[c] = lb;
+            P[p].g[c] = g;
+            P[p].pre[c] = pre;
+            lb += rr.y;
+            pre += (rr.y * g + kWave - 1) / kWave;
+        }
+        P[p].pre[kClasses] = pre;
+    }
+    __syncthreads();
+    const int n = P[0].lb[kClasses - 1] + P[0].len[kClasses - 1];
+    const int m = P[1].lb[kClasses - 1] + P[1].len[kClasses - 1];
+    extern __shared__ __align__(16) unsigned char res_lds[];
+    T* xl = reinterpret_cast<T*>(res_lds);
+    T* xbl = xl + n;
+    T* xsl = xbl + n;
+    T* cl = xsl + n;
+    T* clol = cl + n;
+    T* chil = clol + n;
+    T* sl = chil + n;
+    T* yl = sl + m;
+    T* ysl = yl + m;
+    T* rlol = ysl + m;
+    T* rhil = rlol + m;
+    ResidentChunks<T, kResidentSlotsP, kResidentEntriesP> cp;
+    ResidentChunks<T, kResidentSlotsD, kResidentEntriesD> cd;
+    resident_load(P[0], a.t_ptr, a.t_val, a.t_lidx, cp);
+    resident_load(P[1], a.a_ptr, a.a_val, a.a_lidx, cd);
+    const T tau = T(a.bv.tau[b]), sigma = T(a.bv.sigma[b]);
+    const int since = a.bv.since[b];
+    resident_copy<true>(P[0], a.x, xl);
+    resident_copy<true>(P[0], a.xbar, xbl);
+    resident_copy<true>(P[0], a.xsum, xsl);
+    resident_copy<true>(P[0], a.c, cl);
+    resident_copy<true>(P[0], a.clo, clol);
+    resident_copy<true>(P[0], a.chi, chil);
+    resident_copy<true>(P[1], a.s, sl);
+    resident_copy<true>(P[1], a.y, yl);
+    resident_copy<true>(P[1], a.ysum, ysl);
+    resident_copy<true>(P[1], a.rlo, rlol);
+    resident_copy<true>(P[1], a.rhi, rhil);
+    __syncthreads();
+    for (int k = 0; k < a.count; ++k) {
+        const int first = (k == 0 && since == 0) ? 1 : 0;
+        resident_pass<true>(cp, yl, cl, clol, chil, xl, xbl, xsl, tau, sigma, first);
+        __syncthreads();
+        resident_pass<false>(cd, xbl, static_cast<const T*>(nullptr), rlol, rhil, sl, yl, ysl, tau, sigma, first);
+        __syncthreads();
+    }
+    resident_copy<false>(P[0], a.x, xl);
+    resident_copy<false>(P[0], a.xbar, xbl);
+    resident_copy<false>(P[0], a.xsum, xsl);
+    resident_copy<false>(P[1], a.s, sl);
+    resident_copy<false>(P[1], a.y, yl);
+    resident_copy<false>(P[1], a.ysum, ysl);
 }
 
 // ---------------------------------------------------------------------------
@@ -448,7 +781,7 @@ struct CheckRunner {
     double (&sm)[7];
     double (&mx)[4];
     template <int G>
-    __device__ void run(const ClassView& v) {
+    __device__ __forceinline__ void run(const ClassView& v) {
         op.template run<G>(v, sm, mx);
     }
 };
@@ -800,11 +1133,6 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
 // batches of LPs (block-diagonal in one CSR)
 // ---------------------------------------------------------------------------
 
-// (start, length) of instance b's rows of class c in dimension dim.
-__device__ __forceinline__ int2 batch_range(const int32_t* seg, int b, int dim, int c) {
-    const int32_t* p = seg + (((int64_t)b * 2 + dim) * kClasses + c) * 2;
-    return make_int2(p[0], p[1]);
-}
 
 // Block b reduces instance b's 11 per-row terms of both dimensions: slots
 // 0..6 summed, 7..10 maximised; out[b * 22 + dim * 11 + slot].  Instances
@@ -904,7 +1232,8 @@ __global__ void k_batch_restart_y(int32_t m, const int32_t* __restrict__ inst, c
 
 // Activity bits of every block of one part's step launch (see sov_hip_batch::blk).
 __global__ void k_block_flags(Sched s, int32_t blocks, const int32_t* __restrict__ inst,
-                              const int32_t* __restrict__ active, int32_t* __restrict__ flags) {
+                              const int32_t* __restrict__ active, const int32_t* __restrict__ m32,
+                              const int32_t* __restrict__ m64, int32_t* __restrict__ flags) {
     for (int64_t blk = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; blk < blocks;
          blk += (int64_t)gridDim.x * blockDim.x) {
         int c = 0;
@@ -917,8 +1246,15 @@ __global__ void k_block_flags(Sched s, int32_t blocks, const int32_t* __restrict
         const int64_t r0 = pick(s.offset, c) + local0;
         int f = 0;
         for (int64_t r = 0; r < cnt; ++r) {
-            const int a = active[inst[r0 + r]];
-            f |= a;   // 1 -> bit 0, 2 -> bit 1
+            const int b = inst[r0 + r];
+            const int a = active[b];
+            // bits 0 / 1: a row the CSR step launch of FP32 / FP64 runs (not
+            // one of a resident instance); bits 2 / 3: a row iterating in
+            // FP32 / FP64 (the checks)
+            if (a == 1 && (m32 == nullptr || !m32[b])) f |= 1;
+            if (a == 2 && (m64 == nullptr || !m64[b])) f |= 2;
+            if (a == 1) f |= 4;
+            if (a == 2) f |= 8;
         }
         flags[blk] = f;
     }
@@ -937,31 +1273,209 @@ __global__ void k_compact(int32_t n, const int32_t* __restrict__ scan, int32_t* 
         if (scan[i + 1] != scan[i]) list[scan[i]] = (int32_t)i;
 }
 
+// (Re)build the resident instances when A's or A^T's single part is not the
+// one their indices were built from (or stage_bytes changed: built_from
+// reset).  Host side, from the parts' device-order CSRs.
+static int batch_resident_build(sov_hip_ctx* ctx, sov_hip_batch* bt) {
+    const sov_hip_csr* h = bt->csr;
+    auto& R = bt->res;
+    const bool one = h->a.nparts() == 1 && h->t.nparts() == 1;
+    if (one && R.built_from[0] == h->a.parts_version && R.built_from[1] == h->t.parts_version) return SOV_OK;
+    for (int q = 0; q < 2; ++q) {
+        for (void** p : {(void**)&R.lidx[q], (void**)&R.mask[q], (void**)&R.list[q]}) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+        R.h_inst[q].clear();
+        R.nlist[q] = 0;
+        R.lds[q] = 0;
+    }
+    R.built_from[0] = R.built_from[1] = -1;
+    if (!one) return SOV_OK;
+    const int count = bt->count;
+    auto run = [&](int b, int dim, int c, int k) {
+        return bt->h_seg[((static_cast<size_t>(b) * 2 + dim) * kClasses + c) * 2 + k];
+    };
+    // the device-order CSRs of A (o = 0) and A^T (o = 1) on the host
+    std::vector<int32_t> hptr[2], hidx[2];
+    for (int o = 0; o < 2; ++o) {
+        const Part& part = (o == 0 ? h->a : h->t).part(0);
+        const int32_t rows = (o == 0 ? h->a : h->t).nat.rows, nnz = part.nnz;
+        hptr[o].assign(static_cast<size_t>(rows) + 1, 0);
+        hidx[o].assign(static_cast<size_t>(std::max(nnz, 1)), 0);
+        SOV_HIP_TRY(hipMemcpyAsync(hptr[o].data(), part.ptr, sizeof(int32_t) * hptr[o].size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        if (nnz > 0)
+            SOV_HIP_TRY(hipMemcpyAsync(hidx[o].data(), part.idx, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    }
+    SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // does instance b's pass over orientation o fit the kernel's registers:
+    // at most `slots` chunks per wave, `entries` entries per lane
+    auto fits = [&](int b, int o, int slots, int entries) {
+        const int rdim = o == 0 ? 0 : 1;
+        int64_t chunks = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            const int32_t r0 = run(b, rdim, c, 0), rl = run(b, rdim, c, 1), G = kDefaultLanes[c];
+            chunks += (static_cast<int64_t>(rl) * G + kWave - 1) / kWave;
+            for (int32_t r = r0; r < r0 + rl; ++r) {
+                const int32_t len = hptr[o][static_cast<size_t>(r) + 1] - hptr[o][static_cast<size_t>(r)];
+                if ((len + G - 1) / G > entries) return false;
+            }
+        }
+        return chunks <= static_cast<int64_t>(kResidentWaves) * slots;
+    };
+    std::vector<int32_t> mask[2];
+    for (int q = 0; q < 2; ++q) mask[q].assign(static_cast<size_t>(count), 0);
+    for (int b = 0; b < count; ++b) {
+        int64_t nb = 0, mb = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            nb += run(b, 1, c, 1);
+            mb += run(b, 0, c, 1);
+        }
+        if (nb <= 0 || mb <= 0 || nb > 65535 || mb > 65535) continue;
+        if (!fits(b, 1, kResidentSlotsP, kResidentEntriesP) || !fits(b, 0, kResidentSlotsD, kResidentEntriesD)) continue;
+        for (int q = 0; q < 2; ++q) {
+            // the iterates, objective and bounds, beside the static pass tables
+            const int64_t need = resident_lds_bytes(nb, mb, q == 0 ? 4 : 8);
+            if (need > std::min<int64_t>(bt->stage_bytes, bt->lds_max - 2 * (int64_t)sizeof(ResidentPass))) continue;
+            mask[q][static_cast<size_t>(b)] = 1;
+            R.h_inst[q].push_back(b);
+            R.lds[q] = std::max(R.lds[q], need);
+        }
+    }
+    if (!R.h_inst[0].empty()) {
+        // local indices of both orientations (FP32 residency covers FP64's)
+        for (int o = 0; o < 2; ++o) {
+            const Part& part = (o == 0 ? h->a : h->t).part(0);
+            const int32_t nnz = part.nnz;
+            const int rdim = o == 0 ? 0 : 1, gdim = 1 - rdim;
+            const std::vector<int32_t>& ptr = hptr[o];
+            const std::vector<int32_t>& idx = hidx[o];
+            std::vector<uint16_t> lidx(static_cast<size_t>(std::max(nnz, 1)), 0);
+            for (int b : R.h_inst[0]) {
+                int32_t gs[kClasses], gl[kClasses], go[kClasses];
+                int32_t off = 0;
+                for (int c = 0; c < kClasses; ++c) {
+                    gs[c] = run(b, gdim, c, 0);
+                    gl[c] = run(b, gdim, c, 1);
+                    go[c] = off;
+                    off += gl[c];
+                }
+                for (int c = 0; c < kClasses; ++c) {
+                    const int32_t r0 = run(b, rdim, c, 0), rl = run(b, rdim, c, 1);
+                    for (int32_t r = r0; r < r0 + rl; ++r)
+                        for (int32_t q = ptr[static_cast<size_t>(r)]; q < ptr[static_cast<size_t>(r) + 1]; ++q) {
+                            const int32_t j = idx[static_cast<size_t>(q)];
+                            int k = 0;
+                            while (k < kClasses && !(gl[k] > 0 && j >= gs[k] && j < gs[k] + gl[k])) ++k;
+                            if (k == kClasses)
+                                return fail(SOV_INVALID_ARGUMENT, "batch: an entry outside its instance's block");
+                            lidx[static_cast<size_t>(q)] = static_cast<uint16_t>(go[k] + (j - gs[k]));
+                        }
+                }
+            }
+            SOV_HIP_TRY(hipMalloc(&R.lidx[o], sizeof(uint16_t) * lidx.size()));
+            SOV_HIP_TRY(hipMemcpyAsync(R.lidx[o], lidx.data(), sizeof(uint16_t) * lidx.size(), hipMemcpyHostToDevice,
+                                       ctx->stream));
+            SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));   // lidx is freed next
+        }
+        for (int q = 0; q < 2; ++q) {
+            SOV_HIP_TRY(hipMalloc(&R.mask[q], sizeof(int32_t) * count));
+            SOV_HIP_TRY(hipMemsetAsync(R.mask[q], 0, sizeof(int32_t) * count, ctx->stream));
+            SOV_HIP_TRY(hipMalloc(&R.list[q], sizeof(int32_t) * std::max<size_t>(R.h_inst[q].size(), 1)));
+        }
+    }
+    for (int q = 0; q < 2; ++q) {
+        R.h_mask[q] = std::move(mask[q]);
+        R.on[q] = 0;   // the device masks start all zero
+    }
+    R.built_from[0] = h->a.parts_version;
+    R.built_from[1] = h->t.parts_version;
+    return SOV_OK;
+}
+
+// The resident instances iterating in each precision (h_active), and the
+// device masks the step launches skip.
+static int batch_resident_lists(sov_hip_ctx* ctx, sov_hip_batch* bt) {
+    auto& R = bt->res;
+    for (int q = 0; q < 2; ++q) {
+        // resident only while every instance iterating in precision q fits:
+        // then its intervals make no step launch at all (a resident launch
+        // beside step launches of the same precision measured slower)
+        std::vector<int32_t>& keep = R.h_list[q];   // outlives the copy (the caller synchronises)
+        keep.clear();
+        bool all = R.mask[q] != nullptr;
+        for (size_t b = 0; b < bt->h_active.size() && all; ++b)
+            if (bt->h_active[b] == q + 1) {
+                if (R.h_mask[q][b]) keep.push_back(static_cast<int32_t>(b));
+                else all = false;
+            }
+        if (!all) keep.clear();
+        const int on = keep.empty() ? 0 : 1;
+        // ordered on the context stream, before the block flags that read it
+        if (R.mask[q] != nullptr && on != R.on[q]) {
+            if (on) {
+                SOV_HIP_TRY(hipMemcpyAsync(R.mask[q], R.h_mask[q].data(), sizeof(int32_t) * R.h_mask[q].size(),
+                                           hipMemcpyHostToDevice, ctx->stream));
+            } else {
+                SOV_HIP_TRY(hipMemsetAsync(R.mask[q], 0, sizeof(int32_t) * R.h_mask[q].size(), ctx->stream));
+            }
+            R.on[q] = on;
+        }
+        R.nlist[q] = static_cast<int32_t>(keep.size());
+        if (!keep.empty())
+            SOV_HIP_TRY(hipMemcpyAsync(R.list[q], keep.data(), sizeof(int32_t) * keep.size(), hipMemcpyHostToDevice,
+                                       ctx->stream));
+    }
+    return SOV_OK;
+}
+
+// Per precision: tiled step launches while its iterating instances hold at
+// least tile_frac of the entries (h_active).
+static void batch_tile_choice(sov_hip_batch* bt) {
+    int64_t act[2] = {0, 0};
+    for (size_t b = 0; b < bt->h_nnz.size(); ++b) {
+        const int a = bt->h_active[b];
+        if (a == 1 || a == 2) act[a - 1] += bt->h_nnz[b];
+    }
+    for (int q = 0; q < 2; ++q)
+        bt->tiles_on[q] = bt->tile_frac >= 0.0 && bt->nnz_total > 0 && act[q] > 0 &&
+                          static_cast<double>(act[q]) >= bt->tile_frac * static_cast<double>(bt->nnz_total);
+}
+
 // (Re)compute the block activity bits of every part of both orientations.
 static int batch_block_flags(sov_hip_ctx* ctx, sov_hip_batch* bt) {
     const sov_hip_csr* h = bt->csr;
+    {
+        int rc = batch_resident_build(ctx, bt);
+        if (rc == SOV_OK) rc = batch_resident_lists(ctx, bt);
+        if (rc != SOV_OK) return rc;
+    }
+    batch_tile_choice(bt);
     for (int o = 0; o < 2; ++o) {
         const Orient& orient = o == 0 ? h->a : h->t;
         const int np = orient.nparts();
         if ((int)bt->blk[o].size() < np) {
             bt->blk[o].resize(np, nullptr);
-            bt->list[o].resize(2 * np, nullptr);
-            bt->nlist[o].resize(2 * np, 0);
+            bt->list[o].resize(4 * np, nullptr);
+            bt->nlist[o].resize(4 * np, 0);
             bt->blk_cap[o].resize(np, 0);
         }
         for (int q = 0; q < np; ++q) {
             const Sched sch = orient.part(q).sched();
             const int32_t blocks = sch.block_begin[kClasses];
             if (blocks > bt->blk_cap[o][q]) {
-                for (int32_t** p : {&bt->blk[o][q], &bt->list[o][2 * q], &bt->list[o][2 * q + 1]}) {
+                for (int32_t** p : {&bt->blk[o][q], &bt->list[o][4 * q], &bt->list[o][4 * q + 1],
+                                    &bt->list[o][4 * q + 2], &bt->list[o][4 * q + 3]}) {
                     (void)hipFree(*p);
                     *p = nullptr;
                 }
                 // room for every lane choice: at most one block per 4 rows plus one per class
                 const int64_t cap = std::max<int64_t>(blocks, (int64_t)orient.nat.rows / 4 + kClasses + 1);
                 SOV_HIP_TRY(hipMalloc(&bt->blk[o][q], sizeof(int32_t) * cap));
-                SOV_HIP_TRY(hipMalloc(&bt->list[o][2 * q], sizeof(int32_t) * cap));
-                SOV_HIP_TRY(hipMalloc(&bt->list[o][2 * q + 1], sizeof(int32_t) * cap));
+                for (int bit = 0; bit < 4; ++bit)
+                    SOV_HIP_TRY(hipMalloc(&bt->list[o][4 * q + bit], sizeof(int32_t) * cap));
                 bt->blk_cap[o][q] = cap;
             }
             if (blocks + 1 > bt->scan_cap) {
@@ -971,21 +1485,22 @@ static int batch_block_flags(sov_hip_ctx* ctx, sov_hip_batch* bt) {
                 SOV_HIP_TRY(hipMalloc(&bt->scan, sizeof(int32_t) * (blocks + 1)));
                 bt->scan_cap = blocks + 1;
             }
-            bt->nlist[o][2 * q] = bt->nlist[o][2 * q + 1] = 0;
+            for (int bit = 0; bit < 4; ++bit) bt->nlist[o][4 * q + bit] = 0;
             if (blocks == 0) continue;
             k_block_flags<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(
-                sch, blocks, o == 0 ? bt->row_inst : bt->col_inst, bt->active, bt->blk[o][q]);
-            for (int bit = 0; bit < 2; ++bit) {
+                sch, blocks, o == 0 ? bt->row_inst : bt->col_inst, bt->active, bt->res.mask[0], bt->res.mask[1],
+                bt->blk[o][q]);
+            for (int bit = 0; bit < 4; ++bit) {
                 k_flag_bit<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(blocks, bt->blk[o][q], bit, bt->scan);
                 SOV_HIP_TRY(check_launch());
                 const int rc = inclusive_scan(ctx, bt->scan, (int64_t)blocks + 1);
                 if (rc != SOV_OK) return rc;
-                k_compact<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(blocks, bt->scan, bt->list[o][2 * q + bit]);
+                k_compact<<<launch_blocks(blocks), kBlock, 0, ctx->stream>>>(blocks, bt->scan, bt->list[o][4 * q + bit]);
                 SOV_HIP_TRY(check_launch());
                 int32_t cnt = 0;
                 SOV_HIP_TRY(hipMemcpyAsync(&cnt, bt->scan + blocks, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
                 SOV_HIP_TRY(hipStreamSynchronize(ctx->stream));
-                bt->nlist[o][2 * q + bit] = cnt;
+                bt->nlist[o][4 * q + bit] = cnt;
             }
         }
     }
@@ -994,8 +1509,11 @@ static int batch_block_flags(sov_hip_ctx* ctx, sov_hip_batch* bt) {
     return SOV_OK;
 }
 
+// check: the lists of every block with a row iterating in T (the checks
+// cover every instance); else those the CSR step launches run (resident
+// instances left out).
 template <typename T>
-static BatchView batch_view(const sov_hip_batch* bt, bool rows, int k, int part = -1) {
+static BatchView batch_view(const sov_hip_batch* bt, bool rows, int k, int part = -1, bool check = false) {
     BatchView v;
     v.inst = rows ? bt->row_inst : bt->col_inst;
     v.tau = bt->tau;
@@ -1004,13 +1522,13 @@ static BatchView batch_view(const sov_hip_batch* bt, bool rows, int k, int part 
     v.since = bt->since;
     v.k = k;
     v.want = std::is_same<T, float>::value ? 1 : 2;
-    if (part >= 0) v.list = bt->list[rows ? 0 : 1][2 * part + (v.want - 1)];
+    if (part >= 0) v.list = bt->list[rows ? 0 : 1][4 * part + (check ? 2 : 0) + (v.want - 1)];
     return v;
 }
 
 template <typename T>
-static int batch_blocks(const sov_hip_batch* bt, bool rows, int part) {
-    return bt->nlist[rows ? 0 : 1][2 * part + (std::is_same<T, float>::value ? 0 : 1)];
+static int batch_blocks(const sov_hip_batch* bt, bool rows, int part, bool check = false) {
+    return bt->nlist[rows ? 0 : 1][4 * part + (check ? 2 : 0) + (std::is_same<T, float>::value ? 0 : 1)];
 }
 
 // Move flagged instances from an FP32 state to the FP64 one: their candidate
@@ -1046,7 +1564,7 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
     for (int k = 0; k < npa; ++k) {
         const Part& p = OA.part(k);
         const Sched s = p.sched();
-        const int blocks = batch_blocks<T>(bt, true, k);
+        const int blocks = batch_blocks<T>(bt, true, k, true);
         if (blocks == 0) continue;
         CheckRowOp<T, true> op{};
         op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
@@ -1056,13 +1574,13 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
         op.lo_o = a->orig_row_lo; op.hi_o = a->orig_row_hi; op.rs = a->row_scale;
         op.act_avg = a->act_avg; op.acc = OA.chk_acc; op.rowout = rows_t; op.stride = bt->m;
         op.inv = 1.0; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.mode = seg_mode(k, npa); op.s = s;
-        op.bv = batch_view<T>(bt, true, 0, k);
+        op.bv = batch_view<T>(bt, true, 0, k, true);
         k_check<CheckRowOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     for (int k = 0; k < npt; ++k) {
         const Part& p = OT.part(k);
         const Sched s = p.sched();
-        const int blocks = batch_blocks<T>(bt, false, k);
+        const int blocks = batch_blocks<T>(bt, false, k, true);
         if (blocks == 0) continue;
         CheckColOp<T, true> op{};
         op.ptr = p.ptr; op.idx = p.idx; op.val = vals_of<T>(p);
@@ -1074,7 +1592,7 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
         op.acc = OT.chk_acc; op.rowout = cols_t; op.stride = bt->n;
         op.inv = 1.0; op.tol_s = a->tol_scaled; op.tol_o = a->tol_orig; op.sign = a->obj_sign;
         op.mode = seg_mode(k, npt); op.s = s;
-        op.bv = batch_view<T>(bt, false, 0, k);
+        op.bv = batch_view<T>(bt, false, 0, k, true);
         k_check<CheckColOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     k_batch_reduce<<<bt->count, kBlock, 0, ctx->stream>>>(bt->seg, rows_t, bt->m, cols_t, bt->n, bt->active,
@@ -1249,6 +1767,15 @@ template <typename T>
 int batch_primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
                       int k) {
     const Orient& o = h->t;
+    constexpr bool f64 = std::is_same<T, double>::value;
+    if (bt->tiles_on[f64 ? 1 : 0] && tiles_usable(ctx, o.tiles_of(f64), f64, st->y)) {
+        PrimalOp<T, true> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->y), static_cast<const T*>(st->c),
+                             static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi),
+                             static_cast<T*>(st->x), static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum),
+                             orient_acc<T>(o), T(0), 0, kOnly, Sched{}, batch_view<T>(bt, false, k)};
+        op.bv.skip = bt->res.mask[f64 ? 1 : 0];
+        return launch_tile_step<T>(ctx, o, op, static_cast<const T*>(st->y));
+    }
     const int np = o.nparts();
     for (int q = 0; q < np; ++q) {
         const Part& p = o.part(q);
@@ -1261,6 +1788,7 @@ int batch_primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batc
                              static_cast<const T*>(st->col_lo), static_cast<const T*>(st->col_hi),
                              static_cast<T*>(st->x), static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum),
                              orient_acc<T>(o), T(0), 0, seg_mode(q, np), s, batch_view<T>(bt, false, k, q)};
+        op.bv.skip = bt->res.mask[std::is_same<T, float>::value ? 0 : 1];
         k_step<PrimalOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     SOV_HIP_TRY(check_launch());
@@ -1271,6 +1799,15 @@ template <typename T>
 int batch_dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
                     int k) {
     const Orient& o = h->a;
+    constexpr bool f64 = std::is_same<T, double>::value;
+    if (bt->tiles_on[f64 ? 1 : 0] && tiles_usable(ctx, o.tiles_of(f64), f64, st->x_bar)) {
+        DualOp<T, true> op{nullptr, nullptr, nullptr, static_cast<const T*>(st->x_bar),
+                           static_cast<const T*>(st->row_lo), static_cast<const T*>(st->row_hi),
+                           static_cast<T*>(st->s), static_cast<T*>(st->y), static_cast<T*>(st->y_sum),
+                           orient_acc<T>(o), T(0), T(0), 0, kOnly, Sched{}, batch_view<T>(bt, true, k)};
+        op.bv.skip = bt->res.mask[f64 ? 1 : 0];
+        return launch_tile_step<T>(ctx, o, op, static_cast<const T*>(st->x_bar));
+    }
     const int np = o.nparts();
     for (int q = 0; q < np; ++q) {
         const Part& p = o.part(q);
@@ -1283,9 +1820,56 @@ int batch_dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch*
                            static_cast<const T*>(st->row_hi), static_cast<T*>(st->s), static_cast<T*>(st->y),
                            static_cast<T*>(st->y_sum), orient_acc<T>(o), T(0), T(0), 0, seg_mode(q, np), s,
                            batch_view<T>(bt, true, k, q)};
+        op.bv.skip = bt->res.mask[std::is_same<T, float>::value ? 0 : 1];
         k_step<DualOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
     SOV_HIP_TRY(check_launch());
+    return SOV_OK;
+}
+
+template <typename T>
+int batch_resident_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
+                        int count) {
+    const int q = std::is_same<T, float>::value ? 0 : 1;
+    const auto& R = bt->res;
+    if (R.nlist[q] == 0 || count <= 0) return SOV_OK;
+    const Part& pa = h->a.part(0);
+    const Part& pt = h->t.part(0);
+    ResidentArgs<T> a{};
+    a.a_ptr = pa.ptr;
+    a.a_lidx = R.lidx[0];
+    a.a_val = vals_of<T>(pa);
+    a.t_ptr = pt.ptr;
+    a.t_lidx = R.lidx[1];
+    a.t_val = vals_of<T>(pt);
+    if (a.a_val == nullptr || a.t_val == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch resident: value image missing");
+    a.seg = bt->seg;
+    a.list = R.list[q];
+    a.x = static_cast<T*>(st->x);
+    a.xbar = static_cast<T*>(st->x_bar);
+    a.xsum = static_cast<T*>(st->x_sum);
+    a.c = static_cast<const T*>(st->c);
+    a.clo = static_cast<const T*>(st->col_lo);
+    a.chi = static_cast<const T*>(st->col_hi);
+    a.y = static_cast<T*>(st->y);
+    a.s = static_cast<T*>(st->s);
+    a.ysum = static_cast<T*>(st->y_sum);
+    a.rlo = static_cast<const T*>(st->row_lo);
+    a.rhi = static_cast<const T*>(st->row_hi);
+    a.bv = batch_view<T>(bt, true, 0);
+    a.count = count;
+    for (int c = 0; c < kClasses; ++c) a.ga[c] = a.gt[c] = kDefaultLanes[c];
+    SOV_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_resident<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds[q])));
+    if (!bt->forked) {
+        // the side stream starts after everything queued so far
+        SOV_HIP_TRY(hipEventRecord(bt->ev_fork, ctx->stream));
+        SOV_HIP_TRY(hipStreamWaitEvent(bt->side, bt->ev_fork, 0));
+        bt->forked = true;
+    }
+    k_batch_resident<T><<<R.nlist[q], kResidentThreads, static_cast<size_t>(R.lds[q]), bt->side>>>(a);
+    SOV_HIP_TRY(check_launch());
+    SOV_HIP_TRY(hipEventRecord(bt->ev_join, bt->side));
     return SOV_OK;
 }
 
@@ -1300,6 +1884,9 @@ int batch_ok(const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_s
             for (int q = 0; q < orient.nparts(); ++q)
                 if (orient.part(q).blocks() > bt->blk_cap[o][q])
                     return fail(SOV_INVALID_ARGUMENT, "batch: lanes changed since the last sov_hip_batch_set(active)");
+            // the resident instances index the parts they were built from
+            if (bt->res.lidx[o] != nullptr && bt->res.built_from[o] != orient.parts_version)
+                return fail(SOV_INVALID_ARGUMENT, "batch: segments changed since the last sov_hip_batch_set(active)");
         }
     if (h != nullptr && (bt->m != h->m || bt->n != h->n)) return fail(SOV_DIMENSION_MISMATCH, "batch dims vs csr");
     if (st != nullptr && (st->m != bt->m || st->n != bt->n)) return fail(SOV_DIMENSION_MISMATCH, "batch dims vs state");
@@ -1313,11 +1900,22 @@ void free_batch(sov_hip_batch* bt) {
                     (void*)bt->red})
         (void)hipFree(p);
     if (bt->h_red) (void)hipHostFree(bt->h_red);
+    if (bt->side) {
+        (void)hipStreamSynchronize(bt->side);
+        (void)hipStreamDestroy(bt->side);
+    }
+    if (bt->ev_fork) (void)hipEventDestroy(bt->ev_fork);
+    if (bt->ev_join) (void)hipEventDestroy(bt->ev_join);
     for (int o = 0; o < 2; ++o) {
         for (int32_t* p : bt->blk[o]) (void)hipFree(p);
         for (int32_t* p : bt->list[o]) (void)hipFree(p);
     }
     (void)hipFree(bt->scan);
+    for (int q = 0; q < 2; ++q) {
+        (void)hipFree(bt->res.lidx[q]);
+        (void)hipFree(bt->res.mask[q]);
+        (void)hipFree(bt->res.list[q]);
+    }
     delete bt;
 }
 
@@ -1377,6 +1975,31 @@ int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t count, 
     bt->device = ctx->device;
     bt->stream = ctx->stream;
     bt->csr = h;
+    bt->h_seg = seg;
+    bt->h_active.assign(static_cast<size_t>(count), 2);
+    bt->lds_max = ctx->lds_max;
+    bt->stage_bytes = ctx->lds_max;
+    {
+        // entries of each instance: its row runs' spans of A's device-order CSR
+        std::vector<int32_t> ptr(static_cast<size_t>(h->m) + 1, 0);
+        hipError_t e = hipSuccess;
+        if (h->m > 0) {
+            e = hipMemcpyAsync(ptr.data(), h->a.d_ptr, sizeof(int32_t) * ptr.size(), hipMemcpyDeviceToHost,
+                               ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        }
+        if (e != hipSuccess) {
+            delete bt;
+            return hip_fail(e, "batch_create ptr");
+        }
+        bt->h_nnz.assign(static_cast<size_t>(count), 0);
+        for (int b = 0; b < count; ++b)
+            for (int c = 0; c < kClasses; ++c) {
+                const int32_t* r = &seg[((static_cast<size_t>(b) * 2) * kClasses + c) * 2];
+                if (r[1] > 0) bt->h_nnz[static_cast<size_t>(b)] += ptr[static_cast<size_t>(r[0] + r[1])] - ptr[static_cast<size_t>(r[0])];
+            }
+        for (int64_t v : bt->h_nnz) bt->nnz_total += v;
+    }
     const size_t m1 = std::max<int32_t>(h->m, 1), n1 = std::max<int32_t>(h->n, 1);
     hipError_t e = hipMalloc(&bt->row_inst, sizeof(int32_t) * m1);
     if (e == hipSuccess) e = hipMalloc(&bt->col_inst, sizeof(int32_t) * n1);
@@ -1391,6 +2014,9 @@ int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t count, 
     if (e == hipSuccess) e = hipMalloc(&bt->terms, sizeof(double) * 11 * (m1 + n1));
     if (e == hipSuccess) e = hipMalloc(&bt->red, sizeof(double) * 22 * count);
     if (e == hipSuccess) e = hipHostMalloc(&bt->h_red, sizeof(double) * 22 * count, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->side, hipStreamNonBlocking);   // default priority
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_join, hipEventDisableTiming);
     if (e == hipSuccess && h->m > 0)
         e = hipMemcpy(bt->row_inst, rinst.data(), sizeof(int32_t) * h->m, hipMemcpyHostToDevice);
     if (e == hipSuccess && h->n > 0)
@@ -1428,6 +2054,19 @@ void sov_hip_batch_destroy(sov_hip_batch* bt) {
     free_batch(bt);
 }
 
+int sov_hip_batch_set_stage_bytes(sov_hip_batch* bt, int64_t bytes) {
+    if (bt == nullptr || bytes < 0) return fail(SOV_INVALID_ARGUMENT, "batch_set_stage_bytes args");
+    bt->stage_bytes = std::min<int64_t>(bytes, bt->lds_max);
+    bt->res.built_from[0] = bt->res.built_from[1] = -1;   // rebuilt at the next batch_set(active)
+    return SOV_OK;
+}
+
+int sov_hip_batch_set_tile_share(sov_hip_batch* bt, double share) {
+    if (bt == nullptr || share != share) return fail(SOV_INVALID_ARGUMENT, "batch_set_tile_share args");
+    bt->tile_frac = share;
+    return SOV_OK;
+}
+
 int sov_hip_batch_set(sov_hip_ctx* ctx, sov_hip_batch* bt, const double* tau, const double* sigma,
                       const int32_t* active, const int32_t* since) {
     if (ctx == nullptr || bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch_set args");
@@ -1438,6 +2077,7 @@ int sov_hip_batch_set(sov_hip_ctx* ctx, sov_hip_batch* bt, const double* tau, co
     if (sigma) SOV_HIP_TRY(hipMemcpyAsync(bt->sigma, sigma, sizeof(double) * c, hipMemcpyHostToDevice, ctx->stream));
     if (active) {
         SOV_HIP_TRY(hipMemcpyAsync(bt->active, active, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
+        bt->h_active.assign(active, active + c);
         const int rc = batch_block_flags(ctx, bt);
         if (rc != SOV_OK) return rc;
     }
@@ -1466,6 +2106,26 @@ int sov_hip_batch_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
     DeviceGuard3 g(ctx->device);
     return st->prec == SOV_PREC_F64 ? batch_dual_impl<double>(ctx, h, bt, st, k)
                                     : batch_dual_impl<float>(ctx, h, bt, st, k);
+}
+
+int sov_hip_batch_resident_join(sov_hip_ctx* ctx, const sov_hip_batch* bt) {
+    if (ctx == nullptr || bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch resident_join args");
+    if (!bt->forked) return SOV_OK;
+    DeviceGuard3 g(ctx->device);
+    SOV_HIP_TRY(hipStreamWaitEvent(ctx->stream, bt->ev_join, 0));
+    bt->forked = false;
+    return SOV_OK;
+}
+
+int sov_hip_batch_resident_steps(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,
+                                 const sov_hip_pdhg_state* st, int count) {
+    if (ctx == nullptr || h == nullptr || count < 0) return fail(SOV_INVALID_ARGUMENT, "batch resident_steps args");
+    int rc = state_ok(h, st);
+    if (rc == SOV_OK) rc = batch_ok(h, bt, st);
+    if (rc != SOV_OK) return rc;
+    DeviceGuard3 g(ctx->device);
+    return st->prec == SOV_PREC_F64 ? batch_resident_impl<double>(ctx, h, bt, st, count)
+                                    : batch_resident_impl<float>(ctx, h, bt, st, count);
 }
 
 int sov_hip_batch_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,

@@ -399,7 +399,7 @@ int build_tiles(sov_hip_ctx* ctx, const sov_hip_csr* h, Orient& o, bool f64) {
         e = hipMemcpy(t.units, t.h_units.data(), sizeof(int32_t) * t.h_units.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && t.Q > 1) e = hipMalloc(&t.part, (size_t)(f64 ? 8 : 4) * t.Q * t.nb * t.rmax);
     if (e == hipSuccess && t.Q > 1) e = hipMalloc(&t.cnt, sizeof(int32_t) * t.nb);
-    if (e == hipSuccess && t.Q > 1) e = hipMemset(t.cnt, 0, sizeof(int32_t) * t.nb);
+    if (e == hipSuccess && t.Q > 1) e = hipMemsetAsync(t.cnt, 0, sizeof(int32_t) * t.nb, ctx->stream);
     if (e == hipSuccess && f64) e = hipMalloc(&t.val64, sizeof(double) * e1);
     if (e == hipSuccess && !f64) e = hipMalloc(&t.val32, sizeof(float) * e1);
     if (e == hipSuccess) e = hipMemcpy(t.keys, keys.data(), sizeof(uint32_t) * e1, hipMemcpyHostToDevice);

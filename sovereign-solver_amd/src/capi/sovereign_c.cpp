@@ -195,6 +195,8 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->fp32_refinement = d.fp32_refinement ? 1 : 0;
     o->refinement_start = d.refinement_start;
     o->refinement_max_scale = d.refinement_max_scale;
+    o->batch_stage_bytes = d.batch_stage_bytes;
+    o->batch_tile_share = d.batch_tile_share;
     o->use_primal_weight = 0;
     o->initial_primal_weight = d.initial_primal_weight;
     o->primal_weight_smoothing = d.primal_weight_smoothing;
@@ -234,6 +236,8 @@ sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
     p.fp32_refinement = o->fp32_refinement != 0;
     p.refinement_start = o->refinement_start;
     p.refinement_max_scale = o->refinement_max_scale;
+    p.batch_stage_bytes = o->batch_stage_bytes;
+    p.batch_tile_share = o->batch_tile_share;
     p.use_primal_weight = o->use_primal_weight != 0;
     p.initial_primal_weight = o->initial_primal_weight;
     p.primal_weight_smoothing = o->primal_weight_smoothing;

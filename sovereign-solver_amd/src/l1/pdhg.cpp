@@ -1531,6 +1531,8 @@ PdhgBatchSession::Impl::Impl(PdhgOptions options, const std::vector<const Proble
     hip_ok(sov_hip_csr_order(ctx, csr.h, 1, ord.col.data()), "order");
     hip_ok(sov_hip_batch_create(ctx, csr.h, static_cast<int32_t>(B), row_off.data(), col_off.data(), &bh.b),
            "batch_create");
+    if (o.batch_stage_bytes >= 0) hip_ok(sov_hip_batch_set_stage_bytes(bh.b, o.batch_stage_bytes), "stage_bytes");
+    hip_ok(sov_hip_batch_set_tile_share(bh.b, o.batch_tile_share), "tile_share");
     const std::size_t mb = std::max<std::size_t>(m, 1) * 8, nb = std::max<std::size_t>(n, 1) * 8;
     d_olo_r = DevMem(ctx, mb);
     d_ohi_r = DevMem(ctx, mb);
@@ -1619,8 +1621,12 @@ PdhgBatchSession::Impl::Impl(PdhgOptions options, const std::vector<const Proble
             for (std::size_t i = 0; i < rows; ++i) y_start[r0 + i] = sign * w[i] / rs[r0 + i];
         }
     if (o.autotune && nnz > 0) hip_ok(sov_hip_csr_tune(ctx, csr.h, f32 ? SOV_PREC_F32 : SOV_PREC_F64), "tune");
-    // the batched step kernels run on the CSR parts only
-    for (int orient : {0, 1}) hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, 0), "tiles");
+    // the batched steps run on the tiled layouts while most entries iterate
+    // in one precision (batch_tile_share), else on the CSR parts' block lists
+    for (int orient : {0, 1}) {
+        const bool tiled = o.batch_tile_share >= 0.0 && nnz > 0 && m > 0 && n > 0;
+        hip_ok(sov_hip_csr_set_tiles(ctx, csr.h, orient, tiled ? 1 : 0), "tiles");
+    }
     setup_seconds = seconds_since(t0);
 }
 
@@ -1766,13 +1772,22 @@ PdhgBatchResult PdhgBatchSession::Impl::run() {
         GraphHandle graph;
         long long graph_key = -1, active_version = 0;
         bool has32 = f32, has64 = !f32;
+        // an interval: the resident instances' whole interval in one launch
+        // per precision on the batch's stream, beside the step launches of
+        // the others on the context stream, joined before the check
         auto run_steps = [&](long long count) {
+            for (DeviceState* S : {&S32, &S64}) {
+                if ((S == &S32 && !has32) || (S == &S64 && !has64)) continue;
+                hip_ok(sov_hip_batch_resident_steps(ctx, csr.h, bh.b, &S->st, static_cast<int>(count)),
+                       "batch resident steps");
+            }
             for (long long k = 0; k < count; ++k)
                 for (DeviceState* S : {&S32, &S64}) {
                     if ((S == &S32 && !has32) || (S == &S64 && !has64)) continue;
                     hip_ok(sov_hip_batch_primal_step(ctx, csr.h, bh.b, &S->st, static_cast<int>(k)), "batch primal_step");
                     hip_ok(sov_hip_batch_dual_step(ctx, csr.h, bh.b, &S->st, static_cast<int>(k)), "batch dual_step");
                 }
+            hip_ok(sov_hip_batch_resident_join(ctx, bh.b), "batch resident join");
         };
         const bool adaptive = o.restart_scheme == PdhgRestartScheme::Adaptive;
         std::size_t n_active = B;

@@ -143,6 +143,48 @@ def test_batch_solves_every_instance_to_the_oracles_objective(gpu, prec):
     assert timing["iterations"] == max(r.iterations for r in res)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_resident_intervals_equal_the_step_launches(gpu, prec):
+    """Instances whose iterates fit the LDS budget run each interval resident
+    in one workgroup; the others stay on the CSR step launches.  Both add a
+    row's entries in the same order with the same lanes and update it with
+    the same arithmetic, so a batch solved with every instance on the CSR
+    launches (budget 0), with a 32 KiB budget (the smaller instances
+    resident) and with the full budget takes the same iterations to the
+    same points, bit for bit (tiled steps off: they add in another order)."""
+    lps = c0_models([60, 400, 150, 1000, 90, 230])
+    runs = {sb: S.solve_batch(lps, tolerance=1e-6, precision=prec, autotune=False, max_iterations=400000,
+                              stage_bytes=sb, tile_share=-1.0)[0] for sb in (0, 32768, None)}
+    for sb in (32768, None):
+        for b, (r0, r) in enumerate(zip(runs[0], runs[sb])):
+            assert r.status == r0.status == "optimal", (sb, b)
+            assert r.iterations == r0.iterations and r.objective == r0.objective, (sb, b, r.iterations, r0.iterations)
+            np.testing.assert_array_equal(r.x, r0.x)
+            np.testing.assert_array_equal(r.y, r0.y)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_tiled_batch_steps_reach_the_oracles_optima(gpu, prec):
+    """The batch's steps on the tiled layouts of the whole stacked matrix
+    (share 0: tiled while any instance iterates in the precision) and on the
+    CSR block lists only (share -1) solve every instance to the oracle's
+    objective; the tiled sums differ from the CSR ones in rounding only."""
+    lps = c0_models([60, 400, 150, 1000, 90, 230], seed0=40)
+    runs = {sh: S.solve_batch(lps, tolerance=1e-6, precision=prec, max_iterations=400000, stage_bytes=0,
+                              tile_share=sh, autotune=False)[0] for sh in (0.0, -1.0)}
+    for b, lp in enumerate(lps):
+        o = O.pdhg(O.Model(lp.arrays()), tolerance=1e-6, termination=1, restart_scheme=1, max_iterations=400000)
+        for sh, res in runs.items():
+            r = res[b]
+            assert r.status == "optimal", (sh, b, r)
+            assert max(r.relative_kkt) <= 1e-6 * 1.0001, (sh, b)
+            # both points end with a relative duality gap <= 1e-6 of
+            # 1 + |primal objective| + |dual objective|, so their objectives
+            # agree to twice that
+            tol = 2e-6 * (1.0 + abs(r.objective) + abs(o["objective"]))
+            assert abs(r.objective - o["objective"]) <= tol, (sh, b, r.objective, o["objective"])
+
+
 def test_an_instance_does_not_depend_on_its_batch(gpu):
     lps = c0_models([200, 500, 80])
     alone, _ = S.solve_batch(lps[:1], tolerance=1e-6, precision="f64", autotune=False)
