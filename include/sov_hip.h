@@ -451,6 +451,13 @@ int sov_hip_batch_set_stage_bytes(sov_hip_batch* batch, int64_t bytes);
  * Takes effect at the next sov_hip_batch_set with `active`.  No reference
  * counterpart. */
 int sov_hip_batch_set_tile_share(sov_hip_batch* batch, double share);
+/* Split precisions: when on, the FP64 state's batched step launches run on
+ * the batch's own stream (forked from the context stream at the first such
+ * launch, like sov_hip_batch_resident_steps), beside the FP32 state's on the
+ * context stream; sov_hip_batch_resident_join joins it back and must come
+ * before anything reads the FP64 state.  Default off.  Not between a fork
+ * and its join.  No reference counterpart. */
+int sov_hip_batch_set_split(sov_hip_batch* batch, int on);
 /* `count` iterations (k = 0 .. count-1 of an interval) of the resident
  * instances iterating in st's precision, in one launch: each instance's
  * workgroup keeps its iterates in LDS and its entries in registers for the

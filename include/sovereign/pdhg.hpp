@@ -174,6 +174,9 @@ struct PdhgOptions {
     /// the length classes, so a row group's column slices hold few entries
     /// each and the tiles run slower than the block lists).
     double batch_tile_share = -1.0;
+    /// Batches: the FP64 instances' step launches run on a stream of their
+    /// own, beside the FP32 instances' (joined before each check).
+    bool batch_split = true;
 };
 
 struct PdhgResult {

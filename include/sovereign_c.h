@@ -99,6 +99,7 @@ typedef struct sov_pdhg_opts {
     double refinement_max_scale;     /* PdhgOptions::refinement_max_scale */
     int64_t batch_stage_bytes;       /* PdhgOptions::batch_stage_bytes (default -1) */
     double batch_tile_share;         /* PdhgOptions::batch_tile_share (default -1) */
+    int32_t batch_split;             /* PdhgOptions::batch_split (default 1) */
 } sov_pdhg_opts;
 
 typedef struct sov_pdhg_result {

@@ -383,13 +383,15 @@ struct sov_hip_batch {
         int64_t lds[2] = {0, 0};                   // per precision: LDS bytes of the largest resident instance
         int on[2] = {0, 0};                        // per precision: mask holds h_mask (1) or zeros (0)
     } res;
-    // The resident launches run on the side stream, beside the step launches
-    // of the context stream: sov_hip_batch_resident_steps forks it from the
-    // context stream (ev_fork) and sov_hip_batch_resident_join makes the
-    // context stream wait for it (ev_join).
+    // The resident launches (and with `split` the FP64 step launches) run on
+    // the side stream, beside the context stream's launches: the first such
+    // launch after a join forks it from the context stream (ev_fork) and
+    // sov_hip_batch_resident_join makes the context stream wait for it
+    // (ev_join).
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    mutable bool forked = false;    // a resident launch is queued since the last join
+    mutable bool forked = false;    // work is queued on `side` since the last join
+    bool split = false;             // FP64 step launches on `side` too (sov_hip_batch_set_split)
     int64_t stage_bytes = 0;        // LDS a resident instance may take (0: none resident)
     int64_t lds_max = 0;            // the device's LDS per workgroup
     // Tiled steps: a precision's step launches run on the tiled layouts of

@@ -1763,6 +1763,28 @@ int set_point_impl(sov_hip_ctx* ctx, const sov_hip_pdhg_state* st, const double*
     return SOV_OK;
 }
 
+// The batch's own stream, forked from the context stream at its first use
+// since the last join (sov_hip_batch_resident_join joins it back).
+static hipStream_t batch_side(sov_hip_ctx* ctx, const sov_hip_batch* bt, hipError_t* err) {
+    *err = hipSuccess;
+    if (!bt->forked) {
+        *err = hipEventRecord(bt->ev_fork, ctx->stream);
+        if (*err == hipSuccess) *err = hipStreamWaitEvent(bt->side, bt->ev_fork, 0);
+        if (*err != hipSuccess) return nullptr;
+        bt->forked = true;
+    }
+    return bt->side;
+}
+
+// The stream of a batch step launch: the batch's own stream for the FP64
+// state when the batch splits its precisions, else the context stream.
+template <typename T>
+static hipStream_t batch_step_stream(sov_hip_ctx* ctx, const sov_hip_batch* bt, hipError_t* err) {
+    *err = hipSuccess;
+    if (std::is_same<T, double>::value && bt->split) return batch_side(ctx, bt, err);
+    return ctx->stream;
+}
+
 template <typename T>
 int batch_primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_state* st,
                       int k) {
@@ -1789,7 +1811,10 @@ int batch_primal_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batc
                              static_cast<T*>(st->x), static_cast<T*>(st->x_bar), static_cast<T*>(st->x_sum),
                              orient_acc<T>(o), T(0), 0, seg_mode(q, np), s, batch_view<T>(bt, false, k, q)};
         op.bv.skip = bt->res.mask[std::is_same<T, float>::value ? 0 : 1];
-        k_step<PrimalOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+        hipError_t se;
+        const hipStream_t stream = batch_step_stream<T>(ctx, bt, &se);
+        SOV_HIP_TRY(se);
+        k_step<PrimalOp<T, true>><<<blocks, kBlock, 0, stream>>>(op);
     }
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
@@ -1821,7 +1846,10 @@ int batch_dual_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch*
                            static_cast<T*>(st->y_sum), orient_acc<T>(o), T(0), T(0), 0, seg_mode(q, np), s,
                            batch_view<T>(bt, true, k, q)};
         op.bv.skip = bt->res.mask[std::is_same<T, float>::value ? 0 : 1];
-        k_step<DualOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
+        hipError_t se;
+        const hipStream_t stream = batch_step_stream<T>(ctx, bt, &se);
+        SOV_HIP_TRY(se);
+        k_step<DualOp<T, true>><<<blocks, kBlock, 0, stream>>>(op);
     }
     SOV_HIP_TRY(check_launch());
     return SOV_OK;
@@ -1861,15 +1889,11 @@ int batch_resident_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_ba
     for (int c = 0; c < kClasses; ++c) a.ga[c] = a.gt[c] = kDefaultLanes[c];
     SOV_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_resident<T>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds[q])));
-    if (!bt->forked) {
-        // the side stream starts after everything queued so far
-        SOV_HIP_TRY(hipEventRecord(bt->ev_fork, ctx->stream));
-        SOV_HIP_TRY(hipStreamWaitEvent(bt->side, bt->ev_fork, 0));
-        bt->forked = true;
-    }
-    k_batch_resident<T><<<R.nlist[q], kResidentThreads, static_cast<size_t>(R.lds[q]), bt->side>>>(a);
+    hipError_t se;
+    const hipStream_t side = batch_side(ctx, bt, &se);   // after everything queued so far
+    SOV_HIP_TRY(se);
+    k_batch_resident<T><<<R.nlist[q], kResidentThreads, static_cast<size_t>(R.lds[q]), side>>>(a);
     SOV_HIP_TRY(check_launch());
-    SOV_HIP_TRY(hipEventRecord(bt->ev_join, bt->side));
     return SOV_OK;
 }
 
@@ -2061,6 +2085,13 @@ int sov_hip_batch_set_stage_bytes(sov_hip_batch* bt, int64_t bytes) {
     return SOV_OK;
 }
 
+int sov_hip_batch_set_split(sov_hip_batch* bt, int on) {
+    if (bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch_set_split args");
+    if (bt->forked) return fail(SOV_INVALID_ARGUMENT, "batch_set_split between a fork and its join");
+    bt->split = on != 0;
+    return SOV_OK;
+}
+
 int sov_hip_batch_set_tile_share(sov_hip_batch* bt, double share) {
     if (bt == nullptr || share != share) return fail(SOV_INVALID_ARGUMENT, "batch_set_tile_share args");
     bt->tile_frac = share;
@@ -2112,6 +2143,7 @@ int sov_hip_batch_resident_join(sov_hip_ctx* ctx, const sov_hip_batch* bt) {
     if (ctx == nullptr || bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch resident_join args");
     if (!bt->forked) return SOV_OK;
     DeviceGuard3 g(ctx->device);
+    SOV_HIP_TRY(hipEventRecord(bt->ev_join, bt->side));
     SOV_HIP_TRY(hipStreamWaitEvent(ctx->stream, bt->ev_join, 0));
     bt->forked = false;
     return SOV_OK;

@@ -48,6 +48,7 @@ class PdhgOpts(C.Structure):
         ("fp32_refinement", C.c_int32), ("refinement_start", C.c_double), ("refinement_max_scale", C.c_double),
         ("batch_stage_bytes", C.c_int64),
         ("batch_tile_share", C.c_double),
+        ("batch_split", C.c_int32),
     ]
 
 
@@ -399,7 +400,7 @@ def solve(lp: LP, backend: str = "hip", tolerance=1e-4, max_iterations=1000000, 
 def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, restart_check_period=40,
                 termination="relative", precision="f32", fp64_polish=True, use_graphs=True, autotune=True,
                 restarts="adaptive", fp32_switch_factor=None, fp32_stall_checks=None, verbose=False,
-                warm_duals=None, primal_weights=None, stage_bytes=None, tile_share=None):
+                warm_duals=None, primal_weights=None, stage_bytes=None, tile_share=None, split=None):
     """PdhgBatch(options).solve(problems, hip backend) through
     sov_pdhg_solve_batch: every model solved in one stacked device batch.
     Returns (list of Result, dict of batch timings)."""
@@ -425,6 +426,8 @@ def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, res
         o.batch_stage_bytes = int(stage_bytes)
     if tile_share is not None:    # share of entries from which a precision's steps run tiled (negative: never)
         o.batch_tile_share = float(tile_share)
+    if split is not None:         # FP64 step launches on a stream of their own
+        o.batch_split = int(bool(split))
     count = len(lps)
     handles = (C.c_void_p * count)(*[lp.handle for lp in lps])
     res = (PdhgRes * count)()

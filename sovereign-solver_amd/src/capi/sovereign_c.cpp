@@ -197,6 +197,7 @@ void sov_pdhg_default_opts(sov_pdhg_opts* o) {
     o->refinement_max_scale = d.refinement_max_scale;
     o->batch_stage_bytes = d.batch_stage_bytes;
     o->batch_tile_share = d.batch_tile_share;
+    o->batch_split = d.batch_split ? 1 : 0;
     o->use_primal_weight = 0;
     o->initial_primal_weight = d.initial_primal_weight;
     o->primal_weight_smoothing = d.primal_weight_smoothing;
@@ -238,6 +239,7 @@ sov::PdhgOptions to_options(const sov_pdhg_opts* o) {
     p.refinement_max_scale = o->refinement_max_scale;
     p.batch_stage_bytes = o->batch_stage_bytes;
     p.batch_tile_share = o->batch_tile_share;
+    p.batch_split = o->batch_split != 0;
     p.use_primal_weight = o->use_primal_weight != 0;
     p.initial_primal_weight = o->initial_primal_weight;
     p.primal_weight_smoothing = o->primal_weight_smoothing;

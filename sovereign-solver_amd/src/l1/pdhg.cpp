@@ -1533,6 +1533,7 @@ PdhgBatchSession::Impl::Impl(PdhgOptions options, const std::vector<const Proble
            "batch_create");
     if (o.batch_stage_bytes >= 0) hip_ok(sov_hip_batch_set_stage_bytes(bh.b, o.batch_stage_bytes), "stage_bytes");
     hip_ok(sov_hip_batch_set_tile_share(bh.b, o.batch_tile_share), "tile_share");
+    hip_ok(sov_hip_batch_set_split(bh.b, o.batch_split ? 1 : 0), "split");
     const std::size_t mb = std::max<std::size_t>(m, 1) * 8, nb = std::max<std::size_t>(n, 1) * 8;
     d_olo_r = DevMem(ctx, mb);
     d_ohi_r = DevMem(ctx, mb);
@@ -1773,8 +1774,9 @@ PdhgBatchResult PdhgBatchSession::Impl::run() {
         long long graph_key = -1, active_version = 0;
         bool has32 = f32, has64 = !f32;
         // an interval: the resident instances' whole interval in one launch
-        // per precision on the batch's stream, beside the step launches of
-        // the others on the context stream, joined before the check
+        // per precision and (batch_split) the FP64 step launches on the
+        // batch's stream, beside the FP32 step launches on the context
+        // stream, joined before the check
         auto run_steps = [&](long long count) {
             for (DeviceState* S : {&S32, &S64}) {
                 if ((S == &S32 && !has32) || (S == &S64 && !has64)) continue;
