@@ -152,6 +152,20 @@ def test_hip_library_loads_without_a_device():
         assert L.sov_hip_ctx_create(0, C.byref(ctx)) == 1
 
 
+def test_batch_settings_reject_bad_arguments_without_a_device():
+    # the batch's option setters check their arguments before touching a device
+    import ctypes as C
+    L = S.hip()
+    L.sov_hip_batch_set_stage_bytes.argtypes = [C.c_void_p, C.c_int64]
+    L.sov_hip_batch_set_tile_share.argtypes = [C.c_void_p, C.c_double]
+    L.sov_hip_batch_set_split.argtypes = [C.c_void_p, C.c_int]
+    L.sov_hip_batch_resident_join.argtypes = [C.c_void_p, C.c_void_p]
+    assert L.sov_hip_batch_set_stage_bytes(None, 0) == 3
+    assert L.sov_hip_batch_set_tile_share(None, 0.5) == 3
+    assert L.sov_hip_batch_set_split(None, 1) == 3
+    assert L.sov_hip_batch_resident_join(None, None) == 3
+
+
 def test_time_limit_before_first_check_reports_a_point():
     # reference tests/test_pdhg.cpp:125-147
     text = "Maximize\n obj: x + y\nSubject To\n c1: x + 2 y <= 4\n c2: 4 x + 2 y <= 12\nEnd\n"
