@@ -164,6 +164,25 @@ def test_resident_intervals_equal_the_step_launches(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_streams_and_graphs_do_not_change_the_iterations(gpu, prec):
+    """The FP64 instances' steps and the resident intervals on the batch's
+    own stream (split) or all on the context stream, in captured interval
+    graphs or as plain launches: the same kernels make the same sums, so
+    every instance takes the same iterations to the same point, bit for
+    bit."""
+    lps = c0_models([60, 400, 150, 1000, 90, 230], seed0=70)
+    kw = dict(tolerance=1e-6, precision=prec, autotune=False, max_iterations=400000)
+    base = S.solve_batch(lps, split=True, use_graphs=True, **kw)[0]
+    for split, graphs in ((False, True), (True, False)):
+        res = S.solve_batch(lps, split=split, use_graphs=graphs, **kw)[0]
+        for b, (r0, r) in enumerate(zip(base, res)):
+            assert r.status == r0.status == "optimal", (split, graphs, b)
+            assert r.iterations == r0.iterations and r.objective == r0.objective, (split, graphs, b)
+            np.testing.assert_array_equal(r.x, r0.x)
+            np.testing.assert_array_equal(r.y, r0.y)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_tiled_batch_steps_reach_the_oracles_optima(gpu, prec):
     """The batch's steps on the tiled layouts of the whole stacked matrix
     (share 0: tiled while any instance iterates in the precision) and on the
