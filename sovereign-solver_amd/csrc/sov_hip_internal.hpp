@@ -195,9 +195,9 @@ __host__ __device__ constexpr int tile_slice_bytes(int rmax, int tsize) {
 // between the short columns and the dense ones): 384 stages them, 69.6 us;
 // 800 gathers the sparser ones from memory in one unit, 65.6 us (fewer
 // barrier-ended units); 1200, 65.7 us; the FP64 steps and the FP32 primal
-// step do not move.  Again with the two-buffer 64 KiB slices: 500 / 650 /
-// 800 / 1000 / 1400 give an FP32 dual step of 60.1 / 59.9 / 59.4 / 59.7 /
-// 59.9 us and an FP64 one of 85.1 / 85.0 / 84.9 / 84.5 / 85.2 us.
+// step do not move.  Again with the two-buffer 64 KiB slices: 500 / 800 /
+// 1400 give an FP32 dual step of 61.1 / 59.3 / 59.3 us and an FP64 one of
+// 87.5 / 85.1 / 90.4 us.
 constexpr int kTileStageMin = 800;
 // Whether the tiled kernels of element size tsize add passes in pairs (and
 // the pass list evens every (wave, unit) count for them).  Measured on the

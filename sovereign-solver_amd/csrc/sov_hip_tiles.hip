@@ -288,6 +288,7 @@ static int layout_host(int32_t rows, int32_t cols, const std::vector<int32_t>& p
     t.nb = static_cast<int32_t>(nb);
     t.rmax = static_cast<int32_t>((nchunks + nb - 1) / nb * 64);
     t.W = tile_slice_bytes(t.rmax, tsize) / tsize;
+    if (t.W <= 0) return fail(SOV_INVALID_ARGUMENT, "tiles: the row sums leave no room for a slice");
     t.cols = cols;
     t.S = static_cast<int32_t>((static_cast<int64_t>(cols) + t.W - 1) / t.W);
     // units: a slice with at least kTileStageMin entries per group on
