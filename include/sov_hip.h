@@ -467,8 +467,9 @@ int sov_hip_batch_set_split(sov_hip_batch* batch, int on);
  * The launch runs on the batch's own stream, after everything queued so far
  * on the context stream and beside what is queued there next (the step
  * launches, which skip the resident instances); sov_hip_batch_resident_join
- * makes the context stream wait for it, and must come before anything that
- * reads the iterates (a check, a restart, a copy).  No reference counterpart
+ * makes the context stream wait for it.  sov_hip_batch_check, _restart and
+ * _promote join first by themselves; anything else that reads the iterates
+ * (a copy, a product) must come after a join.  No reference counterpart
  * (a persistent batched launch). */
 int sov_hip_batch_resident_steps(sov_hip_ctx* ctx, const sov_hip_csr* csr, const sov_hip_batch* batch,
                                  const sov_hip_pdhg_state* st, int count);

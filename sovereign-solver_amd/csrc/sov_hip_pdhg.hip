@@ -1776,6 +1776,16 @@ static hipStream_t batch_side(sov_hip_ctx* ctx, const sov_hip_batch* bt, hipErro
     return bt->side;
 }
 
+// The context stream waits for what the batch's own stream holds (a no-op
+// when nothing was forked since the last join).
+static int batch_join(sov_hip_ctx* ctx, const sov_hip_batch* bt) {
+    if (!bt->forked) return SOV_OK;
+    SOV_HIP_TRY(hipEventRecord(bt->ev_join, bt->side));
+    SOV_HIP_TRY(hipStreamWaitEvent(ctx->stream, bt->ev_join, 0));
+    bt->forked = false;
+    return SOV_OK;
+}
+
 // The stream of a batch step launch: the batch's own stream for the FP64
 // state when the batch splits its precisions, else the context stream.
 template <typename T>
@@ -2141,12 +2151,8 @@ int sov_hip_batch_dual_step(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
 
 int sov_hip_batch_resident_join(sov_hip_ctx* ctx, const sov_hip_batch* bt) {
     if (ctx == nullptr || bt == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch resident_join args");
-    if (!bt->forked) return SOV_OK;
     DeviceGuard3 g(ctx->device);
-    SOV_HIP_TRY(hipEventRecord(bt->ev_join, bt->side));
-    SOV_HIP_TRY(hipStreamWaitEvent(ctx->stream, bt->ev_join, 0));
-    bt->forked = false;
-    return SOV_OK;
+    return batch_join(ctx, bt);
 }
 
 int sov_hip_batch_resident_steps(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_batch* bt,
@@ -2172,6 +2178,8 @@ int sov_hip_batch_check(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_ba
         return fail(SOV_INVALID_ARGUMENT, "batch check: null original-space array");
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch check during capture");
     DeviceGuard3 g(ctx->device);
+    rc = batch_join(ctx, bt);   // the iterates may still be in flight on the batch's stream
+    if (rc != SOV_OK) return rc;
     return a->st->prec == SOV_PREC_F64 ? batch_check_impl<double>(ctx, h, bt, a, obj_const, out)
                                        : batch_check_impl<float>(ctx, h, bt, a, obj_const, out);
 }
@@ -2185,6 +2193,8 @@ int sov_hip_batch_restart(sov_hip_ctx* ctx, const sov_hip_batch* bt, const sov_h
     if (st->m > 0 && act == nullptr) return fail(SOV_INVALID_ARGUMENT, "batch restart: act is null");
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch restart during capture");
     DeviceGuard3 g(ctx->device);
+    rc = batch_join(ctx, bt);
+    if (rc != SOV_OK) return rc;
     SOV_HIP_TRY(hipMemcpyAsync(bt->flags, flags, sizeof(int32_t) * bt->count, hipMemcpyHostToDevice, ctx->stream));
     if (st->prec == SOV_PREC_F64) {
         if (st->n > 0)
@@ -2240,6 +2250,8 @@ int sov_hip_batch_promote(sov_hip_ctx* ctx, const sov_hip_batch* bt, const sov_h
         return fail(SOV_INVALID_ARGUMENT, "batch promote: needs an FP32 and an FP64 state");
     if (ctx->capturing) return fail(SOV_INVALID_ARGUMENT, "batch promote during capture");
     DeviceGuard3 g(ctx->device);
+    rc = batch_join(ctx, bt);
+    if (rc != SOV_OK) return rc;
     const size_t c = static_cast<size_t>(bt->count);
     SOV_HIP_TRY(hipMemcpyAsync(bt->flags, flags, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
     SOV_HIP_TRY(hipMemcpyAsync(bt->use_avg, use_avg, sizeof(int32_t) * c, hipMemcpyHostToDevice, ctx->stream));
