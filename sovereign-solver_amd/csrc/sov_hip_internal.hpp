@@ -349,8 +349,8 @@ struct sov_hip_batch {
     double* terms = nullptr;        // 11 * (m + n): per-row check terms
     // per step-kernel block of each part of A (blk[0]) and A^T (blk[1]):
     // bit 0 / 1 = holds a row the CSR step launch of the FP32 / FP64 state
-    // runs (an instance iterating in it and not run by the instance-staged
-    // launch), bit 2 / 3 = holds a row of an instance iterating in FP32 /
+    // runs (an instance iterating in it and not resident now), bit 2 / 3 =
+    // holds a row of an instance iterating in FP32 /
     // FP64; and per part the compacted list of the blocks with bit j set
     // (list[.][4q + j]): bits 0-1 for the step launches, 2-3 for the checks.
     // Recomputed whenever `active` is set.

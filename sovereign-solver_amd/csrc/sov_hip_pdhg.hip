@@ -47,7 +47,7 @@ struct BatchView {
     int k = 0;
     int want = 0;   // the active value this launch's state serves
     const int32_t* list = nullptr;  // the blocks this launch runs (compacted), or null for all
-    const int32_t* skip = nullptr;  // per instance: 1 = its rows run in the instance-staged launch instead
+    const int32_t* skip = nullptr;  // per instance: 1 = resident (its rows run in the resident launch instead)
 };
 
 // The per-row updates of the two steps, shared by every kernel that makes
