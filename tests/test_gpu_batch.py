@@ -164,6 +164,25 @@ def test_resident_intervals_equal_the_step_launches(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_whole_solves_in_resident_intervals(gpu, prec):
+    """Every instance small enough to be resident from the first interval:
+    each solve runs entirely in resident launches and ends where the step
+    launches end, bit for bit, at the oracle's objective."""
+    lps = c0_models([60, 150, 90, 230, 300, 120], seed0=90)
+    kw = dict(tolerance=1e-6, precision=prec, autotune=False, max_iterations=400000, tile_share=-1.0)
+    steps = S.solve_batch(lps, stage_bytes=0, **kw)[0]
+    res = S.solve_batch(lps, **kw)[0]
+    for b, (lp, r0, r) in enumerate(zip(lps, steps, res)):
+        assert r.status == r0.status == "optimal", b
+        assert r.iterations == r0.iterations and r.objective == r0.objective, (b, r.iterations, r0.iterations)
+        np.testing.assert_array_equal(r.x, r0.x)
+        np.testing.assert_array_equal(r.y, r0.y)
+        o = O.pdhg(O.Model(lp.arrays()), tolerance=1e-6, termination=1, restart_scheme=1, max_iterations=400000)
+        tol = 2e-6 * (1.0 + abs(r.objective) + abs(o["objective"]))
+        assert abs(r.objective - o["objective"]) <= tol, (b, r.objective, o["objective"])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_streams_and_graphs_do_not_change_the_iterations(gpu, prec):
     """The FP64 instances' steps and the resident intervals on the batch's
     own stream (split) or all on the context stream, in captured interval
