@@ -71,7 +71,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // GEMM on the matrix cores
 // ---------------------------------------------------------------------------
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // C = op(A) op(B), op(A) M x K, op(B) K x N, BF16 in, FP32 accumulate.
@@ -100,45 +100,79 @@ struct GemmArgs {
     int act;
 };
 
-constexpr int kTile = 64;   // block tile (4 waves of 32 x 32)
+constexpr int kTile = 64;      // block tile (4 waves of 32 x 32)
+constexpr int kTileK = 32;     // k per LDS stage: two 16-deep MFMA steps
+constexpr int kLdsRow = 40;    // LDS row stride in elements (80 bytes: 16-byte aligned, banks staggered)
 
-__device__ __forceinline__ short load_a(const GemmArgs& g, int i, int k) {
-    if (i >= g.M || k >= g.K) return 0;
-    return (short)(g.ta ? g.A[(int64_t)k * g.lda + i] : g.A[(int64_t)i * g.lda + k]);
-}
-
-__device__ __forceinline__ short load_b(const GemmArgs& g, int k, int j) {
-    if (j >= g.N || k >= g.K) return 0;
-    return (short)(g.tb ? g.B[(int64_t)j * g.ldb + k] : g.B[(int64_t)k * g.ldb + j]);
-}
-
-// Each wave computes a 32 x 32 tile with v_mfma_f32_32x32x8_bf16: lane l
-// supplies op(A) row (l % 32) and op(B) column (l % 32) at the 4 k values
-// 4 (l / 32) .. +3 of each 8-wide k step, and holds C rows
-// 8 i + 4 (l / 32) + r (i, r in 0..3) of column l % 32 in acc[4 i + r].
-__global__ void __launch_bounds__(kBlock) k_gemm(GemmArgs g) {
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int row0 = blockIdx.y * kTile + (wave / 2) * 32;
-    const int col0 = blockIdx.x * kTile + (wave % 2) * 32;
-    const int li = lane % 32, kq = (lane / 32) * 4;
-    f32x16 acc = {};
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
-        bf16x4 a, b;
-        a[0] = load_a(g, row0 + li, k0 + kq + 0);
-        a[1] = load_a(g, row0 + li, k0 + kq + 1);
-        a[2] = load_a(g, row0 + li, k0 + kq + 2);
-        a[3] = load_a(g, row0 + li, k0 + kq + 3);
-        b[0] = load_b(g, k0 + kq + 0, col0 + li);
-        b[1] = load_b(g, k0 + kq + 1, col0 + li);
-        b[2] = load_b(g, k0 + kq + 2, col0 + li);
-        b[3] = load_b(g, k0 + kq + 3, col0 + li);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+// Stage op(A) rows [row0, row0 + 64) x k [k0, k0 + 32) into sa[i][k] and
+// op(B) columns [col0, col0 + 64) x the same k into sb[j][k]: k contiguous
+// for both, whatever the storage order, zero outside the matrices.  Each of
+// the 256 threads moves 8 elements of each; consecutive threads read
+// consecutive addresses of the stored operand.
+__device__ __forceinline__ void stage(const GemmArgs& g, int row0, int col0, int k0, uint16_t* sa, uint16_t* sb) {
+    const int t = threadIdx.x;
+    if (!g.ta) {   // A stored M x K: 8 consecutive k of one row
+        const int i = t / 4, kv = (t % 4) * 8, gi = row0 + i;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gk = k0 + kv + q;
+            sa[i * kLdsRow + kv + q] = (gi < g.M && gk < g.K) ? g.A[(int64_t)gi * g.lda + gk] : uint16_t(0);
+        }
+    } else {       // A stored K x M: 8 consecutive rows of one k
+        const int k = t / 8, iv = (t % 8) * 8, gk = k0 + k;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gi = row0 + iv + q;
+            sa[(iv + q) * kLdsRow + k] = (gi < g.M && gk < g.K) ? g.A[(int64_t)gk * g.lda + gi] : uint16_t(0);
+        }
     }
-    const int j = col0 + li;
+    if (g.tb) {    // B stored N x K: 8 consecutive k of one column
+        const int j = t / 4, kv = (t % 4) * 8, gj = col0 + j;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gk = k0 + kv + q;
+            sb[j * kLdsRow + kv + q] = (gj < g.N && gk < g.K) ? g.B[(int64_t)gj * g.ldb + gk] : uint16_t(0);
+        }
+    } else {       // B stored K x N: 8 consecutive columns of one k
+        const int k = t / 8, jv = (t % 8) * 8, gk = k0 + k;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gj = col0 + jv + q;
+            sb[(jv + q) * kLdsRow + k] = (gj < g.N && gk < g.K) ? g.B[(int64_t)gk * g.ldb + gj] : uint16_t(0);
+        }
+    }
+}
+
+// Each wave computes a 32 x 32 tile with v_mfma_f32_32x32x16_bf16: lane l
+// supplies op(A) row (l % 32) and op(B) column (l % 32) at the 8 k values
+// 8 (l / 32) .. +7 of each 16-deep step (one 16-byte LDS read each), and
+// holds C rows 8 i + 4 (l / 32) + r (i, r in 0..3) of column l % 32 in
+// acc[4 i + r].
+__global__ void __launch_bounds__(kBlock) k_gemm(GemmArgs g) {
+    __shared__ __align__(16) uint16_t sa[kTile * kLdsRow];
+    __shared__ __align__(16) uint16_t sb[kTile * kLdsRow];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int brow = blockIdx.y * kTile, bcol = blockIdx.x * kTile;
+    const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;
+    const int li = lane % 32, kq = (lane / 32) * 8;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < g.K; k0 += kTileK) {
+        stage(g, brow, bcol, k0, sa, sb);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kTileK; ks += 16) {
+            const uint4 av = *reinterpret_cast<const uint4*>(sa + (wr + li) * kLdsRow + ks + kq);
+            const uint4 bv = *reinterpret_cast<const uint4*>(sb + (wc + li) * kLdsRow + ks + kq);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                          __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int j = bcol + wc + li;
     if (j >= g.N) return;
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        const int i = row0 + 8 * (t / 4) + (lane / 32) * 4 + (t % 4);
+        const int i = brow + wr + 8 * (t / 4) + (lane / 32) * 4 + (t % 4);
         if (i >= g.M) continue;
         float r = acc[t];
         if (g.bias) r += g.bias[j];
