@@ -1,6 +1,6 @@
 // The learned warm start's network: a fully connected MLP with GELU hidden
 // layers, BF16 weights and activations and FP32 accumulation on the MFMA
-// matrix cores (v_mfma_f32_32x32x8_bf16), trained on the GPU with Adam.
+// matrix cores (v_mfma_f32_32x32x16_bf16), trained on the GPU with Adam.
 //
 // No reference counterpart: the network predicts a dual warm start and a
 // primal weight for PDHG on a parametric LP family.
