@@ -403,7 +403,14 @@ def solve_batch(lps, tolerance=1e-6, max_iterations=1000000, time_limit=0.0, res
                 warm_duals=None, primal_weights=None, stage_bytes=None, tile_share=None, split=None):
     """PdhgBatch(options).solve(problems, hip backend) through
     sov_pdhg_solve_batch: every model solved in one stacked device batch.
-    Returns (list of Result, dict of batch timings)."""
+    Returns (list of Result, dict of batch timings).
+
+    stage_bytes: LDS a resident instance may take (None: the device's 160
+    KiB, 0: none; resident instances run whole check intervals in one
+    workgroup while every instance of their precision fits).  tile_share:
+    the share of the entries from which a precision's steps run on the tiled
+    layouts (None: never).  split: the FP64 instances' steps on a stream of
+    their own (None: on)."""
     L = engine()
     o = PdhgOpts()
     L.sov_pdhg_default_opts(C.byref(o))
