@@ -363,6 +363,10 @@ struct sov_hip_batch {
     int64_t version = 0;            // bumped when the block lists change
     const sov_hip_csr* csr = nullptr;
     double* red = nullptr;          // count * 22
+    int32_t* chunks = nullptr;      // the check reduction's chunk table, n_chunks records of 8
+    int32_t n_chunks = 0;
+    double* chunk_part = nullptr;   // 11 per split chunk: their block results
+    int32_t* chunk_cnt = nullptr;   // count * 2: arrivals per split instance dimension (0 between launches)
     double* h_red = nullptr;        // pinned host copy, count * 22
     // Resident instances: small instances whose iterates, objective and
     // bounds fit stage_bytes of LDS and whose rows fit the resident kernel's

@@ -1134,43 +1134,84 @@ static int check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hip_chec
 // ---------------------------------------------------------------------------
 
 
-// Block b reduces instance b's 11 per-row terms of both dimensions: slots
-// 0..6 summed, 7..10 maximised; out[b * 22 + dim * 11 + slot].  Instances
-// not iterating in the checked state (active[b] != want) are skipped.
+// Reduces each instance's 11 per-row terms of both dimensions: slots 0..6
+// summed, 7..10 maximised; out[b * 22 + dim * 11 + slot].  Instances not
+// iterating in the checked state (active[b] != want) are skipped.  One
+// workgroup per chunk of the chunk table (batch_reduce_chunks):
+// * a whole chunk (c < 0) is all of instance b's entries of one dimension:
+//   one sweep, each term keeping its own accumulation order (classes, then
+//   entries, ascending) and reduction tree, written straight to out;
+// * an instance dimension with more than kReduceChunk entries is split
+//   into chunks of at most kReduceChunk entries of one class, so that a
+//   large instance is read by many CUs at once.  Each chunk's block result
+//   goes to its slot of part with device-scope stores; the chunk that
+//   arrives last at the instance dimension's counter resets it for the next
+//   launch and adds the slots in table order (classes, then entries,
+//   ascending) with device-scope loads, so the sums do not depend on the
+//   arrival order.
+constexpr int kReduceChunk = 8192;
+constexpr int kReduceRec = 8;   // b, dim, c, i0, i1, slot, chunks of (b, dim), first slot
+
 __global__ void __launch_bounds__(kBlock) k_batch_reduce(const int32_t* __restrict__ seg,
                                                          const double* __restrict__ rows_t, int64_t m,
                                                          const double* __restrict__ cols_t, int64_t n,
                                                          const int32_t* __restrict__ active, int want,
+                                                         const int32_t* __restrict__ chunks,
+                                                         double* __restrict__ part, int32_t* __restrict__ cnt,
                                                          double* __restrict__ out) {
-    const int b = blockIdx.x;
+    const int32_t* rec = chunks + (int64_t)blockIdx.x * kReduceRec;
+    const int b = rec[0];
     if (active[b] != want) return;   // uniform per block
-    // one sweep over the instance's entries per dimension, all 11 terms of
-    // an entry loaded together (7 sums, then 4 maxima); each term keeps its
-    // own accumulation order (classes, then entries, ascending) and
-    // reduction tree, so the sums are those of one sweep per term
-    for (int dim = 0; dim < 2; ++dim) {
-        const double* t = dim == 0 ? rows_t : cols_t;
-        const int64_t stride = dim == 0 ? m : n;
-        double sm[7], mx[4];
+    const int dim = rec[1];
+    const double* t = dim == 0 ? rows_t : cols_t;
+    const int64_t stride = dim == 0 ? m : n;
+    double sm[7], mx[4];
 #pragma unroll
-        for (int k = 0; k < 7; ++k) sm[k] = 0.0;
+    for (int k = 0; k < 7; ++k) sm[k] = 0.0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) mx[k] = 0.0;
+    for (int k = 0; k < 4; ++k) mx[k] = 0.0;
+    auto add = [&](int64_t at) {
+        double v[11];
+#pragma unroll
+        for (int k = 0; k < 11; ++k) v[k] = t[k * stride + at];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) sm[k] = sm[k] + v[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mx[k] = nmax(mx[k], v[7 + k]);
+    };
+    if (rec[2] < 0) {
         for (int c = 0; c < kClasses; ++c) {
             const int2 r = batch_range(seg, b, dim, c);
-            for (int i = threadIdx.x; i < r.y; i += kBlock) {
-                const int64_t at = r.x + i;
-                double v[11];
-#pragma unroll
-                for (int k = 0; k < 11; ++k) v[k] = t[k * stride + at];
-#pragma unroll
-                for (int k = 0; k < 7; ++k) sm[k] = sm[k] + v[k];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) mx[k] = nmax(mx[k], v[7 + k]);
-            }
+            for (int i = threadIdx.x; i < r.y; i += kBlock) add(r.x + (int64_t)i);
         }
         block_reduce_write<7, 4>(sm, mx, out + (int64_t)b * 22 + dim * 11);
-        __syncthreads();   // the reduction's LDS is reused by the next dimension
+        return;
+    }
+    const int2 r = batch_range(seg, b, dim, rec[2]);
+    for (int i = rec[3] + threadIdx.x; i < rec[4]; i += kBlock) add(r.x + (int64_t)i);
+    __shared__ double res[11];
+    __shared__ int arrived;
+    block_reduce_write<7, 4>(sm, mx, res);
+    __syncthreads();
+    if (threadIdx.x < 11)
+        __hip_atomic_store(part + (int64_t)rec[5] * 11 + threadIdx.x, res[threadIdx.x], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);   // the stores have completed before the block counts itself in
+    __syncthreads();
+    int32_t* ctr = cnt + (int64_t)b * 2 + dim;
+    if (threadIdx.x == 0) arrived = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (arrived != rec[6] - 1) return;
+    if (threadIdx.x == 0) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 11) {
+        const int k = threadIdx.x;
+        double acc = 0.0;
+        for (int j = 0; j < rec[6]; ++j) {
+            const double v = __hip_atomic_load(part + ((int64_t)rec[7] + j) * 11 + k, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+            acc = k < 7 ? acc + v : nmax(acc, v);
+        }
+        out[(int64_t)b * 22 + dim * 11 + k] = acc;
     }
 }
 
@@ -1595,8 +1636,9 @@ static int batch_check_impl(sov_hip_ctx* ctx, const sov_hip_csr* h, const sov_hi
         op.bv = batch_view<T>(bt, false, 0, k, true);
         k_check<CheckColOp<T, true>><<<blocks, kBlock, 0, ctx->stream>>>(op);
     }
-    k_batch_reduce<<<bt->count, kBlock, 0, ctx->stream>>>(bt->seg, rows_t, bt->m, cols_t, bt->n, bt->active,
-                                                          std::is_same<T, float>::value ? 1 : 2, bt->red);
+    k_batch_reduce<<<bt->n_chunks, kBlock, 0, ctx->stream>>>(bt->seg, rows_t, bt->m, cols_t, bt->n, bt->active,
+                                                             std::is_same<T, float>::value ? 1 : 2, bt->chunks,
+                                                             bt->chunk_part, bt->chunk_cnt, bt->red);
     SOV_HIP_TRY(check_launch());
     SOV_HIP_TRY(hipMemcpyAsync(bt->h_red, bt->red, sizeof(double) * 22 * bt->count, hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -1927,11 +1969,37 @@ int batch_ok(const sov_hip_csr* h, const sov_hip_batch* bt, const sov_hip_pdhg_s
     return SOV_OK;
 }
 
+// The check reduction's chunk table (k_batch_reduce): per instance and
+// dimension, one whole record when it has at most kReduceChunk entries,
+// else one record per kReduceChunk entries of each class, in class order.
+// Returns the split chunks' slot count.
+int32_t batch_reduce_chunks(const std::vector<int32_t>& seg, int count, std::vector<int32_t>& table) {
+    table.clear();
+    int32_t slots = 0;
+    for (int b = 0; b < count; ++b)
+        for (int dim = 0; dim < 2; ++dim) {
+            auto len = [&](int c) { return seg[((static_cast<size_t>(b) * 2 + dim) * kClasses + c) * 2 + 1]; };
+            int64_t total = 0;
+            for (int c = 0; c < kClasses; ++c) total += len(c);
+            if (total <= kReduceChunk) {
+                table.insert(table.end(), {b, dim, -1, 0, 0, 0, 1, 0});
+                continue;
+            }
+            const int32_t first = slots;
+            int32_t nch = 0;
+            for (int c = 0; c < kClasses; ++c) nch += (len(c) + kReduceChunk - 1) / kReduceChunk;
+            for (int c = 0; c < kClasses; ++c)
+                for (int32_t i0 = 0; i0 < len(c); i0 += kReduceChunk)
+                    table.insert(table.end(), {b, dim, c, i0, std::min(len(c), i0 + kReduceChunk), slots++, nch, first});
+        }
+    return slots;
+}
+
 void free_batch(sov_hip_batch* bt) {
     for (void* p : {(void*)bt->row_inst, (void*)bt->col_inst, (void*)bt->seg, (void*)bt->tau, (void*)bt->sigma,
                     (void*)bt->active, (void*)bt->since, (void*)bt->flags, (void*)bt->use_avg, (void*)bt->fac,
-                    (void*)bt->terms,
-                    (void*)bt->red})
+                    (void*)bt->terms, (void*)bt->red, (void*)bt->chunks, (void*)bt->chunk_part,
+                    (void*)bt->chunk_cnt})
         (void)hipFree(p);
     if (bt->h_red) (void)hipHostFree(bt->h_red);
     if (bt->side) {
@@ -2048,6 +2116,15 @@ int sov_hip_batch_create(sov_hip_ctx* ctx, const sov_hip_csr* h, int32_t count, 
     if (e == hipSuccess) e = hipMalloc(&bt->terms, sizeof(double) * 11 * (m1 + n1));
     if (e == hipSuccess) e = hipMalloc(&bt->red, sizeof(double) * 22 * count);
     if (e == hipSuccess) e = hipHostMalloc(&bt->h_red, sizeof(double) * 22 * count, hipHostMallocDefault);
+    std::vector<int32_t> table;
+    const int32_t slots = batch_reduce_chunks(seg, count, table);
+    bt->n_chunks = static_cast<int32_t>(table.size() / kReduceRec);
+    if (e == hipSuccess) e = hipMalloc(&bt->chunks, sizeof(int32_t) * table.size());
+    if (e == hipSuccess) e = hipMalloc(&bt->chunk_part, sizeof(double) * 11 * std::max<int32_t>(slots, 1));
+    if (e == hipSuccess) e = hipMalloc(&bt->chunk_cnt, sizeof(int32_t) * 2 * count);
+    if (e == hipSuccess)
+        e = hipMemcpy(bt->chunks, table.data(), sizeof(int32_t) * table.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(bt->chunk_cnt, 0, sizeof(int32_t) * 2 * count, ctx->stream);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->side, hipStreamNonBlocking);   // default priority
     if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev_join, hipEventDisableTiming);

@@ -64,7 +64,10 @@ def ctx(gpu):
 def test_batch_check_and_norms_match_each_instance(ctx, K):
     rng = np.random.default_rng(17 + K)
     models = [O.Model(S.LP.random_zipf(m, n, seed, kmax=min(m, 200)).arrays()) for m, n, seed in
-              ((300, 700, 1), (40, 90, 2), (900, 2100, 3), (120, 260, 4))]
+              ((300, 700, 1), (40, 90, 2), (900, 2100, 3), (120, 260, 4), (9000, 20000, 5))]
+    # the last instance's rows and columns exceed the check reduction's
+    # 8192-entry chunk, so its sums are split over chunks and finished by
+    # the last chunk to arrive
     d, ro, co = stack(models)
     B = len(models)
     A = Csr(ctx, d["m"], d["n"], d["row_ptr"], d["col_idx"], d["val"])
@@ -73,7 +76,7 @@ def test_batch_check_and_norms_match_each_instance(ctx, K):
     bt = C.c_void_p()
     check(ctx.L.sov_hip_batch_create(ctx.h, A.h, B, ro.ctypes.data, co.ctypes.data, C.byref(bt)))
     pr, pc = A.order(0), A.order(1)
-    since = np.array([3, 5, 1, 7], np.int32)
+    since = np.array([3, 5, 1, 7, 4], np.int32)
     check(ctx.L.sov_hip_batch_set(ctx.h, bt, None, None, None, since.ctypes.data))
     # a point with some columns on their bounds, per-instance running sums
     x = d["col_lo"] + rng.uniform(0, 1, d["n"]) * (d["col_hi"] - d["col_lo"])
@@ -115,7 +118,7 @@ def test_batch_check_and_norms_match_each_instance(ctx, K):
     sq = np.zeros(B)
     check(ctx.L.sov_hip_batch_sumsq(ctx.h, bt, 0, dv, sq.ctypes.data))
     np.testing.assert_allclose(sq, [np.sum(v[ro[b]:ro[b + 1]] ** 2) for b in range(B)], rtol=1e-13)
-    fac = np.array([2.0, -1.0, 0.5, 3.0])
+    fac = np.array([2.0, -1.0, 0.5, 3.0, 1.5])
     check(ctx.L.sov_hip_batch_scale(ctx.h, bt, 0, fac.ctypes.data, dv))
     got = np.empty(d["m"])
     got[pr] = ctx.download(dv, d["m"])
