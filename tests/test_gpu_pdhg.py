@@ -1,7 +1,7 @@
 """GPU parity of the PDHG engine (C++ host loop over the HIP C-ABI) against the
 CPU oracle and the reference's recorded results.
 
-Bar (north star): same termination status, objective within 1e-6 relative,
+Bar: same termination status, objective within 1e-6 relative,
 relative KKT under the run's tolerance; FP64 trajectories agree with the
 oracle to rounding, FP32 runs are stated against the FP64 oracle solve.
 """
