@@ -412,7 +412,7 @@ int sov_hip_pdhg_combine(sov_hip_ctx* ctx, const sov_hip_pdhg_state* correction,
                          const double* x0, const double* y0, int64_t since, double inv_delta);
 
 /* ---- batches of LPs ---------------------------------------------------
- * No reference counterpart (the north star's batched solve).  A batch is a
+ * No reference counterpart (many LPs solved in one launch sequence).  A batch is a
  * block-diagonal stack of independent LPs in ONE csr handle: instance b owns
  * natural rows [row_off[b], row_off[b+1]) and columns [col_off[b],
  * col_off[b+1]) (count + 1 offsets each, covering the matrix).  All state
